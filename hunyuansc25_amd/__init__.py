@@ -1,0 +1,351 @@
+"""hunyuansc25_amd -- Python binding of the MI355X multilevel k-way partitioner.
+
+The product is the C ABI in ``libhgp.so`` (declared in ``include/hunyuangraph.h``);
+this module only loads it with ctypes and converts numpy arrays.  There is no
+Python or CPU fallback: if the HIP library is missing every call raises.
+
+``PartitionGraph`` mirrors the reference entry point ``hunyuangraph_PartitionGraph``
+(hunyuangraph_partitiongraph.h:284): int32 CSR in, ``part`` out.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhgp.so")
+
+HGP_OK = 0
+ERRORS = {
+    -1: "invalid input",
+    -2: "device error",
+    -3: "out of device memory",
+    -4: "unsupported",
+    -5: "no HIP device",
+}
+
+MAX_LEVELS = 64
+
+
+class HgpOptions(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("ub_num", C.c_int32),
+        ("ub_den", C.c_int32),
+        ("match_rounds", C.c_int32),
+        ("trials", C.c_int32),
+        ("fm_passes", C.c_int32),
+        ("patience", C.c_int32),
+        ("validate", C.c_int32),
+        ("device", C.c_int32),
+    ]
+
+
+class HgpStats(C.Structure):
+    _fields_ = [
+        ("cut", C.c_int64),
+        ("max_pwgt", C.c_int64),
+        ("tvwgt", C.c_int64),
+        ("levels", C.c_int32),
+        ("coarsest_n", C.c_int32),
+        ("coarsest_m", C.c_int64),
+        ("init_cut", C.c_int64),
+        ("refine_iters", C.c_int64),
+        ("ms_total", C.c_double),
+        ("ms_coarsen", C.c_double),
+        ("ms_init", C.c_double),
+        ("ms_uncoarsen", C.c_double),
+        ("alg_bytes", C.c_double),
+        ("level_n", C.c_int64 * MAX_LEVELS),
+        ("level_m", C.c_int64 * MAX_LEVELS),
+        ("level_iters", C.c_int32 * MAX_LEVELS),
+    ]
+
+    def to_dict(self) -> dict:
+        L = self.levels
+        return {
+            "cut": int(self.cut),
+            "max_pwgt": int(self.max_pwgt),
+            "tvwgt": int(self.tvwgt),
+            "levels": int(L),
+            "coarsest_n": int(self.coarsest_n),
+            "coarsest_m": int(self.coarsest_m),
+            "init_cut": int(self.init_cut),
+            "refine_iters": int(self.refine_iters),
+            "ms_total": float(self.ms_total),
+            "ms_coarsen": float(self.ms_coarsen),
+            "ms_init": float(self.ms_init),
+            "ms_uncoarsen": float(self.ms_uncoarsen),
+            "alg_bytes": float(self.alg_bytes),
+            "level_n": [int(self.level_n[i]) for i in range(L + 1)],
+            "level_m": [int(self.level_m[i]) for i in range(L + 1)],
+            "level_iters": [int(self.level_iters[i]) for i in range(L + 1)],
+        }
+
+
+class HgpGraph(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32),
+        ("m", C.c_int64),
+        ("xadj", C.POINTER(C.c_int64)),
+        ("adjncy", C.POINTER(C.c_int32)),
+        ("adjwgt", C.POINTER(C.c_int32)),
+        ("vwgt", C.POINTER(C.c_int32)),
+    ]
+
+
+# Every symbol declared in include/hunyuangraph.h, with its ctypes signature.
+_P = C.c_void_p
+_I32, _I64, _U64, _F64 = C.c_int32, C.c_int64, C.c_uint64, C.c_double
+SIGNATURES = {
+    "hunyuangraph_PartitionGraph": (None, [_P] * 9),
+    "hgp_last_error": (C.c_char_p, []),
+    "hgp_version": (C.c_char_p, []),
+    "hgp_device_count": (C.c_int, []),
+    "hgp_default_options": (None, [_P]),
+    "hgp_partition": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
+    "hgp_partition_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
+    "hgp_scan_i32": (C.c_int, [_P, _P, _I64]),
+    "hgp_scan_i64": (C.c_int, [_P, _P, _I64]),
+    "hgp_part_weights": (C.c_int, [_I32, _P, _P, _I32, _P]),
+    "hgp_edge_cut": (C.c_int, [_I32, _P, _P, _P, _P, _P]),
+    "hgp_match": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _U64, _I32, _I32, _P, _P]),
+    "hgp_contract": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "hgp_initpart": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P]),
+    "hgp_refine": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I64, _I32, _P, _P, _P, _P]),
+    "hgp_graph_free": (None, [_P]),
+    "hgp_gen_grid2d": (C.c_int, [_I32, _I32, _P]),
+    "hgp_gen_rgg3d": (C.c_int, [_I32, _F64, _U64, _I32, _I32, _I32, _I32, _P]),
+    "hgp_gen_rmat": (C.c_int, [_I32, _I64, _F64, _F64, _F64, _U64, _P]),
+    "hgp_gen_banded": (C.c_int, [_I32, _I32, _F64, _U64, _P]),
+}
+
+_lib_handle = None
+
+
+def lib() -> C.CDLL:
+    """Loads libhgp.so (raises if it has not been built)."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib_handle = h
+    return _lib_handle
+
+
+class HgpError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hgp error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+def _check(rc: int) -> None:
+    if rc != HGP_OK:
+        raise HgpError(rc, lib().hgp_last_error().decode())
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _i64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def options(**kw) -> HgpOptions:
+    o = HgpOptions()
+    lib().hgp_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def device_count() -> int:
+    return int(lib().hgp_device_count())
+
+
+# ------------------------------------------------------------------ drop-in mirror
+
+def PartitionGraph(nvtxs, xadj, adjncy, vwgt, adjwgt, nparts, tpwgts=None, ubvec=None) -> np.ndarray:
+    """Mirror of hunyuangraph_PartitionGraph (hunyuangraph_partitiongraph.h:284):
+    int32 CSR in, part[] out.  vwgt/adjwgt may be None (unit weights); ubvec is
+    the balance bound (default 1.03); tpwgts are uniform as in the reference."""
+    xadj = _i32(xadj)
+    adjncy = _i32(adjncy)
+    vw = None if vwgt is None else _i32(vwgt)
+    aw = None if adjwgt is None else _i32(adjwgt)
+    n = C.c_int(int(nvtxs))
+    k = C.c_int(int(nparts))
+    ub = None if ubvec is None else (C.c_float * 1)(float(ubvec))
+    part = np.zeros(int(nvtxs), dtype=np.int32)
+    L = lib()
+    L.hunyuangraph_PartitionGraph(C.byref(n), _ptr(xadj), _ptr(adjncy), _ptr(vw), _ptr(aw), C.byref(k), None,
+                                  None if ub is None else C.cast(ub, C.c_void_p), _ptr(part))
+    err = L.hgp_last_error().decode()
+    if err:
+        raise HgpError(-1, err)
+    return part
+
+
+# ------------------------------------------------------------------ versioned API
+
+def partition(xadj, adjncy, k, vwgt=None, adjwgt=None, **opts):
+    """Full pipeline on host arrays; returns (part, stats dict)."""
+    xadj = _i64(xadj)
+    adjncy = _i32(adjncy)
+    n = len(xadj) - 1
+    vw = None if vwgt is None else _i32(vwgt)
+    aw = None if adjwgt is None else _i32(adjwgt)
+    o = options(**opts)
+    st = HgpStats()
+    part = np.zeros(max(n, 0), dtype=np.int32)
+    _check(lib().hgp_partition(n, _ptr(xadj), _ptr(adjncy), _ptr(vw), _ptr(aw), int(k), C.byref(o), _ptr(part),
+                               C.byref(st)))
+    return part, st.to_dict()
+
+
+def partition_device(n, d_xadj, d_adjncy, d_vwgt, d_adjwgt, k, d_part, **opts) -> dict:
+    """Full pipeline on device pointers (ints, e.g. torch tensor data_ptr())."""
+    o = options(**opts)
+    st = HgpStats()
+    _check(lib().hgp_partition_device(int(n), C.c_void_p(d_xadj), C.c_void_p(d_adjncy), C.c_void_p(d_vwgt),
+                                      C.c_void_p(d_adjwgt), int(k), C.byref(o), C.c_void_p(d_part),
+                                      C.byref(st)))
+    return st.to_dict()
+
+
+# ------------------------------------------------------------------ stages
+
+def scan_i32(a) -> np.ndarray:
+    a = _i32(a)
+    out = np.empty_like(a)
+    _check(lib().hgp_scan_i32(_ptr(a), _ptr(out), a.size))
+    return out
+
+
+def scan_i64(a) -> np.ndarray:
+    a = _i64(a)
+    out = np.empty_like(a)
+    _check(lib().hgp_scan_i64(_ptr(a), _ptr(out), a.size))
+    return out
+
+
+def part_weights(vwgt, where, k) -> np.ndarray:
+    vwgt, where = _i32(vwgt), _i32(where)
+    out = np.zeros(int(k), dtype=np.int64)
+    _check(lib().hgp_part_weights(len(vwgt), _ptr(vwgt), _ptr(where), int(k), _ptr(out)))
+    return out
+
+
+def edge_cut(xadj, adjncy, adjwgt, where) -> int:
+    xadj, adjncy, adjwgt, where = _i64(xadj), _i32(adjncy), _i32(adjwgt), _i32(where)
+    out = np.zeros(1, dtype=np.int64)
+    _check(lib().hgp_edge_cut(len(xadj) - 1, _ptr(xadj), _ptr(adjncy), _ptr(adjwgt), _ptr(where), _ptr(out)))
+    return int(out[0])
+
+
+def match(xadj, adjncy, vwgt, adjwgt, maxvwgt, seed, level, rounds):
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    m = np.zeros(n, dtype=np.int32)
+    cnt = np.zeros(1, dtype=np.int64)
+    _check(lib().hgp_match(n, _ptr(xadj), _ptr(adjncy), _ptr(vwgt), _ptr(adjwgt), int(maxvwgt), int(seed),
+                           int(level), int(rounds), _ptr(m), _ptr(cnt)))
+    return m, int(cnt[0])
+
+
+def contract(xadj, adjncy, adjwgt, vwgt, match_arr):
+    """cmap + canonical contraction of a fixed matching; returns
+    (match_normalised, cmap, cxadj, cadjncy, cadjwgt, cvwgt)."""
+    xadj, adjncy, adjwgt, vwgt = _i64(xadj), _i32(adjncy), _i32(adjwgt), _i32(vwgt)
+    mt = _i32(match_arr).copy()
+    n = len(xadj) - 1
+    m = int(xadj[-1])
+    cmap = np.zeros(n, dtype=np.int32)
+    cxadj = np.zeros(n + 1, dtype=np.int64)
+    cadj = np.zeros(max(m, 1), dtype=np.int32)
+    cw = np.zeros(max(m, 1), dtype=np.int32)
+    cvw = np.zeros(max(n, 1), dtype=np.int32)
+    cn = np.zeros(1, dtype=np.int32)
+    cm = np.zeros(1, dtype=np.int64)
+    _check(lib().hgp_contract(n, _ptr(xadj), _ptr(adjncy), _ptr(adjwgt), _ptr(vwgt), _ptr(mt), _ptr(cmap), _ptr(cn),
+                              _ptr(cxadj), _ptr(cadj), _ptr(cw), _ptr(cvw), _ptr(cm)))
+    c, e = int(cn[0]), int(cm[0])
+    return mt, cmap, cxadj[: c + 1].copy(), cadj[:e].copy(), cw[:e].copy(), cvw[:c].copy()
+
+
+def initpart(xadj, adjncy, vwgt, adjwgt, k, **opts) -> np.ndarray:
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    o = options(**opts)
+    where = np.zeros(n, dtype=np.int32)
+    _check(lib().hgp_initpart(n, _ptr(xadj), _ptr(adjncy), _ptr(vwgt), _ptr(adjwgt), int(k), C.byref(o),
+                              _ptr(where)))
+    return where
+
+
+def refine(xadj, adjncy, vwgt, adjwgt, k, tvwgt, is_finest, where, cut, **opts):
+    """One level of Jet refinement; returns (where, cut, iters)."""
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    o = options(**opts)
+    w = _i32(where).copy()
+    c = np.array([cut], dtype=np.int64)
+    it = np.zeros(1, dtype=np.int32)
+    _check(lib().hgp_refine(n, _ptr(xadj), _ptr(adjncy), _ptr(vwgt), _ptr(adjwgt), int(k), int(tvwgt),
+                            int(is_finest), C.byref(o), _ptr(w), _ptr(c), _ptr(it)))
+    return w, int(c[0]), int(it[0])
+
+
+# ------------------------------------------------------------------ generators
+
+def _take_graph(g: HgpGraph):
+    n, m = int(g.n), int(g.m)
+    xadj = np.ctypeslib.as_array(g.xadj, shape=(n + 1,)).copy()
+    adjncy = np.ctypeslib.as_array(g.adjncy, shape=(max(m, 1),))[:m].copy()
+    adjwgt = np.ctypeslib.as_array(g.adjwgt, shape=(max(m, 1),))[:m].copy()
+    vwgt = np.ctypeslib.as_array(g.vwgt, shape=(max(n, 1),))[:n].copy()
+    lib().hgp_graph_free(C.byref(g))
+    return xadj, adjncy, vwgt, adjwgt
+
+
+def gen_grid2d(rows, cols):
+    g = HgpGraph()
+    _check(lib().hgp_gen_grid2d(int(rows), int(cols), C.byref(g)))
+    return _take_graph(g)
+
+
+def gen_rgg3d(n, radius, seed=1, vw=(0, 0), ew=(0, 0)):
+    g = HgpGraph()
+    _check(lib().hgp_gen_rgg3d(int(n), float(radius), int(seed), int(vw[0]), int(vw[1]), int(ew[0]), int(ew[1]),
+                               C.byref(g)))
+    return _take_graph(g)
+
+
+def gen_rmat(scale, nedges, a=0.57, b=0.19, c=0.19, seed=1):
+    g = HgpGraph()
+    _check(lib().hgp_gen_rmat(int(scale), int(nedges), float(a), float(b), float(c), int(seed), C.byref(g)))
+    return _take_graph(g)
+
+
+def gen_banded(n, bw=32, frac=0.1, seed=1):
+    g = HgpGraph()
+    _check(lib().hgp_gen_banded(int(n), int(bw), float(frac), int(seed), C.byref(g)))
+    return _take_graph(g)
+
+
+def rgg_radius(n: int, degree: float = 12.0) -> float:
+    """Radius of a 3-D RGG with the given expected degree before boundary loss."""
+    return (3.0 * degree / (4.0 * np.pi * n)) ** (1.0 / 3.0)

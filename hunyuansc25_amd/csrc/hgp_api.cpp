@@ -1,0 +1,567 @@
+// hgp_api.cpp -- pipeline orchestration (coarsen -> initial partition ->
+// uncoarsen/refine) and the C ABI.
+#include "hunyuangraph.h"
+
+#include "hgp_graph.hpp"
+#include "hgp_prims.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace hgp {
+const char *last_error_text();
+
+namespace {
+
+std::mutex g_api_mu;   // one call at a time per process on the default contexts
+
+struct RunStats {
+    hgp_stats *st;
+};
+
+// Algorithmic HBM bytes of a run: per level l with n_l vertices and m_l
+// directed entries (x = 8, int64 offsets):
+//   CSR_l = 8(n_l+1) + 8 m_l + 4 n_l
+//   coarsening (l < L): 2 CSR_l + 8 m_l + 8 n_l + CSR_{l+1}
+//   refinement: (1 + I_l)(CSR_l + 4 m_l + 8 n_l)
+//   projection (l < L): 12 n_l
+double alg_bytes(const std::vector<int64_t> &ns, const std::vector<int64_t> &ms, const std::vector<int32_t> &iters)
+{
+    auto csr = [&](size_t l) { return 8.0 * (ns[l] + 1) + 8.0 * ms[l] + 4.0 * ns[l]; };
+    double b = 0;
+    size_t L = ns.size() - 1;
+    for (size_t l = 0; l <= L; l++) {
+        if (l < L) b += 2 * csr(l) + 8.0 * ms[l] + 8.0 * ns[l] + csr(l + 1) + 12.0 * ns[l];
+        b += (1.0 + iters[l]) * (csr(l) + 4.0 * ms[l] + 8.0 * ns[l]);
+    }
+    return b;
+}
+
+size_t estimate_arena(int32_t n, int64_t m, int32_t k, bool host_copy)
+{
+    size_t b = (size_t)64 * (size_t)m + (size_t)192 * (size_t)n + ((size_t)64 << 20) + (size_t)k * 4096;
+    if (host_copy) b += (size_t)8 * (size_t)(n + 1) + (size_t)8 * (size_t)m + (size_t)8 * (size_t)n;
+    return b;
+}
+
+// The multilevel pipeline on a device-resident level-0 graph
+// (hunyuangraph_kway_partition, hunyuangraph_partitiongraph.h:13-149).
+void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_t *d_part, hgp_stats *st)
+{
+    int32_t n = g0.n;
+    int64_t *dsc = c.arena.lalloc<int64_t>(8);
+    HGP_CHECK_HIP(hipEventRecord(c.ev[0], c.stream));
+    reduce_sum(c, g0.vwgt, n, dsc);
+    read_scalars(c, dsc, sizeof(int64_t));
+    int64_t tvwgt = c.pinned[0];
+    if (k == 1) {
+        fill<int32_t>(c, d_part, n, 0);
+        HGP_CHECK_HIP(hipEventRecord(c.ev[1], c.stream));
+        HGP_CHECK_HIP(hipEventRecord(c.ev[2], c.stream));
+        HGP_CHECK_HIP(hipEventRecord(c.ev[3], c.stream));
+        HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+        if (st) { st->tvwgt = tvwgt; st->max_pwgt = tvwgt; }
+        return;
+    }
+    int32_t thr = coarsen_threshold(k);
+    int32_t maxvwgt = coarsen_maxvwgt(tvwgt, thr);
+
+    std::vector<DevGraph> lv{g0};
+    std::vector<int32_t *> cmaps;
+    // hunyuangarph_coarsen (hunyuangraph_GPU_coarsen.h:163-253): do-while with
+    // the stop rule n > thr && n < 0.85 n_finer && m > n/2 (exact integers).
+    for (;;) {
+        const DevGraph &f = lv.back();
+        int32_t *cmap = c.arena.lalloc<int32_t>((size_t)f.n);
+        size_t rmark = c.arena.right_mark();
+        int32_t *match = c.arena.ralloc<int32_t>((size_t)f.n);
+        int32_t level = (int32_t)lv.size() - 1;
+        match_level(c, f, maxvwgt, p.seed, level, p.match_rounds, match);
+        int32_t cn = build_cmap(c, f.n, match, cmap);
+        DevGraph cg = contract(c, f, match, cmap, cn);
+        c.arena.right_release(rmark);
+        cmaps.push_back(cmap);
+        lv.push_back(cg);
+        int32_t nf = f.n;
+        bool go = cg.n > thr && 100 * (int64_t)cg.n < 85 * (int64_t)nf && cg.m > cg.n / 2;
+        if (!go || (int)lv.size() >= HGP_MAX_LEVELS - 1) break;
+    }
+    HGP_CHECK_HIP(hipEventRecord(c.ev[1], c.stream));
+    int32_t L = (int32_t)lv.size() - 1;
+    const DevGraph &cg = lv[L];
+    int32_t *where = c.arena.lalloc<int32_t>((size_t)cg.n);
+    initial_partition(c, cg, k, p, where);
+    int64_t *d_cut = dsc + 1;
+    edge_cut(c, cg, where, d_cut);
+    HGP_CHECK_HIP(hipEventRecord(c.ev[2], c.stream));
+    read_scalars(c, d_cut, sizeof(int64_t));
+    int64_t init_cut = c.pinned[0];
+    std::vector<int32_t> iters(lv.size(), 0);
+    for (int32_t l = L; l >= 0; l--) {
+        RefineStats rs = refine_level(c, lv[l], k, tvwgt, l == 0, p, where, d_cut);
+        iters[l] = rs.iters;
+        if (l == 0) break;
+        int32_t *wf = (l - 1 == 0) ? d_part : c.arena.lalloc<int32_t>((size_t)lv[l - 1].n);
+        project(c, lv[l - 1].n, cmaps[l - 1], where, wf);
+        where = wf;
+    }
+    if (L == 0) HGP_CHECK_HIP(hipMemcpyAsync(d_part, where, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
+    HGP_CHECK_HIP(hipEventRecord(c.ev[3], c.stream));
+    // final statistics
+    int64_t *pw = c.arena.lalloc<int64_t>((size_t)k);
+    part_weights(c, n, g0.vwgt, d_part, k, pw);
+    std::vector<int64_t> hpw(k);
+    HGP_CHECK_HIP(hipMemcpyAsync(hpw.data(), pw, sizeof(int64_t) * (size_t)k, hipMemcpyDeviceToHost, c.stream));
+    read_scalars(c, d_cut, sizeof(int64_t));
+    if (st) {
+        float t01 = 0, t12 = 0, t23 = 0;
+        HGP_CHECK_HIP(hipEventElapsedTime(&t01, c.ev[0], c.ev[1]));
+        HGP_CHECK_HIP(hipEventElapsedTime(&t12, c.ev[1], c.ev[2]));
+        HGP_CHECK_HIP(hipEventElapsedTime(&t23, c.ev[2], c.ev[3]));
+        st->cut = c.pinned[0];
+        int64_t mx = 0;
+        for (int64_t x : hpw) mx = x > mx ? x : mx;
+        st->max_pwgt = mx;
+        st->tvwgt = tvwgt;
+        st->levels = L;
+        st->coarsest_n = cg.n;
+        st->coarsest_m = cg.m;
+        st->init_cut = init_cut;
+        st->ms_coarsen = t01;
+        st->ms_init = t12;
+        st->ms_uncoarsen = t23;
+        st->ms_total = (double)t01 + t12 + t23;
+        std::vector<int64_t> ns, ms;
+        int64_t tot = 0;
+        for (size_t l = 0; l < lv.size(); l++) {
+            ns.push_back(lv[l].n);
+            ms.push_back(lv[l].m);
+            if (l < HGP_MAX_LEVELS) {
+                st->level_n[l] = lv[l].n;
+                st->level_m[l] = lv[l].m;
+                st->level_iters[l] = iters[l];
+            }
+            tot += iters[l];
+        }
+        st->refine_iters = tot;
+        st->alg_bytes = alg_bytes(ns, ms, iters);
+    }
+}
+
+Params params_from(const hgp_options *o)
+{
+    Params p;
+    if (!o) return p;
+    p.seed = o->seed;
+    p.ub_num = o->ub_num;
+    p.ub_den = o->ub_den;
+    p.match_rounds = o->match_rounds;
+    p.trials = o->trials;
+    p.fm_passes = o->fm_passes;
+    p.patience = o->patience;
+    return p;
+}
+
+void check_params(const Params &p, int32_t k)
+{
+    if (k < 1) throw Error{-1, "nparts must be >= 1"};
+    if (k > 4096) throw Error{-4, "nparts > 4096 is not supported"};
+    if (p.ub_num <= 0 || p.ub_den <= 0 || p.ub_num < p.ub_den) throw Error{-1, "balance bound must be >= 1"};
+    if (p.match_rounds < 0 || p.trials < 1 || p.fm_passes < 0 || p.patience < 1)
+        throw Error{-1, "invalid option values"};
+}
+
+// Host-side CSR checks (the invariants of check_graph, hunyuangraph_GPU_match.h:1718-1800).
+void validate_host(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                   const int32_t *adjwgt, bool symmetric)
+{
+    if (n < 0) throw Error{-1, "nvtxs < 0"};
+    if (!xadj) throw Error{-1, "xadj is NULL"};
+    if (xadj[0] != 0) throw Error{-1, "xadj[0] != 0"};
+    for (int32_t v = 0; v < n; v++)
+        if (xadj[v + 1] < xadj[v]) throw Error{-1, "xadj is not monotone at vertex " + std::to_string(v)};
+    int64_t m = xadj[n];
+    if (m > INT32_MAX) throw Error{-4, "more than 2^31-1 directed entries"};
+    if (m > 0 && !adjncy) throw Error{-1, "adjncy is NULL"};
+    int64_t wsum = 0;
+    for (int32_t v = 0; v < n; v++) {
+        if (vwgt && vwgt[v] <= 0) throw Error{-1, "vertex weight <= 0 at vertex " + std::to_string(v)};
+        for (int64_t j = xadj[v]; j < xadj[v + 1]; j++) {
+            int32_t u = adjncy[j];
+            if (u < 0 || u >= n) throw Error{-1, "neighbour id out of range in row " + std::to_string(v)};
+            if (u == v) throw Error{-1, "self-loop at vertex " + std::to_string(v)};
+            int32_t w = adjwgt ? adjwgt[j] : 1;
+            if (w <= 0) throw Error{-1, "edge weight <= 0 in row " + std::to_string(v)};
+            wsum += w;
+        }
+    }
+    if (wsum > INT32_MAX) throw Error{-4, "total edge weight exceeds 2^31-1"};
+    if (symmetric) {
+        // every (v,u,w) must have a matching (u,v,w): compare sorted triples
+        std::vector<std::pair<uint64_t, int32_t>> a, b;
+        a.reserve((size_t)m);
+        b.reserve((size_t)m);
+        for (int32_t v = 0; v < n; v++)
+            for (int64_t j = xadj[v]; j < xadj[v + 1]; j++) {
+                int32_t u = adjncy[j], w = adjwgt ? adjwgt[j] : 1;
+                a.push_back({((uint64_t)v << 32) | (uint32_t)u, w});
+                b.push_back({((uint64_t)u << 32) | (uint32_t)v, w});
+            }
+        std::sort(a.begin(), a.end());
+        std::sort(b.begin(), b.end());
+        if (a != b) throw Error{-1, "graph is not symmetric"};
+    }
+}
+
+template <class F>
+int guarded(F f)
+{
+    clear_error();
+    try {
+        f();
+        return HGP_OK;
+    } catch (const Error &e) {
+        set_error(e.code, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error(-3, "host out of memory");
+        return HGP_ERR_OOM;
+    } catch (const std::exception &e) {
+        set_error(-2, e.what());
+        return HGP_ERR_DEVICE;
+    }
+}
+
+// Runs body with an arena of at least `bytes`; on arena overflow the arena
+// grows and the body is re-run (the body must be restartable).
+template <class F>
+void with_arena(Ctx &c, size_t bytes, F body)
+{
+    for (int attempt = 0;; attempt++) {
+        c.arena.reset();
+        c.arena.reserve(bytes);
+        try {
+            body();
+            HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+            c.arena.reset();
+            return;
+        } catch (const Error &e) {
+            (void)hipStreamSynchronize(c.stream);
+            c.arena.reset();
+            if (e.code != -3 || attempt >= 3) throw;
+            bytes *= 2;
+        }
+    }
+}
+
+template <class T>
+T *to_dev(Ctx &c, const T *h, size_t n)
+{
+    T *d = c.arena.lalloc<T>(n);
+    if (n) HGP_CHECK_HIP(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, c.stream));
+    return d;
+}
+
+template <class T>
+void to_host(Ctx &c, T *h, const T *d, size_t n)
+{
+    if (n) HGP_CHECK_HIP(hipMemcpyAsync(h, d, sizeof(T) * n, hipMemcpyDeviceToHost, c.stream));
+}
+
+DevGraph upload(Ctx &c, int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                const int32_t *adjwgt)
+{
+    DevGraph g;
+    g.n = n;
+    g.m = xadj[n];
+    g.xadj = to_dev(c, xadj, (size_t)n + 1);
+    g.adjncy = to_dev(c, adjncy, (size_t)g.m);
+    if (adjwgt) g.adjwgt = to_dev(c, adjwgt, (size_t)g.m);
+    else { g.adjwgt = c.arena.lalloc<wgt_t>((size_t)g.m); fill<int32_t>(c, g.adjwgt, g.m, 1); }
+    if (vwgt) g.vwgt = to_dev(c, vwgt, (size_t)n);
+    else { g.vwgt = c.arena.lalloc<wgt_t>((size_t)n); fill<int32_t>(c, g.vwgt, n, 1); }
+    return g;
+}
+
+}  // namespace
+}  // namespace hgp
+
+using namespace hgp;
+
+extern "C" {
+
+const char *hgp_last_error(void) { return hgp::last_error_text(); }
+
+const char *hgp_version(void) { return "hgp 0.1.0 (gfx950)"; }
+
+int hgp_device_count(void)
+{
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return cnt;
+}
+
+void hgp_default_options(hgp_options *o)
+{
+    if (!o) return;
+    o->seed = 1;
+    o->ub_num = 103;
+    o->ub_den = 100;
+    o->match_rounds = 8;
+    o->trials = 1024;
+    o->fm_passes = 10;
+    o->patience = 6;
+    o->validate = 0;
+    o->device = 0;
+}
+
+int hgp_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                  const int32_t *adjwgt, int32_t k, const hgp_options *opt, int32_t *part, hgp_stats *stats)
+{
+    return guarded([&] {
+        hgp_options o;
+        if (opt) o = *opt; else hgp_default_options(&o);
+        Params p = params_from(&o);
+        check_params(p, k);
+        if (stats) memset(stats, 0, sizeof(*stats));
+        validate_host(n, xadj, adjncy, vwgt, adjwgt, o.validate != 0);
+        if (n == 0) return;
+        if (!part) throw Error{-1, "part is NULL"};
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(o.device);
+        HGP_CHECK_HIP(hipSetDevice(c.device));
+        with_arena(c, estimate_arena(n, xadj[n], k, true), [&] {
+            DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
+            int32_t *d_part = c.arena.lalloc<int32_t>((size_t)n);
+            run_pipeline(c, g, k, p, d_part, stats);
+            to_host(c, part, d_part, (size_t)n);
+        });
+    });
+}
+
+int hgp_partition_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy, const int32_t *d_vwgt,
+                         const int32_t *d_adjwgt, int32_t k, const hgp_options *opt, int32_t *d_part,
+                         hgp_stats *stats)
+{
+    return guarded([&] {
+        hgp_options o;
+        if (opt) o = *opt; else hgp_default_options(&o);
+        Params p = params_from(&o);
+        check_params(p, k);
+        if (stats) memset(stats, 0, sizeof(*stats));
+        if (n < 0) throw Error{-1, "nvtxs < 0"};
+        if (n == 0) return;
+        if (!d_xadj || !d_adjncy || !d_vwgt || !d_adjwgt || !d_part) throw Error{-1, "NULL device pointer"};
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(o.device);
+        HGP_CHECK_HIP(hipSetDevice(c.device));
+        int64_t m = 0;
+        HGP_CHECK_HIP(hipMemcpy(&m, d_xadj + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+        if (m < 0 || m > INT32_MAX) throw Error{-4, "directed entry count out of range"};
+        with_arena(c, estimate_arena(n, m, k, false), [&] {
+            DevGraph g;
+            g.n = n;
+            g.m = m;
+            g.xadj = const_cast<eid_t *>(d_xadj);
+            g.adjncy = const_cast<vid_t *>(d_adjncy);
+            g.adjwgt = const_cast<wgt_t *>(d_adjwgt);
+            g.vwgt = const_cast<wgt_t *>(d_vwgt);
+            run_pipeline(c, g, k, p, d_part, stats);
+        });
+    });
+}
+
+void hunyuangraph_PartitionGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, int *adjwgt, int *nparts,
+                                 float *tpwgts, float *ubvec, int *part)
+{
+    (void)tpwgts;   // uniform target weights, as in the reference (hunyuangraph_admin.h:19-22)
+    guarded([&] {
+        if (!nvtxs || !xadj || !nparts) throw Error{-1, "NULL argument"};
+        int32_t n = *nvtxs;
+        std::vector<int64_t> x64((size_t)n + 1);
+        for (int32_t i = 0; i <= n; i++) x64[i] = xadj[i];
+        hgp_options o;
+        hgp_default_options(&o);
+        if (ubvec && *ubvec > 1.0f) {
+            o.ub_num = (int32_t)std::lround((double)*ubvec * 1000.0);
+            o.ub_den = 1000;
+        }
+        int rc = hgp_partition(n, x64.data(), adjncy, vwgt, adjwgt, *nparts, &o, part, nullptr);
+        if (rc != HGP_OK) throw Error{rc, hgp_last_error()};
+    });
+}
+
+// ---------------------------------------------------------------- stages
+
+int hgp_scan_i32(const int32_t *in, int32_t *out, int64_t n)
+{
+    return guarded([&] {
+        if (n < 0) throw Error{-1, "n < 0"};
+        if (n == 0) return;
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(0);
+        with_arena(c, (size_t)n * 12 + (16 << 20), [&] {
+            int32_t *d = to_dev(c, in, (size_t)n);
+            scan_inclusive(c, d, d, n);
+            to_host(c, out, d, (size_t)n);
+        });
+    });
+}
+
+int hgp_scan_i64(const int64_t *in, int64_t *out, int64_t n)
+{
+    return guarded([&] {
+        if (n < 0) throw Error{-1, "n < 0"};
+        if (n == 0) return;
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(0);
+        with_arena(c, (size_t)n * 24 + (16 << 20), [&] {
+            int64_t *d = to_dev(c, in, (size_t)n);
+            scan_inclusive(c, d, d, n);
+            to_host(c, out, d, (size_t)n);
+        });
+    });
+}
+
+int hgp_part_weights(int32_t n, const int32_t *vwgt, const int32_t *where, int32_t k, int64_t *pwgts)
+{
+    return guarded([&] {
+        if (n < 0 || k < 1) throw Error{-1, "bad sizes"};
+        for (int32_t v = 0; v < n; v++)
+            if (where[v] < 0 || where[v] >= k) throw Error{-1, "part id out of range"};
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(0);
+        with_arena(c, (size_t)n * 16 + (size_t)k * 16 + (16 << 20), [&] {
+            int32_t *dv = to_dev(c, vwgt, (size_t)n);
+            int32_t *dw = to_dev(c, where, (size_t)n);
+            int64_t *dp = c.arena.lalloc<int64_t>((size_t)k);
+            part_weights(c, n, dv, dw, k, dp);
+            to_host(c, pwgts, dp, (size_t)k);
+        });
+    });
+}
+
+int hgp_edge_cut(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt, const int32_t *where,
+                 int64_t *cut)
+{
+    return guarded([&] {
+        validate_host(n, xadj, adjncy, nullptr, adjwgt, false);
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(0);
+        with_arena(c, estimate_arena(n, xadj[n], 1, true), [&] {
+            DevGraph g = upload(c, n, xadj, adjncy, nullptr, adjwgt);
+            int32_t *dw = to_dev(c, where, (size_t)n);
+            int64_t *dc = c.arena.lalloc<int64_t>(1);
+            edge_cut(c, g, dw, dc);
+            to_host(c, cut, dc, 1);
+        });
+    });
+}
+
+int hgp_match(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt, const int32_t *adjwgt,
+              int32_t maxvwgt, uint64_t seed, int32_t level, int32_t rounds, int32_t *match, int64_t *matched)
+{
+    return guarded([&] {
+        validate_host(n, xadj, adjncy, vwgt, adjwgt, false);
+        if (n == 0) { if (matched) *matched = 0; return; }
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(0);
+        with_arena(c, estimate_arena(n, xadj[n], 1, true), [&] {
+            DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
+            int32_t *dm = c.arena.lalloc<int32_t>((size_t)n);
+            int64_t mc = match_level(c, g, maxvwgt, seed, level, rounds, dm);
+            to_host(c, match, dm, (size_t)n);
+            if (matched) *matched = mc;
+        });
+    });
+}
+
+int hgp_contract(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt, const int32_t *vwgt,
+                 int32_t *match, int32_t *cmap, int32_t *cn, int64_t *cxadj, int32_t *cadjncy, int32_t *cadjwgt,
+                 int32_t *cvwgt, int64_t *cm)
+{
+    return guarded([&] {
+        validate_host(n, xadj, adjncy, vwgt, adjwgt, false);
+        for (int32_t v = 0; v < n; v++) {
+            int32_t u = match[v];
+            if (u < -1 || u >= n) throw Error{-1, "match entry out of range"};
+            if (u >= 0 && u != v && match[u] != v) throw Error{-1, "match is not symmetric"};
+        }
+        if (n == 0) { *cn = 0; *cm = 0; cxadj[0] = 0; return; }
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(0);
+        with_arena(c, estimate_arena(n, xadj[n], 1, true), [&] {
+            DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
+            int32_t *dm = to_dev(c, match, (size_t)n);
+            int32_t *dc = c.arena.lalloc<int32_t>((size_t)n);
+            int32_t ncn = build_cmap(c, n, dm, dc);
+            DevGraph cg = contract(c, g, dm, dc, ncn);
+            HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+            to_host(c, match, dm, (size_t)n);
+            to_host(c, cmap, dc, (size_t)n);
+            to_host(c, cxadj, cg.xadj, (size_t)ncn + 1);
+            to_host(c, cadjncy, cg.adjncy, (size_t)cg.m);
+            to_host(c, cadjwgt, cg.adjwgt, (size_t)cg.m);
+            to_host(c, cvwgt, cg.vwgt, (size_t)ncn);
+            *cn = ncn;
+            *cm = cg.m;
+        });
+    });
+}
+
+int hgp_initpart(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt, const int32_t *adjwgt,
+                 int32_t k, const hgp_options *opt, int32_t *where)
+{
+    return guarded([&] {
+        hgp_options o;
+        if (opt) o = *opt; else hgp_default_options(&o);
+        Params p = params_from(&o);
+        check_params(p, k);
+        validate_host(n, xadj, adjncy, vwgt, adjwgt, false);
+        if (n == 0) return;
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(o.device);
+        size_t extra = (size_t)std::min(p.trials, n) * (size_t)n * 16;
+        with_arena(c, estimate_arena(n, xadj[n], k, true) + extra, [&] {
+            DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
+            int32_t *dw = c.arena.lalloc<int32_t>((size_t)n);
+            initial_partition(c, g, k, p, dw);
+            to_host(c, where, dw, (size_t)n);
+        });
+    });
+}
+
+int hgp_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt, const int32_t *adjwgt,
+               int32_t k, int64_t tvwgt, int32_t is_finest, const hgp_options *opt, int32_t *where, int64_t *cut,
+               int32_t *iters)
+{
+    return guarded([&] {
+        hgp_options o;
+        if (opt) o = *opt; else hgp_default_options(&o);
+        Params p = params_from(&o);
+        check_params(p, k);
+        validate_host(n, xadj, adjncy, vwgt, adjwgt, false);
+        for (int32_t v = 0; v < n; v++)
+            if (where[v] < 0 || where[v] >= k) throw Error{-1, "part id out of range"};
+        if (n == 0) return;
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(o.device);
+        with_arena(c, estimate_arena(n, xadj[n], k, true), [&] {
+            DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
+            int32_t *dw = to_dev(c, where, (size_t)n);
+            int64_t *dc = to_dev(c, cut, 1);
+            RefineStats rs = refine_level(c, g, k, tvwgt, is_finest != 0, p, dw, dc);
+            to_host(c, where, dw, (size_t)n);
+            to_host(c, cut, dc, 1);
+            if (iters) *iters = rs.iters;
+        });
+    });
+}
+
+}  // extern "C"

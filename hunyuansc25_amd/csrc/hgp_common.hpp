@@ -1,0 +1,191 @@
+// hgp_common.hpp -- shared types, error handling, device arena and wave64
+// helpers of the MI355X partitioner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace hgp {
+
+using eid_t = int64_t;   // edge offsets (CSR xadj), 64-bit so m may exceed 2^31
+using vid_t = int32_t;   // vertex ids
+using wgt_t = int32_t;   // vertex and edge weights (int32, as in the reference)
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kMaxBuckets = 50;    // Jet gain buckets (hunyuangraph_GPU_krefine.h:2473)
+constexpr int kMidBucket = 25;
+
+// ------------------------------------------------------------------ errors
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+void set_error(int code, const std::string &msg);
+void clear_error();
+
+#define HGP_CHECK_HIP(expr)                                                        \
+    do {                                                                           \
+        hipError_t e_ = (expr);                                                    \
+        if (e_ != hipSuccess) throw ::hgp::Error{-2, std::string(#expr) + ": " +   \
+                                                     hipGetErrorString(e_)};       \
+    } while (0)
+
+#define HGP_LAUNCH_CHECK() HGP_CHECK_HIP(hipGetLastError())
+
+// ------------------------------------------------------------------ arena
+
+// Two-ended stack over one device allocation (after hunyuangraph_GPU_memory.h:34-376):
+// the left end holds data that lives across pipeline phases (the level
+// hierarchy), the right end holds per-phase temporaries.  Frees are LIFO per
+// end; a mark/release pair frees everything above a mark at once.  A request
+// that does not fit throws Error{-3}.
+class Arena {
+  public:
+    Arena() = default;
+    ~Arena();
+    void reserve(size_t bytes);          // (re)allocate the block if smaller
+    void *left(size_t bytes);
+    void *right(size_t bytes);
+    size_t left_mark() const { return lo_; }
+    size_t right_mark() const { return hi_; }
+    void left_release(size_t mark);
+    void right_release(size_t mark);
+    void reset();                        // free everything (keeps the block)
+    size_t capacity() const { return cap_; }
+    size_t high_water() const { return peak_; }
+
+    template <class T> T *lalloc(size_t n) { return (T *)left(sizeof(T) * (n ? n : 1)); }
+    template <class T> T *ralloc(size_t n) { return (T *)right(sizeof(T) * (n ? n : 1)); }
+
+  private:
+    char *base_ = nullptr;
+    size_t cap_ = 0, lo_ = 0, hi_ = 0, peak_ = 0;
+};
+
+// ------------------------------------------------------------------ context
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Arena arena;
+    int64_t *pinned = nullptr;     // pinned host buffer: scalar read-backs + upload staging
+    hipEvent_t ev[8] = {};
+};
+
+constexpr size_t kPinnedBytes = 1 << 20;
+constexpr size_t kStageOffset = 64 << 10;   // staging area starts here
+
+Ctx &context(int device);         // per-device default context (lazily created)
+
+// Copies `bytes` from device to the pinned buffer and waits.
+void read_scalars(Ctx &c, const void *dsrc, size_t bytes);
+
+// Host-to-device copy through the pinned staging area; waits for the copy, so
+// the host source may die right after the call.
+void upload_staged(Ctx &c, void *ddst, const void *hsrc, size_t bytes);
+
+// ------------------------------------------------------------------ hashing
+// Identical to the oracle's generators (oracle_dga.c): SplitMix64 finaliser.
+
+__host__ __device__ inline uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__host__ __device__ inline uint64_t rng3(uint64_t seed, uint64_t a, uint64_t b, uint64_t c)
+{
+    return mix64(mix64(mix64(seed ^ a) ^ b) ^ c);
+}
+
+__host__ __device__ inline uint32_t edge_hash(uint64_t seed, int32_t level, int32_t lo, int32_t hi)
+{
+    return (uint32_t)(rng3(seed, 0x100000000ull + (uint64_t)(uint32_t)level, (uint64_t)(uint32_t)lo,
+                           (uint64_t)(uint32_t)hi) >> 32);
+}
+
+__host__ __device__ inline uint64_t trial_rng(uint64_t seed, uint32_t node, int32_t trial, int32_t draw)
+{
+    return rng3(seed, 0x200000000ull + node, (uint64_t)(uint32_t)trial, (uint64_t)(uint32_t)draw);
+}
+
+// ------------------------------------------------------------------ launch
+
+inline unsigned grid_for(int64_t n, int block)
+{
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 0x7fffffff) g = 0x7fffffff;
+    return (unsigned)g;
+}
+
+// ------------------------------------------------------------------ device helpers
+
+__device__ inline int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+__device__ inline uint64_t lanemask_lt()
+{
+    int l = lane_id();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+template <class T>
+__device__ inline T wave_sum(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+template <class T>
+__device__ inline T wave_max(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        T u = __shfl_xor(v, o, kWave);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+// Inclusive wave scan (Hillis-Steele over 64 lanes).
+template <class T>
+__device__ inline T wave_inclusive_scan(T v)
+{
+    int l = lane_id();
+    for (int o = 1; o < kWave; o <<= 1) {
+        T u = __shfl_up(v, o, kWave);
+        if (l >= o) v += u;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread (blockDim.x <= 1024,
+// multiple of 64).  `sh` needs blockDim.x/64 + 1 elements.  Returns the
+// exclusive prefix; *total receives the block total.
+template <class T>
+__device__ inline T block_exclusive_scan(T v, T *sh, T *total)
+{
+    int w = threadIdx.x / kWave, nw = blockDim.x / kWave;
+    T inc = wave_inclusive_scan(v);
+    if (lane_id() == kWave - 1) sh[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T acc = 0;
+        for (int i = 0; i < nw; i++) { T t = sh[i]; sh[i] = acc; acc += t; }
+        sh[nw] = acc;
+    }
+    __syncthreads();
+    T r = sh[w] + inc - v;
+    if (total) *total = sh[nw];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace hgp

@@ -1,0 +1,69 @@
+// hgp_graph.hpp -- device graph record and the phase entry points of the
+// multilevel pipeline.
+#pragma once
+
+#include "hgp_common.hpp"
+
+namespace hgp {
+
+// One level of the hierarchy in HBM: CSR with 64-bit offsets, int32 ids and
+// weights (xadj[n+1], adjncy[m], adjwgt[m], vwgt[n]).
+struct DevGraph {
+    int32_t n = 0;
+    int64_t m = 0;
+    eid_t *xadj = nullptr;
+    vid_t *adjncy = nullptr;
+    wgt_t *adjwgt = nullptr;
+    wgt_t *vwgt = nullptr;
+};
+
+struct Params {
+    uint64_t seed = 1;
+    int32_t ub_num = 103, ub_den = 100;
+    int32_t match_rounds = 8;
+    int32_t trials = 1024;
+    int32_t fm_passes = 10;
+    int32_t patience = 6;
+};
+
+// ---------------------------------------------------------------- coarsening
+
+// hunyuangraph_partitiongraph.h:298-302 and hunyuangraph_GPU_coarsen.h:165
+int32_t coarsen_threshold(int32_t k);
+int32_t coarsen_maxvwgt(int64_t tvwgt, int32_t thr);
+
+// One level of matching (device match[n] out: partner or -1).  Returns the
+// number of matched vertices.
+int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, int32_t level,
+                    int32_t rounds, int32_t *match);
+
+// Normalises match (unmatched -> self) and builds cmap; returns cnvtxs.
+int32_t build_cmap(Ctx &c, int32_t n, int32_t *match, int32_t *cmap);
+
+// Contracts g by a normalised matching; the coarse arrays are allocated on
+// the left end of the arena.
+DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t *cmap, int32_t cn);
+
+// ---------------------------------------------------------------- initial partition
+
+// Recursive bisection of g into k parts; where (device, n) out.
+void initial_partition(Ctx &c, const DevGraph &g, int32_t k, const Params &p, int32_t *where);
+
+// ---------------------------------------------------------------- refinement
+
+struct RefineStats {
+    int32_t iters = 0;
+};
+
+// Jet refinement of one level in place.  *d_cut (device int64) in/out.
+RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bool finest,
+                         const Params &p, int32_t *where, int64_t *d_cut);
+
+// where_f[v] = where_c[cmap[v]]
+void project(Ctx &c, int32_t n, const int32_t *cmap, const int32_t *where_c, int32_t *where_f);
+
+// Exact cut into a device int64 (overwritten), part weights into pwgts[k].
+void edge_cut(Ctx &c, const DevGraph &g, const int32_t *where, int64_t *d_cut);
+void part_weights(Ctx &c, int32_t n, const wgt_t *vwgt, const int32_t *where, int32_t k, int64_t *pwgts);
+
+}  // namespace hgp

@@ -1,0 +1,571 @@
+// hgp_initpart.hip -- recursive-bisection initial partition of the coarsest
+// graph (hunyuangraph_gpu_initialpartition / _RecursiveBisection,
+// hunyuangraph_GPU_initialpartition.h:3913-4496).
+//
+// The recursion is run depth by depth.  All bisection nodes of one depth live
+// in one "forest": every vertex of the coarsest graph sits in the contiguous
+// range of the tree node that owns it, rows hold forest positions, and edges
+// never cross nodes.  One launch runs every trial of every node of the depth
+// (one wave per GGGP + FM trial), so k-1 bisections take ceil(log2 k) launches
+// and no host round-trips.  Tree node ids are heap ids (root 1, children 2i
+// and 2i+1), which also key the trial random streams.
+#include "hgp_graph.hpp"
+#include "hgp_prims.hpp"
+
+#include <cmath>
+
+namespace hgp {
+
+namespace {
+
+constexpr int kTrialBytesPerVertex = 14;   // where, moved (u8) + ed, id, queue (i32)
+constexpr int kLdsBudget = 60 * 1024;
+
+struct Node {
+    int32_t start, end, k, fpart;
+    int64_t tv, onemin, onemax, ideal0;
+    int32_t T, trial_off, best, cnt0;
+    int64_t woff;       // offset of the node's trial-where block (bytes)
+    int64_t soff;       // offset of the node's global scratch block (bytes)
+};
+
+__global__ void k_ip_init(int32_t n, const wgt_t *__restrict__ vwgt, int32_t *__restrict__ node_of,
+                          int32_t *__restrict__ label, wgt_t *__restrict__ fvw, Node *nodes)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (v < n) {
+        node_of[v] = 1;
+        label[v] = v;
+        fvw[v] = vwgt[v];
+    }
+    if (v == 0) {
+        nodes[1].start = 0;
+        nodes[1].end = n;
+    }
+}
+
+// Per active node: total weight and the bisection bounds
+// (:3986-3987 with tp1 = kr/k, ub = 1.03^(1/ln k) computed on the host).
+__global__ void __launch_bounds__(256) k_ip_setup(const int32_t *__restrict__ act, Node *nodes,
+                                                  const wgt_t *__restrict__ fvw, double ub, int32_t trials)
+{
+    __shared__ int64_t sh[4];
+    Node &nd = nodes[act[blockIdx.x]];
+    int64_t s = 0;
+    for (int32_t v = nd.start + threadIdx.x; v < nd.end; v += blockDim.x) s += fvw[v];
+    s = wave_sum(s);
+    if (lane_id() == 0) sh[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t tv = sh[0] + sh[1] + sh[2] + sh[3];
+        int32_t k = nd.k, kl = k >> 1, kr = k - kl;
+        double t1 = ((double)tv * (double)kr) / (double)k;
+        nd.tv = tv;
+        nd.onemax = (int64_t)floor(ub * t1);
+        nd.onemin = (int64_t)floor(t1 / ub);
+        nd.ideal0 = (tv * kl) / k;
+        int32_t nn = nd.end - nd.start;
+        nd.T = trials < nn ? trials : nn;
+        nd.best = -1;
+    }
+}
+
+// Exclusive offsets of trials and scratch over the active nodes (one block).
+__global__ void k_ip_offsets(const int32_t *__restrict__ act, int32_t nact, Node *nodes, int32_t lds_cap,
+                             int32_t *total)
+{
+    if (threadIdx.x != 0) return;
+    int32_t t = 0;
+    int64_t w = 0, s = 0;
+    for (int32_t a = 0; a < nact; a++) {
+        Node &nd = nodes[act[a]];
+        int64_t nn = nd.end - nd.start;
+        nd.trial_off = t;
+        nd.woff = w;
+        nd.soff = s;
+        t += nd.T;
+        w += (int64_t)nd.T * nn;
+        if (nn > lds_cap) s += (int64_t)nd.T * ((nn * kTrialBytesPerVertex + 255) & ~255ll);
+    }
+    *total = t;
+}
+
+struct TrialMem {
+    uint8_t *where, *moved;
+    int32_t *ed, *id, *queue;
+};
+
+__device__ inline void fence_wave()
+{
+    __threadfence_block();
+    __syncthreads();
+}
+
+// One GGGP + FM trial per wave (hunyuangraph_gpu_BFS_warp_2wayrefine_noedit[_sampling],
+// :2426-3659).  Lane 0 runs the serial BFS; vertex scans, the FM arg-max
+// (gain desc, id asc) and neighbour updates are spread over the 64 lanes.
+// The boundary flag is derived: bnd(v) = deg(v) == 0 || ed[v] > 0, which is
+// the invariant the reference's incremental updates maintain.
+__global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__ act, int32_t nact,
+                                                     const Node *__restrict__ nodes, const int32_t *total,
+                                                     const eid_t *__restrict__ fx, const vid_t *__restrict__ fadj,
+                                                     const wgt_t *__restrict__ fw, const wgt_t *__restrict__ fvw,
+                                                     uint64_t seed, int32_t fm_passes, int32_t lds_cap,
+                                                     uint8_t *__restrict__ trial_where, char *__restrict__ gscratch,
+                                                     int64_t *__restrict__ tcut)
+{
+    extern __shared__ char lds[];
+    int32_t gw = blockIdx.x;
+    if (gw >= *total) return;
+    int lane = threadIdx.x;
+    int32_t lo = 0, hi = nact - 1;
+    while (lo < hi) {
+        int32_t mid = (lo + hi + 1) >> 1;
+        if (nodes[act[mid]].trial_off <= gw) lo = mid; else hi = mid - 1;
+    }
+    const int32_t nid = act[lo];
+    const Node nd = nodes[nid];
+    const int32_t t = gw - nd.trial_off;
+    const int32_t n = nd.end - nd.start, s0 = nd.start;
+    char *base;
+    if (n <= lds_cap) base = lds;
+    else base = gscratch + nd.soff + (int64_t)t * (((int64_t)n * kTrialBytesPerVertex + 255) & ~255ll);
+    TrialMem M;
+    M.where = (uint8_t *)base;
+    M.moved = M.where + n;
+    M.ed = (int32_t *)(base + (((int64_t)2 * n + 3) & ~3ll));
+    M.id = M.ed + n;
+    M.queue = M.id + n;
+    const int64_t tv = nd.tv, onemin = nd.onemin, onemax = nd.onemax, ideal0 = nd.ideal0;
+
+    for (int32_t i = lane; i < n; i += kWave) { M.where[i] = 1; M.moved[i] = 0; }
+    fence_wave();
+
+    int64_t pw0 = 0, pw1 = tv;
+    if (lane == 0) {
+        int32_t draw = 0, start;
+        if (nd.T >= n) {
+            start = t;
+        } else {
+            int32_t len = n / nd.T, rem = n % nd.T, left;
+            if (t < rem) { len++; left = t * len; }
+            else left = rem * (len + 1) + (t - rem) * len;
+            start = left + (int32_t)(trial_rng(seed, (uint32_t)nid, t, draw++) % (uint64_t)len);
+        }
+        int32_t first = 0, last = 1, nleft = n - 1, drain = 0;
+        M.queue[0] = start;
+        M.moved[start] = 1;
+        for (;;) {
+            if (first == last) {
+                if (nleft == 0 || drain) break;
+                int32_t r = (int32_t)(trial_rng(seed, (uint32_t)nid, t, draw++) % (uint64_t)nleft);
+                int32_t v;
+                for (v = 0; v < n; v++) {
+                    if (M.moved[v] == 0) {
+                        if (r == 0) break;
+                        r--;
+                    }
+                }
+                M.queue[0] = v;
+                M.moved[v] = 1;
+                first = 0;
+                last = 1;
+                nleft--;
+            }
+            int32_t v = M.queue[first++];
+            int32_t wv = fvw[s0 + v];
+            if (pw0 > 0 && pw1 - wv < onemin) { drain = 1; continue; }
+            M.where[v] = 0;
+            pw0 += wv;
+            pw1 -= wv;
+            if (pw1 <= onemax) break;
+            drain = 0;
+            for (eid_t j = fx[s0 + v]; j < fx[s0 + v + 1]; j++) {
+                int32_t x = fadj[j] - s0;
+                if (M.moved[x] == 0) { M.queue[last++] = x; M.moved[x] = 1; nleft--; }
+            }
+        }
+    }
+    pw0 = __shfl(pw0, 0, kWave);
+    pw1 = __shfl(pw1, 0, kWave);
+    fence_wave();
+
+    // ed / id and the cut (compute_v_ed_id_bnd, :23-62)
+    int64_t cutacc = 0;
+    for (int32_t v = lane; v < n; v += kWave) {
+        int32_t e = 0, in = 0;
+        uint8_t me = M.where[v];
+        for (eid_t j = fx[s0 + v]; j < fx[s0 + v + 1]; j++) {
+            if (M.where[fadj[j] - s0] != me) e += fw[j]; else in += fw[j];
+        }
+        M.ed[v] = e;
+        M.id[v] = in;
+        M.moved[v] = 0;
+        cutacc += e;
+    }
+    int64_t cut = wave_sum(cutacc) / 2;
+    fence_wave();
+
+    int64_t pw[2] = {pw0, pw1};
+    const int64_t ideal[2] = {ideal0, tv - ideal0};
+    int64_t limit = n / 100;
+    if (limit < 15) limit = 15;
+    if (limit > 100) limit = 100;
+    int32_t *swaps = M.queue;
+
+    for (int32_t pass = 0; pass < fm_passes; pass++) {
+        int64_t mincutorder = -1;
+        int64_t newcut = cut, mincut = cut, initcut = cut;
+        int64_t d0 = ideal[0] - pw[0];
+        int64_t mindiff = d0 < 0 ? -d0 : d0;
+        int32_t nswaps;
+        for (nswaps = 0; nswaps < n; nswaps++) {
+            int from = (ideal[0] - pw[0] < ideal[1] - pw[1]) ? 0 : 1;
+            int to = from ^ 1;
+            int32_t bv = -1, bg = 0;
+            for (int32_t u = lane; u < n; u += kWave) {
+                if (M.where[u] != from || M.moved[u]) continue;
+                int32_t e = M.ed[u];
+                bool bnd = e > 0 || fx[s0 + u] == fx[s0 + u + 1];
+                if (!bnd) continue;
+                int32_t g = e - M.id[u];
+                if (bv == -1 || g > bg) { bv = u; bg = g; }
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                int32_t ov = __shfl_xor(bv, o, kWave), og = __shfl_xor(bg, o, kWave);
+                if (ov != -1 && (bv == -1 || og > bg || (og == bg && ov < bv))) { bv = ov; bg = og; }
+            }
+            if (bv == -1) break;
+            int32_t wv = fvw[s0 + bv];
+            newcut -= bg;
+            int64_t dd = ideal[0] - pw[0];
+            dd = dd < 0 ? -dd : dd;
+            if ((newcut < mincut && pw[to] + wv <= onemax && pw[from] - wv <= onemax) ||
+                (newcut == mincut && dd < mindiff)) {
+                mincut = newcut;
+                mindiff = dd;
+                mincutorder = nswaps;
+            } else if (nswaps - mincutorder > limit) {
+                newcut += bg;
+                break;
+            }
+            pw[to] += wv;
+            pw[from] -= wv;
+            if (lane == 0) {
+                M.where[bv] = (uint8_t)to;
+                M.moved[bv] = 1;
+                swaps[nswaps] = bv;
+                int32_t tmp = M.ed[bv];
+                M.ed[bv] = M.id[bv];
+                M.id[bv] = tmp;
+            }
+            fence_wave();
+            for (eid_t j = fx[s0 + bv] + lane; j < fx[s0 + bv + 1]; j += kWave) {
+                int32_t x = fadj[j] - s0;
+                int32_t kw = (M.where[x] == from) ? -fw[j] : fw[j];
+                atomicAdd(&M.id[x], kw);
+                atomicAdd(&M.ed[x], -kw);
+            }
+            fence_wave();
+        }
+        for (int32_t i = lane; i < nswaps; i += kWave) M.moved[swaps[i]] = 0;
+        fence_wave();
+        for (nswaps--; nswaps > mincutorder; nswaps--) {
+            int32_t v = swaps[nswaps];
+            int from = M.where[v];
+            int to = from ^ 1;
+            int32_t wv = fvw[s0 + v];
+            fence_wave();
+            if (lane == 0) {
+                M.where[v] = (uint8_t)to;
+                int32_t tmp = M.ed[v];
+                M.ed[v] = M.id[v];
+                M.id[v] = tmp;
+            }
+            pw[to] += wv;
+            pw[from] -= wv;
+            fence_wave();
+            for (eid_t j = fx[s0 + v] + lane; j < fx[s0 + v + 1]; j += kWave) {
+                int32_t x = fadj[j] - s0;
+                int32_t kw = (M.where[x] == from) ? -fw[j] : fw[j];
+                atomicAdd(&M.id[x], kw);
+                atomicAdd(&M.ed[x], -kw);
+            }
+            fence_wave();
+        }
+        cut = mincut;
+        if (mincutorder <= 0 || mincut == initcut) break;
+    }
+
+    uint8_t *out = trial_where + nd.woff + (int64_t)t * n;
+    for (int32_t i = lane; i < n; i += kWave) out[i] = M.where[i];
+    if (lane == 0) tcut[gw] = cut;
+}
+
+// Arg-min cut over a node's trials, first minimum wins (:3713-3812).
+__global__ void __launch_bounds__(256) k_ip_select(const int32_t *__restrict__ act, Node *nodes,
+                                                   const int64_t *__restrict__ tcut)
+{
+    __shared__ int64_t sc[256];
+    __shared__ int32_t si[256];
+    Node &nd = nodes[act[blockIdx.x]];
+    int64_t bc = INT64_MAX;
+    int32_t bi = -1;
+    for (int32_t t = threadIdx.x; t < nd.T; t += blockDim.x) {
+        int64_t c = tcut[nd.trial_off + t];
+        if (c < bc) { bc = c; bi = t; }
+    }
+    sc[threadIdx.x] = bc;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            int64_t oc = sc[threadIdx.x + s];
+            int32_t oi = si[threadIdx.x + s];
+            if (oi != -1 && (si[threadIdx.x] == -1 || oc < sc[threadIdx.x] ||
+                             (oc == sc[threadIdx.x] && oi < si[threadIdx.x]))) {
+                sc[threadIdx.x] = oc;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) nd.best = si[0];
+}
+
+// Write the winning bisection through the labels (:3896-3902) and record
+// each vertex's side for the split (-1 for vertices of finished nodes).
+__global__ void k_ip_side(int32_t n, const int32_t *__restrict__ node_of, const Node *__restrict__ nodes,
+                          const uint8_t *__restrict__ trial_where, const int32_t *__restrict__ label,
+                          int32_t *__restrict__ part, int8_t *__restrict__ side)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= n) return;
+    const Node &nd = nodes[node_of[v]];
+    if (nd.k < 2 || nd.best < 0) { side[v] = -1; return; }
+    int32_t nn = nd.end - nd.start;
+    uint8_t w = trial_where[nd.woff + (int64_t)nd.best * nn + (v - nd.start)];
+    part[label[v]] = nd.fpart + w;
+    side[v] = (int8_t)w;
+}
+
+__global__ void __launch_bounds__(256) k_ip_count0(const int32_t *__restrict__ act, Node *nodes, int32_t ntab,
+                                                   const int8_t *__restrict__ side)
+{
+    __shared__ int32_t sh[4];
+    int32_t id = act[blockIdx.x];
+    Node &nd = nodes[id];
+    int32_t s = 0;
+    for (int32_t v = nd.start + threadIdx.x; v < nd.end; v += blockDim.x) s += side[v] == 0;
+    s = wave_sum(s);
+    if (lane_id() == 0) sh[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t c0 = sh[0] + sh[1] + sh[2] + sh[3];
+        nd.cnt0 = c0;
+        if (2 * id + 1 < ntab) {
+            nodes[2 * id].start = nd.start;
+            nodes[2 * id].end = nd.start + c0;
+            nodes[2 * id + 1].start = nd.start + c0;
+            nodes[2 * id + 1].end = nd.end;
+        }
+    }
+}
+
+__global__ void k_ip_f0(int32_t n, const int8_t *__restrict__ side, int32_t *__restrict__ f0)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (v < n) f0[v] = side[v] == 0;
+}
+
+// Order-preserving split (hunyuangraph_GPU_splitgraph.h:802-944): side-0
+// vertices first, then side-1, each in their old order, inside the node's
+// range.  Rows keep only same-side entries; finished nodes drop their rows.
+__global__ void k_ip_pos(int32_t n, const int32_t *__restrict__ node_of, const Node *__restrict__ nodes,
+                         const int8_t *__restrict__ side, const int32_t *__restrict__ s0, const eid_t *__restrict__ fx,
+                         const vid_t *__restrict__ fadj, int32_t *__restrict__ pos, int32_t *__restrict__ nnode_of,
+                         int32_t *__restrict__ nlabel, const int32_t *__restrict__ label, wgt_t *__restrict__ nfvw,
+                         const wgt_t *__restrict__ fvw, int64_t *__restrict__ ndeg)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= n) return;
+    int32_t id = node_of[v];
+    int8_t sd = side[v];
+    int32_t p;
+    int64_t d = 0;
+    if (sd < 0) {
+        p = v;
+        nnode_of[p] = id;
+    } else {
+        const Node &nd = nodes[id];
+        int32_t before = nd.start > 0 ? s0[nd.start - 1] : 0;
+        int32_t rank0 = (s0[v] - (sd == 0 ? 1 : 0)) - before;
+        int32_t rank1 = (v - nd.start) - rank0;
+        p = sd == 0 ? nd.start + rank0 : nd.start + nd.cnt0 + rank1;
+        nnode_of[p] = 2 * id + sd;
+        for (eid_t j = fx[v]; j < fx[v + 1]; j++) d += side[fadj[j]] == sd;
+    }
+    pos[v] = p;
+    nlabel[p] = label[v];
+    nfvw[p] = fvw[v];
+    ndeg[p] = d;
+}
+
+__global__ void k_ip_edges(int32_t n, const int8_t *__restrict__ side, const int32_t *__restrict__ pos,
+                           const eid_t *__restrict__ fx, const vid_t *__restrict__ fadj, const wgt_t *__restrict__ fw,
+                           const eid_t *__restrict__ nfx, vid_t *__restrict__ nfadj, wgt_t *__restrict__ nfw)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= n) return;
+    int8_t sd = side[v];
+    if (sd < 0) return;
+    eid_t o = nfx[pos[v]];
+    for (eid_t j = fx[v]; j < fx[v + 1]; j++) {
+        int32_t x = fadj[j];
+        if (side[x] != sd) continue;
+        nfadj[o] = pos[x];
+        nfw[o] = fw[j];
+        o++;
+    }
+}
+
+struct TreeNode { int32_t id, k, fpart; };
+
+}  // namespace
+
+void initial_partition(Ctx &c, const DevGraph &g, int32_t k, const Params &p, int32_t *where)
+{
+    int32_t n = g.n;
+    if (n <= 0) return;
+    fill<int32_t>(c, where, n, 0);
+    if (k <= 1) return;
+
+    // tree shape depends on k only
+    std::vector<std::vector<TreeNode>> depth_nodes;
+    std::vector<TreeNode> cur{{1, k, 0}};
+    int32_t maxid = 1;
+    while (!cur.empty()) {
+        depth_nodes.push_back(cur);
+        std::vector<TreeNode> nxt;
+        for (auto &t : cur) {
+            int32_t kl = t.k >> 1, kr = t.k - kl;
+            maxid = std::max(maxid, 2 * t.id + 1);
+            if (t.k > 2) {
+                if (kl >= 2) nxt.push_back({2 * t.id, kl, t.fpart});
+                if (kr >= 2) nxt.push_back({2 * t.id + 1, kr, t.fpart + kl});
+            }
+        }
+        cur.swap(nxt);
+    }
+    int32_t ntab = maxid + 1;
+    std::vector<Node> htab(ntab);
+    for (auto &t : htab) { t.start = t.end = 0; t.k = 1; t.fpart = 0; t.best = -1; t.T = 0; }
+    {
+        // k and fpart of every tree node (leaves included)
+        std::vector<TreeNode> stack{{1, k, 0}};
+        while (!stack.empty()) {
+            TreeNode t = stack.back();
+            stack.pop_back();
+            if (t.id >= ntab) continue;
+            htab[t.id].k = t.k;
+            htab[t.id].fpart = t.fpart;
+            if (t.k >= 2) {
+                stack.push_back({2 * t.id, t.k >> 1, t.fpart});
+                stack.push_back({2 * t.id + 1, t.k - (t.k >> 1), t.fpart + (t.k >> 1)});
+            }
+        }
+    }
+
+    double ub = std::pow((double)p.ub_num / (double)p.ub_den, 1.0 / std::log((double)k));
+    int32_t lds_cap = kLdsBudget / kTrialBytesPerVertex;
+    int32_t lds_n = std::min(n, lds_cap);
+    size_t lds_bytes = (size_t)(((int64_t)2 * lds_n + 3) & ~3ll) + (size_t)lds_n * 12 + 16;
+
+    size_t mark = c.arena.right_mark();
+    Node *nodes = c.arena.ralloc<Node>((size_t)ntab);
+    // active node ids of every depth, back to back
+    std::vector<int32_t> hact;
+    std::vector<size_t> act_off;
+    for (auto &dn : depth_nodes) {
+        act_off.push_back(hact.size());
+        for (auto &t : dn) hact.push_back(t.id);
+    }
+    int32_t *act_all = c.arena.ralloc<int32_t>(hact.size());
+    upload_staged(c, nodes, htab.data(), sizeof(Node) * (size_t)ntab);
+    upload_staged(c, act_all, hact.data(), sizeof(int32_t) * hact.size());
+    int32_t *total = c.arena.ralloc<int32_t>(1);
+    int64_t maxtrials = (int64_t)std::min(p.trials, n);
+    uint8_t *trial_where = c.arena.ralloc<uint8_t>((size_t)(maxtrials * n));
+    char *gscratch = nullptr;
+    if (n > lds_cap) {
+        // sum over nodes of T * round256(14 nn) <= maxtrials * (14 n + 256 * (n / lds_cap + 1))
+        size_t gs = (size_t)maxtrials * ((size_t)n * kTrialBytesPerVertex + 256 * ((size_t)(n / lds_cap) + 1));
+        gscratch = c.arena.ralloc<char>(gs);
+    }
+    // trials of one depth: sum of min(trials, n_node) <= n
+    int64_t *tcut = c.arena.ralloc<int64_t>((size_t)n + 1);
+    int32_t *node_of[2], *label[2];
+    wgt_t *fvw[2], *fw[2];
+    eid_t *fx[2];
+    vid_t *fadj[2];
+    int64_t mm = g.m > 0 ? g.m : 1;
+    for (int b = 0; b < 2; b++) {
+        node_of[b] = c.arena.ralloc<int32_t>((size_t)n);
+        label[b] = c.arena.ralloc<int32_t>((size_t)n);
+        fvw[b] = c.arena.ralloc<wgt_t>((size_t)n);
+        fx[b] = c.arena.ralloc<eid_t>((size_t)n + 1);
+        fadj[b] = c.arena.ralloc<vid_t>((size_t)mm);
+        fw[b] = c.arena.ralloc<wgt_t>((size_t)mm);
+    }
+    int8_t *side = c.arena.ralloc<int8_t>((size_t)n);
+    int32_t *f0 = c.arena.ralloc<int32_t>((size_t)n);
+    int32_t *pos = c.arena.ralloc<int32_t>((size_t)n);
+    int64_t *ndeg = c.arena.ralloc<int64_t>((size_t)n);
+
+    unsigned gv = grid_for(n, 256);
+    k_ip_init<<<gv, 256, 0, c.stream>>>(n, g.vwgt, node_of[0], label[0], fvw[0], nodes);
+    HGP_LAUNCH_CHECK();
+    HGP_CHECK_HIP(hipMemcpyAsync(fx[0], g.xadj, sizeof(eid_t) * ((size_t)n + 1), hipMemcpyDeviceToDevice, c.stream));
+    if (g.m > 0) {
+        HGP_CHECK_HIP(hipMemcpyAsync(fadj[0], g.adjncy, sizeof(vid_t) * (size_t)g.m, hipMemcpyDeviceToDevice, c.stream));
+        HGP_CHECK_HIP(hipMemcpyAsync(fw[0], g.adjwgt, sizeof(wgt_t) * (size_t)g.m, hipMemcpyDeviceToDevice, c.stream));
+    }
+
+    int cb = 0;
+    for (size_t d = 0; d < depth_nodes.size(); d++) {
+        const auto &dn = depth_nodes[d];
+        int32_t nact = (int32_t)dn.size();
+        const int32_t *act = act_all + act_off[d];
+        k_ip_setup<<<nact, 256, 0, c.stream>>>(act, nodes, fvw[cb], ub, p.trials);
+        HGP_LAUNCH_CHECK();
+        k_ip_offsets<<<1, 64, 0, c.stream>>>(act, nact, nodes, lds_cap, total);
+        HGP_LAUNCH_CHECK();
+        int64_t waves = std::min<int64_t>((int64_t)nact * p.trials, n);
+        k_ip_trials<<<(unsigned)std::max<int64_t>(waves, 1), kWave, lds_bytes, c.stream>>>(
+            act, nact, nodes, total, fx[cb], fadj[cb], fw[cb], fvw[cb], p.seed, p.fm_passes, lds_cap, trial_where,
+            gscratch, tcut);
+        HGP_LAUNCH_CHECK();
+        k_ip_select<<<nact, 256, 0, c.stream>>>(act, nodes, tcut);
+        HGP_LAUNCH_CHECK();
+        k_ip_side<<<gv, 256, 0, c.stream>>>(n, node_of[cb], nodes, trial_where, label[cb], where, side);
+        HGP_LAUNCH_CHECK();
+        if (d + 1 == depth_nodes.size()) break;
+        k_ip_count0<<<nact, 256, 0, c.stream>>>(act, nodes, ntab, side);
+        HGP_LAUNCH_CHECK();
+        k_ip_f0<<<gv, 256, 0, c.stream>>>(n, side, f0);
+        HGP_LAUNCH_CHECK();
+        scan_inclusive(c, f0, f0, n);
+        int nb = cb ^ 1;
+        k_ip_pos<<<gv, 256, 0, c.stream>>>(n, node_of[cb], nodes, side, f0, fx[cb], fadj[cb], pos, node_of[nb],
+                                           label[nb], label[cb], fvw[nb], fvw[cb], ndeg);
+        HGP_LAUNCH_CHECK();
+        offsets_from_counts(c, ndeg, fx[nb], n);
+        k_ip_edges<<<gv, 256, 0, c.stream>>>(n, side, pos, fx[cb], fadj[cb], fw[cb], fx[nb], fadj[nb], fw[nb]);
+        HGP_LAUNCH_CHECK();
+        cb = nb;
+    }
+    c.arena.right_release(mark);
+}
+
+}  // namespace hgp

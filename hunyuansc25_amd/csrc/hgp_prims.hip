@@ -1,0 +1,330 @@
+// hgp_prims.hip -- scans, stable radix sort, fills, reductions (wave64).
+#include "hgp_prims.hpp"
+
+namespace hgp {
+
+namespace {
+
+constexpr int kScanBS = 256;    // threads per block
+constexpr int kScanR = 8;       // rounds per tile: tile = 2048 items
+
+// Per-tile sums of a striped tile (item r*256+t of the tile in round r).
+template <class T>
+__global__ void __launch_bounds__(kScanBS) k_tile_reduce(const T *__restrict__ in, int64_t n, T *__restrict__ sums)
+{
+    __shared__ T sh[kScanBS / kWave];
+    int64_t base = (int64_t)blockIdx.x * (kScanBS * kScanR);
+    T acc = 0;
+#pragma unroll
+    for (int r = 0; r < kScanR; r++) {
+        int64_t i = base + r * kScanBS + threadIdx.x;
+        if (i < n) acc += in[i];
+    }
+    acc = wave_sum(acc);
+    if (lane_id() == 0) sh[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T t = 0;
+        for (int w = 0; w < kScanBS / kWave; w++) t += sh[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// Inclusive scan of each tile plus the inclusive prefix of the previous
+// tiles (incl_sums[tile-1]).
+template <class T>
+__global__ void __launch_bounds__(kScanBS) k_tile_scan(const T *in, T *out, int64_t n, const T *__restrict__ incl_sums)
+{
+    __shared__ T sh[kScanBS / kWave + 1];
+    int64_t base = (int64_t)blockIdx.x * (kScanBS * kScanR);
+    T carry = (incl_sums && blockIdx.x > 0) ? incl_sums[blockIdx.x - 1] : (T)0;
+    for (int r = 0; r < kScanR; r++) {
+        int64_t i = base + r * kScanBS + threadIdx.x;
+        T v = i < n ? in[i] : (T)0;
+        T tot;
+        T ex = block_exclusive_scan(v, sh, &tot);
+        if (i < n) out[i] = carry + ex + v;
+        carry += tot;
+        if (base + (r + 1) * kScanBS >= n) break;   // uniform across the block
+    }
+}
+
+template <class T>
+void scan_impl(Ctx &c, const T *in, T *out, int64_t n)
+{
+    if (n <= 0) return;
+    int64_t tile = kScanBS * kScanR;
+    int64_t tiles = (n + tile - 1) / tile;
+    if (tiles == 1) {
+        k_tile_scan<T><<<1, kScanBS, 0, c.stream>>>(in, out, n, nullptr);
+        HGP_LAUNCH_CHECK();
+        return;
+    }
+    size_t mark = c.arena.right_mark();
+    T *sums = c.arena.ralloc<T>((size_t)tiles);
+    k_tile_reduce<T><<<(unsigned)tiles, kScanBS, 0, c.stream>>>(in, n, sums);
+    HGP_LAUNCH_CHECK();
+    scan_impl<T>(c, sums, sums, tiles);
+    k_tile_scan<T><<<(unsigned)tiles, kScanBS, 0, c.stream>>>(in, out, n, sums);
+    HGP_LAUNCH_CHECK();
+    c.arena.right_release(mark);
+}
+
+template <class TI, class TO>
+__global__ void k_copy_shift(const TI *__restrict__ in, TO *__restrict__ out, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i + 1] = (TO)in[i];
+    if (i == 0) out[0] = 0;
+}
+
+template <class T>
+__global__ void k_fill(T *p, int64_t n, T v)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_iota(int32_t *p, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
+__global__ void k_reduce_i32(const int32_t *__restrict__ in, int64_t n, unsigned long long *out)
+{
+    __shared__ int64_t sh[4];
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t acc = 0;
+    for (; i < n; i += stride) acc += in[i];
+    acc = wave_sum(acc);
+    if (lane_id() == 0) sh[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); w++) t += sh[w];
+        atomicAdd(out, (unsigned long long)t);
+    }
+}
+
+// ------------------------------------------------------------ radix sort
+
+constexpr int kRsBS = 256;
+constexpr int kRsIPT = 16;                 // rounds per tile
+constexpr int kRsTile = kRsBS * kRsIPT;    // 4096 items
+constexpr int kRsWaves = kRsBS / kWave;
+
+template <class K>
+__global__ void __launch_bounds__(kRsBS) k_rs_hist(const K *__restrict__ keys, int64_t n, int shift, int ntiles,
+                                                   uint32_t *__restrict__ hist)
+{
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsIPT; r++) {
+        int64_t i = base + r * kRsBS + threadIdx.x;
+        if (i < n) atomicAdd(&cnt[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter: items of a tile are ranked in index order; ranks inside a
+// wave come from an 8-ballot digit match, ranks across waves and rounds from
+// LDS counters.  incl = inclusive scan of hist (digit-major), so the tile's
+// first slot for digit d is incl[d*ntiles+tile] - hist[d*ntiles+tile].
+template <class K>
+__global__ void __launch_bounds__(kRsBS) k_rs_scatter(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                                      K *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
+                                                      int shift, int ntiles, const uint32_t *__restrict__ hist,
+                                                      const uint32_t *__restrict__ incl)
+{
+    __shared__ uint32_t base_d[256];
+    __shared__ uint32_t wcnt[kRsWaves][256];
+    int t = threadIdx.x, w = t / kWave;
+    {
+        int64_t h = (int64_t)t * ntiles + blockIdx.x;
+        base_d[t] = incl[h] - hist[h];
+    }
+    int64_t tbase = (int64_t)blockIdx.x * kRsTile;
+    for (int r = 0; r < kRsIPT; r++) {
+        if (tbase + (int64_t)r * kRsBS >= n) break;      // uniform across the block
+        int64_t i = tbase + (int64_t)r * kRsBS + t;
+        bool valid = i < n;
+        K key = valid ? kin[i] : (K)0;
+        uint32_t val = valid ? vin[i] : 0u;
+        uint32_t d = (uint32_t)(key >> shift) & 255u;
+#pragma unroll
+        for (int ww = 0; ww < kRsWaves; ww++) wcnt[ww][t] = 0;
+        __syncthreads();
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            bool bit = (d >> b) & 1u;
+            uint64_t bb = __ballot(bit);
+            m &= bit ? bb : ~bb;
+        }
+        uint64_t lower = m & lanemask_lt();
+        uint32_t rank = (uint32_t)__popcll(lower);
+        if (valid && lower == 0) wcnt[w][d] = (uint32_t)__popcll(m);
+        __syncthreads();
+        {
+            uint32_t acc = base_d[t];
+#pragma unroll
+            for (int ww = 0; ww < kRsWaves; ww++) {
+                uint32_t c = wcnt[ww][t];
+                wcnt[ww][t] = acc;
+                acc += c;
+            }
+            base_d[t] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = wcnt[w][d] + rank;
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+    }
+}
+
+template <class K>
+void radix_impl(Ctx &c, K *keys, uint32_t *vals, int64_t n, int lo, int hi)
+{
+    if (n <= 1 || hi <= lo) return;
+    if (n > 0x7fffffffll) throw Error{-4, "radix_sort: more than 2^31 items"};
+    size_t mark = c.arena.right_mark();
+    int ntiles = (int)((n + kRsTile - 1) / kRsTile);
+    K *k2 = c.arena.ralloc<K>((size_t)n);
+    uint32_t *v2 = c.arena.ralloc<uint32_t>((size_t)n);
+    uint32_t *hist = c.arena.ralloc<uint32_t>((size_t)ntiles * 256);
+    uint32_t *incl = c.arena.ralloc<uint32_t>((size_t)ntiles * 256);
+    K *src_k = keys, *dst_k = k2;
+    uint32_t *src_v = vals, *dst_v = v2;
+    int passes = 0;
+    for (int shift = lo; shift < hi; shift += 8) {
+        k_rs_hist<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, n, shift, ntiles, hist);
+        HGP_LAUNCH_CHECK();
+        scan_impl<int32_t>(c, (const int32_t *)hist, (int32_t *)incl, (int64_t)ntiles * 256);
+        k_rs_scatter<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, src_v, dst_k, dst_v, n, shift, ntiles, hist, incl);
+        HGP_LAUNCH_CHECK();
+        std::swap(src_k, dst_k);
+        std::swap(src_v, dst_v);
+        passes++;
+    }
+    if (passes & 1) {
+        HGP_CHECK_HIP(hipMemcpyAsync(keys, src_k, sizeof(K) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
+        HGP_CHECK_HIP(hipMemcpyAsync(vals, src_v, sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
+    }
+    c.arena.right_release(mark);
+}
+
+__global__ void k_flag_to_i32(const uint8_t *__restrict__ f, int64_t n, int32_t *__restrict__ o)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = f[i] != 0;
+}
+
+__global__ void k_compact(const uint8_t *__restrict__ f, const int32_t *__restrict__ incl, int64_t n,
+                          int32_t *__restrict__ idx, int64_t *count)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && f[i]) idx[incl[i] - 1] = (int32_t)i;
+    if (i == n - 1) *count = incl[i];
+}
+
+}  // namespace
+
+void scan_inclusive(Ctx &c, const int32_t *in, int32_t *out, int64_t n) { scan_impl<int32_t>(c, in, out, n); }
+void scan_inclusive(Ctx &c, const int64_t *in, int64_t *out, int64_t n) { scan_impl<int64_t>(c, in, out, n); }
+
+void offsets_from_counts(Ctx &c, const int64_t *cnt, int64_t *off, int64_t n)
+{
+    k_copy_shift<int64_t, int64_t><<<grid_for(n > 0 ? n : 1, 256), 256, 0, c.stream>>>(cnt, off, n);
+    HGP_LAUNCH_CHECK();
+    scan_impl<int64_t>(c, off + 1, off + 1, n);
+}
+
+void offsets_from_counts(Ctx &c, const int32_t *cnt, int64_t *off, int64_t n)
+{
+    k_copy_shift<int32_t, int64_t><<<grid_for(n > 0 ? n : 1, 256), 256, 0, c.stream>>>(cnt, off, n);
+    HGP_LAUNCH_CHECK();
+    scan_impl<int64_t>(c, off + 1, off + 1, n);
+}
+
+void offsets_from_counts(Ctx &c, const int32_t *cnt, int32_t *off, int64_t n)
+{
+    k_copy_shift<int32_t, int32_t><<<grid_for(n > 0 ? n : 1, 256), 256, 0, c.stream>>>(cnt, off, n);
+    HGP_LAUNCH_CHECK();
+    scan_impl<int32_t>(c, off + 1, off + 1, n);
+}
+
+void radix_sort(Ctx &c, uint32_t *keys, uint32_t *vals, int64_t n, int lo, int hi)
+{
+    radix_impl<uint32_t>(c, keys, vals, n, lo, hi);
+}
+
+void radix_sort(Ctx &c, uint64_t *keys, uint32_t *vals, int64_t n, int lo, int hi)
+{
+    radix_impl<uint64_t>(c, keys, vals, n, lo, hi);
+}
+
+template <class T>
+void fill(Ctx &c, T *p, int64_t n, T v)
+{
+    if (n <= 0) return;
+    unsigned g = grid_for(n, 256);
+    if (g > 65536) g = 65536;
+    k_fill<T><<<g, 256, 0, c.stream>>>(p, n, v);
+    HGP_LAUNCH_CHECK();
+}
+
+template void fill<int32_t>(Ctx &, int32_t *, int64_t, int32_t);
+template void fill<int64_t>(Ctx &, int64_t *, int64_t, int64_t);
+template void fill<uint8_t>(Ctx &, uint8_t *, int64_t, uint8_t);
+template void fill<uint32_t>(Ctx &, uint32_t *, int64_t, uint32_t);
+
+void iota(Ctx &c, int32_t *p, int64_t n)
+{
+    if (n <= 0) return;
+    k_iota<<<grid_for(n, 256), 256, 0, c.stream>>>(p, n);
+    HGP_LAUNCH_CHECK();
+}
+
+void reduce_sum(Ctx &c, const int32_t *in, int64_t n, int64_t *out)
+{
+    fill<int64_t>(c, out, 1, 0);
+    if (n <= 0) return;
+    unsigned g = grid_for(n, 256);
+    if (g > 4096) g = 4096;
+    k_reduce_i32<<<g, 256, 0, c.stream>>>(in, n, (unsigned long long *)out);
+    HGP_LAUNCH_CHECK();
+}
+
+void compact_indices(Ctx &c, const uint8_t *flag, int64_t n, int32_t *idx, int64_t *count)
+{
+    if (n <= 0) {
+        fill<int64_t>(c, count, 1, 0);
+        return;
+    }
+    size_t mark = c.arena.right_mark();
+    int32_t *incl = c.arena.ralloc<int32_t>((size_t)n);
+    k_flag_to_i32<<<grid_for(n, 256), 256, 0, c.stream>>>(flag, n, incl);
+    HGP_LAUNCH_CHECK();
+    scan_impl<int32_t>(c, incl, incl, n);
+    k_compact<<<grid_for(n, 256), 256, 0, c.stream>>>(flag, incl, n, idx, count);
+    HGP_LAUNCH_CHECK();
+    c.arena.right_release(mark);
+}
+
+int bits_for(uint64_t maxval)
+{
+    int b = 0;
+    while (b < 64 && (maxval >> b) != 0) b++;
+    return b;
+}
+
+}  // namespace hgp

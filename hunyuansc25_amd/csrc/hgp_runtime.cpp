@@ -1,0 +1,135 @@
+// hgp_runtime.cpp -- error state, device contexts and the two-ended arena.
+#include "hgp_common.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+
+namespace hgp {
+
+namespace {
+thread_local int g_err_code = 0;
+thread_local std::string g_err_msg;
+std::mutex g_ctx_mu;
+std::map<int, std::unique_ptr<Ctx>> g_ctx;
+}  // namespace
+
+void set_error(int code, const std::string &msg)
+{
+    g_err_code = code;
+    g_err_msg = msg;
+}
+
+void clear_error()
+{
+    g_err_code = 0;
+    g_err_msg.clear();
+}
+
+const char *last_error_text() { return g_err_msg.c_str(); }
+
+// ------------------------------------------------------------------ arena
+
+static size_t round_up(size_t b) { return (b + 255) & ~(size_t)255; }
+
+Arena::~Arena()
+{
+    if (base_) (void)hipFree(base_);
+}
+
+void Arena::reserve(size_t bytes)
+{
+    bytes = round_up(bytes);
+    if (bytes <= cap_) return;
+    if (lo_ != 0 || hi_ != cap_) throw Error{-3, "arena: reserve while allocations are live"};
+    if (base_) HGP_CHECK_HIP(hipFree(base_));
+    base_ = nullptr;
+    cap_ = lo_ = hi_ = 0;
+    hipError_t e = hipMalloc((void **)&base_, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        base_ = nullptr;
+        throw Error{-3, "arena: hipMalloc of " + std::to_string(bytes) + " bytes failed"};
+    }
+    cap_ = bytes;
+    hi_ = cap_;
+}
+
+// An allocation that does not fit throws Error{-3}; the caller resets the
+// arena, reserves more and retries the whole call.
+void *Arena::left(size_t bytes)
+{
+    bytes = round_up(bytes);
+    if (lo_ + bytes > hi_)
+        throw Error{-3, "arena: left allocation of " + std::to_string(bytes) + " bytes does not fit"};
+    void *p = base_ + lo_;
+    lo_ += bytes;
+    if (lo_ + (cap_ - hi_) > peak_) peak_ = lo_ + (cap_ - hi_);
+    return p;
+}
+
+void *Arena::right(size_t bytes)
+{
+    bytes = round_up(bytes);
+    if (lo_ + bytes > hi_)
+        throw Error{-3, "arena: right allocation of " + std::to_string(bytes) + " bytes does not fit"};
+    hi_ -= bytes;
+    if (lo_ + (cap_ - hi_) > peak_) peak_ = lo_ + (cap_ - hi_);
+    return base_ + hi_;
+}
+
+void Arena::left_release(size_t mark) { lo_ = mark; }
+void Arena::right_release(size_t mark) { hi_ = mark; }
+
+void Arena::reset()
+{
+    lo_ = 0;
+    hi_ = cap_;
+}
+
+// ------------------------------------------------------------------ context
+
+Ctx &context(int device)
+{
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    auto it = g_ctx.find(device);
+    if (it != g_ctx.end()) return *it->second;
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) {
+        (void)hipGetLastError();
+        throw Error{-5, "no HIP device"};
+    }
+    if (device < 0 || device >= cnt) throw Error{-1, "device ordinal out of range"};
+    HGP_CHECK_HIP(hipSetDevice(device));
+    auto c = std::make_unique<Ctx>();
+    c->device = device;
+    HGP_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HGP_CHECK_HIP(hipHostMalloc((void **)&c->pinned, kPinnedBytes, hipHostMallocDefault));
+    for (auto &e : c->ev) HGP_CHECK_HIP(hipEventCreate(&e));
+    Ctx &ref = *c;
+    g_ctx[device] = std::move(c);
+    return ref;
+}
+
+void upload_staged(Ctx &c, void *ddst, const void *hsrc, size_t bytes)
+{
+    char *stage = (char *)c.pinned + kStageOffset;
+    size_t cap = kPinnedBytes - kStageOffset;
+    for (size_t off = 0; off < bytes; off += cap) {
+        size_t b = std::min(cap, bytes - off);
+        HGP_CHECK_HIP(hipStreamSynchronize(c.stream));   // staging area free again
+        memcpy(stage, (const char *)hsrc + off, b);
+        HGP_CHECK_HIP(hipMemcpyAsync((char *)ddst + off, stage, b, hipMemcpyHostToDevice, c.stream));
+    }
+    HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+}
+
+void read_scalars(Ctx &c, const void *dsrc, size_t bytes)
+{
+    HGP_CHECK_HIP(hipMemcpyAsync(c.pinned, dsrc, bytes, hipMemcpyDeviceToHost, c.stream));
+    HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+}
+
+}  // namespace hgp

@@ -1,0 +1,172 @@
+/*
+ * hunyuangraph.h -- C ABI of the MI355X multilevel k-way graph partitioner.
+ *
+ * The first symbol is the drop-in replacement of the reference's entry point;
+ * everything else is the versioned API (64-bit edge offsets, honoured weights,
+ * a seed, status codes) plus stage entry points used by the parity tests.
+ * All pointers are plain host pointers unless a function says "device".
+ * No function exits the process; errors are returned as negative status codes
+ * and described by hgp_last_error().
+ */
+#ifndef HUNYUANGRAPH_H
+#define HUNYUANGRAPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ drop-in */
+
+/* Replaces hunyuangraph_PartitionGraph (hunyuangraph_partitiongraph.h:284).
+ * Same signature and CSR contract: int32 xadj/adjncy, nvtxs vertices,
+ * *nparts parts, part[] out.  Unlike the reference, vwgt and adjwgt are
+ * honoured (NULL means unit weights), *ubvec (if non-NULL and > 1) is the
+ * balance bound (default 1.03, enforced exactly), tpwgts must be NULL or
+ * uniform.  On error part[] is left unspecified and hgp_last_error() says why
+ * (the reference exits the process instead). */
+void hunyuangraph_PartitionGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, int *adjwgt,
+                                 int *nparts, float *tpwgts, float *ubvec, int *part);
+
+/* ------------------------------------------------------------ status */
+
+enum {
+    HGP_OK = 0,
+    HGP_ERR_INVALID = -1,     /* malformed CSR, k < 1, weights <= 0, ...          */
+    HGP_ERR_DEVICE = -2,      /* HIP runtime error                                 */
+    HGP_ERR_OOM = -3,         /* device memory or HBM budget exhausted             */
+    HGP_ERR_UNSUPPORTED = -4, /* outside the supported range (e.g. k too large)    */
+    HGP_ERR_NODEVICE = -5     /* no HIP device present                              */
+};
+
+/* Text of the last error on the calling thread ("" if none). */
+const char *hgp_last_error(void);
+/* Library version string. */
+const char *hgp_version(void);
+/* Number of HIP devices (0 if none; never errors). */
+int hgp_device_count(void);
+
+/* ------------------------------------------------------------ options */
+
+typedef struct hgp_options {
+    uint64_t seed;          /* random stream of matching ties and trial starts   */
+    int32_t ub_num;         /* balance bound ub_num/ub_den (default 103/100)     */
+    int32_t ub_den;
+    int32_t match_rounds;   /* handshake rounds per level (default 8)            */
+    int32_t trials;         /* initial bisection trials per node (default 1024)  */
+    int32_t fm_passes;      /* FM passes per trial (default 10)                  */
+    int32_t patience;       /* Jet non-improving rounds (default 6)              */
+    int32_t validate;       /* 1: check CSR symmetry on the host before running  */
+    int32_t device;         /* HIP device ordinal (default 0)                    */
+} hgp_options;
+
+#define HGP_MAX_LEVELS 64
+
+typedef struct hgp_stats {
+    int64_t cut;            /* final edge cut (exact)                            */
+    int64_t max_pwgt;       /* heaviest part                                     */
+    int64_t tvwgt;          /* total vertex weight                               */
+    int32_t levels;         /* coarse levels built                               */
+    int32_t coarsest_n;
+    int64_t coarsest_m;
+    int64_t init_cut;       /* cut of the initial partition                      */
+    int64_t refine_iters;   /* Jet iterations over all levels                    */
+    double ms_total;        /* device time, CSR resident -> part resident        */
+    double ms_coarsen, ms_init, ms_uncoarsen;
+    double alg_bytes;       /* algorithmic HBM bytes of the run                  */
+    int64_t level_n[HGP_MAX_LEVELS];
+    int64_t level_m[HGP_MAX_LEVELS];
+    int32_t level_iters[HGP_MAX_LEVELS];
+} hgp_stats;
+
+void hgp_default_options(hgp_options *opt);
+
+/* ------------------------------------------------------------ partition */
+
+/* Full pipeline on host arrays (copied to the device and back).
+ * xadj has n+1 int64 offsets; vwgt/adjwgt may be NULL (unit weights). */
+int hgp_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                  const int32_t *adjwgt, int32_t k, const hgp_options *opt, int32_t *part,
+                  hgp_stats *stats);
+
+/* Full pipeline on device-resident arrays (device pointers on opt->device);
+ * d_part (device, n entries) receives the partition.  The caller's arrays are
+ * not modified.  ms_total in stats is the device time of this call. */
+int hgp_partition_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy,
+                         const int32_t *d_vwgt, const int32_t *d_adjwgt, int32_t k,
+                         const hgp_options *opt, int32_t *d_part, hgp_stats *stats);
+
+/* ------------------------------------------------------------ stages
+ * Host-array entry points of single stages, for parity tests.  Each copies
+ * its inputs to the device, runs the GPU stage and copies the result back. */
+
+/* Inclusive scans (hunyuangraph_GPU_prefixsum.h:141-188). in and out may alias. */
+int hgp_scan_i32(const int32_t *in, int32_t *out, int64_t n);
+int hgp_scan_i64(const int64_t *in, int64_t *out, int64_t n);
+
+/* Part weights (hunyuangraph_GPU_uncoarsen.h:19-45) and edge cut
+ * (hunyuangraph_graph.h:184-268). */
+int hgp_part_weights(int32_t n, const int32_t *vwgt, const int32_t *where, int32_t k,
+                     int64_t *pwgts);
+int hgp_edge_cut(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
+                 const int32_t *where, int64_t *cut);
+
+/* One level of matching (hunyuangraph_gpu_match, hunyuangraph_GPU_match.h:1846):
+ * match[v] = partner or -1 (unmatched), before normalisation. */
+int hgp_match(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+              const int32_t *adjwgt, int32_t maxvwgt, uint64_t seed, int32_t level,
+              int32_t rounds, int32_t *match, int64_t *matched);
+
+/* cmap of a matching plus contraction into the canonical coarse CSR
+ * (hunyuangraph_gpu_create_cgraph, hunyuangraph_GPU_contraction.h:715).
+ * match (n) is normalised in place (unmatched -> self).  Output buffers:
+ * cmap n, cxadj n+1, cadjncy/cadjwgt xadj[n], cvwgt n.  *cn, *cm out. */
+int hgp_contract(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *adjwgt,
+                 const int32_t *vwgt, int32_t *match, int32_t *cmap, int32_t *cn,
+                 int64_t *cxadj, int32_t *cadjncy, int32_t *cadjwgt, int32_t *cvwgt,
+                 int64_t *cm);
+
+/* Recursive-bisection initial partition (hunyuangraph_gpu_initialpartition,
+ * hunyuangraph_GPU_initialpartition.h:4405) of a graph into k parts. */
+int hgp_initpart(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                 const int32_t *adjwgt, int32_t k, const hgp_options *opt, int32_t *where);
+
+/* Jet refinement of one level (k_refine, hunyuangraph_GPU_krefine.h:4434);
+ * where and *cut in/out, *iters out. */
+int hgp_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+               const int32_t *adjwgt, int32_t k, int64_t tvwgt, int32_t is_finest,
+               const hgp_options *opt, int32_t *where, int64_t *cut, int32_t *iters);
+
+/* ------------------------------------------------------------ graphs
+ * Seeded synthetic generators (host code).  The library allocates the
+ * arrays; release them with hgp_graph_free. */
+typedef struct hgp_graph {
+    int32_t n;
+    int64_t m;              /* directed entries = 2 x undirected edges */
+    int64_t *xadj;
+    int32_t *adjncy;
+    int32_t *adjwgt;
+    int32_t *vwgt;
+} hgp_graph;
+
+void hgp_graph_free(hgp_graph *g);
+/* 5-point grid rows x cols, rows sorted, unit weights. */
+int hgp_gen_grid2d(int32_t rows, int32_t cols, hgp_graph *g);
+/* 3-D random geometric graph: n points in [0,1)^3, radius r; optional
+ * weights vwgt U{vw_lo..vw_hi}, adjwgt U{ew_lo..ew_hi} (0,0 = unit). */
+int hgp_gen_rgg3d(int32_t n, double radius, uint64_t seed, int32_t vw_lo, int32_t vw_hi,
+                  int32_t ew_lo, int32_t ew_hi, hgp_graph *g);
+/* R-MAT with 2^scale vertices and exactly nedges unique undirected edges,
+ * self-loops and duplicates removed, seeded vertex relabelling. */
+int hgp_gen_rmat(int32_t scale, int64_t nedges, double a, double b, double c, uint64_t seed,
+                 hgp_graph *g);
+/* Banded FEM-like pattern: |i-j| <= bw plus frac*n*bw random symmetric
+ * off-band entries. */
+int hgp_gen_banded(int32_t n, int32_t bw, double frac, uint64_t seed, hgp_graph *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

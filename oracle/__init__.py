@@ -1,0 +1,266 @@
+"""CPU oracle of the partitioning path -- TEST INFRASTRUCTURE ONLY.
+
+ctypes loader for ``oracle/libhgp_oracle.so`` (C sources in this folder, see
+``hgp_oracle.h``).  Only ``tests/``, ``__graft_entry__.smoke()`` and
+``bench.py``'s CPU-baseline leg use it, as the checker or as the timed host
+path; the product library never loads it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhgp_oracle.so")
+MAX_LEVELS = 64
+
+
+class OracleOpts(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("ub_num", C.c_int32),
+        ("ub_den", C.c_int32),
+        ("match_rounds", C.c_int32),
+        ("trials", C.c_int32),
+        ("fm_passes", C.c_int32),
+        ("patience", C.c_int32),
+    ]
+
+
+class OracleStats(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int32),
+        ("coarsest_n", C.c_int32),
+        ("coarsest_m", C.c_int64),
+        ("init_cut", C.c_int64),
+        ("cut", C.c_int64),
+        ("max_pwgt", C.c_int64),
+        ("refine_iters", C.c_int64),
+        ("level_n", C.c_int64 * MAX_LEVELS),
+        ("level_m", C.c_int64 * MAX_LEVELS),
+        ("level_iters", C.c_int32 * MAX_LEVELS),
+    ]
+
+    def to_dict(self):
+        L = self.levels
+        return {
+            "levels": int(L),
+            "coarsest_n": int(self.coarsest_n),
+            "coarsest_m": int(self.coarsest_m),
+            "init_cut": int(self.init_cut),
+            "cut": int(self.cut),
+            "max_pwgt": int(self.max_pwgt),
+            "refine_iters": int(self.refine_iters),
+            "level_n": [int(self.level_n[i]) for i in range(L + 1)],
+            "level_m": [int(self.level_m[i]) for i in range(L + 1)],
+            "level_iters": [int(self.level_iters[i]) for i in range(L + 1)],
+        }
+
+
+_P = C.c_void_p
+_I32, _I64, _U64, _U32 = C.c_int32, C.c_int64, C.c_uint64, C.c_uint32
+_SIG = {
+    "oracle_scan_i32": (None, [_P, _I64]),
+    "oracle_scan_i64": (None, [_P, _I64]),
+    "oracle_part_weights": (None, [_I32, _P, _P, _I32, _P]),
+    "oracle_edge_cut": (_I64, [_I32, _P, _P, _P, _P]),
+    "oracle_balanced": (C.c_int, [_I32, _P, _P, _I32, _I64, _I64]),
+    "oracle_cmap": (_I32, [_I32, _P, _P]),
+    "oracle_contract": (_I64, [_I32, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _P]),
+    "oracle_default_opts": (None, [_P]),
+    "oracle_dga_partition": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
+    "oracle_coarsen_threshold": (_I32, [_I32]),
+    "oracle_maxvwgt": (_I32, [_I64, _I32]),
+    "oracle_dga_match": (_I64, [_I32, _P, _P, _P, _P, _I32, _U64, _I32, _I32, _P]),
+    "oracle_edge_hash": (_U32, [_U64, _I32, _I32, _I32]),
+    "oracle_dga_initpart": (None, [_I32, _P, _P, _P, _P, _I32, _P, _P]),
+    "oracle_dga_trial": (_I64, [_I32, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I32, _I32, _U64, _U32, _I32, _P]),
+    "oracle_dga_refine": (_I32, [_I32, _P, _P, _P, _P, _I32, _I64, _I32, _P, _P, _P]),
+    "oracle_jet_lp_step": (_I64, [_I32, _P, _P, _P, _I32, _P, _I32, _I32, _P, _P]),
+    "oracle_gain_bucket": (_I32, [_I64, _I64]),
+    "oracle_cpu_rb_partition": (_I64, [_I32, _P, _P, _P, _P, _I32, _U64, _P]),
+}
+
+_lib = None
+
+
+def build() -> None:
+    """Compiles the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        h = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIG.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = h
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _i64(a):
+    return np.ascontiguousarray(a, dtype=np.int64)
+
+
+def opts(**kw) -> OracleOpts:
+    o = OracleOpts()
+    lib().oracle_default_opts(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def scan_i32(a):
+    a = _i32(a).copy()
+    lib().oracle_scan_i32(_p(a), a.size)
+    return a
+
+
+def scan_i64(a):
+    a = _i64(a).copy()
+    lib().oracle_scan_i64(_p(a), a.size)
+    return a
+
+
+def part_weights(vwgt, where, k):
+    vwgt, where = _i32(vwgt), _i32(where)
+    out = np.zeros(int(k), dtype=np.int64)
+    lib().oracle_part_weights(len(vwgt), _p(vwgt), _p(where), int(k), _p(out))
+    return out
+
+
+def edge_cut(xadj, adjncy, adjwgt, where) -> int:
+    xadj, adjncy, adjwgt, where = _i64(xadj), _i32(adjncy), _i32(adjwgt), _i32(where)
+    return int(lib().oracle_edge_cut(len(xadj) - 1, _p(xadj), _p(adjncy), _p(adjwgt), _p(where)))
+
+
+def balanced(vwgt, where, k, ub_num=103, ub_den=100) -> bool:
+    vwgt, where = _i32(vwgt), _i32(where)
+    return bool(lib().oracle_balanced(len(vwgt), _p(vwgt), _p(where), int(k), int(ub_num), int(ub_den)))
+
+
+def cmap(match):
+    m = _i32(match).copy()
+    cm = np.zeros(len(m), dtype=np.int32)
+    cn = lib().oracle_cmap(len(m), _p(m), _p(cm))
+    return m, cm, int(cn)
+
+
+def contract(xadj, adjncy, adjwgt, vwgt, match):
+    """(match_normalised, cmap, cxadj, cadjncy, cadjwgt, cvwgt)"""
+    xadj, adjncy, adjwgt, vwgt = _i64(xadj), _i32(adjncy), _i32(adjwgt), _i32(vwgt)
+    m, cm, cn = cmap(match)
+    n = len(xadj) - 1
+    mm = int(xadj[-1])
+    cx = np.zeros(cn + 1, dtype=np.int64)
+    ca = np.zeros(max(mm, 1), dtype=np.int32)
+    cw = np.zeros(max(mm, 1), dtype=np.int32)
+    cv = np.zeros(max(cn, 1), dtype=np.int32)
+    e = lib().oracle_contract(n, _p(xadj), _p(adjncy), _p(adjwgt), _p(vwgt), _p(m), _p(cm), cn, _p(cx), _p(ca),
+                              _p(cw), _p(cv))
+    return m, cm, cx, ca[:e].copy(), cw[:e].copy(), cv[:cn].copy()
+
+
+def coarsen_threshold(k):
+    return int(lib().oracle_coarsen_threshold(int(k)))
+
+
+def maxvwgt(tvwgt, thr):
+    return int(lib().oracle_maxvwgt(int(tvwgt), int(thr)))
+
+
+def edge_hash(seed, level, lo, hi):
+    return int(lib().oracle_edge_hash(int(seed), int(level), int(lo), int(hi)))
+
+
+def dga_match(xadj, adjncy, vwgt, adjwgt, maxvwgt_, seed, level, rounds):
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    m = np.zeros(n, dtype=np.int32)
+    cnt = lib().oracle_dga_match(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(maxvwgt_), int(seed), int(level),
+                                 int(rounds), _p(m))
+    return m, int(cnt)
+
+
+def dga_partition(xadj, adjncy, k, vwgt=None, adjwgt=None, **kw):
+    xadj, adjncy = _i64(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    vwgt = np.ones(n, np.int32) if vwgt is None else _i32(vwgt)
+    adjwgt = np.ones(len(adjncy), np.int32) if adjwgt is None else _i32(adjwgt)
+    o = opts(**kw)
+    st = OracleStats()
+    part = np.zeros(n, dtype=np.int32)
+    rc = lib().oracle_dga_partition(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(k), C.byref(o), _p(part),
+                                    C.byref(st))
+    if rc != 0:
+        raise ValueError(f"oracle_dga_partition failed: {rc}")
+    return part, st.to_dict()
+
+
+def dga_initpart(xadj, adjncy, vwgt, adjwgt, k, **kw):
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    o = opts(**kw)
+    where = np.zeros(n, dtype=np.int32)
+    lib().oracle_dga_initpart(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(k), C.byref(o), _p(where))
+    return where
+
+
+def dga_trial(xadj, adjncy, vwgt, adjwgt, tvwgt, onemin, onemax, ideal0, ntrials, trial, seed, node_id, fm_passes):
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    w = np.zeros(n, dtype=np.uint8)
+    cut = lib().oracle_dga_trial(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(tvwgt), int(onemin), int(onemax),
+                                 int(ideal0), int(ntrials), int(trial), int(seed), int(node_id), int(fm_passes), _p(w))
+    return w, int(cut)
+
+
+def dga_refine(xadj, adjncy, vwgt, adjwgt, k, tvwgt, is_finest, where, cut, **kw):
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    o = opts(**kw)
+    w = _i32(where).copy()
+    c = np.array([cut], dtype=np.int64)
+    it = lib().oracle_dga_refine(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(k), int(tvwgt), int(is_finest),
+                                 C.byref(o), _p(w), _p(c))
+    return w, int(c[0]), int(it)
+
+
+def jet_lp_step(xadj, adjncy, adjwgt, k, where, filter_num, filter_den, lock):
+    xadj, adjncy, adjwgt, where = _i64(xadj), _i32(adjncy), _i32(adjwgt), _i32(where)
+    n = len(xadj) - 1
+    lk = np.ascontiguousarray(lock, dtype=np.uint8).copy()
+    dest = np.zeros(n, dtype=np.int32)
+    moves = lib().oracle_jet_lp_step(n, _p(xadj), _p(adjncy), _p(adjwgt), int(k), _p(where), int(filter_num),
+                                     int(filter_den), _p(lk), _p(dest))
+    return dest, lk, int(moves)
+
+
+def gain_bucket(gain, vwgt):
+    return int(lib().oracle_gain_bucket(int(gain), int(vwgt)))
+
+
+def cpu_rb_partition(xadj, adjncy, k, vwgt=None, adjwgt=None, seed=1):
+    xadj, adjncy = _i64(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    vwgt = np.ones(n, np.int32) if vwgt is None else _i32(vwgt)
+    adjwgt = np.ones(len(adjncy), np.int32) if adjwgt is None else _i32(adjwgt)
+    part = np.zeros(n, dtype=np.int32)
+    cut = lib().oracle_cpu_rb_partition(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(k), int(seed), _p(part))
+    return part, int(cut)

@@ -1,0 +1,174 @@
+/*
+ * hgp_oracle.h -- CPU oracle for the multilevel k-way partitioning path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+ * calls this code.  Only tests/, __graft_entry__.smoke() and bench.py's CPU
+ * baseline leg may use it, and only as the checker (or, for the CPU baseline,
+ * as the timed host path), never as the thing measured on the GPU.
+ *
+ * Three groups of functions:
+ *
+ *  (1) Exact primitives -- closed-form integer arithmetic that the GPU must
+ *      reproduce bit for bit: inclusive scans, part weights, edge cut, the
+ *      exact 1.03 balance test, cmap of a fixed matching and contraction of a
+ *      fixed matching into the canonical coarse CSR.
+ *
+ *  (2) "DGA" -- a serial emulation of the deterministic GPU algorithm: the
+ *      reference GPU pipeline (hunyuangraph_PartitionGraph,
+ *      hunyuangraph_partitiongraph.h:284) with every atomics-ordered or
+ *      CAS-raced decision replaced by a deterministic rule.  The GPU library
+ *      implements the same rules, so its part[] must equal oracle_dga_partition
+ *      bit for bit on the same inputs and seed.
+ *
+ *  (3) "CPU RB" -- a restatement of the reference CPU path
+ *      hunyuangraph_rbbisection (hunyuangraph_CPU_initialpartition.h:361):
+ *      serial METIS-style multilevel recursive bisection.  It is the timed
+ *      host-CPU baseline.  The reference draws from glibc rand() (never
+ *      seeded, hunyuangraph_common.h:34-47); this restatement draws from a
+ *      seeded xorshift generator instead, so its random sequences (visit
+ *      order, GGGP start vertices, FM permutations) are "parity unpinned".
+ *
+ * Index conventions: vertex ids int32, edge offsets (xadj) int64, weights
+ * int32, sums int64.  CSR is symmetric, 0-based, no self-loops.
+ */
+#ifndef HGP_ORACLE_H
+#define HGP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- (1) */
+
+/* Inclusive in-place scan (hunyuangraph_GPU_prefixsum.h:141-188 computes an
+ * inclusive in-place scan; int32 wraps exactly as two's complement). */
+void oracle_scan_i32(int32_t *a, int64_t n);
+void oracle_scan_i64(int64_t *a, int64_t n);
+
+/* pwgts[p] = sum of vwgt[v] over where[v]==p  (hunyuangraph_GPU_uncoarsen.h:19-45). */
+void oracle_part_weights(int32_t n, const int32_t *vwgt, const int32_t *where,
+                         int32_t k, int64_t *pwgts);
+
+/* Edge cut = sum over directed entries with where[u]!=where[v] of adjwgt, /2
+ * (hunyuangraph_graph.h:144-155). */
+int64_t oracle_edge_cut(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                        const int32_t *adjwgt, const int32_t *where);
+
+/* Exact balance test: 100*k*max_p pwgts[p] <= 103*tvwgt, generalised to
+ * ub_num/ub_den.  Returns 1 if balanced. */
+int oracle_balanced(int32_t n, const int32_t *vwgt, const int32_t *where, int32_t k,
+                    int64_t ub_num, int64_t ub_den);
+
+/* cmap of a fixed matching (hunyuangraph_GPU_match.h:1012-1024 + scan :3088):
+ * match[v]==-1 or an unreciprocated entry is treated as a singleton and
+ * rewritten to v; cmap[v] = number of representatives (v<=match[v]) in
+ * 1..v (vertex 0 maps to 0); the partner's cmap copies the representative's.
+ * Returns cnvtxs.  match is normalised in place. */
+int32_t oracle_cmap(int32_t n, int32_t *match, int32_t *cmap);
+
+/* Contraction of a fixed (normalised) matching into the canonical coarse CSR
+ * (hunyuangraph_CPU_contraction.h:9-98, hunyuangraph_GPU_contraction.h:715-1247):
+ * rows = concatenated rows of both endpoints renamed by cmap, self entries
+ * dropped, duplicates merged by summing weights, sorted by coarse id.
+ * Output arrays must hold cn+1, m (fine m is an upper bound), cn entries.
+ * Returns the coarse directed entry count. */
+int64_t oracle_contract(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                        const int32_t *adjwgt, const int32_t *vwgt,
+                        const int32_t *match, const int32_t *cmap, int32_t cn,
+                        int64_t *cxadj, int32_t *cadjncy, int32_t *cadjwgt,
+                        int32_t *cvwgt);
+
+/* ---------------------------------------------------------------- (2) */
+
+typedef struct oracle_opts {
+    uint64_t seed;
+    int32_t ub_num, ub_den;     /* final balance bound, default 103/100          */
+    int32_t match_rounds;       /* handshake rounds per level, default 8         */
+    int32_t trials;             /* initial-bisection trials, default 1024        */
+    int32_t fm_passes;          /* FM passes per trial, default 10               */
+    int32_t patience;           /* Jet non-improving rounds, default 6           */
+} oracle_opts;
+
+typedef struct oracle_stats {
+    int32_t levels;             /* number of coarse levels built                 */
+    int32_t coarsest_n;
+    int64_t coarsest_m;
+    int64_t init_cut;           /* cut of the initial partition (coarsest level) */
+    int64_t cut;                /* final cut (exact)                             */
+    int64_t max_pwgt;
+    int64_t refine_iters;       /* total Jet iterations over all levels          */
+    int64_t level_n[64];
+    int64_t level_m[64];
+    int32_t level_iters[64];    /* Jet iterations per level (index = level)      */
+} oracle_stats;
+
+void oracle_default_opts(oracle_opts *o);
+
+/* Full deterministic pipeline.  part[] out.  Returns 0 on success, negative on
+ * invalid input. */
+int oracle_dga_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                         const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
+                         const oracle_opts *opts, int32_t *part, oracle_stats *st);
+
+/* Stage entry points of the DGA, exposed for stage-by-stage parity tests. */
+
+/* Coarsening parameters (hunyuangraph_partitiongraph.h:298-302,
+ * hunyuangraph_GPU_coarsen.h:165). */
+int32_t oracle_coarsen_threshold(int32_t k);
+int32_t oracle_maxvwgt(int64_t tvwgt, int32_t threshold);
+
+/* One level of matching, before normalisation: match[v] = partner or -1.
+ * Returns the number of matched vertices. */
+int64_t oracle_dga_match(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                         const int32_t *vwgt, const int32_t *adjwgt, int32_t maxvwgt,
+                         uint64_t seed, int32_t level, int32_t rounds, int32_t *match);
+
+/* Edge priority hash of the handshake matching. */
+uint32_t oracle_edge_hash(uint64_t seed, int32_t level, int32_t lo, int32_t hi);
+
+/* Recursive-bisection initial partition of a (coarsest) graph into k parts. */
+void oracle_dga_initpart(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                         const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
+                         const oracle_opts *opts, int32_t *where);
+
+/* One GGGP+FM trial on a graph (exposed for tests): returns the cut, fills
+ * where (0/1).  node_id and trial select the random stream. */
+int64_t oracle_dga_trial(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                         const int32_t *vwgt, const int32_t *adjwgt,
+                         int64_t tvwgt, int64_t onemin, int64_t onemax, int64_t ideal0,
+                         int32_t ntrials, int32_t trial, uint64_t seed, uint32_t node_id,
+                         int32_t fm_passes, uint8_t *where);
+
+/* Jet refinement of one level in place.  Returns the number of iterations.
+ * cut is in/out. */
+int32_t oracle_dga_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                          const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
+                          int64_t tvwgt, int32_t is_finest, const oracle_opts *opts,
+                          int32_t *where, int64_t *cut);
+
+/* One Jet label-propagation step (select + afterburner) without applying the
+ * moves: dest[v] = new part for selected vertices, else where[v].
+ * lock is in/out with the reference's lock semantics.  Returns #moves. */
+int64_t oracle_jet_lp_step(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                           const int32_t *adjwgt, int32_t k, const int32_t *where,
+                           int32_t filter_num, int32_t filter_den,
+                           uint8_t *lock, int32_t *dest);
+
+/* Integer restatement of gain_bucket (hunyuangraph_GPU_krefine.h:2477-2513). */
+int32_t oracle_gain_bucket(int64_t gain, int64_t vwgt);
+
+/* ---------------------------------------------------------------- (3) */
+
+/* Reference CPU path restated: multilevel recursive bisection with the
+ * standalone parameters of hunyuangraph_rbbisection (threshold 200, 1 cut,
+ * GGGP x5/7, FM x10, ubfactor 1.03 per bisection).  Returns the cut. */
+int64_t oracle_cpu_rb_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                                const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
+                                uint64_t seed, int32_t *part);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,198 @@
+"""GPU parity: every stage of the HIP path, called through the C ABI, must equal
+the oracle bit for bit (integer arithmetic throughout)."""
+import numpy as np
+import pytest
+
+from conftest import csr_from_edges, random_graph
+
+pytestmark = pytest.mark.gpu
+
+
+def graphs(hgp):
+    """Small graphs covering the shapes the pipeline meets."""
+    out = {}
+    out["grid32"] = hgp.gen_grid2d(32, 32)
+    x, a, vw, aw = random_graph(300, 0.03, seed=3, wmax=9, vmax=4)
+    out["er300w"] = (x, a, vw, aw)
+    out["rmat12"] = hgp.gen_rmat(12, 1 << 14, seed=5)
+    out["rgg2k"] = hgp.gen_rgg3d(2000, hgp.rgg_radius(2000, 12), seed=7, vw=(1, 4), ew=(1, 16))
+    out["band1k"] = hgp.gen_banded(1000, 8, 0.1, seed=9)
+    # star with leaves plus isolated vertices: exercises leaf/isolate/forced
+    edges = [(0, i) for i in range(1, 40)] + [(40, 41), (41, 42)]
+    x, a, w = csr_from_edges(60, edges)
+    out["star_iso"] = (x, a, np.ones(60, np.int32), w)
+    return out
+
+
+@pytest.mark.parametrize("n", [1, 2, 255, 2047, 2048, 2049, 100003, (1 << 20) + 7])
+def test_scan_i32(hgp, orc, n):
+    rng = np.random.default_rng(n)
+    a = rng.integers(-1000, 1000, size=n).astype(np.int32)
+    np.testing.assert_array_equal(hgp.scan_i32(a), orc.scan_i32(a))
+    np.testing.assert_array_equal(hgp.scan_i32(a), np.cumsum(a, dtype=np.int64).astype(np.int32))
+
+
+def test_scan_i32_wraps(hgp, orc):
+    a = np.full(5000, 2**30, dtype=np.int32)
+    np.testing.assert_array_equal(hgp.scan_i32(a), orc.scan_i32(a))
+
+
+@pytest.mark.parametrize("n", [1, 3000, 4194305])
+def test_scan_i64(hgp, orc, n):
+    rng = np.random.default_rng(n + 1)
+    a = rng.integers(-(2**40), 2**40, size=n).astype(np.int64)
+    np.testing.assert_array_equal(hgp.scan_i64(a), orc.scan_i64(a))
+    np.testing.assert_array_equal(hgp.scan_i64(a), np.cumsum(a))
+
+
+def test_part_weights_and_cut(hgp, orc):
+    for name, (x, a, vw, aw) in graphs(hgp).items():
+        n = len(x) - 1
+        for k in (2, 7, 64):
+            where = np.random.default_rng(k).integers(0, k, size=n).astype(np.int32)
+            np.testing.assert_array_equal(hgp.part_weights(vw, where, k), orc.part_weights(vw, where, k), err_msg=name)
+            assert hgp.edge_cut(x, a, aw, where) == orc.edge_cut(x, a, aw, where), name
+
+
+@pytest.mark.parametrize("level", [0, 3])
+def test_match_bit_exact(hgp, orc, level):
+    for name, (x, a, vw, aw) in graphs(hgp).items():
+        tv = int(vw.sum())
+        for k in (2, 64):
+            mv = orc.maxvwgt(tv, orc.coarsen_threshold(k))
+            m_g, c_g = hgp.match(x, a, vw, aw, mv, 11, level, 8)
+            m_o, c_o = orc.dga_match(x, a, vw, aw, mv, 11, level, 8)
+            np.testing.assert_array_equal(m_g, m_o, err_msg=f"{name} k={k}")
+            assert c_g == c_o
+
+
+def test_contract_bit_exact(hgp, orc):
+    for name, (x, a, vw, aw) in graphs(hgp).items():
+        tv = int(vw.sum())
+        mv = orc.maxvwgt(tv, orc.coarsen_threshold(4))
+        mt, _ = orc.dga_match(x, a, vw, aw, mv, 1, 0, 8)
+        got = hgp.contract(x, a, aw, vw, mt)
+        exp = orc.contract(x, a, aw, vw, mt)
+        for g_arr, e_arr, what in zip(got, exp, ["match", "cmap", "cxadj", "cadjncy", "cadjwgt", "cvwgt"]):
+            np.testing.assert_array_equal(g_arr, e_arr, err_msg=f"{name}: {what}")
+
+
+def test_contract_random_matchings(hgp, orc):
+    """Contraction of arbitrary (non-adjacent too) fixed matchings."""
+    x, a, vw, aw = random_graph(500, 0.02, seed=21, wmax=5, vmax=3)
+    n = len(x) - 1
+    rng = np.random.default_rng(4)
+    for trial in range(3):
+        perm = rng.permutation(n)
+        mt = -np.ones(n, np.int32)
+        npair = n // 3
+        for i in range(npair):
+            u, v = perm[2 * i], perm[2 * i + 1]
+            mt[u], mt[v] = v, u
+        got = hgp.contract(x, a, aw, vw, mt)
+        exp = orc.contract(x, a, aw, vw, mt)
+        for g_arr, e_arr in zip(got, exp):
+            np.testing.assert_array_equal(g_arr, e_arr)
+
+
+@pytest.mark.parametrize("k", [2, 3, 5, 8, 16])
+def test_initpart_bit_exact(hgp, orc, k):
+    for name, (x, a, vw, aw) in graphs(hgp).items():
+        if len(x) - 1 > 1200:
+            continue
+        w_g = hgp.initpart(x, a, vw, aw, k, seed=3, trials=64)
+        w_o = orc.dga_initpart(x, a, vw, aw, k, seed=3, trials=64)
+        np.testing.assert_array_equal(w_g, w_o, err_msg=f"{name} k={k}")
+
+
+def test_initpart_exhaustive_and_sampled(hgp, orc):
+    x, a, vw, aw = random_graph(200, 0.05, seed=8, wmax=4, vmax=3)
+    for trials in (1024, 37):     # n <= trials: exhaustive; n > trials: sampled starts
+        w_g = hgp.initpart(x, a, vw, aw, 6, seed=9, trials=trials)
+        w_o = orc.dga_initpart(x, a, vw, aw, 6, seed=9, trials=trials)
+        np.testing.assert_array_equal(w_g, w_o)
+
+
+@pytest.mark.parametrize("k", [4, 8, 64])
+@pytest.mark.parametrize("finest", [0, 1])
+def test_refine_bit_exact(hgp, orc, k, finest):
+    for name, (x, a, vw, aw) in graphs(hgp).items():
+        n = len(x) - 1
+        tv = int(vw.sum())
+        where = np.random.default_rng(k).integers(0, k, size=n).astype(np.int32)
+        cut = orc.edge_cut(x, a, aw, where)
+        w_g, c_g, i_g = hgp.refine(x, a, vw, aw, k, tv, finest, where, cut)
+        w_o, c_o, i_o = orc.dga_refine(x, a, vw, aw, k, tv, finest, where, cut)
+        np.testing.assert_array_equal(w_g, w_o, err_msg=f"{name} k={k}")
+        assert (c_g, i_g) == (c_o, i_o), name
+        assert c_g == orc.edge_cut(x, a, aw, w_g)
+
+
+@pytest.mark.parametrize("k", [2, 8, 32])
+def test_pipeline_bit_exact(hgp, orc, k):
+    for name, (x, a, vw, aw) in graphs(hgp).items():
+        p_g, st_g = hgp.partition(x, a, k, vw, aw, seed=5)
+        p_o, st_o = orc.dga_partition(x, a, k, vw, aw, seed=5)
+        np.testing.assert_array_equal(p_g, p_o, err_msg=f"{name} k={k}")
+        assert st_g["cut"] == st_o["cut"] == orc.edge_cut(x, a, aw, p_g)
+        assert st_g["levels"] == st_o["levels"]
+        assert st_g["level_iters"] == st_o["level_iters"]
+
+
+def test_pipeline_grid256_k8(hgp, orc):
+    """The plain small config: 256x256 5-point grid, k=8, ub 1.03, seed 1."""
+    x, a, vw, aw = hgp.gen_grid2d(256, 256)
+    p_g, st = hgp.partition(x, a, 8, vw, aw, seed=1)
+    p_o, st_o = orc.dga_partition(x, a, 8, vw, aw, seed=1)
+    np.testing.assert_array_equal(p_g, p_o)
+    assert st["cut"] == orc.edge_cut(x, a, aw, p_g)
+    assert orc.balanced(vw, p_g, 8, 103, 100)
+
+
+def test_dropin_matches_versioned(hgp):
+    x, a, vw, aw = hgp.gen_grid2d(40, 50)
+    n = len(x) - 1
+    p1 = hgp.PartitionGraph(n, x.astype(np.int32), a, vw, aw, 6)
+    p2, _ = hgp.partition(x, a, 6, vw, aw, seed=1)
+    np.testing.assert_array_equal(p1, p2)
+
+
+def test_deterministic(hgp):
+    x, a, vw, aw = hgp.gen_rmat(13, 1 << 15, seed=2)
+    p1, s1 = hgp.partition(x, a, 16, seed=7)
+    p2, s2 = hgp.partition(x, a, 16, seed=7)
+    np.testing.assert_array_equal(p1, p2)
+    assert s1["cut"] == s2["cut"]
+
+
+def test_edge_cases(hgp, orc):
+    # single vertex, no edges
+    x = np.array([0, 0], np.int64)
+    a = np.zeros(0, np.int32)
+    p, st = hgp.partition(x, a, 4)
+    assert p.tolist() == [0] or p[0] in range(4)
+    # k > n
+    x, a, w = csr_from_edges(5, [(0, 1), (1, 2), (2, 3), (3, 4)])
+    p, _ = hgp.partition(x, a, 8)
+    q, _ = orc.dga_partition(x, a, 8)
+    np.testing.assert_array_equal(p, q)
+    # k = 1
+    p, _ = hgp.partition(x, a, 1)
+    assert (p == 0).all()
+    # edgeless graph
+    x = np.zeros(101, np.int64)
+    p, _ = hgp.partition(x, np.zeros(0, np.int32), 4)
+    q, _ = orc.dga_partition(x, np.zeros(0, np.int32), 4)
+    np.testing.assert_array_equal(p, q)
+
+
+def test_invalid_inputs_raise(hgp):
+    x, a, w = csr_from_edges(4, [(0, 1), (1, 2)])
+    bad = a.copy()
+    bad[0] = 9
+    with pytest.raises(hgp.HgpError):
+        hgp.partition(x, bad, 2)
+    with pytest.raises(hgp.HgpError):
+        hgp.partition(x, a, 0)
+    with pytest.raises(hgp.HgpError):
+        hgp.partition(x, a, 2, vwgt=np.array([1, 0, 1, 1], np.int32))
