@@ -108,6 +108,8 @@ SIGNATURES = {
     "hgp_default_options": (None, [_P]),
     "hgp_partition": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "hgp_partition_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
+    "hgp_ktimer_enable": (C.c_int, [C.c_int]),
+    "hgp_ktimer_report": (C.c_int64, [_P, _I64]),
     "hgp_scan_i32": (C.c_int, [_P, _P, _I64]),
     "hgp_scan_i64": (C.c_int, [_P, _P, _I64]),
     "hgp_part_weights": (C.c_int, [_I32, _P, _P, _I32, _P]),
@@ -224,6 +226,23 @@ def partition_device(n, d_xadj, d_adjncy, d_vwgt, d_adjwgt, k, d_part, **opts) -
                                       C.c_void_p(d_adjwgt), int(k), C.byref(o), C.c_void_p(d_part),
                                       C.byref(st)))
     return st.to_dict()
+
+
+# ------------------------------------------------------------------ kernel timer
+
+def ktimer_enable(on: bool = True) -> None:
+    _check(lib().hgp_ktimer_enable(1 if on else 0))
+
+
+def ktimer_report() -> dict:
+    """{kernel: (launches, total_ms, algorithmic_bytes)} of the recorded launches."""
+    import json
+    need = lib().hgp_ktimer_report(None, 0)
+    if need < 0:
+        _check(int(need))
+    buf = C.create_string_buffer(int(need) + 16)
+    lib().hgp_ktimer_report(buf, len(buf))
+    return {k: tuple(v) for k, v in json.loads(buf.value.decode()).items()}
 
 
 # ------------------------------------------------------------------ stages

@@ -397,6 +397,56 @@ void hunyuangraph_PartitionGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, 
     });
 }
 
+// ---------------------------------------------------------------- kernel timer
+
+int hgp_ktimer_enable(int on)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        KTimer &t = ktimer();
+        t.on = on != 0;
+        t.recs.clear();
+        t.used = 0;
+    });
+}
+
+int64_t hgp_ktimer_report(char *buf, int64_t len)
+{
+    std::string out = "{";
+    try {
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        KTimer &t = ktimer();
+        struct Agg { int64_t n = 0; double ms = 0, bytes = 0; };
+        std::vector<std::pair<std::string, Agg>> agg;
+        for (auto &r : t.recs) {
+            HGP_CHECK_HIP(hipEventSynchronize(r.b));
+            float ms = 0;
+            HGP_CHECK_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            auto it = std::find_if(agg.begin(), agg.end(), [&](auto &x) { return x.first == r.name; });
+            if (it == agg.end()) { agg.push_back({r.name, Agg{}}); it = agg.end() - 1; }
+            it->second.n++;
+            it->second.ms += ms;
+            it->second.bytes += r.bytes;
+        }
+        char tmp[256];
+        for (size_t i = 0; i < agg.size(); i++) {
+            snprintf(tmp, sizeof(tmp), "%s\"%s\": [%lld, %.6f, %.1f]", i ? ", " : "", agg[i].first.c_str(),
+                     (long long)agg[i].second.n, agg[i].second.ms, agg[i].second.bytes);
+            out += tmp;
+        }
+    } catch (const Error &e) {
+        set_error(e.code, e.msg);
+        return e.code;
+    }
+    out += "}";
+    if (buf && len > 0) {
+        size_t nb = std::min((size_t)len - 1, out.size());
+        memcpy(buf, out.data(), nb);
+        buf[nb] = 0;
+    }
+    return (int64_t)out.size() + 1;
+}
+
 // ---------------------------------------------------------------- stages
 
 int hgp_scan_i32(const int32_t *in, int32_t *out, int64_t n)

@@ -404,9 +404,13 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
     unsigned gv = grid_for(n, kBS), gg = grid_for((int64_t)n * kG, kBS);
     for (int r = 0; r < rounds; r++) {
         const int32_t *prev = r > 0 ? progress + r - 1 : nullptr;
-        k_propose<<<gg, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, maxvwgt, seed, level,
-                                            prop, prev);
-        HGP_LAUNCH_CHECK();
+        {
+            // per level: rows (adjncy, adjwgt) + gathers of match/vwgt per entry; xadj, match, vwgt, prop per vertex
+            KScope ks(c, "match_propose", 16.0 * (double)g.m + 28.0 * n);
+            k_propose<<<gg, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, maxvwgt, seed, level,
+                                                prop, prev);
+            HGP_LAUNCH_CHECK();
+        }
         k_accept<<<gv, kBS, 0, c.stream>>>(n, prop, match, progress + r, prev);
         HGP_LAUNCH_CHECK();
     }
@@ -535,9 +539,13 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     if (m > 0) {
         uint64_t *key = c.arena.ralloc<uint64_t>((size_t)m);
         uint32_t *val = c.arena.ralloc<uint32_t>((size_t)m);
-        k_gather<<<grid_for((int64_t)cn * kG, kBS), kBS, 0, c.stream>>>(cn, rep, match, g.xadj, g.adjncy, g.adjwgt,
-                                                                       cmap, toff, key, val);
-        HGP_LAUNCH_CHECK();
+        {
+            // per fine entry: adjncy, adjwgt, cmap gather, key+val write; per coarse vertex: rep, match, toff, 2 xadj
+            KScope ks(c, "contract_gather", 24.0 * (double)m + 40.0 * cn);
+            k_gather<<<grid_for((int64_t)cn * kG, kBS), kBS, 0, c.stream>>>(cn, rep, match, g.xadj, g.adjncy,
+                                                                           g.adjwgt, cmap, toff, key, val);
+            HGP_LAUNCH_CHECK();
+        }
         int bc = bits_for((uint64_t)cn);
         int br = bits_for((uint64_t)(cn > 0 ? cn - 1 : 0));
         radix_sort(c, key, val, m, 0, bc);

@@ -83,6 +83,39 @@ constexpr size_t kStageOffset = 64 << 10;   // staging area starts here
 
 Ctx &context(int device);         // per-device default context (lazily created)
 
+// ------------------------------------------------------------------ kernel timer
+// Optional per-kernel timing with HIP events on the context's stream: a
+// KScope records an event pair around the launches in its scope and the
+// algorithmic HBM bytes those launches must move.  Off unless enabled.
+struct KRec {
+    const char *name;
+    hipEvent_t a, b;
+    double bytes;
+};
+
+struct KTimer {
+    bool on = false;
+    std::vector<KRec> recs;
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    hipEvent_t get();
+};
+
+KTimer &ktimer();
+
+class KScope {
+  public:
+    KScope(Ctx &c, const char *name, double bytes);
+    ~KScope();
+
+  private:
+    Ctx &c_;
+    bool on_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
+    const char *name_;
+    double bytes_;
+};
+
 // Copies `bytes` from device to the pinned buffer and waits.
 void read_scalars(Ctx &c, const void *dsrc, size_t bytes);
 

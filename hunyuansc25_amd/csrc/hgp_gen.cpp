@@ -13,6 +13,8 @@
 #include <cstring>
 #include <thread>
 #include <vector>
+#include <chrono>
+#include <cstdio>
 
 namespace {
 
@@ -30,6 +32,15 @@ inline uint64_t rnd(uint64_t seed, uint64_t stream, uint64_t i)
 }
 
 inline double unit(uint64_t r) { return (double)(r >> 11) * (1.0 / 9007199254740992.0); }
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+bool gen_timing() { return getenv("HGP_GEN_TIMING") != nullptr; }
+
+#define GEN_T(label) do { if (gen_timing()) fprintf(stderr, "gen %-12s %.3f s\n", label, now_s() - t0); } while (0)
 
 int nthreads()
 {
@@ -80,48 +91,32 @@ void radix_sort_u64(std::vector<uint64_t> &a)
     }
 }
 
-// Stable LSD radix sort of (key, val) pairs by key.
-void radix_sort_pairs_u64(std::vector<uint64_t> &k, std::vector<uint64_t> &v)
-{
-    std::vector<uint64_t> k2(k.size()), v2(v.size());
-    for (int shift = 0; shift < 64; shift += 8) {
-        size_t cnt[256] = {0};
-        for (uint64_t x : k) cnt[(x >> shift) & 255]++;
-        bool trivial = false;
-        for (int d = 0; d < 256; d++) if (cnt[d] == k.size()) trivial = true;
-        if (trivial) continue;
-        size_t off = 0;
-        for (int d = 0; d < 256; d++) { size_t c = cnt[d]; cnt[d] = off; off += c; }
-        for (size_t i = 0; i < k.size(); i++) {
-            size_t p = cnt[(k[i] >> shift) & 255]++;
-            k2[p] = k[i];
-            v2[p] = v[i];
-        }
-        k.swap(k2);
-        v.swap(v2);
-    }
-}
-
 // Builds a symmetric CSR from unique undirected pairs (lo < hi) packed as
-// lo << 32 | hi; rows come out sorted.
+// lo << 32 | hi; rows come out sorted, so the result does not depend on the
+// order of the pairs or on thread scheduling.
 void csr_from_pairs(int32_t n, const std::vector<uint64_t> &pairs, hgp_graph *g)
 {
-    std::vector<int64_t> deg(n + 1, 0);
-    for (uint64_t p : pairs) { deg[p >> 32]++; deg[p & 0xffffffffu]++; }
-    alloc_graph(g, n, (int64_t)pairs.size() * 2);
+    int64_t P = (int64_t)pairs.size();
+    std::vector<std::atomic<int64_t>> deg(n);
+    parallel_for(n, [&](int64_t v) { deg[v].store(0, std::memory_order_relaxed); });
+    parallel_for(P, [&](int64_t i) {
+        deg[pairs[i] >> 32].fetch_add(1, std::memory_order_relaxed);
+        deg[pairs[i] & 0xffffffffu].fetch_add(1, std::memory_order_relaxed);
+    });
+    alloc_graph(g, n, 2 * P);
     g->xadj[0] = 0;
-    for (int32_t v = 0; v < n; v++) g->xadj[v + 1] = g->xadj[v] + deg[v];
-    std::vector<int64_t> pos(g->xadj, g->xadj + n);
-    for (uint64_t p : pairs) {
-        int32_t a = (int32_t)(p >> 32), b = (int32_t)(p & 0xffffffffu);
-        g->adjncy[pos[a]++] = b;
-        g->adjncy[pos[b]++] = a;
-    }
+    for (int32_t v = 0; v < n; v++) g->xadj[v + 1] = g->xadj[v] + deg[v].load(std::memory_order_relaxed);
+    parallel_for(n, [&](int64_t v) { deg[v].store(g->xadj[v], std::memory_order_relaxed); });
+    parallel_for(P, [&](int64_t i) {
+        int32_t a = (int32_t)(pairs[i] >> 32), b = (int32_t)(pairs[i] & 0xffffffffu);
+        g->adjncy[deg[a].fetch_add(1, std::memory_order_relaxed)] = b;
+        g->adjncy[deg[b].fetch_add(1, std::memory_order_relaxed)] = a;
+    });
     parallel_for(n, [&](int64_t v) {
         std::sort(g->adjncy + g->xadj[v], g->adjncy + g->xadj[v + 1]);
+        g->vwgt[v] = 1;
     });
-    for (int64_t j = 0; j < g->m; j++) g->adjwgt[j] = 1;
-    for (int32_t v = 0; v < n; v++) g->vwgt[v] = 1;
+    parallel_for(2 * P, [&](int64_t j) { g->adjwgt[j] = 1; });
 }
 
 }  // namespace
@@ -260,46 +255,121 @@ int hgp_gen_rmat(int32_t scale, int64_t nedges, double a, double b, double c, ui
         uint64_t lo = u < v ? u : v, hi = u < v ? v : u;
         return (lo << 32) | hi;
     };
+    double t0 = now_s();
     // The edge set is the first nedges distinct pairs in draw order.  Draws are
-    // made in batches; (pair, first index) is kept per distinct pair.
-    std::vector<uint64_t> keys;      // pair
+    // made in batches; every distinct pair keeps the index of its first draw.
+    std::vector<uint64_t> keys;      // distinct pairs
     std::vector<uint64_t> firsts;    // first draw index, parallel to keys
     uint64_t drawn = 0;
+    const int nt = nthreads();
     while ((int64_t)keys.size() < nedges) {
-        uint64_t want = (uint64_t)((double)(nedges - (int64_t)keys.size()) * 1.15) + 1024;
+        uint64_t want = (uint64_t)((double)(nedges - (int64_t)keys.size()) * 1.3) + 1024;
         std::vector<uint64_t> batch(want);
         parallel_for((int64_t)want, [&](int64_t i) { batch[i] = draw(drawn + (uint64_t)i); });
-        std::vector<uint64_t> idx;
-        idx.reserve(want);
-        std::vector<uint64_t> prs;
-        prs.reserve(want);
-        for (uint64_t i = 0; i < want; i++) if (batch[i] != ~0ull) { prs.push_back(batch[i]); idx.push_back(drawn + i); }
-        drawn += want;
-        // merge with the pairs already kept (their indices are smaller), then a
-        // stable radix sort by pair keeps the first occurrence of each pair
-        std::vector<uint64_t> allp(keys);
-        std::vector<uint64_t> alli(firsts);
-        allp.insert(allp.end(), prs.begin(), prs.end());
-        alli.insert(alli.end(), idx.begin(), idx.end());
-        radix_sort_pairs_u64(allp, alli);
-        keys.clear();
-        firsts.clear();
-        for (size_t t = 0; t < allp.size(); t++) {
-            if (!keys.empty() && keys.back() == allp[t]) continue;
-            keys.push_back(allp[t]);
-            firsts.push_back(alli[t]);
+        GEN_T("draw");
+        // MSD buckets by the top key bits over (kept pairs, new draws), in parallel
+        const int bb = 10, shift = 32 + scale - bb;   // keys are lo << 32 | hi, lo < 2^scale
+        const size_t nb = (size_t)1 << bb;
+        const size_t K = keys.size(), N = K + want;
+        auto key_at = [&](size_t i) { return i < K ? keys[i] : batch[i - K]; };
+        auto idx_at = [&](size_t i) { return i < K ? firsts[i] : drawn + (i - K); };
+        std::vector<std::vector<size_t>> cnt(nt, std::vector<size_t>(nb + 1, 0));
+        auto chunk = [&](int t, size_t &lo, size_t &hi) { lo = N * t / nt; hi = N * (t + 1) / nt; };
+        {
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; t++)
+                th.emplace_back([&, t] {
+                    size_t lo, hi;
+                    chunk(t, lo, hi);
+                    for (size_t i = lo; i < hi; i++) {
+                        uint64_t kk = key_at(i);
+                        if (kk != ~0ull) cnt[t][kk >> shift]++;
+                    }
+                });
+            for (auto &x : th) x.join();
         }
+        std::vector<size_t> bstart(nb + 1, 0);
+        size_t off = 0;
+        for (size_t b = 0; b < nb; b++) {
+            bstart[b] = off;
+            for (int t = 0; t < nt; t++) { size_t c = cnt[t][b]; cnt[t][b] = off; off += c; }
+        }
+        bstart[nb] = off;
+        std::vector<std::pair<uint64_t, uint64_t>> P(off);
+        {
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; t++)
+                th.emplace_back([&, t] {
+                    size_t lo, hi;
+                    chunk(t, lo, hi);
+                    for (size_t i = lo; i < hi; i++) {
+                        uint64_t kk = key_at(i);
+                        if (kk != ~0ull) P[cnt[t][kk >> shift]++] = {kk, idx_at(i)};
+                    }
+                });
+            for (auto &x : th) x.join();
+        }
+        batch.clear();
+        batch.shrink_to_fit();
+        GEN_T("pack");
+        // sort each bucket by (pair, index) and keep the first entry of each pair
+        std::vector<size_t> ucnt(nb, 0);
+        {
+            std::atomic<size_t> next(0);
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; t++)
+                th.emplace_back([&] {
+                    for (size_t b; (b = next.fetch_add(1)) < nb;) {
+                        auto lo = P.begin() + bstart[b], hi = P.begin() + bstart[b + 1];
+                        std::sort(lo, hi);
+                        size_t u = 0;
+                        for (auto it = lo; it != hi; ++it)
+                            if (it == lo || it->first != (it - 1)->first) u++;
+                        ucnt[b] = u;
+                    }
+                });
+            for (auto &x : th) x.join();
+        }
+        GEN_T("sort");
+        std::vector<size_t> uoff(nb + 1, 0);
+        for (size_t b = 0; b < nb; b++) uoff[b + 1] = uoff[b] + ucnt[b];
+        keys.assign(uoff[nb], 0);
+        firsts.assign(uoff[nb], 0);
+        parallel_for((int64_t)nb, [&](int64_t b) {
+            size_t o = uoff[b];
+            auto lo = P.begin() + bstart[b], hi = P.begin() + bstart[b + 1];
+            for (auto it = lo; it != hi; ++it)
+                if (it == lo || it->first != (it - 1)->first) { keys[o] = it->first; firsts[o] = it->second; o++; }
+        });
+        drawn += want;
     }
+    GEN_T("dedup");
     if ((int64_t)keys.size() > nedges) {
-        // keep the nedges pairs with the smallest first-draw index
-        std::vector<uint64_t> order(keys.size());
-        for (size_t i = 0; i < order.size(); i++) order[i] = i;
-        std::nth_element(order.begin(), order.begin() + nedges, order.end(),
-                         [&](uint64_t x, uint64_t y) { return firsts[x] < firsts[y]; });
-        std::vector<uint64_t> kept(nedges);
-        for (int64_t i = 0; i < nedges; i++) kept[i] = keys[order[i]];
+        // keep the nedges pairs with the smallest first-draw index: mark first
+        // draws in a bitmap over draw indices and find the cut-off index
+        std::vector<uint64_t> bits((drawn + 63) / 64, 0);
+        for (uint64_t f : firsts) bits[f >> 6] |= 1ull << (f & 63);
+        int64_t need = nedges, acc = 0;
+        uint64_t cutoff = drawn;
+        for (size_t w = 0; w < bits.size(); w++) {
+            int c = __builtin_popcountll(bits[w]);
+            if (acc + c >= need) {
+                uint64_t x = bits[w];
+                for (int b = 0; b < 64; b++) {
+                    if (x >> b & 1) {
+                        if (++acc == need) { cutoff = w * 64 + b + 1; break; }
+                    }
+                }
+                break;
+            }
+            acc += c;
+        }
+        std::vector<uint64_t> kept;
+        kept.reserve(nedges);
+        for (size_t i = 0; i < keys.size(); i++) if (firsts[i] < cutoff) kept.push_back(keys[i]);
         keys.swap(kept);
     }
+    GEN_T("trim");
     // seeded relabelling (Fisher-Yates)
     std::vector<int32_t> perm(n);
     for (int64_t v = 0; v < n; v++) perm[v] = (int32_t)v;
@@ -312,7 +382,9 @@ int hgp_gen_rmat(int32_t scale, int64_t nedges, double a, double b, double c, ui
         uint64_t u = (uint64_t)perm[p >> 32], v = (uint64_t)perm[p & 0xffffffffu];
         keys[i] = u < v ? (u << 32) | v : (v << 32) | u;
     });
+    GEN_T("relabel");
     csr_from_pairs((int32_t)n, keys, g);
+    GEN_T("csr");
     return HGP_OK;
 }
 

@@ -65,8 +65,11 @@ void scan_impl(Ctx &c, const T *in, T *out, int64_t n)
     k_tile_reduce<T><<<(unsigned)tiles, kScanBS, 0, c.stream>>>(in, n, sums);
     HGP_LAUNCH_CHECK();
     scan_impl<T>(c, sums, sums, tiles);
-    k_tile_scan<T><<<(unsigned)tiles, kScanBS, 0, c.stream>>>(in, out, n, sums);
-    HGP_LAUNCH_CHECK();
+    {
+        KScope ks(c, "scan_tiles", 2.0 * sizeof(T) * (double)n);
+        k_tile_scan<T><<<(unsigned)tiles, kScanBS, 0, c.stream>>>(in, out, n, sums);
+        HGP_LAUNCH_CHECK();
+    }
     c.arena.right_release(mark);
 }
 
@@ -206,11 +209,17 @@ void radix_impl(Ctx &c, K *keys, uint32_t *vals, int64_t n, int lo, int hi)
     uint32_t *src_v = vals, *dst_v = v2;
     int passes = 0;
     for (int shift = lo; shift < hi; shift += 8) {
-        k_rs_hist<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, n, shift, ntiles, hist);
-        HGP_LAUNCH_CHECK();
+        {
+            KScope ks(c, "radix_hist", (double)sizeof(K) * (double)n);
+            k_rs_hist<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, n, shift, ntiles, hist);
+            HGP_LAUNCH_CHECK();
+        }
         scan_impl<int32_t>(c, (const int32_t *)hist, (int32_t *)incl, (int64_t)ntiles * 256);
-        k_rs_scatter<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, src_v, dst_k, dst_v, n, shift, ntiles, hist, incl);
-        HGP_LAUNCH_CHECK();
+        {
+            KScope ks(c, "radix_scatter", 2.0 * (double)(sizeof(K) + 4) * (double)n);
+            k_rs_scatter<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, src_v, dst_k, dst_v, n, shift, ntiles, hist, incl);
+            HGP_LAUNCH_CHECK();
+        }
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);
         passes++;

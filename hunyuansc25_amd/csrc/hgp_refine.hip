@@ -612,11 +612,21 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         read_scalars(c, ctl, sizeof(JetCtl));
         JetCtl h = *(JetCtl *)c.pinned;
         if (h.done) break;
-        if (big_k) launch_select<64>(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key);
-        else launch_select<16>(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key);
+        {
+            // LP: every row (adjncy, adjwgt, where gather) + xadj, where, lock, dest, pregain per vertex.
+            // RW/RS: the same per-vertex traffic; rows only of overloaded parts (not counted).
+            bool lp = h.mode == kModeLP;
+            KScope ks(c, lp ? "jet_select_lp" : "jet_select_rb", (lp ? 12.0 * (double)g.m : 0.0) + 21.0 * n);
+            if (big_k) launch_select<64>(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key);
+            else launch_select<16>(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key);
+        }
         if (h.mode == kModeLP) {
-            k_afterburner<16><<<gg, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl, sel);
-            HGP_LAUNCH_CHECK();
+            {
+                KScope ks(c, "jet_afterburner", 5.0 * n);
+                k_afterburner<16><<<gg, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl,
+                                                            sel);
+                HGP_LAUNCH_CHECK();
+            }
             k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock);
             HGP_LAUNCH_CHECK();
         } else {
@@ -641,8 +651,12 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                 HGP_LAUNCH_CHECK();
             }
         }
-        k_move_delta<16><<<gg, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts);
-        HGP_LAUNCH_CHECK();
+        {
+            KScope ks(c, "jet_move_delta", 8.0 * n);
+            k_move_delta<16><<<gg, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl,
+                                                       pwgts);
+            HGP_LAUNCH_CHECK();
+        }
         k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where);
         HGP_LAUNCH_CHECK();
         k_jet_end<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts);

@@ -113,6 +113,37 @@ Ctx &context(int device)
     return ref;
 }
 
+KTimer &ktimer()
+{
+    static KTimer t;
+    return t;
+}
+
+hipEvent_t KTimer::get()
+{
+    if (used == pool.size()) {
+        hipEvent_t e;
+        HGP_CHECK_HIP(hipEventCreate(&e));
+        pool.push_back(e);
+    }
+    return pool[used++];
+}
+
+KScope::KScope(Ctx &c, const char *name, double bytes) : c_(c), on_(ktimer().on), name_(name), bytes_(bytes)
+{
+    if (!on_) return;
+    a_ = ktimer().get();
+    b_ = ktimer().get();
+    HGP_CHECK_HIP(hipEventRecord(a_, c_.stream));
+}
+
+KScope::~KScope()
+{
+    if (!on_) return;
+    if (hipEventRecord(b_, c_.stream) != hipSuccess) return;
+    ktimer().recs.push_back({name_, a_, b_, bytes_});
+}
+
 void upload_staged(Ctx &c, void *ddst, const void *hsrc, size_t bytes)
 {
     char *stage = (char *)c.pinned + kStageOffset;

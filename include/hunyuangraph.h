@@ -98,6 +98,15 @@ int hgp_partition_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjn
                          const int32_t *d_vwgt, const int32_t *d_adjwgt, int32_t k,
                          const hgp_options *opt, int32_t *d_part, hgp_stats *stats);
 
+/* ------------------------------------------------------------ kernel timer
+ * Per-kernel timing with HIP events on the library's stream, for
+ * measurement: when enabled, the main kernels record an event pair per launch
+ * and the algorithmic HBM bytes the launch must move.  enable(0/1) clears the
+ * records.  report() writes JSON {"kernel": [launches, total_ms, bytes], ...}
+ * into buf (NUL-terminated, truncated to len) and returns the length needed. */
+int hgp_ktimer_enable(int on);
+int64_t hgp_ktimer_report(char *buf, int64_t len);
+
 /* ------------------------------------------------------------ stages
  * Host-array entry points of single stages, for parity tests.  Each copies
  * its inputs to the device, runs the GPU stage and copies the result back. */
