@@ -70,11 +70,12 @@ void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_
     int32_t maxvwgt = coarsen_maxvwgt(tvwgt, thr);
 
     std::vector<DevGraph> lv{g0};
+    attach_bins(c, lv[0]);
     std::vector<int32_t *> cmaps;
     // hunyuangarph_coarsen (hunyuangraph_GPU_coarsen.h:163-253): do-while with
     // the stop rule n > thr && n < 0.85 n_finer && m > n/2 (exact integers).
     for (;;) {
-        const DevGraph &f = lv.back();
+        const DevGraph f = lv.back();   // a copy: lv.push_back below may reallocate
         int32_t *cmap = c.arena.lalloc<int32_t>((size_t)f.n);
         size_t rmark = c.arena.right_mark();
         int32_t *match = c.arena.ralloc<int32_t>((size_t)f.n);
@@ -83,6 +84,7 @@ void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_
         int32_t cn = build_cmap(c, f.n, match, cmap);
         DevGraph cg = contract(c, f, match, cmap, cn);
         c.arena.right_release(rmark);
+        attach_bins(c, cg);
         cmaps.push_back(cmap);
         lv.push_back(cg);
         int32_t nf = f.n;
@@ -506,6 +508,7 @@ int hgp_edge_cut(int32_t n, const int64_t *xadj, const int32_t *adjncy, const in
         Ctx &c = context(0);
         with_arena(c, estimate_arena(n, xadj[n], 1, true), [&] {
             DevGraph g = upload(c, n, xadj, adjncy, nullptr, adjwgt);
+            attach_bins(c, g);
             int32_t *dw = to_dev(c, where, (size_t)n);
             int64_t *dc = c.arena.lalloc<int64_t>(1);
             edge_cut(c, g, dw, dc);
@@ -524,6 +527,7 @@ int hgp_match(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32
         Ctx &c = context(0);
         with_arena(c, estimate_arena(n, xadj[n], 1, true), [&] {
             DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
+            attach_bins(c, g);
             int32_t *dm = c.arena.lalloc<int32_t>((size_t)n);
             int64_t mc = match_level(c, g, maxvwgt, seed, level, rounds, dm);
             to_host(c, match, dm, (size_t)n);
@@ -604,6 +608,7 @@ int hgp_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int3
         Ctx &c = context(o.device);
         with_arena(c, estimate_arena(n, xadj[n], k, true), [&] {
             DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
+            attach_bins(c, g);
             int32_t *dw = to_dev(c, where, (size_t)n);
             int64_t *dc = to_dev(c, cut, 1);
             RefineStats rs = refine_level(c, g, k, tvwgt, is_finest != 0, p, dw, dc);

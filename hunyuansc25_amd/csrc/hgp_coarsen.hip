@@ -5,7 +5,11 @@
 #include "hgp_graph.hpp"
 #include "hgp_prims.hpp"
 
+#include <type_traits>
+
 namespace hgp {
+
+void attach_bins(Ctx &c, DevGraph &g) { g.bins = row_bins(c, g.n, g.xadj, nullptr, true); }
 
 int32_t coarsen_threshold(int32_t k)
 {
@@ -38,47 +42,51 @@ __device__ inline bool better(int32_t w, uint32_t h, int32_t lo, int32_t hi, int
     return hi < bhi;
 }
 
+struct EdgeKey {
+    int32_t w, lo, hi;
+    uint32_t h;
+};
+
+struct BetterEdge {
+    __device__ bool operator()(const EdgeKey &a, const EdgeKey &b) const
+    {
+        return better(a.w, a.h, a.lo, a.hi, b.w, b.h, b.lo, b.hi);
+    }
+};
+
 // Handshake proposal (replaces random_match_conflict, hunyuangraph_GPU_match.h:454-510,
 // and set_receive_send_topk_one, :800-861): every free vertex proposes to its
-// best free neighbour u with vwgt[u]+vwgt[v] < maxvwgt.  kG lanes per vertex,
-// reduction by xor-shuffles inside the lane group.
-__global__ void __launch_bounds__(kBS) k_propose(int32_t n, const eid_t *__restrict__ xadj,
-                                                 const vid_t *__restrict__ adjncy, const wgt_t *__restrict__ adjwgt,
-                                                 const wgt_t *__restrict__ vwgt, const int32_t *__restrict__ match,
-                                                 int32_t maxvwgt, uint64_t seed, int32_t level,
-                                                 int32_t *__restrict__ prop, const int32_t *prev_progress)
+// best free neighbour u with vwgt[u]+vwgt[v] < maxvwgt.  G lanes per row.
+template <int G>
+__global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ ids, int64_t cnt,
+                                                    const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                    const wgt_t *__restrict__ adjwgt, const wgt_t *__restrict__ vwgt,
+                                                    const int32_t *__restrict__ match, int32_t maxvwgt,
+                                                    uint64_t seed, int32_t level, int32_t *__restrict__ prop,
+                                                    const int32_t *prev_progress)
 {
     if (prev_progress && *prev_progress == 0) return;
-    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t v = (int32_t)(gt / kG);
-    int lane = (int)(threadIdx.x % kG);
-    if (v >= n) return;
+    int64_t gi = gindex<G>();
+    int lane = glane<G>();
+    if (gi >= cnt) return;
+    int32_t v = ids[gi];
     if (match[v] != -1) {
         if (lane == 0) prop[v] = -1;
         return;
     }
     int32_t wv = vwgt[v];
-    int32_t bw = -1, blo = 0, bhi = 0;
-    uint32_t bh = 0;
+    EdgeKey best{-1, 0, 0, 0u};
     eid_t end = xadj[v + 1];
-    for (eid_t j = xadj[v] + lane; j < end; j += kG) {
+    for (eid_t j = xadj[v] + lane; j < end; j += G) {
         int32_t u = adjncy[j];
         if (u == v || match[u] != -1) continue;
         if ((int64_t)vwgt[u] + wv >= maxvwgt) continue;
-        int32_t w = adjwgt[j];
-        int32_t lo = u < v ? u : v, hi = u < v ? v : u;
-        uint32_t h = edge_hash(seed, level, lo, hi);
-        if (better(w, h, lo, hi, bw, bh, blo, bhi)) { bw = w; bh = h; blo = lo; bhi = hi; }
+        EdgeKey e{adjwgt[j], u < v ? u : v, u < v ? v : u, 0u};
+        e.h = edge_hash(seed, level, e.lo, e.hi);
+        if (BetterEdge()(e, best)) best = e;
     }
-#pragma unroll
-    for (int o = kG / 2; o > 0; o >>= 1) {
-        int32_t ow = __shfl_xor(bw, o, kG);
-        uint32_t oh = __shfl_xor(bh, o, kG);
-        int32_t olo = __shfl_xor(blo, o, kG);
-        int32_t ohi = __shfl_xor(bhi, o, kG);
-        if (better(ow, oh, olo, ohi, bw, bh, blo, bhi)) { bw = ow; bh = oh; blo = olo; bhi = ohi; }
-    }
-    if (lane == 0) prop[v] = bw < 0 ? -1 : (blo == v ? bhi : blo);
+    best = group_best<G>(best, BetterEdge());
+    if (lane == 0) prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
 }
 
 __global__ void k_accept(int32_t n, const int32_t *__restrict__ prop, int32_t *__restrict__ match, int32_t *progress,
@@ -110,21 +118,54 @@ __global__ void k_leaf_count(int32_t n, const eid_t *__restrict__ xadj, const vi
     if (xadj[v + 1] - xadj[v] == 1) atomicAdd(&cnt[adjncy[xadj[v]]], 1);
 }
 
-// Hub h pairs its unmatched degree-1 neighbours consecutively in row order;
-// a leaf has one neighbour, so only its hub touches it.
-__global__ void k_leaf_pair(int32_t n, const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
-                            const wgt_t *__restrict__ vwgt, int32_t maxvwgt, int32_t *match,
-                            const int32_t *__restrict__ cnt)
+// Consecutive pairing in row order: the eligible entries of hub h's row form
+// a sequence that is paired (0,1), (2,3), ...; a pair whose weight is not
+// below maxvwgt is skipped.  A G-lane group walks the row in chunks; lane
+// ranks come from ballots, the unpaired last entry of a chunk is carried.
+// Leaf (hunyuangraph_GPU_match.h:1374-1423): eligible = unmatched degree-1
+// neighbour.  Relative (:1597-1680): eligible = unmatched neighbour whose
+// dist is h.  Each eligible vertex belongs to exactly one hub.
+template <int G, bool kLeaf>
+__global__ void __launch_bounds__(kRowBS) k_pair_row(const int32_t *__restrict__ ids, int64_t cnt,
+                                                     const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                     const wgt_t *__restrict__ vwgt, int32_t maxvwgt,
+                                                     const int32_t *__restrict__ dist,
+                                                     const int32_t *__restrict__ hubcnt, int32_t *match)
 {
-    int32_t h = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (h >= n || cnt[h] <= 1) return;
-    int32_t flag = -1;
-    for (eid_t j = xadj[h]; j < xadj[h + 1]; j++) {
-        int32_t u = adjncy[j];
-        if (match[u] != -1 || xadj[u + 1] - xadj[u] != 1) continue;
-        if (flag == -1) { flag = u; continue; }
-        if ((int64_t)vwgt[flag] + vwgt[u] < maxvwgt) { match[flag] = u; match[u] = flag; }
-        flag = -1;
+    int64_t gi = gindex<G>();
+    int lane = glane<G>();
+    if (gi >= cnt) return;
+    int32_t h = ids[gi];
+    if (hubcnt[h] <= 1) return;
+    int32_t pending = -1;
+    eid_t e = xadj[h + 1];
+    for (eid_t s = xadj[h]; s < e; s += G) {
+        eid_t j = s + lane;
+        int32_t u = j < e ? adjncy[j] : -1;
+        bool elig = false;
+        if (u >= 0 && match[u] == -1) {
+            if (kLeaf) elig = xadj[u + 1] - xadj[u] == 1;
+            else elig = dist[u] == h;
+        }
+        uint64_t mask = group_ballot<G>(elig);
+        int npend = pending >= 0 ? 1 : 0;
+        // shuffles run on every lane of the group (a shuffle from an inactive
+        // lane would not deliver its value): the previous eligible lane's
+        // vertex, and the last eligible vertex of the chunk
+        uint64_t below = mask & group_lanemask_lt<G>();
+        int32_t prev_u = __shfl(u, below ? 63 - __clzll((long long)below) : 0, G);
+        int32_t last_u = __shfl(u, mask ? 63 - __clzll((long long)mask) : 0, G);
+        if (elig) {
+            int t = __popcll(below) + npend;   // index in this chunk's sequence
+            if (t & 1) {
+                int32_t partner = (t == 1 && npend) ? pending : prev_u;
+                if ((int64_t)vwgt[partner] + vwgt[u] < maxvwgt) { match[partner] = u; match[u] = partner; }
+            }
+        }
+        // uniform update of the carried entry
+        int total = __popcll(mask) + npend;
+        if (total & 1) pending = mask ? last_u : pending;
+        else pending = -1;
     }
 }
 
@@ -226,39 +267,45 @@ __global__ void k_twin_pair(int64_t cap, const int64_t *__restrict__ count, cons
 
 // ---- relative matches (hunyuangraph_GPU_match.h:1597-1680): dist = the
 // neighbour of largest degree, then larger edge weight, then first in row.
-__global__ void k_rel_dist(int32_t n, const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
-                           const wgt_t *__restrict__ adjwgt, const int32_t *__restrict__ match,
-                           int32_t *__restrict__ dist, int32_t *__restrict__ cnt)
-{
-    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (v >= n) return;
-    int32_t d = -1;
-    if (match[v] == -1) {
-        int64_t bl = -1;
-        int32_t bw = 0;
-        for (eid_t j = xadj[v]; j < xadj[v + 1]; j++) {
-            int32_t u = adjncy[j];
-            int64_t l = xadj[u + 1] - xadj[u];
-            if (d == -1 || l > bl || (l == bl && adjwgt[j] > bw)) { d = u; bl = l; bw = adjwgt[j]; }
-        }
-        if (d != -1) atomicAdd(&cnt[d], 1);
-    }
-    dist[v] = d;
-}
+struct DistKey {
+    int32_t l, w, r, u;
+};
 
-__global__ void k_rel_pair(int32_t n, const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
-                           const wgt_t *__restrict__ vwgt, int32_t maxvwgt, const int32_t *__restrict__ dist,
-                           const int32_t *__restrict__ cnt, int32_t *match)
+struct BetterDist {
+    __device__ bool operator()(const DistKey &a, const DistKey &b) const
+    {
+        if (a.l != b.l) return a.l > b.l;
+        if (a.w != b.w) return a.w > b.w;
+        return a.r < b.r;
+    }
+};
+
+template <int G>
+__global__ void __launch_bounds__(kRowBS) k_rel_dist(const int32_t *__restrict__ ids, int64_t cnt,
+                                                     const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                     const wgt_t *__restrict__ adjwgt,
+                                                     const int32_t *__restrict__ match, int32_t *__restrict__ dist,
+                                                     int32_t *__restrict__ hubcnt)
 {
-    int32_t h = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (h >= n || cnt[h] <= 1) return;
-    int32_t flag = -1;
-    for (eid_t j = xadj[h]; j < xadj[h + 1]; j++) {
+    int64_t gi = gindex<G>();
+    int lane = glane<G>();
+    if (gi >= cnt) return;
+    int32_t v = ids[gi];
+    if (match[v] != -1) {
+        if (lane == 0) dist[v] = -1;
+        return;
+    }
+    DistKey best{-1, 0, 0, -1};
+    eid_t b = xadj[v], e = xadj[v + 1];
+    for (eid_t j = b + lane; j < e; j += G) {
         int32_t u = adjncy[j];
-        if (dist[u] != h || match[u] != -1 || u == flag) continue;
-        if (flag == -1) { flag = u; continue; }
-        if ((int64_t)vwgt[flag] + vwgt[u] < maxvwgt) { match[flag] = u; match[u] = flag; }
-        flag = -1;
+        DistKey k{(int32_t)(xadj[u + 1] - xadj[u]), adjwgt[j], (int32_t)(j - b), u};
+        if (BetterDist()(k, best)) best = k;
+    }
+    best = group_best<G>(best, BetterDist());
+    if (lane == 0) {
+        dist[v] = best.u;
+        if (best.u >= 0) atomicAdd(&hubcnt[best.u], 1);
     }
 }
 
@@ -300,21 +347,23 @@ __global__ void k_rep(int32_t n, const eid_t *__restrict__ xadj, const wgt_t *__
 
 // Concatenated rows of both endpoints renamed by cmap; key = row << 32 | cc,
 // with the row's own id replaced by cn so internal entries sort last.
-__global__ void __launch_bounds__(kBS) k_gather(int32_t cn, const int32_t *__restrict__ rep,
-                                                const int32_t *__restrict__ match, const eid_t *__restrict__ xadj,
-                                                const vid_t *__restrict__ adjncy, const wgt_t *__restrict__ adjwgt,
-                                                const int32_t *__restrict__ cmap, const int64_t *__restrict__ toff,
-                                                uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+template <int G>
+__global__ void __launch_bounds__(kRowBS) k_gather(const int32_t *__restrict__ ids, int64_t cnt, int32_t cn,
+                                                   const int32_t *__restrict__ rep, const int32_t *__restrict__ match,
+                                                   const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                   const wgt_t *__restrict__ adjwgt, const int32_t *__restrict__ cmap,
+                                                   const int64_t *__restrict__ toff, uint64_t *__restrict__ key,
+                                                   uint32_t *__restrict__ val)
 {
-    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t c = (int32_t)(gt / kG);
-    int lane = (int)(threadIdx.x % kG);
-    if (c >= cn) return;
+    int64_t gi = gindex<G>();
+    int lane = glane<G>();
+    if (gi >= cnt) return;
+    int32_t c = ids[gi];
     int32_t v = rep[c], u = match[v];
     int64_t o = toff[c];
     uint64_t hi = (uint64_t)(uint32_t)c << 32;
     eid_t b = xadj[v], e = xadj[v + 1];
-    for (eid_t j = b + lane; j < e; j += kG) {
+    for (eid_t j = b + lane; j < e; j += G) {
         int32_t cc = cmap[adjncy[j]];
         key[o + (j - b)] = hi | (uint32_t)(cc == c ? cn : cc);
         val[o + (j - b)] = (uint32_t)adjwgt[j];
@@ -323,7 +372,7 @@ __global__ void __launch_bounds__(kBS) k_gather(int32_t cn, const int32_t *__res
         o += e - b;
         b = xadj[u];
         e = xadj[u + 1];
-        for (eid_t j = b + lane; j < e; j += kG) {
+        for (eid_t j = b + lane; j < e; j += G) {
             int32_t cc = cmap[adjncy[j]];
             key[o + (j - b)] = hi | (uint32_t)(cc == c ? cn : cc);
             val[o + (j - b)] = (uint32_t)adjwgt[j];
@@ -385,6 +434,16 @@ int64_t count_matched(Ctx &c, int32_t n, const int32_t *match, int64_t *d_cnt)
 
 inline bool need_more(int64_t matched, int32_t n) { return 4 * matched < 3 * (int64_t)n; }
 
+// Calls f(integral_constant<G>, ids, count) for every non-empty degree class;
+// MaxG caps the group size (ballot-based kernels need G <= 64).
+template <int MaxG = 256, class F>
+void for_bins(const RowBins &b, F f)
+{
+    if (b.count(0)) f(std::integral_constant<int, 8>{}, b.list(0), b.count(0));
+    if (b.count(1)) f(std::integral_constant<int, 64>{}, b.list(1), b.count(1));
+    if (b.count(2)) f(std::integral_constant<int, MaxG < 256 ? MaxG : 256>{}, b.list(2), b.count(2));
+}
+
 }  // namespace
 
 // hunyuangraph_gpu_match (hunyuangraph_GPU_match.h:1846-3214) restated with
@@ -401,15 +460,19 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
     int64_t *d_cnt = c.arena.ralloc<int64_t>(2);
     fill<int32_t>(c, match, n, -1);
     fill<int32_t>(c, progress, rounds > 0 ? rounds : 1, 0);
-    unsigned gv = grid_for(n, kBS), gg = grid_for((int64_t)n * kG, kBS);
+    unsigned gv = grid_for(n, kBS);
     for (int r = 0; r < rounds; r++) {
         const int32_t *prev = r > 0 ? progress + r - 1 : nullptr;
         {
             // per level: rows (adjncy, adjwgt) + gathers of match/vwgt per entry; xadj, match, vwgt, prop per vertex
             KScope ks(c, "match_propose", 16.0 * (double)g.m + 28.0 * n);
-            k_propose<<<gg, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, maxvwgt, seed, level,
-                                                prop, prev);
-            HGP_LAUNCH_CHECK();
+            for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t cnt) {
+                constexpr int G = decltype(gc)::value;
+                k_propose<G><<<grid_groups(cnt, G), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
+                                                                           g.vwgt, match, maxvwgt, seed, level,
+                                                                           prop, prev);
+                HGP_LAUNCH_CHECK();
+            });
         }
         k_accept<<<gv, kBS, 0, c.stream>>>(n, prop, match, progress + r, prev);
         HGP_LAUNCH_CHECK();
@@ -420,8 +483,12 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
         fill<int32_t>(c, cnt, n, 0);
         k_leaf_count<<<gv, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, match, cnt);
         HGP_LAUNCH_CHECK();
-        k_leaf_pair<<<gv, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.vwgt, maxvwgt, match, cnt);
-        HGP_LAUNCH_CHECK();
+        for_bins<64>(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
+            constexpr int G = decltype(gc)::value;
+            k_pair_row<G, true><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.vwgt,
+                                                                            maxvwgt, nullptr, cnt, match);
+            HGP_LAUNCH_CHECK();
+        });
         matched = count_matched(c, n, match, d_cnt);
     }
     if (need_more(matched, n)) {
@@ -477,10 +544,18 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
         int32_t *dist = c.arena.ralloc<int32_t>((size_t)n);
         int32_t *cnt = prop;
         fill<int32_t>(c, cnt, n, 0);
-        k_rel_dist<<<gv, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, match, dist, cnt);
-        HGP_LAUNCH_CHECK();
-        k_rel_pair<<<gv, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.vwgt, maxvwgt, dist, cnt, match);
-        HGP_LAUNCH_CHECK();
+        for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
+            constexpr int G = decltype(gc)::value;
+            k_rel_dist<G><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt, match,
+                                                                       dist, cnt);
+            HGP_LAUNCH_CHECK();
+        });
+        for_bins<64>(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
+            constexpr int G = decltype(gc)::value;
+            k_pair_row<G, false><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.vwgt,
+                                                                             maxvwgt, dist, cnt, match);
+            HGP_LAUNCH_CHECK();
+        });
         c.arena.right_release(m2);
         matched = count_matched(c, n, match, d_cnt);
     }
@@ -541,10 +616,14 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
         uint32_t *val = c.arena.ralloc<uint32_t>((size_t)m);
         {
             // per fine entry: adjncy, adjwgt, cmap gather, key+val write; per coarse vertex: rep, match, toff, 2 xadj
+            RowBins cb = row_bins(c, cn, nullptr, len, false);
             KScope ks(c, "contract_gather", 24.0 * (double)m + 40.0 * cn);
-            k_gather<<<grid_for((int64_t)cn * kG, kBS), kBS, 0, c.stream>>>(cn, rep, match, g.xadj, g.adjncy,
-                                                                           g.adjwgt, cmap, toff, key, val);
-            HGP_LAUNCH_CHECK();
+            for_bins(cb, [&](auto gc, const int32_t *ids, int64_t nb) {
+                constexpr int G = decltype(gc)::value;
+                k_gather<G><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, cn, rep, match, g.xadj, g.adjncy,
+                                                                         g.adjwgt, cmap, toff, key, val);
+                HGP_LAUNCH_CHECK();
+            });
         }
         int bc = bits_for((uint64_t)cn);
         int br = bits_for((uint64_t)(cn > 0 ? cn - 1 : 0));

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "hgp_common.hpp"
+#include "hgp_group.hpp"
 
 namespace hgp {
 
@@ -15,7 +16,11 @@ struct DevGraph {
     vid_t *adjncy = nullptr;
     wgt_t *adjwgt = nullptr;
     wgt_t *vwgt = nullptr;
+    RowBins bins;             // degree classes of the rows (set by attach_bins)
 };
+
+// Computes g.bins (ids on the left end of the arena: they live with the level).
+void attach_bins(Ctx &c, DevGraph &g);
 
 struct Params {
     uint64_t seed = 1;

@@ -1,0 +1,122 @@
+// hgp_group.hpp -- lane groups of G threads working on one CSR row.
+//
+// Row-parallel kernels take their vertices from degree classes (RowBins):
+// rows of <= 8 entries get 8-lane groups, rows of <= 512 entries one wave,
+// longer rows a 256-thread block.  Group reductions are xor-shuffles inside a
+// wave for G <= 64 and go through LDS across the block's four waves for
+// G = 256.  Every reduction here uses a strict total order (or an integer
+// sum), so results do not depend on which lane saw which entry.
+#pragma once
+
+#include "hgp_common.hpp"
+
+namespace hgp {
+
+constexpr int kRowBS = 256;             // block size of row-parallel kernels
+constexpr int64_t kBinSmall = 8;        // rows <= 8 entries
+constexpr int64_t kBinMid = 512;        // rows <= 512 entries
+constexpr int kBinG[3] = {8, 64, 256};  // lanes per row of each class
+
+struct RowBins {
+    int32_t *ids = nullptr;   // row ids grouped by class (order inside a class is arbitrary)
+    int64_t off[4] = {0, 0, 0, 0};
+    int64_t count(int c) const { return off[c + 1] - off[c]; }
+    const int32_t *list(int c) const { return ids + off[c]; }
+};
+
+// Classes of the rows of xadj (or of the lengths len, if xadj is null); ids is
+// allocated on the left end of the arena when `left`, else on the right.
+RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool left);
+
+template <int G>
+__device__ inline int glane() { return (int)(threadIdx.x % G); }
+
+template <int G>
+__device__ inline int64_t gindex() { return ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; }
+
+template <class T>
+__device__ inline T shfl_xor_pod(T v, int o, int width)
+{
+    static_assert(sizeof(T) % 4 == 0, "pod shuffle needs 4-byte multiples");
+    T r;
+    const int32_t *p = reinterpret_cast<const int32_t *>(&v);
+    int32_t *q = reinterpret_cast<int32_t *>(&r);
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); i++) q[i] = __shfl_xor(p[i], o, width);
+    return r;
+}
+
+// All lanes of the group receive the best value under `better` (a strict
+// total order on distinct values).
+template <int G, class T, class Better>
+__device__ inline T group_best(T v, Better better)
+{
+    if constexpr (G <= kWave) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            T u = shfl_xor_pod(v, o, G);
+            if (better(u, v)) v = u;
+        }
+        return v;
+    } else {
+        __shared__ T sh[G / kWave];
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            T u = shfl_xor_pod(v, o, kWave);
+            if (better(u, v)) v = u;
+        }
+        if (lane_id() == 0) sh[threadIdx.x / kWave] = v;
+        __syncthreads();
+        T r = sh[0];
+#pragma unroll
+        for (int i = 1; i < G / kWave; i++)
+            if (better(sh[i], r)) r = sh[i];
+        __syncthreads();
+        return r;
+    }
+}
+
+template <int G, class T>
+__device__ inline T group_sum(T v)
+{
+    if constexpr (G <= kWave) {
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+        return v;
+    } else {
+        __shared__ T sh[G / kWave];
+        v = wave_sum(v);
+        if (lane_id() == 0) sh[threadIdx.x / kWave] = v;
+        __syncthreads();
+        T r = 0;
+#pragma unroll
+        for (int i = 0; i < G / kWave; i++) r += sh[i];
+        __syncthreads();
+        return r;
+    }
+}
+
+// Ballot restricted to the G lanes of the caller's group (G <= 64), bit i =
+// group lane i.
+template <int G>
+__device__ inline uint64_t group_ballot(bool p)
+{
+    uint64_t b = __ballot(p);
+    if constexpr (G == kWave) return b;
+    int base = lane_id() & ~(G - 1);
+    return (b >> base) & ((1ull << G) - 1);
+}
+
+template <int G>
+__device__ inline uint64_t group_lanemask_lt()
+{
+    int l = glane<G>();
+    return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+inline unsigned grid_groups(int64_t groups, int G)
+{
+    return grid_for(groups * G, kRowBS);
+}
+
+}  // namespace hgp

@@ -1,5 +1,6 @@
 // hgp_prims.hip -- scans, stable radix sort, fills, reductions (wave64).
 #include "hgp_prims.hpp"
+#include "hgp_group.hpp"
 
 namespace hgp {
 
@@ -245,7 +246,68 @@ __global__ void k_compact(const uint8_t *__restrict__ f, const int32_t *__restri
     if (i == n - 1) *count = incl[i];
 }
 
+// ------------------------------------------------------------ degree classes
+
+__device__ inline int row_class(int64_t d) { return d <= kBinSmall ? 0 : (d <= kBinMid ? 1 : 2); }
+
+__global__ void k_bin_count(int32_t n, const eid_t *__restrict__ xadj, const int64_t *__restrict__ len,
+                            unsigned long long *cnt)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    bool valid = v < n;
+    int cls = -1;
+    if (valid) cls = row_class(xadj ? xadj[v + 1] - xadj[v] : len[v]);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        uint64_t b = __ballot(cls == c);
+        if (lane_id() == 0 && b) atomicAdd(&cnt[c], (unsigned long long)__popcll(b));
+    }
+}
+
+__global__ void k_bin_fill(int32_t n, const eid_t *__restrict__ xadj, const int64_t *__restrict__ len,
+                           int64_t o0, int64_t o1, int64_t o2, unsigned long long *cursor, int32_t *__restrict__ ids)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    bool valid = v < n;
+    int cls = -1;
+    if (valid) cls = row_class(xadj ? xadj[v + 1] - xadj[v] : len[v]);
+    const int64_t base[3] = {o0, o1, o2};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        uint64_t b = __ballot(cls == c);
+        if (!b) continue;
+        int leader = __ffsll((long long)b) - 1;
+        unsigned long long slot = 0;
+        if (lane_id() == leader) slot = atomicAdd(&cursor[c], (unsigned long long)__popcll(b));
+        slot = __shfl(slot, leader, kWave);
+        if (cls == c) ids[base[c] + (int64_t)slot + __popcll(b & lanemask_lt())] = v;
+    }
+}
+
 }  // namespace
+
+RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool left)
+{
+    RowBins b;
+    b.ids = left ? c.arena.lalloc<int32_t>((size_t)n) : c.arena.ralloc<int32_t>((size_t)n);
+    if (n <= 0) return b;
+    size_t mark = c.arena.right_mark();
+    unsigned long long *cnt = c.arena.ralloc<unsigned long long>(6);
+    fill<uint32_t>(c, (uint32_t *)cnt, 12, 0u);
+    unsigned g = grid_for(n, 256);
+    k_bin_count<<<g, 256, 0, c.stream>>>(n, xadj, len, cnt);
+    HGP_LAUNCH_CHECK();
+    read_scalars(c, cnt, 3 * sizeof(int64_t));
+    int64_t k0 = c.pinned[0], k1 = c.pinned[1], k2 = c.pinned[2];
+    b.off[0] = 0;
+    b.off[1] = k0;
+    b.off[2] = k0 + k1;
+    b.off[3] = k0 + k1 + k2;
+    k_bin_fill<<<g, 256, 0, c.stream>>>(n, xadj, len, b.off[0], b.off[1], b.off[2], cnt + 3, b.ids);
+    HGP_LAUNCH_CHECK();
+    c.arena.right_release(mark);
+    return b;
+}
 
 void scan_inclusive(Ctx &c, const int32_t *in, int32_t *out, int64_t n) { scan_impl<int32_t>(c, in, out, n); }
 void scan_inclusive(Ctx &c, const int64_t *in, int64_t *out, int64_t n) { scan_impl<int64_t>(c, in, out, n); }

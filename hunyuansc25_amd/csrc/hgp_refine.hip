@@ -10,6 +10,9 @@
 #include "hgp_graph.hpp"
 #include "hgp_prims.hpp"
 
+#include <algorithm>
+#include <type_traits>
+
 namespace hgp {
 
 namespace {
@@ -102,36 +105,55 @@ __global__ void __launch_bounds__(kBS) k_jet_begin(JetCtl *ctl, int32_t k, const
 
 // ---------------------------------------------------------------- selection
 
+struct PartKey {
+    int32_t c, q;      // connectivity, part (q = -1: none)
+};
+
+// Larger connectivity first, ties to the smaller part id, only c > 0 counts.
+struct BetterPart {
+    __device__ bool operator()(const PartKey &a, const PartKey &b) const
+    {
+        return a.c > b.c || (a.c == b.c && a.c > 0 && a.q < b.q);
+    }
+};
+
 // Per-vertex decision of the iteration's mode.
 //  LP : select_dest_part (:1995-2046) + filter_potential_vertex (:2114-2135)
 //  RW : select_dest_parts_rw (:3184-3241), bucket of assign_move_scores (:3243-3266)
 //  RS : assign_move_scores_part1 (:2515-2556) with its effective loss -conn[p]
+// Each G-lane group owns a dense row of k ints in LDS (set_gain_val_warp,
+// :4332-4432, keeps the same per-warp k-wide accumulator): phase 1 adds edge
+// weights per neighbour part, phase 2 reads them and reduces the arg-max,
+// phase 3 clears the touched entries.  Blocks loop over the row list.
 template <int G>
-__global__ void __launch_bounds__(kBS) k_jet_select(int32_t n, int32_t k, const eid_t *__restrict__ xadj,
-                                                    const vid_t *__restrict__ adjncy,
-                                                    const wgt_t *__restrict__ adjwgt, const wgt_t *__restrict__ vwgt,
-                                                    const int32_t *__restrict__ where,
-                                                    const int64_t *__restrict__ pwgts,
-                                                    const uint8_t *__restrict__ lock,
-                                                    const int32_t *__restrict__ under, JetCtl *ctl,
-                                                    int32_t *__restrict__ dest, int32_t *__restrict__ pregain,
-                                                    uint32_t *__restrict__ key)
+__global__ void __launch_bounds__(kRowBS) k_jet_select(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
+                                                       const eid_t *__restrict__ xadj,
+                                                       const vid_t *__restrict__ adjncy,
+                                                       const wgt_t *__restrict__ adjwgt,
+                                                       const wgt_t *__restrict__ vwgt,
+                                                       const int32_t *__restrict__ where,
+                                                       const int64_t *__restrict__ pwgts,
+                                                       const uint8_t *__restrict__ lock,
+                                                       const int32_t *__restrict__ under, JetCtl *ctl,
+                                                       int32_t *__restrict__ dest, int32_t *__restrict__ pregain,
+                                                       uint32_t *__restrict__ key)
 {
     extern __shared__ int32_t acc[];
     const JetCtl &C = *ctl;
     if (C.done) return;
     const int32_t mode = C.mode;
     const int64_t maxwgt = C.maxwgt, opt = C.opt;
-    constexpr int NG = kBS / G;
-    const int g = threadIdx.x / G, lane = threadIdx.x % G;
+    constexpr int NG = kRowBS / G;
+    const int g = threadIdx.x / G, lane = glane<G>();
     int32_t *my = acc + (size_t)g * k;
-    for (int i = threadIdx.x; i < NG * k; i += kBS) acc[i] = 0;
+    for (int i = threadIdx.x; i < NG * k; i += kRowBS) acc[i] = 0;
     __syncthreads();
     const uint32_t nokey = (uint32_t)k * kMaxBuckets;
     int32_t ncand_local = 0;
-    for (int64_t base = (int64_t)blockIdx.x * NG; base < n; base += (int64_t)gridDim.x * NG) {
-        int32_t v = (int32_t)(base + g);
-        bool live = v < n;
+    for (int64_t base = (int64_t)blockIdx.x * NG; base < cnt; base += (int64_t)gridDim.x * NG) {
+        int64_t gi = base + g;
+        bool live = gi < cnt;
+        int32_t v = live ? ids[gi] : 0;
         int32_t p = live ? where[v] : 0;
         bool eligible = live;
         if (live && mode != kModeLP) {
@@ -144,25 +166,22 @@ __global__ void __launch_bounds__(kBS) k_jet_select(int32_t n, int32_t k, const 
         if (eligible)
             for (eid_t j = b + lane; j < e; j += G) atomicAdd(&my[where[adjncy[j]]], adjwgt[j]);
         __syncthreads();
-        int32_t bc = 0, bq = -1;
+        PartKey best{0, -1};
         int32_t pc = eligible ? my[p] : 0;
         if (eligible && mode != kModeRS) {
             for (eid_t j = b + lane; j < e; j += G) {
                 int32_t q = where[adjncy[j]];
                 if (q == p) continue;
                 if (mode == kModeRW && !(pwgts[q] < maxwgt)) continue;
-                int32_t cq = my[q];
-                if (cq > bc || (cq == bc && cq > 0 && q < bq)) { bc = cq; bq = q; }
+                PartKey cand{my[q], q};
+                if (BetterPart()(cand, best)) best = cand;
             }
         }
-#pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            int32_t oc = __shfl_xor(bc, o, G), oq = __shfl_xor(bq, o, G);
-            if (oc > bc || (oc == bc && oc > 0 && oq < bq)) { bc = oc; bq = oq; }
-        }
+        best = group_best<G>(best, BetterPart());
         __syncthreads();
         if (eligible)
             for (eid_t j = b + lane; j < e; j += G) my[where[adjncy[j]]] = 0;
+        int32_t bc = best.c, bq = best.q;
         if (live && lane == 0) {
             int32_t d = p;
             if (mode == kModeLP) {
@@ -209,19 +228,20 @@ __global__ void __launch_bounds__(kBS) k_jet_select(int32_t n, int32_t k, const 
 // gain assuming every higher-priority neighbour (greater gain, ties to the
 // smaller id) has already moved.
 template <int G>
-__global__ void __launch_bounds__(kBS) k_afterburner(int32_t n, const eid_t *__restrict__ xadj,
-                                                     const vid_t *__restrict__ adjncy,
-                                                     const wgt_t *__restrict__ adjwgt,
-                                                     const int32_t *__restrict__ where,
-                                                     const int32_t *__restrict__ dest,
-                                                     const int32_t *__restrict__ pregain, const JetCtl *ctl,
-                                                     uint8_t *__restrict__ sel)
+__global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restrict__ ids, int64_t cnt,
+                                                        const eid_t *__restrict__ xadj,
+                                                        const vid_t *__restrict__ adjncy,
+                                                        const wgt_t *__restrict__ adjwgt,
+                                                        const int32_t *__restrict__ where,
+                                                        const int32_t *__restrict__ dest,
+                                                        const int32_t *__restrict__ pregain, const JetCtl *ctl,
+                                                        uint8_t *__restrict__ sel)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
-    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t v = (int32_t)(gt / G);
-    int lane = (int)(threadIdx.x % G);
-    if (v >= n) return;
+    int64_t gi = gindex<G>();
+    int lane = glane<G>();
+    if (gi >= cnt) return;
+    int32_t v = ids[gi];
     int32_t ig = pregain[v];
     if (ig == kNoGain) {
         if (lane == 0) sel[v] = 0;
@@ -240,8 +260,7 @@ __global__ void __launch_bounds__(kBS) k_afterburner(int32_t n, const eid_t *__r
             if (nw == p) change += w; else if (nw == best) change -= w;
         }
     }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) change += __shfl_xor(change, o, G);
+    change = group_sum<G>(change);
     if (lane == 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
 }
 
@@ -380,36 +399,52 @@ __global__ void k_gather_w(const int64_t *__restrict__ ne_ptr, const int32_t *__
 // Exact cut change of applying dest[] (edges between two moved vertices are
 // seen from both ends, so each side counts once and one-sided edges twice),
 // plus part-weight updates (perform_moves, :4034-4155).
+// Persistent blocks over the row list; the cut change and the part-weight
+// changes are summed in LDS (a k-entry histogram) and flushed with one global
+// atomic per block and touched part.
 template <int G>
-__global__ void __launch_bounds__(kBS) k_move_delta(int32_t n, const eid_t *__restrict__ xadj,
-                                                    const vid_t *__restrict__ adjncy,
-                                                    const wgt_t *__restrict__ adjwgt, const wgt_t *__restrict__ vwgt,
-                                                    const int32_t *__restrict__ where,
-                                                    const int32_t *__restrict__ dest, JetCtl *ctl,
-                                                    int64_t *__restrict__ pwgts)
+__global__ void __launch_bounds__(kRowBS) k_move_delta(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
+                                                       const eid_t *__restrict__ xadj,
+                                                       const vid_t *__restrict__ adjncy,
+                                                       const wgt_t *__restrict__ adjwgt,
+                                                       const wgt_t *__restrict__ vwgt,
+                                                       const int32_t *__restrict__ where,
+                                                       const int32_t *__restrict__ dest, JetCtl *ctl,
+                                                       int64_t *__restrict__ pwgts)
 {
+    extern __shared__ unsigned long long pdelta[];
+    __shared__ unsigned long long bdelta;
     if (ctl->done) return;
-    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t v = (int32_t)(gt / G);
-    int lane = (int)(threadIdx.x % G);
-    if (v >= n) return;
-    int32_t pv = where[v], dv = dest[v];
-    if (pv == dv) return;
-    int64_t d2 = 0;
-    for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) {
-        int32_t u = adjncy[j];
-        int32_t pu = where[u], du = dest[u];
-        bool um = pu != du;
-        int32_t before = pv != pu, after = dv != (um ? du : pu);
-        d2 += (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
+    for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
+    if (threadIdx.x == 0) bdelta = 0;
+    __syncthreads();
+    constexpr int NG = kRowBS / G;
+    const int g = threadIdx.x / G, lane = glane<G>();
+    for (int64_t base = (int64_t)blockIdx.x * NG; base < cnt; base += (int64_t)gridDim.x * NG) {
+        int64_t gi = base + g;
+        int32_t v = gi < cnt ? ids[gi] : 0;
+        int32_t pv = gi < cnt ? where[v] : 0, dv = gi < cnt ? dest[v] : 0;
+        int64_t d2 = 0;
+        if (pv != dv) {
+            for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) {
+                int32_t u = adjncy[j];
+                int32_t pu = where[u], du = dest[u];
+                bool um = pu != du;
+                int32_t before = pv != pu, after = dv != (um ? du : pu);
+                d2 += (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
+            }
+        }
+        d2 = group_sum<G>(d2);
+        if (lane == 0 && pv != dv) {
+            if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
+            atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
+            atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
+        }
     }
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) d2 += __shfl_xor(d2, o, G);
-    if (lane == 0) {
-        atomicAdd(&ctl->delta2, (unsigned long long)d2);
-        atomicAdd((unsigned long long *)&pwgts[pv], (unsigned long long)(-(int64_t)vwgt[v]));
-        atomicAdd((unsigned long long *)&pwgts[dv], (unsigned long long)(int64_t)vwgt[v]);
-    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += kRowBS)
+        if (pdelta[i]) atomicAdd((unsigned long long *)&pwgts[i], pdelta[i]);
+    if (threadIdx.x == 0 && bdelta) atomicAdd(&ctl->delta2, bdelta);
 }
 
 __global__ void k_move_commit(int32_t n, const JetCtl *ctl, const int32_t *__restrict__ dest, int32_t *__restrict__ where)
@@ -496,20 +531,28 @@ __global__ void k_jet_fini(const JetCtl *ctl, int64_t *d_cut) { *d_cut = ctl->be
 // ---------------------------------------------------------------- utilities
 
 template <int G>
-__global__ void __launch_bounds__(kBS) k_cut(int32_t n, const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
-                                             const wgt_t *__restrict__ adjwgt, const int32_t *__restrict__ where,
-                                             unsigned long long *out)
+__global__ void __launch_bounds__(kRowBS) k_cut(const int32_t *__restrict__ ids, int64_t cnt,
+                                                const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                const wgt_t *__restrict__ adjwgt, const int32_t *__restrict__ where,
+                                                unsigned long long *out)
 {
-    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int32_t v = (int32_t)(gt / G);
-    int lane = (int)(threadIdx.x % G);
+    int64_t gi = gindex<G>();
+    int lane = glane<G>();
+    if (gi >= cnt) return;
+    int32_t v = ids[gi];
+    int32_t p = where[v];
     int64_t s = 0;
-    if (v < n) {
-        int32_t p = where[v];
-        for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) s += where[adjncy[j]] != p ? adjwgt[j] : 0;
-    }
-    s = wave_sum(s);
-    if (lane_id() == 0 && s) atomicAdd(out, (unsigned long long)s);
+    for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) s += where[adjncy[j]] != p ? adjwgt[j] : 0;
+    s = group_sum<G>(s);
+    if (lane == 0 && s) atomicAdd(out, (unsigned long long)s);
+}
+
+template <int MaxG = 256, class F>
+void for_bins(const RowBins &b, F f)
+{
+    if (b.count(0)) f(std::integral_constant<int, 8>{}, b.list(0), b.count(0));
+    if (b.count(1)) f(std::integral_constant<int, 64>{}, b.list(1), b.count(1));
+    if (b.count(2)) f(std::integral_constant<int, MaxG < 256 ? MaxG : 256>{}, b.list(2), b.count(2));
 }
 
 __global__ void k_half(int64_t *d) { *d /= 2; }
@@ -529,18 +572,33 @@ __global__ void k_project(int32_t n, const int32_t *__restrict__ cmap, const int
 }
 
 template <int G>
+void launch_select_bin(Ctx &c, const DevGraph &g, int32_t k, const int32_t *ids, int64_t cnt, const int32_t *where,
+                       const int64_t *pwgts, const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest,
+                       int32_t *pregain, uint32_t *key)
+{
+    constexpr int NG = kRowBS / G;
+    size_t lds = sizeof(int32_t) * (size_t)NG * (size_t)k;
+    int64_t blocks = (cnt + NG - 1) / NG;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) blocks = 1;
+    k_jet_select<G><<<(unsigned)blocks, kRowBS, lds, c.stream>>>(ids, cnt, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt,
+                                                                where, pwgts, lock, under, ctl, dest, pregain, key);
+    HGP_LAUNCH_CHECK();
+}
+
+// The small class uses 8-lane groups while 32 k-int rows fit in 64 KB of LDS
+// (k <= 512), else one wave per row.
 void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, const int64_t *pwgts,
                    const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest, int32_t *pregain,
                    uint32_t *key)
 {
-    constexpr int NG = kBS / G;
-    size_t lds = sizeof(int32_t) * (size_t)NG * (size_t)k;
-    int64_t blocks = (g.n + NG - 1) / NG;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks < 1) blocks = 1;
-    k_jet_select<G><<<(unsigned)blocks, kBS, lds, c.stream>>>(g.n, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where,
-                                                             pwgts, lock, under, ctl, dest, pregain, key);
-    HGP_LAUNCH_CHECK();
+    const RowBins &b = g.bins;
+    if (b.count(0)) {
+        if (k <= 512) launch_select_bin<8>(c, g, k, b.list(0), b.count(0), where, pwgts, lock, under, ctl, dest, pregain, key);
+        else launch_select_bin<64>(c, g, k, b.list(0), b.count(0), where, pwgts, lock, under, ctl, dest, pregain, key);
+    }
+    if (b.count(1)) launch_select_bin<64>(c, g, k, b.list(1), b.count(1), where, pwgts, lock, under, ctl, dest, pregain, key);
+    if (b.count(2)) launch_select_bin<256>(c, g, k, b.list(2), b.count(2), where, pwgts, lock, under, ctl, dest, pregain, key);
 }
 
 }  // namespace
@@ -549,9 +607,12 @@ void edge_cut(Ctx &c, const DevGraph &g, const int32_t *where, int64_t *d_cut)
 {
     fill<int64_t>(c, d_cut, 1, 0);
     if (g.n <= 0) return;
-    k_cut<16><<<grid_for((int64_t)g.n * 16, kBS), kBS, 0, c.stream>>>(g.n, g.xadj, g.adjncy, g.adjwgt, where,
-                                                                     (unsigned long long *)d_cut);
-    HGP_LAUNCH_CHECK();
+    for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
+        constexpr int G = decltype(gc)::value;
+        k_cut<G><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt, where,
+                                                             (unsigned long long *)d_cut);
+        HGP_LAUNCH_CHECK();
+    });
     k_half<<<1, 1, 0, c.stream>>>(d_cut);
     HGP_LAUNCH_CHECK();
 }
@@ -604,8 +665,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     part_weights(c, n, g.vwgt, where, k, pwgts);
     k_jet_init<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, d_cut, maxwgt, opt, p.patience, finest ? 1 : 3);
     HGP_LAUNCH_CHECK();
-    unsigned gv = grid_for(n, kBS), gg = grid_for((int64_t)n * 16, kBS);
-    bool big_k = (size_t)k * 16 * sizeof(int32_t) > 64 * 1024;
+    unsigned gv = grid_for(n, kBS);
     for (;;) {
         k_jet_begin<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under);
         HGP_LAUNCH_CHECK();
@@ -617,15 +677,18 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             // RW/RS: the same per-vertex traffic; rows only of overloaded parts (not counted).
             bool lp = h.mode == kModeLP;
             KScope ks(c, lp ? "jet_select_lp" : "jet_select_rb", (lp ? 12.0 * (double)g.m : 0.0) + 21.0 * n);
-            if (big_k) launch_select<64>(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key);
-            else launch_select<16>(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key);
+            launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key);
         }
         if (h.mode == kModeLP) {
             {
                 KScope ks(c, "jet_afterburner", 5.0 * n);
-                k_afterburner<16><<<gg, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl,
-                                                            sel);
-                HGP_LAUNCH_CHECK();
+                for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
+                    constexpr int G = decltype(gc)::value;
+                    k_afterburner<G><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy,
+                                                                                 g.adjwgt, where, dest, pregain,
+                                                                                 ctl, sel);
+                    HGP_LAUNCH_CHECK();
+                });
             }
             k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock);
             HGP_LAUNCH_CHECK();
@@ -653,9 +716,13 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         }
         {
             KScope ks(c, "jet_move_delta", 8.0 * n);
-            k_move_delta<16><<<gg, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl,
-                                                       pwgts);
-            HGP_LAUNCH_CHECK();
+            for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
+                constexpr int G = decltype(gc)::value;
+                int64_t blocks = std::min<int64_t>((nb + kRowBS / G - 1) / (kRowBS / G), 2048);
+                k_move_delta<G><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                    ids, nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts);
+                HGP_LAUNCH_CHECK();
+            });
         }
         k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where);
         HGP_LAUNCH_CHECK();

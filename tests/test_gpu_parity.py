@@ -21,6 +21,17 @@ def graphs(hgp):
     edges = [(0, i) for i in range(1, 40)] + [(40, 41), (41, 42)]
     x, a, w = csr_from_edges(60, edges)
     out["star_iso"] = (x, a, np.ones(60, np.int32), w)
+    # hub with 300 leaves: leaf pairing over several 64-entry chunks
+    edges = [(0, i) for i in range(1, 301)] + [(301, 302)]
+    x, a, w = csr_from_edges(303, edges)
+    out["star300"] = (x, a, np.ones(303, np.int32), w)
+    # comb: hub 0 -- teeth 1..200, tooth i -- heavy private vertex 200+i; with
+    # k=64 the heavy vertices cannot be matched, so the relative pass pairs teeth
+    edges = [(0, i) for i in range(1, 201)] + [(i, 200 + i) for i in range(1, 201)]
+    x, a, w = csr_from_edges(401, edges)
+    vw = np.ones(401, np.int32)
+    vw[201:] = 50
+    out["comb"] = (x, a, vw, w)
     return out
 
 
