@@ -121,107 +121,239 @@ struct BetterPart {
 //  LP : select_dest_part (:1995-2046) + filter_potential_vertex (:2114-2135)
 //  RW : select_dest_parts_rw (:3184-3241), bucket of assign_move_scores (:3243-3266)
 //  RS : assign_move_scores_part1 (:2515-2556) with its effective loss -conn[p]
-// Each G-lane group owns a dense row of k ints in LDS (set_gain_val_warp,
-// :4332-4432, keeps the same per-warp k-wide accumulator): phase 1 adds edge
-// weights per neighbour part, phase 2 reads them and reduces the arg-max,
-// phase 3 clears the touched entries.  Blocks loop over the row list.
-template <int G>
-__global__ void __launch_bounds__(kRowBS) k_jet_select(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
-                                                       const eid_t *__restrict__ xadj,
-                                                       const vid_t *__restrict__ adjncy,
-                                                       const wgt_t *__restrict__ adjwgt,
-                                                       const wgt_t *__restrict__ vwgt,
-                                                       const int32_t *__restrict__ where,
-                                                       const int64_t *__restrict__ pwgts,
-                                                       const uint8_t *__restrict__ lock,
-                                                       const int32_t *__restrict__ under, JetCtl *ctl,
-                                                       int32_t *__restrict__ dest, int32_t *__restrict__ pregain,
-                                                       uint32_t *__restrict__ key)
+// The three kernels below differ only in how they find, for a row, the
+// connectivity pc to its own part and the best other part (c, q) under
+// BetterPart; every part adjacent to the row is a candidate, so the result
+// is the same whichever kernel serves a row.
+
+struct SelArgs {
+    int32_t k;
+    const eid_t *xadj;
+    const vid_t *adjncy;
+    const wgt_t *adjwgt;
+    const wgt_t *vwgt;
+    const int32_t *where;
+    const int64_t *pwgts;
+    const uint8_t *lock;
+    const int32_t *under;
+    JetCtl *ctl;
+    int32_t *dest;
+    int32_t *pregain;
+    uint32_t *key;
+};
+
+__device__ inline bool sel_eligible(const SelArgs &A, int32_t mode, int64_t maxwgt, int64_t opt, int32_t v,
+                                    int32_t p)
 {
-    extern __shared__ int32_t acc[];
-    const JetCtl &C = *ctl;
+    if (mode == kModeLP) return true;
+    int64_t pw = A.pwgts[p];
+    if (mode == kModeRW) return pw > maxwgt && 2 * (int64_t)A.vwgt[v] < 3 * (pw - opt);
+    return pw > maxwgt;
+}
+
+// Whether part q may receive v in this mode (q != p).
+__device__ inline bool sel_target(const SelArgs &A, int32_t mode, int64_t maxwgt, int32_t q)
+{
+    if (mode == kModeRS) return false;
+    return mode != kModeRW || A.pwgts[q] < maxwgt;
+}
+
+// Writes dest/pregain/key of v from (pc, best); returns 1 for an LP candidate.
+__device__ inline int32_t sel_decide(const SelArgs &A, const JetCtl &C, int32_t mode, int32_t v, int32_t p,
+                                     bool eligible, int32_t pc, PartKey best)
+{
+    const uint32_t nokey = (uint32_t)A.k * kMaxBuckets;
+    int32_t bc = best.c, bq = best.q;
+    int32_t d = p, cand = 0;
+    if (mode == kModeLP) {
+        int32_t gain = kNoGain;
+        if (bq >= 0 && bc > 0) {
+            int32_t fn = C.filt_num;
+            if (bc >= pc || (int64_t)(pc - bc) < ((int64_t)fn * pc) / 4) {
+                d = bq;
+                if (!A.lock[v]) { gain = bc - pc; cand = 1; }
+            }
+        }
+        A.pregain[v] = gain;
+    } else if (mode == kModeRW) {
+        uint32_t kk = nokey;
+        if (eligible) {
+            int64_t gain = 0;
+            bool mv = false;
+            if (bq >= 0 && bc > 0) { d = bq; gain = (int64_t)bc - pc; mv = true; }
+            else if (C.nunder > 0) { d = A.under[v % C.nunder]; gain = -(int64_t)pc; mv = true; }
+            if (mv && d != p) kk = (uint32_t)p * kMaxBuckets + (uint32_t)gain_bucket(gain, A.vwgt[v]);
+            else d = p;
+        }
+        A.key[v] = kk;
+    } else {
+        uint32_t kk = nokey;
+        if (eligible) {
+            int32_t bkt = gain_bucket(-(int64_t)pc, A.vwgt[v]);
+            if (bkt < kMaxBuckets && (int64_t)A.vwgt[v] < 2 * (A.pwgts[p] - C.opt))
+                kk = (uint32_t)p * kMaxBuckets + (uint32_t)bkt;
+        }
+        A.key[v] = kk;
+    }
+    A.dest[v] = d;
+    return cand;
+}
+
+__device__ inline void sel_count(const SelArgs &A, int32_t mode, int32_t ncand)
+{
+    if (mode != kModeLP) return;
+    int32_t s = wave_sum(ncand);
+    if (lane_id() == 0 && s) atomicAdd(&A.ctl->ncand, s);
+}
+
+// Orders this wave's LDS accesses before and after the point (the wave runs
+// in lockstep, so completing its own LDS operations is enough).
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Rows of <= 8 entries: 8 lanes, one entry each; a lane's connectivity to its
+// entry's part is summed over the group with shuffles (no LDS).
+__global__ void __launch_bounds__(kRowBS) k_jet_select_s(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
+{
+    const JetCtl &C = *A.ctl;
     if (C.done) return;
     const int32_t mode = C.mode;
     const int64_t maxwgt = C.maxwgt, opt = C.opt;
-    constexpr int NG = kRowBS / G;
-    const int g = threadIdx.x / G, lane = glane<G>();
-    int32_t *my = acc + (size_t)g * k;
-    for (int i = threadIdx.x; i < NG * k; i += kRowBS) acc[i] = 0;
-    __syncthreads();
-    const uint32_t nokey = (uint32_t)k * kMaxBuckets;
-    int32_t ncand_local = 0;
-    for (int64_t base = (int64_t)blockIdx.x * NG; base < cnt; base += (int64_t)gridDim.x * NG) {
-        int64_t gi = base + g;
-        bool live = gi < cnt;
-        int32_t v = live ? ids[gi] : 0;
-        int32_t p = live ? where[v] : 0;
-        bool eligible = live;
-        if (live && mode != kModeLP) {
-            int64_t pw = pwgts[p];
-            int64_t wv = vwgt[v];
-            if (mode == kModeRW) eligible = pw > maxwgt && 2 * wv < 3 * (pw - opt);
-            else eligible = pw > maxwgt;
+    int64_t gi = gindex<8>();
+    int lane = glane<8>();
+    int32_t ncand = 0;
+    if (gi < cnt) {
+        int32_t v = ids[gi];
+        int32_t p = A.where[v];
+        bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
+        eid_t j = A.xadj[v] + lane;
+        int32_t q = -1, w = 0;
+        if (eligible && j < A.xadj[v + 1]) { q = A.where[A.adjncy[j]]; w = A.adjwgt[j]; }
+        int32_t conn = w;
+#pragma unroll
+        for (int o = 1; o < 8; o++) {
+            int32_t qq = __shfl_xor(q, o, 8), ww = __shfl_xor(w, o, 8);
+            if (qq == q) conn += ww;
         }
-        eid_t b = live ? xadj[v] : 0, e = live ? xadj[v + 1] : 0;
-        if (eligible)
-            for (eid_t j = b + lane; j < e; j += G) atomicAdd(&my[where[adjncy[j]]], adjwgt[j]);
-        __syncthreads();
+        int32_t pc = group_sum<8>(q == p ? w : 0);
         PartKey best{0, -1};
+        if (q >= 0 && q != p && sel_target(A, mode, maxwgt, q)) best = PartKey{conn, q};
+        best = group_best<8>(best, BetterPart());
+        if (lane == 0) ncand = sel_decide(A, C, mode, v, p, eligible, pc, best);
+    }
+    sel_count(A, mode, ncand);
+}
+
+// Rows of 9..512 entries: one wave per row, entries cached in registers (8 per
+// lane), a k-int LDS accumulator per wave.  Waves loop over rows on their own.
+__global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
+{
+    extern __shared__ int32_t acc[];
+    constexpr int S = (int)(kBinMid / kWave);
+    const JetCtl &C = *A.ctl;
+    if (C.done) return;
+    const int32_t mode = C.mode;
+    const int64_t maxwgt = C.maxwgt, opt = C.opt;
+    const int k = A.k, lane = lane_id();
+    const int wv = threadIdx.x / kWave, nw = kRowBS / kWave;
+    int32_t *my = acc + (size_t)wv * k;
+    for (int i = lane; i < k; i += kWave) my[i] = 0;
+    wave_lds_sync();
+    int32_t ncand = 0;
+    for (int64_t gi = (int64_t)blockIdx.x * nw + wv; gi < cnt; gi += (int64_t)gridDim.x * nw) {
+        int32_t v = ids[gi];
+        int32_t p = A.where[v];
+        bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
+        eid_t b = A.xadj[v], e = A.xadj[v + 1];
+        int32_t q[S], w[S];
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+            eid_t j = b + lane + (eid_t)t * kWave;
+            q[t] = -1;
+            w[t] = 0;
+            if (eligible && j < e) {
+                q[t] = A.where[A.adjncy[j]];
+                w[t] = A.adjwgt[j];
+                atomicAdd(&my[q[t]], w[t]);
+            }
+        }
+        wave_lds_sync();
         int32_t pc = eligible ? my[p] : 0;
-        if (eligible && mode != kModeRS) {
-            for (eid_t j = b + lane; j < e; j += G) {
-                int32_t q = where[adjncy[j]];
-                if (q == p) continue;
-                if (mode == kModeRW && !(pwgts[q] < maxwgt)) continue;
-                PartKey cand{my[q], q};
+        PartKey best{0, -1};
+#pragma unroll
+        for (int t = 0; t < S; t++) {
+            if (q[t] >= 0 && q[t] != p && sel_target(A, mode, maxwgt, q[t])) {
+                PartKey cand{my[q[t]], q[t]};
                 if (BetterPart()(cand, best)) best = cand;
             }
         }
-        best = group_best<G>(best, BetterPart());
-        __syncthreads();
+        best = group_best<kWave>(best, BetterPart());
+        wave_lds_sync();
+#pragma unroll
+        for (int t = 0; t < S; t++)
+            if (q[t] >= 0) my[q[t]] = 0;
+        wave_lds_sync();
+        if (lane == 0) ncand += sel_decide(A, C, mode, v, p, eligible, pc, best);
+    }
+    sel_count(A, mode, ncand);
+}
+
+// Rows of > 512 entries: one block per row and a k-int LDS accumulator.  The
+// arg-max and the clearing walk the k parts when the row is longer than k,
+// else the row again.
+__global__ void __launch_bounds__(kRowBS) k_jet_select_b(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
+{
+    extern __shared__ int32_t acc[];
+    const JetCtl &C = *A.ctl;
+    if (C.done) return;
+    const int32_t mode = C.mode;
+    const int64_t maxwgt = C.maxwgt, opt = C.opt;
+    const int k = A.k;
+    for (int i = threadIdx.x; i < k; i += kRowBS) acc[i] = 0;
+    __syncthreads();
+    int32_t ncand = 0;
+    for (int64_t gi = blockIdx.x; gi < cnt; gi += gridDim.x) {
+        int32_t v = ids[gi];
+        int32_t p = A.where[v];
+        bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
+        eid_t b = A.xadj[v], e = A.xadj[v + 1];
+        bool walk_parts = e - b > (eid_t)k;
         if (eligible)
-            for (eid_t j = b + lane; j < e; j += G) my[where[adjncy[j]]] = 0;
-        int32_t bc = best.c, bq = best.q;
-        if (live && lane == 0) {
-            int32_t d = p;
-            if (mode == kModeLP) {
-                int32_t gain = kNoGain;
-                if (bq >= 0 && bc > 0) {
-                    int32_t fn = C.filt_num;
-                    if (bc >= pc || (int64_t)(pc - bc) < ((int64_t)fn * pc) / 4) {
-                        d = bq;
-                        if (!lock[v]) { gain = bc - pc; ncand_local++; }
-                    }
+            for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) atomicAdd(&acc[A.where[A.adjncy[j]]], A.adjwgt[j]);
+        __syncthreads();
+        int32_t pc = eligible ? acc[p] : 0;
+        PartKey best{0, -1};
+        if (eligible) {
+            if (walk_parts) {
+                for (int q = threadIdx.x; q < k; q += kRowBS) {
+                    if (q == p || !sel_target(A, mode, maxwgt, q)) continue;
+                    PartKey cand{acc[q], q};
+                    if (BetterPart()(cand, best)) best = cand;
                 }
-                pregain[v] = gain;
-            } else if (mode == kModeRW) {
-                uint32_t kk = nokey;
-                if (eligible) {
-                    int64_t gain = 0;
-                    bool mv = false;
-                    if (bq >= 0 && bc > 0) { d = bq; gain = (int64_t)bc - pc; mv = true; }
-                    else if (C.nunder > 0) { d = under[v % C.nunder]; gain = -(int64_t)pc; mv = true; }
-                    if (mv && d != p) kk = (uint32_t)p * kMaxBuckets + (uint32_t)gain_bucket(gain, vwgt[v]);
-                    else d = p;
-                }
-                key[v] = kk;
             } else {
-                uint32_t kk = nokey;
-                if (eligible) {
-                    int32_t bkt = gain_bucket(-(int64_t)pc, vwgt[v]);
-                    if (bkt < kMaxBuckets && (int64_t)vwgt[v] < 2 * (pwgts[p] - opt))
-                        kk = (uint32_t)p * kMaxBuckets + (uint32_t)bkt;
+                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) {
+                    int32_t q = A.where[A.adjncy[j]];
+                    if (q == p || !sel_target(A, mode, maxwgt, q)) continue;
+                    PartKey cand{acc[q], q};
+                    if (BetterPart()(cand, best)) best = cand;
                 }
-                key[v] = kk;
             }
-            dest[v] = d;
+        }
+        best = group_best<kRowBS>(best, BetterPart());
+        if (eligible) {
+            if (walk_parts) {
+                for (int q = threadIdx.x; q < k; q += kRowBS) acc[q] = 0;
+            } else {
+                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) acc[A.where[A.adjncy[j]]] = 0;
+            }
         }
         __syncthreads();
+        if (threadIdx.x == 0) ncand += sel_decide(A, C, mode, v, p, eligible, pc, best);
     }
-    if (mode == kModeLP) {
-        int32_t s = wave_sum(ncand_local);
-        if (lane_id() == 0 && s) atomicAdd(&ctl->ncand, s);
-    }
+    sel_count(A, mode, ncand);
 }
 
 // Afterburner (afterburner_heuristic, :2137-2178): recompute a candidate's
@@ -571,34 +703,28 @@ __global__ void k_project(int32_t n, const int32_t *__restrict__ cmap, const int
     if (v < n) wf[v] = wc[cmap[v]];
 }
 
-template <int G>
-void launch_select_bin(Ctx &c, const DevGraph &g, int32_t k, const int32_t *ids, int64_t cnt, const int32_t *where,
-                       const int64_t *pwgts, const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest,
-                       int32_t *pregain, uint32_t *key)
-{
-    constexpr int NG = kRowBS / G;
-    size_t lds = sizeof(int32_t) * (size_t)NG * (size_t)k;
-    int64_t blocks = (cnt + NG - 1) / NG;
-    if (blocks > 8192) blocks = 8192;
-    if (blocks < 1) blocks = 1;
-    k_jet_select<G><<<(unsigned)blocks, kRowBS, lds, c.stream>>>(ids, cnt, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt,
-                                                                where, pwgts, lock, under, ctl, dest, pregain, key);
-    HGP_LAUNCH_CHECK();
-}
-
-// The small class uses 8-lane groups while 32 k-int rows fit in 64 KB of LDS
-// (k <= 512), else one wave per row.
 void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, const int64_t *pwgts,
                    const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest, int32_t *pregain,
                    uint32_t *key)
 {
     const RowBins &b = g.bins;
+    SelArgs A{k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl, dest, pregain, key};
     if (b.count(0)) {
-        if (k <= 512) launch_select_bin<8>(c, g, k, b.list(0), b.count(0), where, pwgts, lock, under, ctl, dest, pregain, key);
-        else launch_select_bin<64>(c, g, k, b.list(0), b.count(0), where, pwgts, lock, under, ctl, dest, pregain, key);
+        k_jet_select_s<<<grid_groups(b.count(0), 8), kRowBS, 0, c.stream>>>(b.list(0), b.count(0), A);
+        HGP_LAUNCH_CHECK();
     }
-    if (b.count(1)) launch_select_bin<64>(c, g, k, b.list(1), b.count(1), where, pwgts, lock, under, ctl, dest, pregain, key);
-    if (b.count(2)) launch_select_bin<256>(c, g, k, b.list(2), b.count(2), where, pwgts, lock, under, ctl, dest, pregain, key);
+    if (b.count(1)) {
+        int64_t blocks = std::min<int64_t>((b.count(1) + 3) / 4, 4096);
+        k_jet_select_w<<<(unsigned)blocks, kRowBS, sizeof(int32_t) * 4 * (size_t)k, c.stream>>>(b.list(1),
+                                                                                               b.count(1), A);
+        HGP_LAUNCH_CHECK();
+    }
+    if (b.count(2)) {
+        int64_t blocks = std::min<int64_t>(b.count(2), 4096);
+        k_jet_select_b<<<(unsigned)blocks, kRowBS, sizeof(int32_t) * (size_t)k, c.stream>>>(b.list(2), b.count(2),
+                                                                                           A);
+        HGP_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace

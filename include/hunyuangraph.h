@@ -99,8 +99,9 @@ int hgp_partition_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjn
                          const hgp_options *opt, int32_t *d_part, hgp_stats *stats);
 
 /* ------------------------------------------------------------ kernel timer
- * Per-kernel timing with HIP events on the library's stream, for
- * measurement: when enabled, the main kernels record an event pair per launch
+ * No counterpart entry point in the reference (its phase timers are process
+ * globals, hunyuangraph_timer.h:9-319).  Per-kernel timing with HIP events on
+ * the library's stream, for measurement: when enabled, the main kernels record an event pair per launch
  * and the algorithmic HBM bytes the launch must move.  enable(0/1) clears the
  * records.  report() writes JSON {"kernel": [launches, total_ms, bytes], ...}
  * into buf (NUL-terminated, truncated to len) and returns the length needed. */
@@ -149,7 +150,8 @@ int hgp_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int3
                const hgp_options *opt, int32_t *where, int64_t *cut, int32_t *iters);
 
 /* ------------------------------------------------------------ graphs
- * Seeded synthetic generators (host code).  The library allocates the
+ * No counterpart in the reference, which reads graphs from files
+ * (hunyuangraph_io.h:25-186).  Seeded synthetic generators (host code).  The library allocates the
  * arrays; release them with hgp_graph_free. */
 typedef struct hgp_graph {
     int32_t n;

@@ -14,7 +14,6 @@
 #include <string.h>
 #include <math.h>
 #include <limits.h>
-#include <stdio.h>
 
 /* ------------------------------------------------------------------ rng */
 
@@ -915,17 +914,9 @@ int32_t oracle_dga_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
             else jet_rs_step(&c, where, pwgts, maxwgt, opt, dest);
             balance_counter++;
         }
-        int mode_dbg = curmax <= maxwgt ? 0 : (balance_counter <= 2 ? 1 : 2);
-        int64_t before_max = curmax;
-        int64_t nmv = 0;
-        for (int32_t v = 0; v < n; v++) nmv += dest[v] != where[v];
         cur_cut += apply_moves(n, xadj, adjncy, adjwgt, vwgt, where, dest, pwgts);
         iters++;
         curmax = max_of(pwgts, k);
-        if (getenv("HGP_ORACLE_TRACE"))
-            fprintf(stderr, "n=%d it=%d mode=%d moves=%lld max %lld->%lld maxwgt=%lld cut=%lld count=%d\n", n, iters,
-                    mode_dbg, (long long)nmv, (long long)before_max, (long long)curmax, (long long)maxwgt,
-                    (long long)cur_cut, count);
         if (best_max > maxwgt && curmax < best_max) {
             best_max = curmax;
             best_cut = cur_cut;
