@@ -108,6 +108,12 @@ SIGNATURES = {
     "hgp_default_options": (None, [_P]),
     "hgp_partition": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "hgp_partition_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
+    "hgp_partition_batch": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P]),
+    "hgp_server_create": (C.c_int, [_P, _I32, _I64, _P]),
+    "hgp_server_submit": (C.c_int64, [_P, _I32, _P, _P, _P, _P, _I32, _U64, _P]),
+    "hgp_server_wait": (C.c_int, [_P, _I64, _P, _P]),
+    "hgp_server_info": (C.c_int, [_P, _P, _P, _P]),
+    "hgp_server_destroy": (C.c_int, [_P]),
     "hgp_ktimer_enable": (C.c_int, [C.c_int]),
     "hgp_ktimer_report": (C.c_int64, [_P, _I64]),
     "hgp_scan_i32": (C.c_int, [_P, _P, _I64]),
@@ -216,6 +222,87 @@ def partition(xadj, adjncy, k, vwgt=None, adjwgt=None, **opts):
     _check(lib().hgp_partition(n, _ptr(xadj), _ptr(adjncy), _ptr(vw), _ptr(aw), int(k), C.byref(o), _ptr(part),
                                C.byref(st)))
     return part, st.to_dict()
+
+
+def partition_batch(graphs, ks, lanes=0, **opts):
+    """Partitions many graphs in one call.  graphs: list of (xadj, adjncy,
+    vwgt, adjwgt) with vwgt/adjwgt possibly None; ks: parts per graph.
+    Returns (list of part arrays, list of stats dicts)."""
+    cnt = len(graphs)
+    if len(ks) != cnt:
+        raise ValueError("one k per graph")
+    keep = []
+    ns = np.zeros(max(cnt, 1), np.int32)
+    arrs = {name: (C.c_void_p * max(cnt, 1))() for name in ("x", "a", "vw", "aw", "p")}
+    parts = []
+    for i, (x, a, vw, aw) in enumerate(graphs):
+        x, a = _i64(x), _i32(a)
+        vw = None if vw is None else _i32(vw)
+        aw = None if aw is None else _i32(aw)
+        p = np.zeros(max(len(x) - 1, 0), np.int32)
+        keep += [x, a, vw, aw]
+        parts.append(p)
+        ns[i] = len(x) - 1
+        arrs["x"][i], arrs["a"][i] = x.ctypes.data, a.ctypes.data
+        arrs["vw"][i] = None if vw is None else vw.ctypes.data
+        arrs["aw"][i] = None if aw is None else aw.ctypes.data
+        arrs["p"][i] = p.ctypes.data
+    kk = np.ascontiguousarray(ks, np.int32)
+    o = options(**opts)
+    status = np.zeros(max(cnt, 1), np.int32)
+    st = (HgpStats * max(cnt, 1))()
+    rc = lib().hgp_partition_batch(cnt, _ptr(ns), arrs["x"], arrs["a"], arrs["vw"], arrs["aw"], _ptr(kk),
+                                   C.byref(o), int(lanes), arrs["p"], _ptr(status), st)
+    _check(rc)
+    return parts, [st[i].to_dict() for i in range(cnt)]
+
+
+class Server:
+    """The serving loop: submit() queues a job and returns a ticket, wait()
+    returns (part, stats, latency_ms).  Arrays are kept alive until waited."""
+
+    def __init__(self, slots=4, budget_bytes=200 << 30, **opts):
+        self._h = C.c_void_p()
+        o = options(**opts)
+        _check(lib().hgp_server_create(C.byref(o), int(slots), int(budget_bytes), C.byref(self._h)))
+        self._live = {}
+
+    def submit(self, xadj, adjncy, k, vwgt=None, adjwgt=None, seed=1) -> int:
+        x, a = _i64(xadj), _i32(adjncy)
+        vw = None if vwgt is None else _i32(vwgt)
+        aw = None if adjwgt is None else _i32(adjwgt)
+        part = np.zeros(max(len(x) - 1, 0), np.int32)
+        t = lib().hgp_server_submit(self._h, len(x) - 1, _ptr(x), _ptr(a), _ptr(vw), _ptr(aw), int(k), int(seed),
+                                    _ptr(part))
+        if t < 0:
+            _check(int(t))
+        self._live[t] = (x, a, vw, aw, part)
+        return t
+
+    def wait(self, ticket):
+        st = HgpStats()
+        lat = C.c_double()
+        rc = lib().hgp_server_wait(self._h, int(ticket), C.byref(st), C.byref(lat))
+        arrs = self._live.pop(ticket, None)
+        _check(rc)
+        return arrs[4], st.to_dict(), lat.value
+
+    def info(self) -> dict:
+        r, p, c = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().hgp_server_info(self._h, C.byref(r), C.byref(p), C.byref(c)))
+        return {"reserved": r.value, "peak_reserved": p.value, "completed": c.value}
+
+    def close(self) -> None:
+        if self._h:
+            _check(lib().hgp_server_destroy(self._h))
+            self._h = C.c_void_p()
+            self._live.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def partition_device(n, d_xadj, d_adjncy, d_vwgt, d_adjwgt, k, d_part, **opts) -> dict:

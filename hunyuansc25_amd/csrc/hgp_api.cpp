@@ -4,15 +4,19 @@
 
 #include "hgp_graph.hpp"
 #include "hgp_prims.hpp"
+#include "hgp_host.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <numeric>
+#include <thread>
 #include <vector>
 
 namespace hgp {
-const char *last_error_text();
 
 namespace {
 
@@ -218,47 +222,6 @@ void validate_host(int32_t n, const int64_t *xadj, const int32_t *adjncy, const 
     }
 }
 
-template <class F>
-int guarded(F f)
-{
-    clear_error();
-    try {
-        f();
-        return HGP_OK;
-    } catch (const Error &e) {
-        set_error(e.code, e.msg);
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        set_error(-3, "host out of memory");
-        return HGP_ERR_OOM;
-    } catch (const std::exception &e) {
-        set_error(-2, e.what());
-        return HGP_ERR_DEVICE;
-    }
-}
-
-// Runs body with an arena of at least `bytes`; on arena overflow the arena
-// grows and the body is re-run (the body must be restartable).
-template <class F>
-void with_arena(Ctx &c, size_t bytes, F body)
-{
-    for (int attempt = 0;; attempt++) {
-        c.arena.reset();
-        c.arena.reserve(bytes);
-        try {
-            body();
-            HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
-            c.arena.reset();
-            return;
-        } catch (const Error &e) {
-            (void)hipStreamSynchronize(c.stream);
-            c.arena.reset();
-            if (e.code != -3 || attempt >= 3) throw;
-            bytes *= 2;
-        }
-    }
-}
-
 template <class T>
 T *to_dev(Ctx &c, const T *h, size_t n)
 {
@@ -288,7 +251,44 @@ DevGraph upload(Ctx &c, int32_t n, const int64_t *xadj, const int32_t *adjncy, c
     return g;
 }
 
+// Private contexts of the batch lanes, per device, kept for reuse (their
+// arenas keep their size between calls).
+std::vector<Ctx *> lane_contexts(int device, int count)
+{
+    static std::map<int, std::vector<std::unique_ptr<Ctx>>> pool;
+    auto &v = pool[device];
+    while ((int)v.size() < count) v.push_back(create_context(device));
+    std::vector<Ctx *> out;
+    for (int i = 0; i < count; i++) out.push_back(v[i].get());
+    return out;
+}
+
 }  // namespace
+
+size_t job_bytes(const HostJob &j) { return estimate_arena(j.n, j.n > 0 ? j.xadj[j.n] : 0, j.k, true); }
+
+void check_job(const hgp_options &o, const HostJob &j)
+{
+    Params p = params_from(&o);
+    check_params(p, j.k);
+    if (j.stats) memset(j.stats, 0, sizeof(*j.stats));
+    validate_host(j.n, j.xadj, j.adjncy, j.vwgt, j.adjwgt, o.validate != 0);
+    if (j.n > 0 && !j.part) throw Error{-1, "part is NULL"};
+}
+
+void exec_job(Ctx &c, const hgp_options &o, const HostJob &j)
+{
+    if (j.n == 0) return;
+    Params p = params_from(&o);
+    HGP_CHECK_HIP(hipSetDevice(c.device));
+    with_arena(c, job_bytes(j), [&] {
+        DevGraph g = upload(c, j.n, j.xadj, j.adjncy, j.vwgt, j.adjwgt);
+        int32_t *d_part = c.arena.lalloc<int32_t>((size_t)j.n);
+        run_pipeline(c, g, j.k, p, d_part, j.stats);
+        to_host(c, j.part, d_part, (size_t)j.n);
+    });
+}
+
 }  // namespace hgp
 
 using namespace hgp;
@@ -329,21 +329,59 @@ int hgp_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy, const i
     return guarded([&] {
         hgp_options o;
         if (opt) o = *opt; else hgp_default_options(&o);
-        Params p = params_from(&o);
-        check_params(p, k);
-        if (stats) memset(stats, 0, sizeof(*stats));
-        validate_host(n, xadj, adjncy, vwgt, adjwgt, o.validate != 0);
-        if (n == 0) return;
-        if (!part) throw Error{-1, "part is NULL"};
         std::lock_guard<std::mutex> lk(g_api_mu);
-        Ctx &c = context(o.device);
-        HGP_CHECK_HIP(hipSetDevice(c.device));
-        with_arena(c, estimate_arena(n, xadj[n], k, true), [&] {
-            DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
-            int32_t *d_part = c.arena.lalloc<int32_t>((size_t)n);
-            run_pipeline(c, g, k, p, d_part, stats);
-            to_host(c, part, d_part, (size_t)n);
-        });
+        HostJob j{n, xadj, adjncy, vwgt, adjwgt, k, part, stats};
+        check_job(o, j);
+        if (n > 0) exec_job(context(o.device), o, j);
+    });
+}
+
+int hgp_partition_batch(int32_t count, const int32_t *n, const int64_t *const *xadj, const int32_t *const *adjncy,
+                        const int32_t *const *vwgt, const int32_t *const *adjwgt, const int32_t *k,
+                        const hgp_options *opt, int32_t lanes, int32_t *const *part, int32_t *status,
+                        hgp_stats *stats)
+{
+    return guarded([&] {
+        if (count < 0) throw Error{-1, "count < 0"};
+        if (count == 0) return;
+        if (!n || !xadj || !adjncy || !k || !part) throw Error{-1, "NULL array argument"};
+        hgp_options o;
+        if (opt) o = *opt; else hgp_default_options(&o);
+        int nl = lanes > 0 ? lanes : 8;
+        nl = std::min(std::min(nl, 64), count);
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        std::vector<Ctx *> ctx = lane_contexts(o.device, nl);
+        // largest graphs first, so the lanes finish together
+        std::vector<int32_t> order((size_t)count);
+        std::iota(order.begin(), order.end(), 0);
+        auto size_of = [&](int32_t i) { return xadj[i] && n[i] >= 0 ? xadj[i][n[i]] + n[i] : 0; };
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return size_of(a) > size_of(b); });
+        std::atomic<int32_t> next{0};
+        std::vector<int32_t> st((size_t)count, 0);
+        std::vector<std::string> msg((size_t)count);
+        auto work = [&](Ctx *c) {
+            for (;;) {
+                int32_t t = next.fetch_add(1);
+                if (t >= count) break;
+                int32_t i = order[t];
+                int rc = guarded([&] {
+                    if (!xadj[i]) throw Error{-1, "xadj is NULL"};
+                    HostJob j{n[i], xadj[i], adjncy[i], vwgt ? vwgt[i] : nullptr, adjwgt ? adjwgt[i] : nullptr, k[i],
+                              part[i], stats ? stats + i : nullptr};
+                    check_job(o, j);
+                    exec_job(*c, o, j);
+                });
+                st[i] = rc;
+                if (rc != HGP_OK) msg[i] = last_error_text();
+            }
+        };
+        std::vector<std::thread> th;
+        for (int l = 1; l < nl; l++) th.emplace_back(work, ctx[l]);
+        work(ctx[0]);
+        for (auto &t : th) t.join();
+        if (status) std::copy(st.begin(), st.end(), status);
+        for (int32_t i = 0; i < count; i++)
+            if (st[i] != HGP_OK) throw Error{st[i], "graph " + std::to_string(i) + ": " + msg[i]};
     });
 }
 

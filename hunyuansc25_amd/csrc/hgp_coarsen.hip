@@ -5,6 +5,7 @@
 #include "hgp_graph.hpp"
 #include "hgp_prims.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace hgp {
@@ -345,39 +346,242 @@ __global__ void k_rep(int32_t n, const eid_t *__restrict__ xadj, const wgt_t *__
     rep[c] = v;
 }
 
-// Concatenated rows of both endpoints renamed by cmap; key = row << 32 | cc,
-// with the row's own id replaced by cn so internal entries sort last.
+// ---- segmented sort-and-merge of the coarse rows
+//
+// Coarse row c is the concatenation of the rows of its fine vertices, renamed
+// by cmap.  Each row is sorted by coarse id, entries equal to c are dropped
+// and equal ids are merged by summing weights; the merged row is written at
+// toff[c] (the concatenation's offset) of tadj/twgt and its length to ccnt[c].
+// Rows of <= 64 entries sort in registers (bitonic over xor-shuffles), rows of
+// <= kSegLds entries in LDS (bitonic, one block per row), longer rows through
+// the device radix sort.
+
+constexpr uint32_t kSent = 0xFFFFFFFFu;   // dropped entry; sorts last
+constexpr int kSegLds = 4096;
+constexpr int kSegBS = 256;
+constexpr int kSegE = kSegLds / kSegBS;   // entries per thread in the LDS merge
+
+// Entry i of coarse row c's concatenation (fine rows of v, then of u).
+struct CatRow {
+    eid_t bv, dv, bu;
+    __device__ CatRow(const eid_t *xadj, int32_t v, int32_t u)
+    {
+        bv = xadj[v];
+        dv = xadj[v + 1] - bv;
+        bu = u != v ? xadj[u] : 0;
+    }
+    __device__ eid_t at(eid_t i) const { return i < dv ? bv + i : bu + (i - dv); }
+};
+
 template <int G>
-__global__ void __launch_bounds__(kRowBS) k_gather(const int32_t *__restrict__ ids, int64_t cnt, int32_t cn,
-                                                   const int32_t *__restrict__ rep, const int32_t *__restrict__ match,
-                                                   const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
-                                                   const wgt_t *__restrict__ adjwgt, const int32_t *__restrict__ cmap,
-                                                   const int64_t *__restrict__ toff, uint64_t *__restrict__ key,
-                                                   uint32_t *__restrict__ val)
+__device__ inline void group_bitonic(uint32_t &key, int32_t &val)
+{
+    const int lane = glane<G>();
+#pragma unroll
+    for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            uint32_t ok = __shfl_xor(key, j, G);
+            int32_t ov = __shfl_xor(val, j, G);
+            bool asc = (lane & k) == 0;
+            bool lower = (lane & j) == 0;
+            bool take = (lower == asc) ? ok < key : ok > key;
+            if (take) {
+                key = ok;
+                val = ov;
+            }
+        }
+    }
+}
+
+template <int G>
+__device__ inline int32_t group_inclusive_scan(int32_t v)
+{
+    const int lane = glane<G>();
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+        int32_t t = __shfl_up(v, o, G);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Rows of <= G entries, one entry per lane.
+template <int G>
+__global__ void __launch_bounds__(kRowBS) k_seg_reg(const int32_t *__restrict__ ids, int64_t cnt,
+                                                    const int32_t *__restrict__ rep, const int32_t *__restrict__ match,
+                                                    const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                    const wgt_t *__restrict__ adjwgt, const int32_t *__restrict__ cmap,
+                                                    const int64_t *__restrict__ toff, int32_t *__restrict__ tadj,
+                                                    int32_t *__restrict__ twgt, int32_t *__restrict__ ccnt)
 {
     int64_t gi = gindex<G>();
     int lane = glane<G>();
-    if (gi >= cnt) return;
+    if (gi >= cnt) return;   // whole groups leave together
     int32_t c = ids[gi];
     int32_t v = rep[c], u = match[v];
-    int64_t o = toff[c];
-    uint64_t hi = (uint64_t)(uint32_t)c << 32;
-    eid_t b = xadj[v], e = xadj[v + 1];
-    for (eid_t j = b + lane; j < e; j += G) {
+    CatRow row(xadj, v, u);
+    eid_t len = row.dv + (u != v ? xadj[u + 1] - row.bu : 0);
+    uint32_t key = kSent;
+    int32_t w = 0;
+    if (lane < len) {
+        eid_t j = row.at(lane);
         int32_t cc = cmap[adjncy[j]];
-        key[o + (j - b)] = hi | (uint32_t)(cc == c ? cn : cc);
-        val[o + (j - b)] = (uint32_t)adjwgt[j];
-    }
-    if (u != v) {
-        o += e - b;
-        b = xadj[u];
-        e = xadj[u + 1];
-        for (eid_t j = b + lane; j < e; j += G) {
-            int32_t cc = cmap[adjncy[j]];
-            key[o + (j - b)] = hi | (uint32_t)(cc == c ? cn : cc);
-            val[o + (j - b)] = (uint32_t)adjwgt[j];
+        if (cc != c) {
+            key = (uint32_t)cc;
+            w = adjwgt[j];
         }
     }
+    group_bitonic<G>(key, w);
+    uint32_t nxt = __shfl_down(key, 1, G);
+    bool tail = key != kSent && (lane == G - 1 || nxt != key);
+    int32_t incl = group_inclusive_scan<G>(w);
+    uint64_t tm = group_ballot<G>(tail);
+    uint64_t below = tm & group_lanemask_lt<G>();
+    int32_t prev = __shfl(incl, below ? 63 - __clzll(below) : 0, G);
+    if (tail) {
+        int64_t o = toff[c] + __popcll(below);
+        tadj[o] = (int32_t)key;
+        twgt[o] = incl - (below ? prev : 0);
+    }
+    if (lane == 0) ccnt[c] = __popcll(tm);
+}
+
+// Rows of 65..kSegLds entries: one block per row, bitonic sort in LDS padded
+// to a power of two, then tails (last entry of each run of equal ids) are
+// ranked by a block scan and their inclusive weight prefixes differenced.
+__global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ ids, int64_t cnt,
+                                                    const int32_t *__restrict__ rep,
+                                                    const int32_t *__restrict__ match,
+                                                    const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                    const wgt_t *__restrict__ adjwgt,
+                                                    const int32_t *__restrict__ cmap, const int64_t *__restrict__ toff,
+                                                    int32_t *__restrict__ tadj, int32_t *__restrict__ twgt,
+                                                    int32_t *__restrict__ ccnt)
+{
+    __shared__ uint32_t sk[kSegLds];
+    __shared__ int32_t sw[kSegLds];
+    __shared__ int64_t sh[kSegBS / kWave + 1];
+    const int tid = threadIdx.x;
+    for (int64_t gi = blockIdx.x; gi < cnt; gi += gridDim.x) {
+        int32_t c = ids[gi];
+        int32_t v = rep[c], u = match[v];
+        CatRow row(xadj, v, u);
+        int len = (int)(row.dv + (u != v ? xadj[u + 1] - row.bu : 0));
+        int P = 64;
+        while (P < len) P <<= 1;
+        for (int i = tid; i < P; i += kSegBS) {
+            uint32_t key = kSent;
+            int32_t w = 0;
+            if (i < len) {
+                eid_t j = row.at(i);
+                int32_t cc = cmap[adjncy[j]];
+                if (cc != c) {
+                    key = (uint32_t)cc;
+                    w = adjwgt[j];
+                }
+            }
+            sk[i] = key;
+            sw[i] = w;
+        }
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int t = tid; t < P / 2; t += kSegBS) {
+                    int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
+                    int l = i | j;
+                    uint32_t a = sk[i], b = sk[l];
+                    bool asc = (i & k) == 0;
+                    if (asc ? a > b : a < b) {
+                        sk[i] = b;
+                        sk[l] = a;
+                        int32_t x = sw[i];
+                        sw[i] = sw[l];
+                        sw[l] = x;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // Thread tid owns entries [tid*E, tid*E + E), E = P / kSegBS (one each when P < kSegBS).
+        const int E = P >= kSegBS ? P / kSegBS : 1;
+        const int first = tid * E;
+        uint32_t kk[kSegE];
+        int32_t ww[kSegE];
+        uint32_t after = first + E < P ? sk[first + E] : kSent;
+#pragma unroll
+        for (int e = 0; e < kSegE; e++) {
+            int i = first + e;
+            bool own = e < E && i < P;
+            kk[e] = own ? sk[i] : kSent;
+            ww[e] = own ? sw[i] : 0;
+        }
+        int32_t nt = 0, wsum = 0;
+#pragma unroll
+        for (int e = 0; e < kSegE; e++) {
+            uint32_t nx = e + 1 < E ? kk[e + 1 < kSegE ? e + 1 : e] : after;
+            if (e < E && kk[e] != kSent && nx != kk[e]) nt++;
+            wsum += ww[e];
+        }
+        int64_t tot;
+        int64_t ex = block_exclusive_scan(((int64_t)nt << 32) + wsum, sh, &tot);   // weight sums < 2^31
+        int32_t rank = (int32_t)(ex >> 32), run = (int32_t)(ex & 0xFFFFFFFFll);
+        const int32_t ntot = (int32_t)(tot >> 32);
+        __syncthreads();   // every entry has been read: sk/sw now hold the merged row
+#pragma unroll
+        for (int e = 0; e < kSegE; e++) {
+            uint32_t nx = e + 1 < E ? kk[e + 1 < kSegE ? e + 1 : e] : after;
+            run += ww[e];
+            if (e < E && kk[e] != kSent && nx != kk[e]) {
+                sk[rank] = kk[e];
+                sw[rank] = run;
+                rank++;
+            }
+        }
+        __syncthreads();
+        const int64_t o = toff[c];
+        for (int r = tid; r < ntot; r += kSegBS) {
+            tadj[o + r] = (int32_t)sk[r];
+            twgt[o + r] = sw[r] - (r > 0 ? sw[r - 1] : 0);
+        }
+        if (tid == 0) ccnt[c] = ntot;
+        __syncthreads();
+    }
+}
+
+// Rows longer than kSegLds: keys (list index << 32 | coarse id) of all such
+// rows, laid out at loff, go through the device radix sort.
+__global__ void __launch_bounds__(kRowBS) k_long_gather(const int32_t *__restrict__ ids, int64_t cnt, int32_t cn,
+                                                        const int32_t *__restrict__ rep,
+                                                        const int32_t *__restrict__ match,
+                                                        const eid_t *__restrict__ xadj,
+                                                        const vid_t *__restrict__ adjncy,
+                                                        const wgt_t *__restrict__ adjwgt,
+                                                        const int32_t *__restrict__ cmap,
+                                                        const int64_t *__restrict__ loff, uint64_t *__restrict__ key,
+                                                        uint32_t *__restrict__ val)
+{
+    for (int64_t li = blockIdx.x; li < cnt; li += gridDim.x) {
+        int32_t c = ids[li];
+        int32_t v = rep[c], u = match[v];
+        CatRow row(xadj, v, u);
+        eid_t len = row.dv + (u != v ? xadj[u + 1] - row.bu : 0);
+        uint64_t hi = (uint64_t)li << 32;
+        int64_t o = loff[li];
+        for (eid_t i = threadIdx.x; i < len; i += blockDim.x) {
+            eid_t j = row.at(i);
+            int32_t cc = cmap[adjncy[j]];
+            key[o + i] = hi | (uint32_t)(cc == c ? cn : cc);
+            val[o + i] = (uint32_t)adjwgt[j];
+        }
+    }
+}
+
+__global__ void k_long_len(int64_t cnt, const int32_t *__restrict__ ids, const int64_t *__restrict__ len,
+                           int64_t *__restrict__ llen)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) llen[i] = len[ids[i]];
 }
 
 __global__ void k_heads(int64_t m, int32_t cn, const uint64_t *__restrict__ key, int32_t *__restrict__ h)
@@ -397,30 +601,58 @@ __global__ void k_run_start(int64_t m, const int32_t *__restrict__ hsc, int32_t 
     if (hsc[i] != prev) start[hsc[i] - 1] = (int32_t)i;
 }
 
-__global__ void k_run_write(int64_t m, int32_t cn, const uint64_t *__restrict__ key, const int32_t *__restrict__ hsc,
-                            const int32_t *__restrict__ wsc, const int32_t *__restrict__ start,
-                            vid_t *__restrict__ cadjncy, wgt_t *__restrict__ cadjwgt)
+// Last entry of each run writes the merged entry at its row's toff.
+__global__ void k_long_write(int64_t tl, int32_t cn, const int32_t *__restrict__ ids, const uint64_t *__restrict__ key,
+                             const int32_t *__restrict__ hsc, const int32_t *__restrict__ wsc,
+                             const int32_t *__restrict__ start, const int64_t *__restrict__ loff,
+                             const int64_t *__restrict__ toff, int32_t *__restrict__ tadj,
+                             int32_t *__restrict__ twgt)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+    if (i >= tl) return;
     uint64_t k = key[i];
     uint32_t cc = (uint32_t)k;
     if (cc == (uint32_t)cn) return;
-    if (i + 1 < m && key[i + 1] == k) return;          // not the last of its run
+    if (i + 1 < tl && key[i + 1] == k) return;
+    int64_t li = (int64_t)(k >> 32);
+    int64_t b = loff[li];
+    int32_t before = b > 0 ? hsc[b - 1] : 0;
     int32_t r = hsc[i] - 1;
     int32_t s = start[r];
-    cadjncy[r] = (vid_t)cc;
-    cadjwgt[r] = wsc[i] - (s > 0 ? wsc[s - 1] : 0);
+    int64_t o = toff[ids[li]] + (r - before);
+    tadj[o] = (int32_t)cc;
+    twgt[o] = wsc[i] - (s > 0 ? wsc[s - 1] : 0);
 }
 
-__global__ void k_cxadj(int32_t cn, int64_t cm, const int64_t *__restrict__ toff, const int32_t *__restrict__ hsc,
-                        eid_t *__restrict__ cxadj)
+__global__ void k_long_count(int64_t cnt, const int32_t *__restrict__ ids, const int64_t *__restrict__ loff,
+                             const int32_t *__restrict__ hsc, int32_t *__restrict__ ccnt)
 {
-    int32_t c = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (c > cn) return;
-    if (c == cn) { cxadj[c] = cm; return; }
-    int64_t t = toff[c];
-    cxadj[c] = t > 0 ? hsc[t - 1] : 0;
+    int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= cnt) return;
+    int64_t b = loff[li], e = loff[li + 1];
+    int32_t before = b > 0 ? hsc[b - 1] : 0;
+    ccnt[ids[li]] = (e > b ? hsc[e - 1] : before) - before;
+}
+
+// Moves each merged row from toff[c] to cxadj[c].
+template <int G>
+__global__ void __launch_bounds__(kRowBS) k_seg_compact(const int32_t *__restrict__ ids, int64_t cnt,
+                                                        const int64_t *__restrict__ toff,
+                                                        const eid_t *__restrict__ cxadj,
+                                                        const int32_t *__restrict__ tadj,
+                                                        const int32_t *__restrict__ twgt, vid_t *__restrict__ cadjncy,
+                                                        wgt_t *__restrict__ cadjwgt)
+{
+    int64_t gi = gindex<G>();
+    int lane = glane<G>();
+    if (gi >= cnt) return;
+    int32_t c = ids[gi];
+    int64_t src = toff[c];
+    eid_t dst = cxadj[c], l = cxadj[c + 1] - dst;
+    for (eid_t j = lane; j < l; j += G) {
+        cadjncy[dst + j] = tadj[src + j];
+        cadjwgt[dst + j] = twgt[src + j];
+    }
 }
 
 int64_t count_matched(Ctx &c, int32_t n, const int32_t *match, int64_t *d_cnt)
@@ -609,48 +841,95 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     unsigned gv = grid_for(g.n, kBS);
     k_rep<<<gv, kBS, 0, c.stream>>>(g.n, g.xadj, g.vwgt, match, cmap, len, cg.vwgt, rep);
     HGP_LAUNCH_CHECK();
-    offsets_from_counts(c, len, toff, cn);
-    int64_t cm = 0;
-    if (m > 0) {
-        uint64_t *key = c.arena.ralloc<uint64_t>((size_t)m);
-        uint32_t *val = c.arena.ralloc<uint32_t>((size_t)m);
-        {
-            // per fine entry: adjncy, adjwgt, cmap gather, key+val write; per coarse vertex: rep, match, toff, 2 xadj
-            RowBins cb = row_bins(c, cn, nullptr, len, false);
-            KScope ks(c, "contract_gather", 24.0 * (double)m + 40.0 * cn);
-            for_bins(cb, [&](auto gc, const int32_t *ids, int64_t nb) {
-                constexpr int G = decltype(gc)::value;
-                k_gather<G><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, cn, rep, match, g.xadj, g.adjncy,
-                                                                         g.adjwgt, cmap, toff, key, val);
-                HGP_LAUNCH_CHECK();
-            });
-        }
-        int bc = bits_for((uint64_t)cn);
-        int br = bits_for((uint64_t)(cn > 0 ? cn - 1 : 0));
-        radix_sort(c, key, val, m, 0, bc);
-        if (br > 0) radix_sort(c, key, val, m, 32, 32 + br);
-        int32_t *hsc = c.arena.ralloc<int32_t>((size_t)m);
-        int32_t *wsc = c.arena.ralloc<int32_t>((size_t)m);
-        int32_t *start = c.arena.ralloc<int32_t>((size_t)m);
-        unsigned gm = grid_for(m, kBS);
-        k_heads<<<gm, kBS, 0, c.stream>>>(m, cn, key, hsc);
-        HGP_LAUNCH_CHECK();
-        scan_inclusive(c, hsc, hsc, m);
-        scan_inclusive(c, (const int32_t *)val, wsc, m);
-        k_run_start<<<gm, kBS, 0, c.stream>>>(m, hsc, start);
-        HGP_LAUNCH_CHECK();
-        read_scalars(c, hsc + m - 1, sizeof(int32_t));
-        cm = ((int32_t *)c.pinned)[0];
-        cg.adjncy = c.arena.lalloc<vid_t>((size_t)cm);
-        cg.adjwgt = c.arena.lalloc<wgt_t>((size_t)cm);
-        k_run_write<<<gm, kBS, 0, c.stream>>>(m, cn, key, hsc, wsc, start, cg.adjncy, cg.adjwgt);
-        HGP_LAUNCH_CHECK();
-        k_cxadj<<<grid_for((int64_t)cn + 1, kBS), kBS, 0, c.stream>>>(cn, cm, toff, hsc, cg.xadj);
-        HGP_LAUNCH_CHECK();
-    } else {
+    if (m <= 0) {
         cg.adjncy = c.arena.lalloc<vid_t>(1);
         cg.adjwgt = c.arena.lalloc<wgt_t>(1);
         fill<int64_t>(c, cg.xadj, (int64_t)cn + 1, 0);
+        cg.m = 0;
+        c.arena.right_release(mark);
+        return cg;
+    }
+    offsets_from_counts(c, len, toff, cn);   // the concatenations fill [0, m) exactly
+    int32_t *tadj = c.arena.ralloc<int32_t>((size_t)m);
+    int32_t *twgt = c.arena.ralloc<int32_t>((size_t)m);
+    int32_t *ccnt = c.arena.ralloc<int32_t>((size_t)cn);
+    RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegLds);
+    {
+        // per fine entry: adjncy, adjwgt, cmap gather, merged write (upper bound); per coarse row: rep, match, 2 xadj, toff, count
+        KScope ks(c, "contract_sort", 20.0 * (double)m + 36.0 * cn);
+        if (cb.count(0)) {
+            k_seg_reg<8><<<grid_groups(cb.count(0), 8), kRowBS, 0, c.stream>>>(
+                cb.list(0), cb.count(0), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt);
+            HGP_LAUNCH_CHECK();
+        }
+        if (cb.count(1)) {
+            k_seg_reg<64><<<grid_groups(cb.count(1), 64), kRowBS, 0, c.stream>>>(
+                cb.list(1), cb.count(1), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt);
+            HGP_LAUNCH_CHECK();
+        }
+        if (cb.count(2)) {
+            unsigned blocks = (unsigned)std::min<int64_t>(cb.count(2), 8192);
+            k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(2), cb.count(2), rep, match, g.xadj, g.adjncy,
+                                                       g.adjwgt, cmap, toff, tadj, twgt, ccnt);
+            HGP_LAUNCH_CHECK();
+        }
+    }
+    if (int64_t nl = cb.count(3)) {
+        size_t m2 = c.arena.right_mark();
+        const int32_t *lids = cb.list(3);
+        int64_t *llen = c.arena.ralloc<int64_t>((size_t)nl);
+        int64_t *loff = c.arena.ralloc<int64_t>((size_t)nl + 1);
+        k_long_len<<<grid_for(nl, kBS), kBS, 0, c.stream>>>(nl, lids, len, llen);
+        HGP_LAUNCH_CHECK();
+        offsets_from_counts(c, llen, loff, nl);
+        read_scalars(c, loff + nl, sizeof(int64_t));
+        int64_t tl = c.pinned[0];
+        uint64_t *key = c.arena.ralloc<uint64_t>((size_t)tl);
+        uint32_t *val = c.arena.ralloc<uint32_t>((size_t)tl);
+        int32_t *hsc = c.arena.ralloc<int32_t>((size_t)tl);
+        int32_t *wsc = c.arena.ralloc<int32_t>((size_t)tl);
+        int32_t *start = c.arena.ralloc<int32_t>((size_t)tl);
+        {
+            KScope ks(c, "contract_long_gather", 20.0 * (double)tl + 36.0 * nl);
+            k_long_gather<<<(unsigned)std::min<int64_t>(nl, 8192), kRowBS, 0, c.stream>>>(
+                lids, nl, cn, rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, loff, key, val);
+            HGP_LAUNCH_CHECK();
+        }
+        radix_sort(c, key, val, tl, 0, bits_for((uint64_t)cn));
+        if (nl > 1) radix_sort(c, key, val, tl, 32, 32 + bits_for((uint64_t)(nl - 1)));
+        unsigned gt = grid_for(tl, kBS);
+        k_heads<<<gt, kBS, 0, c.stream>>>(tl, cn, key, hsc);
+        HGP_LAUNCH_CHECK();
+        scan_inclusive(c, hsc, hsc, tl);
+        scan_inclusive(c, (const int32_t *)val, wsc, tl);
+        k_run_start<<<gt, kBS, 0, c.stream>>>(tl, hsc, start);
+        HGP_LAUNCH_CHECK();
+        k_long_write<<<gt, kBS, 0, c.stream>>>(tl, cn, lids, key, hsc, wsc, start, loff, toff, tadj, twgt);
+        HGP_LAUNCH_CHECK();
+        k_long_count<<<grid_for(nl, kBS), kBS, 0, c.stream>>>(nl, lids, loff, hsc, ccnt);
+        HGP_LAUNCH_CHECK();
+        c.arena.right_release(m2);
+    }
+    offsets_from_counts(c, ccnt, cg.xadj, cn);
+    read_scalars(c, cg.xadj + cn, sizeof(int64_t));
+    int64_t cm = c.pinned[0];
+    cg.adjncy = c.arena.lalloc<vid_t>((size_t)(cm > 0 ? cm : 1));
+    cg.adjwgt = c.arena.lalloc<wgt_t>((size_t)(cm > 0 ? cm : 1));
+    {
+        KScope ks(c, "contract_compact", 16.0 * (double)cm + 20.0 * cn);
+        for (int cl = 0; cl < 4; cl++) {
+            if (!cb.count(cl)) continue;
+            if (cl == 0)
+                k_seg_compact<8><<<grid_groups(cb.count(cl), 8), kRowBS, 0, c.stream>>>(
+                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt);
+            else if (cl == 1)
+                k_seg_compact<64><<<grid_groups(cb.count(cl), 64), kRowBS, 0, c.stream>>>(
+                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt);
+            else
+                k_seg_compact<256><<<grid_groups(cb.count(cl), 256), kRowBS, 0, c.stream>>>(
+                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt);
+            HGP_LAUNCH_CHECK();
+        }
     }
     cg.m = cm;
     c.arena.right_release(mark);

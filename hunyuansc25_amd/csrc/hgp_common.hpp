@@ -6,6 +6,8 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -57,6 +59,7 @@ class Arena {
     void left_release(size_t mark);
     void right_release(size_t mark);
     void reset();                        // free everything (keeps the block)
+    void free_block();                   // return the block to HIP (nothing may be live)
     size_t capacity() const { return cap_; }
     size_t high_water() const { return peak_; }
 
@@ -81,7 +84,9 @@ struct Ctx {
 constexpr size_t kPinnedBytes = 1 << 20;
 constexpr size_t kStageOffset = 64 << 10;   // staging area starts here
 
-Ctx &context(int device);         // per-device default context (lazily created)
+Ctx &context(int device);                        // per-device default context (lazily created)
+std::unique_ptr<Ctx> create_context(int device);   // a private context (own stream, arena, staging)
+void destroy_context(std::unique_ptr<Ctx> c);      // releases a private context's resources
 
 // ------------------------------------------------------------------ kernel timer
 // Optional per-kernel timing with HIP events on the context's stream: a
@@ -95,6 +100,7 @@ struct KRec {
 
 struct KTimer {
     bool on = false;
+    std::mutex mu;               // scopes may close on several host threads
     std::vector<KRec> recs;
     std::vector<hipEvent_t> pool;
     size_t used = 0;

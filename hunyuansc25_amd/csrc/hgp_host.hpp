@@ -1,0 +1,75 @@
+// hgp_host.hpp -- host-side job plumbing shared by the C entry points: error
+// capture, arena sizing and the validated host-array job.
+#pragma once
+
+#include "hunyuangraph.h"
+
+#include "hgp_common.hpp"
+
+#include <new>
+#include <stdexcept>
+
+namespace hgp {
+
+const char *last_error_text();
+
+// One graph given as host arrays (the arguments of hgp_partition).
+struct HostJob {
+    int32_t n;
+    const int64_t *xadj;
+    const int32_t *adjncy, *vwgt, *adjwgt;
+    int32_t k;
+    int32_t *part;
+    hgp_stats *stats;
+};
+
+// Throws Error{-1/-4} for invalid options or CSR; touches no device.
+void check_job(const hgp_options &o, const HostJob &j);
+// Device arena bytes the job is run with first.
+size_t job_bytes(const HostJob &j);
+// Upload, partition, download on context c (after check_job).
+void exec_job(Ctx &c, const hgp_options &o, const HostJob &j);
+
+// Runs f, turning exceptions into a status code and the thread's error text.
+template <class F>
+int guarded(F f)
+{
+    clear_error();
+    try {
+        f();
+        return HGP_OK;
+    } catch (const Error &e) {
+        set_error(e.code, e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error(-3, "host out of memory");
+        return HGP_ERR_OOM;
+    } catch (const std::exception &e) {
+        set_error(-2, e.what());
+        return HGP_ERR_DEVICE;
+    }
+}
+
+// Runs body with an arena of at least `bytes`; on arena overflow the arena
+// grows and the body is re-run (the body must be restartable).
+template <class F>
+void with_arena(Ctx &c, size_t bytes, F body)
+{
+    for (int attempt = 0;; attempt++) {
+        c.arena.reset();
+        c.arena.reserve(bytes);
+        try {
+            body();
+            HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+            c.arena.reset();
+            return;
+        } catch (const Error &e) {
+            (void)hipStreamSynchronize(c.stream);
+            c.arena.reset();
+            if (e.code != -3 || attempt >= 3) throw;
+            bytes *= 2;
+        }
+    }
+}
+
+}  // namespace hgp

@@ -248,62 +248,71 @@ __global__ void k_compact(const uint8_t *__restrict__ f, const int32_t *__restri
 
 // ------------------------------------------------------------ degree classes
 
-__device__ inline int row_class(int64_t d) { return d <= kBinSmall ? 0 : (d <= kBinMid ? 1 : 2); }
+struct BinThr {
+    int64_t t0, t1, t2;
+    __device__ int cls(int64_t d) const { return d <= t0 ? 0 : (d <= t1 ? 1 : (d <= t2 ? 2 : 3)); }
+};
 
-__global__ void k_bin_count(int32_t n, const eid_t *__restrict__ xadj, const int64_t *__restrict__ len,
+__global__ void k_bin_count(int32_t n, const eid_t *__restrict__ xadj, const int64_t *__restrict__ len, BinThr t,
                             unsigned long long *cnt)
 {
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     bool valid = v < n;
     int cls = -1;
-    if (valid) cls = row_class(xadj ? xadj[v + 1] - xadj[v] : len[v]);
+    if (valid) cls = t.cls(xadj ? xadj[v + 1] - xadj[v] : len[v]);
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < 4; c++) {
         uint64_t b = __ballot(cls == c);
         if (lane_id() == 0 && b) atomicAdd(&cnt[c], (unsigned long long)__popcll(b));
     }
 }
 
-__global__ void k_bin_fill(int32_t n, const eid_t *__restrict__ xadj, const int64_t *__restrict__ len,
-                           int64_t o0, int64_t o1, int64_t o2, unsigned long long *cursor, int32_t *__restrict__ ids)
+struct BinBase {
+    int64_t o[4];
+};
+
+__global__ void k_bin_fill(int32_t n, const eid_t *__restrict__ xadj, const int64_t *__restrict__ len, BinThr t,
+                           BinBase base, unsigned long long *cursor, int32_t *__restrict__ ids)
 {
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     bool valid = v < n;
     int cls = -1;
-    if (valid) cls = row_class(xadj ? xadj[v + 1] - xadj[v] : len[v]);
-    const int64_t base[3] = {o0, o1, o2};
+    if (valid) cls = t.cls(xadj ? xadj[v + 1] - xadj[v] : len[v]);
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < 4; c++) {
         uint64_t b = __ballot(cls == c);
         if (!b) continue;
         int leader = __ffsll((long long)b) - 1;
         unsigned long long slot = 0;
         if (lane_id() == leader) slot = atomicAdd(&cursor[c], (unsigned long long)__popcll(b));
         slot = __shfl(slot, leader, kWave);
-        if (cls == c) ids[base[c] + (int64_t)slot + __popcll(b & lanemask_lt())] = v;
+        if (cls == c) ids[base.o[c] + (int64_t)slot + __popcll(b & lanemask_lt())] = v;
     }
 }
 
 }  // namespace
 
-RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool left)
+RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool left, int64_t t0, int64_t t1,
+                 int64_t t2)
 {
     RowBins b;
     b.ids = left ? c.arena.lalloc<int32_t>((size_t)n) : c.arena.ralloc<int32_t>((size_t)n);
     if (n <= 0) return b;
     size_t mark = c.arena.right_mark();
-    unsigned long long *cnt = c.arena.ralloc<unsigned long long>(6);
-    fill<uint32_t>(c, (uint32_t *)cnt, 12, 0u);
+    unsigned long long *cnt = c.arena.ralloc<unsigned long long>(8);
+    fill<uint32_t>(c, (uint32_t *)cnt, 16, 0u);
     unsigned g = grid_for(n, 256);
-    k_bin_count<<<g, 256, 0, c.stream>>>(n, xadj, len, cnt);
+    BinThr thr{t0, t1, t2};
+    k_bin_count<<<g, 256, 0, c.stream>>>(n, xadj, len, thr, cnt);
     HGP_LAUNCH_CHECK();
-    read_scalars(c, cnt, 3 * sizeof(int64_t));
-    int64_t k0 = c.pinned[0], k1 = c.pinned[1], k2 = c.pinned[2];
+    read_scalars(c, cnt, 4 * sizeof(int64_t));
+    BinBase base;
     b.off[0] = 0;
-    b.off[1] = k0;
-    b.off[2] = k0 + k1;
-    b.off[3] = k0 + k1 + k2;
-    k_bin_fill<<<g, 256, 0, c.stream>>>(n, xadj, len, b.off[0], b.off[1], b.off[2], cnt + 3, b.ids);
+    for (int i = 0; i < 4; i++) {
+        base.o[i] = b.off[i];
+        b.off[i + 1] = b.off[i] + c.pinned[i];
+    }
+    k_bin_fill<<<g, 256, 0, c.stream>>>(n, xadj, len, thr, base, cnt + 4, b.ids);
     HGP_LAUNCH_CHECK();
     c.arena.right_release(mark);
     return b;

@@ -83,6 +83,14 @@ void *Arena::right(size_t bytes)
 void Arena::left_release(size_t mark) { lo_ = mark; }
 void Arena::right_release(size_t mark) { hi_ = mark; }
 
+void Arena::free_block()
+{
+    if (lo_ != 0 || hi_ != cap_) throw Error{-3, "arena: free_block while allocations are live"};
+    if (base_) HGP_CHECK_HIP(hipFree(base_));
+    base_ = nullptr;
+    cap_ = lo_ = hi_ = 0;
+}
+
 void Arena::reset()
 {
     lo_ = 0;
@@ -91,11 +99,8 @@ void Arena::reset()
 
 // ------------------------------------------------------------------ context
 
-Ctx &context(int device)
+std::unique_ptr<Ctx> create_context(int device)
 {
-    std::lock_guard<std::mutex> g(g_ctx_mu);
-    auto it = g_ctx.find(device);
-    if (it != g_ctx.end()) return *it->second;
     int cnt = 0;
     if (hipGetDeviceCount(&cnt) != hipSuccess || cnt == 0) {
         (void)hipGetLastError();
@@ -108,6 +113,27 @@ Ctx &context(int device)
     HGP_CHECK_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HGP_CHECK_HIP(hipHostMalloc((void **)&c->pinned, kPinnedBytes, hipHostMallocDefault));
     for (auto &e : c->ev) HGP_CHECK_HIP(hipEventCreate(&e));
+    return c;
+}
+
+void destroy_context(std::unique_ptr<Ctx> c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->arena.reset();
+    c->arena.free_block();
+    for (auto &e : c->ev) (void)hipEventDestroy(e);
+    (void)hipHostFree(c->pinned);
+    (void)hipStreamDestroy(c->stream);
+}
+
+Ctx &context(int device)
+{
+    std::lock_guard<std::mutex> g(g_ctx_mu);
+    auto it = g_ctx.find(device);
+    if (it != g_ctx.end()) return *it->second;
+    auto c = create_context(device);
     Ctx &ref = *c;
     g_ctx[device] = std::move(c);
     return ref;
@@ -121,6 +147,7 @@ KTimer &ktimer()
 
 hipEvent_t KTimer::get()
 {
+    std::lock_guard<std::mutex> g(mu);
     if (used == pool.size()) {
         hipEvent_t e;
         HGP_CHECK_HIP(hipEventCreate(&e));
@@ -141,6 +168,7 @@ KScope::~KScope()
 {
     if (!on_) return;
     if (hipEventRecord(b_, c_.stream) != hipSuccess) return;
+    std::lock_guard<std::mutex> g(ktimer().mu);
     ktimer().recs.push_back({name_, a_, b_, bytes_});
 }
 

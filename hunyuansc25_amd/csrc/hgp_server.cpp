@@ -1,0 +1,193 @@
+// hgp_server.cpp -- the serving loop: a FIFO of partition jobs admitted by
+// free slot and device-memory budget, run concurrently on per-slot streams.
+#include "hgp_host.hpp"
+
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <thread>
+
+using Clock = std::chrono::steady_clock;
+
+struct hgp_server {
+    struct Job {
+        hgp::HostJob j;
+        hgp_options o;
+        hgp_stats stats;
+        size_t bytes = 0;
+        int status = 0;
+        bool done = false;
+        std::string msg;
+        Clock::time_point t_submit, t_done;
+    };
+
+    int device = 0;
+    hgp_options base;
+    size_t budget = 0;
+    std::vector<std::unique_ptr<hgp::Ctx>> ctx;
+    std::vector<char> busy;
+    std::deque<int64_t> queue;
+    std::map<int64_t, std::unique_ptr<Job>> jobs;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    bool stop = false;
+    int64_t next_ticket = 0, completed = 0;
+    size_t peak = 0;
+    std::vector<std::thread> workers;
+
+    size_t reserved() const
+    {
+        size_t r = 0;
+        for (auto &c : ctx) r += c->arena.capacity();
+        return r;
+    }
+
+    // Whether slot w may start a job of `need` bytes; may release the arenas
+    // of idle slots (and its own) to make room.  Called with mu held.
+    bool admit(int w, size_t need)
+    {
+        auto others = [&] {
+            size_t r = 0;
+            for (size_t i = 0; i < ctx.size(); i++)
+                if ((int)i != w) r += ctx[i]->arena.capacity();
+            return r;
+        };
+        size_t own = ctx[w]->arena.capacity();
+        if (others() + std::max(own, need) <= budget) return true;
+        for (size_t i = 0; i < ctx.size(); i++)
+            if ((int)i != w && !busy[i] && ctx[i]->arena.capacity() > 0) ctx[i]->arena.free_block();
+        if (others() + std::max(own, need) <= budget) return true;
+        if (own > need && others() + need <= budget) {
+            ctx[w]->arena.free_block();
+            return true;
+        }
+        return false;
+    }
+
+    void worker(int w)
+    {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            cv_work.wait(lk, [&] {
+                return (stop && queue.empty()) || (!queue.empty() && admit(w, jobs[queue.front()]->bytes));
+            });
+            if (queue.empty()) break;   // stopping and drained
+            int64_t t = queue.front();
+            queue.pop_front();
+            Job *job = jobs[t].get();
+            busy[w] = 1;
+            lk.unlock();
+            int rc = hgp::guarded([&] { hgp::exec_job(*ctx[w], job->o, job->j); });
+            std::string msg = rc ? hgp::last_error_text() : "";
+            lk.lock();
+            job->status = rc;
+            job->msg = msg;
+            job->t_done = Clock::now();
+            job->done = true;
+            completed++;
+            busy[w] = 0;
+            peak = std::max(peak, reserved());
+            if (reserved() > budget) ctx[w]->arena.free_block();   // an arena retry grew past the budget
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+int hgp_server_create(const hgp_options *opt, int32_t slots, int64_t budget_bytes, hgp_server **out)
+{
+    return hgp::guarded([&] {
+        if (!out) throw hgp::Error{-1, "out is NULL"};
+        *out = nullptr;
+        if (slots < 1 || slots > 64) throw hgp::Error{-1, "slots must be 1..64"};
+        if (budget_bytes <= 0) throw hgp::Error{-1, "budget must be positive"};
+        auto s = std::make_unique<hgp_server>();
+        if (opt) s->base = *opt; else hgp_default_options(&s->base);
+        s->device = s->base.device;
+        s->budget = (size_t)budget_bytes;
+        for (int i = 0; i < slots; i++) s->ctx.push_back(hgp::create_context(s->device));
+        s->busy.assign((size_t)slots, 0);
+        for (int i = 0; i < slots; i++) s->workers.emplace_back([p = s.get(), i] {
+            (void)hipSetDevice(p->device);
+            p->worker(i);
+        });
+        *out = s.release();
+    });
+}
+
+int64_t hgp_server_submit(hgp_server *s, int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                          const int32_t *adjwgt, int32_t k, uint64_t seed, int32_t *part)
+{
+    int64_t ticket = -1;
+    int rc = hgp::guarded([&] {
+        if (!s) throw hgp::Error{-1, "server is NULL"};
+        auto job = std::make_unique<hgp_server::Job>();
+        job->o = s->base;
+        job->o.seed = seed;
+        job->j = hgp::HostJob{n, xadj, adjncy, vwgt, adjwgt, k, part, &job->stats};
+        hgp::check_job(job->o, job->j);
+        job->bytes = hgp::job_bytes(job->j);
+        if (n > 0 && job->bytes > s->budget) throw hgp::Error{-3, "job needs more device memory than the budget"};
+        job->t_submit = Clock::now();
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) throw hgp::Error{-1, "server is stopping"};
+        ticket = s->next_ticket++;
+        s->jobs[ticket] = std::move(job);
+        s->queue.push_back(ticket);
+        s->cv_work.notify_all();
+    });
+    return rc == HGP_OK ? ticket : rc;
+}
+
+int hgp_server_wait(hgp_server *s, int64_t ticket, hgp_stats *stats, double *latency_ms)
+{
+    int status = HGP_OK;
+    std::string msg;
+    int rc = hgp::guarded([&] {
+        if (!s) throw hgp::Error{-1, "server is NULL"};
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(ticket);
+        if (it == s->jobs.end()) throw hgp::Error{-1, "unknown ticket"};
+        hgp_server::Job *job = it->second.get();
+        s->cv_done.wait(lk, [&] { return job->done; });
+        if (stats) *stats = job->stats;
+        if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(job->t_done - job->t_submit).count();
+        status = job->status;
+        msg = job->msg;
+        s->jobs.erase(it);
+    });
+    if (rc != HGP_OK) return rc;
+    if (status != HGP_OK) hgp::set_error(status, msg);
+    return status;
+}
+
+int hgp_server_info(hgp_server *s, int64_t *reserved, int64_t *peak_reserved, int64_t *completed)
+{
+    return hgp::guarded([&] {
+        if (!s) throw hgp::Error{-1, "server is NULL"};
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (reserved) *reserved = (int64_t)s->reserved();
+        if (peak_reserved) *peak_reserved = (int64_t)std::max(s->peak, s->reserved());
+        if (completed) *completed = s->completed;
+    });
+}
+
+int hgp_server_destroy(hgp_server *s)
+{
+    return hgp::guarded([&] {
+        if (!s) return;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+            s->cv_work.notify_all();
+        }
+        for (auto &t : s->workers) t.join();
+        for (auto &c : s->ctx) hgp::destroy_context(std::move(c));
+        delete s;
+    });
+}
+
+}  // extern "C"

@@ -98,6 +98,51 @@ int hgp_partition_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjn
                          const int32_t *d_vwgt, const int32_t *d_adjwgt, int32_t k,
                          const hgp_options *opt, int32_t *d_part, hgp_stats *stats);
 
+/* ------------------------------------------------------------ batches
+ * No counterpart in the reference, which partitions one graph per call.
+ * Partitions `count` independent graphs: graph i is the CSR n[i], xadj[i]
+ * (int64, n[i]+1 entries), adjncy[i], vwgt[i], adjwgt[i] with k[i] parts, and
+ * part[i] (n[i] entries) receives its partition.  vwgt/adjwgt may be NULL, or
+ * hold NULL entries, for unit weights.  The graphs run concurrently on
+ * `lanes` streams of opt->device (<= 0: 8), each lane with its own arena and
+ * staging buffers; part[i] equals what hgp_partition returns for graph i
+ * alone with the same options.  status[i] (optional) receives each graph's
+ * status and stats[i] (optional) its statistics.  Returns HGP_OK when every
+ * graph succeeded, else the status of the lowest-index failing graph, which
+ * hgp_last_error() describes. */
+int hgp_partition_batch(int32_t count, const int32_t *n, const int64_t *const *xadj,
+                        const int32_t *const *adjncy, const int32_t *const *vwgt,
+                        const int32_t *const *adjwgt, const int32_t *k, const hgp_options *opt,
+                        int32_t lanes, int32_t *const *part, int32_t *status, hgp_stats *stats);
+
+/* ------------------------------------------------------------ serving
+ * No counterpart in the reference.  A request loop on one device: up to
+ * `slots` jobs run at once, each on its own stream and arena.  Jobs are
+ * admitted first come, first served, when a slot is free and the arenas of
+ * all slots fit in `budget_bytes` of device memory (arenas of idle slots are
+ * released when a new job needs the room).  Jobs that arrive while others run
+ * overlap with them on the device, and each job's part[] equals what
+ * hgp_partition returns for the same graph and options. */
+typedef struct hgp_server hgp_server;
+
+/* opt (may be NULL) gives the device and the defaults of every job. */
+int hgp_server_create(const hgp_options *opt, int32_t slots, int64_t budget_bytes, hgp_server **out);
+/* Queues a job (validated now).  The caller keeps every array alive until
+ * hgp_server_wait returns for it.  Returns a ticket >= 0 or a negative status;
+ * a job whose arena alone exceeds the budget is refused with HGP_ERR_OOM. */
+int64_t hgp_server_submit(hgp_server *s, int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                          const int32_t *vwgt, const int32_t *adjwgt, int32_t k, uint64_t seed,
+                          int32_t *part);
+/* Blocks until the job is done and returns its status; stats and latency_ms
+ * (submit to result, host clock) are optional outputs.  A ticket is waited
+ * for once. */
+int hgp_server_wait(hgp_server *s, int64_t ticket, hgp_stats *stats, double *latency_ms);
+/* Device bytes held by the slots' arenas now and at most so far, and the
+ * number of finished jobs (any output may be NULL). */
+int hgp_server_info(hgp_server *s, int64_t *reserved, int64_t *peak_reserved, int64_t *completed);
+/* Finishes queued jobs, stops the workers and frees the device memory. */
+int hgp_server_destroy(hgp_server *s);
+
 /* ------------------------------------------------------------ kernel timer
  * No counterpart entry point in the reference (its phase timers are process
  * globals, hunyuangraph_timer.h:9-319).  Per-kernel timing with HIP events on
