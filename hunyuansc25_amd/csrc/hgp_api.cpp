@@ -46,7 +46,9 @@ double alg_bytes(const std::vector<int64_t> &ns, const std::vector<int64_t> &ms,
 
 size_t estimate_arena(int32_t n, int64_t m, int32_t k, bool host_copy)
 {
-    size_t b = (size_t)64 * (size_t)m + (size_t)192 * (size_t)n + ((size_t)64 << 20) + (size_t)k * 4096;
+    // + the k-int rows of hub rows cut into segments (at most m/4096 rows)
+    size_t b = (size_t)64 * (size_t)m + (size_t)192 * (size_t)n + ((size_t)64 << 20) + (size_t)k * 4096 +
+               ((size_t)m / 1024 + 1) * (size_t)k;
     if (host_copy) b += (size_t)8 * (size_t)(n + 1) + (size_t)8 * (size_t)m + (size_t)8 * (size_t)n;
     return b;
 }
@@ -276,17 +278,17 @@ void check_job(const hgp_options &o, const HostJob &j)
     if (j.n > 0 && !j.part) throw Error{-1, "part is NULL"};
 }
 
-void exec_job(Ctx &c, const hgp_options &o, const HostJob &j)
+void exec_job(Ctx &c, const hgp_options &o, const HostJob &j, size_t bytes, size_t max_bytes)
 {
     if (j.n == 0) return;
     Params p = params_from(&o);
     HGP_CHECK_HIP(hipSetDevice(c.device));
-    with_arena(c, job_bytes(j), [&] {
+    with_arena(c, bytes, [&] {
         DevGraph g = upload(c, j.n, j.xadj, j.adjncy, j.vwgt, j.adjwgt);
         int32_t *d_part = c.arena.lalloc<int32_t>((size_t)j.n);
         run_pipeline(c, g, j.k, p, d_part, j.stats);
         to_host(c, j.part, d_part, (size_t)j.n);
-    });
+    }, max_bytes);
 }
 
 }  // namespace hgp
@@ -332,7 +334,7 @@ int hgp_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy, const i
         std::lock_guard<std::mutex> lk(g_api_mu);
         HostJob j{n, xadj, adjncy, vwgt, adjwgt, k, part, stats};
         check_job(o, j);
-        if (n > 0) exec_job(context(o.device), o, j);
+        if (n > 0) exec_job(context(o.device), o, j, job_bytes(j));
     });
 }
 
@@ -369,7 +371,7 @@ int hgp_partition_batch(int32_t count, const int32_t *n, const int64_t *const *x
                     HostJob j{n[i], xadj[i], adjncy[i], vwgt ? vwgt[i] : nullptr, adjwgt ? adjwgt[i] : nullptr, k[i],
                               part[i], stats ? stats + i : nullptr};
                     check_job(o, j);
-                    exec_job(*c, o, j);
+                    exec_job(*c, o, j, job_bytes(j));
                 });
                 st[i] = rc;
                 if (rc != HGP_OK) msg[i] = last_error_text();

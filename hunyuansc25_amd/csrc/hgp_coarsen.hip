@@ -10,7 +10,11 @@
 
 namespace hgp {
 
-void attach_bins(Ctx &c, DevGraph &g) { g.bins = row_bins(c, g.n, g.xadj, nullptr, true); }
+void attach_bins(Ctx &c, DevGraph &g)
+{
+    g.bins = row_bins(c, g.n, g.xadj, nullptr, true);
+    g.segs = row_segs(c, g.xadj, g.bins);
+}
 
 int32_t coarsen_threshold(int32_t k)
 {
@@ -90,83 +94,154 @@ __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ 
     if (lane == 0) prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
 }
 
+// Class-2 rows by segments (a hub row is shared by several blocks): the
+// segment of a single-segment row writes the proposal; the segments of a
+// multi row leave their best edge in sbest for k_propose_fin.
+__global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *__restrict__ xadj,
+                                                        const vid_t *__restrict__ adjncy,
+                                                        const wgt_t *__restrict__ adjwgt,
+                                                        const wgt_t *__restrict__ vwgt,
+                                                        const int32_t *__restrict__ match, int32_t maxvwgt,
+                                                        uint64_t seed, int32_t level, int32_t *__restrict__ prop,
+                                                        EdgeKey *__restrict__ sbest, const int32_t *prev_progress)
+{
+    if (prev_progress && *prev_progress == 0) return;
+    for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
+        int32_t v = S.row[s];
+        int32_t mi = S.mi[s];
+        if (match[v] != -1) {
+            if (threadIdx.x == 0 && mi < 0) prop[v] = -1;
+            continue;
+        }
+        int32_t wv = vwgt[v];
+        EdgeKey best{-1, 0, 0, 0u};
+        eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
+        for (eid_t j = j0 + threadIdx.x; j < j1; j += kRowBS) {
+            int32_t u = adjncy[j];
+            if (u == v || match[u] != -1) continue;
+            if ((int64_t)vwgt[u] + wv >= maxvwgt) continue;
+            EdgeKey e{adjwgt[j], u < v ? u : v, u < v ? v : u, 0u};
+            e.h = edge_hash(seed, level, e.lo, e.hi);
+            if (BetterEdge()(e, best)) best = e;
+        }
+        best = group_best<kRowBS>(best, BetterEdge());
+        if (threadIdx.x == 0) {
+            if (mi < 0) prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
+            else sbest[s] = best;
+        }
+    }
+}
+
+__global__ void k_propose_fin(RowSegs S, const int32_t *__restrict__ match, const EdgeKey *__restrict__ sbest,
+                              int32_t *__restrict__ prop, const int32_t *prev_progress)
+{
+    if (prev_progress && *prev_progress == 0) return;
+    int32_t t = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= S.nmulti) return;
+    int32_t v = S.mrow[t];
+    if (match[v] != -1) {
+        prop[v] = -1;
+        return;
+    }
+    EdgeKey best{-1, 0, 0, 0u};
+    for (int64_t s = S.mseg[2 * t]; s < S.mseg[2 * t + 1]; s++)
+        if (BetterEdge()(sbest[s], best)) best = sbest[s];
+    prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
+}
+
 __global__ void k_accept(int32_t n, const int32_t *__restrict__ prop, int32_t *__restrict__ match, int32_t *progress,
                          const int32_t *prev_progress)
 {
     if (prev_progress && *prev_progress == 0) return;
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (v >= n || match[v] != -1) return;
-    int32_t u = prop[v];
-    if (u >= 0 && prop[u] == v) {
-        match[v] = u;
-        atomicAdd(progress, 1);
+    bool got = false;
+    if (v < n && match[v] == -1) {
+        int32_t u = prop[v];
+        if (u >= 0 && prop[u] == v) {
+            match[v] = u;
+            got = true;
+        }
+    }
+    uint64_t b = __ballot(got);
+    if (lane_id() == 0 && b) atomicAdd(progress, (int32_t)__popcll(b));
+}
+
+__global__ void __launch_bounds__(kBS) k_count_matched(int32_t n, const int32_t *__restrict__ match,
+                                                       unsigned long long *cnt)
+{
+    __shared__ uint32_t part[kBS / kWave];
+    uint32_t c = 0;
+    for (int64_t v = (int64_t)blockIdx.x * kBS + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBS)
+        c += match[v] >= 0;
+    c = wave_sum(c);
+    if (lane_id() == 0) part[threadIdx.x / kWave] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int i = 0; i < kBS / kWave; i++) t += part[i];
+        if (t) atomicAdd(cnt, (unsigned long long)t);
     }
 }
 
-__global__ void k_count_matched(int32_t n, const int32_t *__restrict__ match, unsigned long long *cnt)
+// ---- leaf (hunyuangraph_GPU_match.h:1374-1423) and relative (:1597-1680)
+// matches: vertices that share a hub are paired consecutively in increasing
+// id order, (0,1), (2,3), ...; a pair whose weight is not below maxvwgt stays
+// unmatched.  hub[v] = -1 for vertices that take no part.
+__global__ void k_leaf_hub(int32_t n, const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                           const int32_t *__restrict__ match, int32_t *__restrict__ hub, uint8_t *__restrict__ f)
 {
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    uint64_t b = __ballot(v < n && match[v] >= 0);
-    if (lane_id() == 0 && b) atomicAdd(cnt, (unsigned long long)__popcll(b));
+    if (v >= n) return;
+    bool leaf = match[v] == -1 && xadj[v + 1] - xadj[v] == 1;
+    hub[v] = leaf ? adjncy[xadj[v]] : -1;
+    f[v] = leaf;
 }
 
-// ---- leaf matches (hunyuangraph_GPU_match.h:1374-1423)
-__global__ void k_leaf_count(int32_t n, const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
-                             const int32_t *__restrict__ match, int32_t *__restrict__ cnt)
+__global__ void k_hub_flag(int32_t n, const int32_t *__restrict__ hub, uint8_t *__restrict__ f)
 {
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (v >= n || match[v] != -1) return;
-    if (xadj[v + 1] - xadj[v] == 1) atomicAdd(&cnt[adjncy[xadj[v]]], 1);
+    if (v < n) f[v] = hub[v] >= 0;
 }
 
-// Consecutive pairing in row order: the eligible entries of hub h's row form
-// a sequence that is paired (0,1), (2,3), ...; a pair whose weight is not
-// below maxvwgt is skipped.  A G-lane group walks the row in chunks; lane
-// ranks come from ballots, the unpaired last entry of a chunk is carried.
-// Leaf (hunyuangraph_GPU_match.h:1374-1423): eligible = unmatched degree-1
-// neighbour.  Relative (:1597-1680): eligible = unmatched neighbour whose
-// dist is h.  Each eligible vertex belongs to exactly one hub.
-template <int G, bool kLeaf>
-__global__ void __launch_bounds__(kRowBS) k_pair_row(const int32_t *__restrict__ ids, int64_t cnt,
-                                                     const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
-                                                     const wgt_t *__restrict__ vwgt, int32_t maxvwgt,
-                                                     const int32_t *__restrict__ dist,
-                                                     const int32_t *__restrict__ hubcnt, int32_t *match)
+__global__ void k_hub_keys(const int64_t *__restrict__ cnt, const int32_t *__restrict__ list,
+                           const int32_t *__restrict__ hub, uint32_t *__restrict__ key, uint32_t *__restrict__ val)
 {
-    int64_t gi = gindex<G>();
-    int lane = glane<G>();
-    if (gi >= cnt) return;
-    int32_t h = ids[gi];
-    if (hubcnt[h] <= 1) return;
-    int32_t pending = -1;
-    eid_t e = xadj[h + 1];
-    for (eid_t s = xadj[h]; s < e; s += G) {
-        eid_t j = s + lane;
-        int32_t u = j < e ? adjncy[j] : -1;
-        bool elig = false;
-        if (u >= 0 && match[u] == -1) {
-            if (kLeaf) elig = xadj[u + 1] - xadj[u] == 1;
-            else elig = dist[u] == h;
-        }
-        uint64_t mask = group_ballot<G>(elig);
-        int npend = pending >= 0 ? 1 : 0;
-        // shuffles run on every lane of the group (a shuffle from an inactive
-        // lane would not deliver its value): the previous eligible lane's
-        // vertex, and the last eligible vertex of the chunk
-        uint64_t below = mask & group_lanemask_lt<G>();
-        int32_t prev_u = __shfl(u, below ? 63 - __clzll((long long)below) : 0, G);
-        int32_t last_u = __shfl(u, mask ? 63 - __clzll((long long)mask) : 0, G);
-        if (elig) {
-            int t = __popcll(below) + npend;   // index in this chunk's sequence
-            if (t & 1) {
-                int32_t partner = (t == 1 && npend) ? pending : prev_u;
-                if ((int64_t)vwgt[partner] + vwgt[u] < maxvwgt) { match[partner] = u; match[u] = partner; }
-            }
-        }
-        // uniform update of the carried entry
-        int total = __popcll(mask) + npend;
-        if (total & 1) pending = mask ? last_u : pending;
-        else pending = -1;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *cnt) return;
+    int32_t v = list[i];
+    key[i] = (uint32_t)hub[v];
+    val[i] = (uint32_t)v;
+}
+
+__global__ void k_hub_heads(const int64_t *__restrict__ cnt, const uint32_t *__restrict__ key,
+                            int32_t *__restrict__ h)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *cnt) return;
+    h[i] = (i == 0 || key[i - 1] != key[i]) ? 1 : 0;
+}
+
+__global__ void k_hub_start(const int64_t *__restrict__ cnt, const int32_t *__restrict__ gid,
+                            int32_t *__restrict__ start)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *cnt) return;
+    if (i == 0 || gid[i - 1] != gid[i]) start[gid[i] - 1] = (int32_t)i;
+}
+
+__global__ void k_hub_pair(const int64_t *__restrict__ cnt, const uint32_t *__restrict__ key,
+                           const uint32_t *__restrict__ val, const int32_t *__restrict__ gid,
+                           const int32_t *__restrict__ start, const wgt_t *__restrict__ vwgt, int32_t maxvwgt,
+                           int32_t *__restrict__ match)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nl = *cnt;
+    if (i + 1 >= nl) return;
+    if (((i - start[gid[i] - 1]) & 1) != 0 || key[i + 1] != key[i]) return;
+    int32_t u = (int32_t)val[i], v = (int32_t)val[i + 1];
+    if ((int64_t)vwgt[u] + vwgt[v] < maxvwgt) {
+        match[u] = v;
+        match[v] = u;
     }
 }
 
@@ -285,8 +360,7 @@ template <int G>
 __global__ void __launch_bounds__(kRowBS) k_rel_dist(const int32_t *__restrict__ ids, int64_t cnt,
                                                      const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
                                                      const wgt_t *__restrict__ adjwgt,
-                                                     const int32_t *__restrict__ match, int32_t *__restrict__ dist,
-                                                     int32_t *__restrict__ hubcnt)
+                                                     const int32_t *__restrict__ match, int32_t *__restrict__ dist)
 {
     int64_t gi = gindex<G>();
     int lane = glane<G>();
@@ -304,10 +378,7 @@ __global__ void __launch_bounds__(kRowBS) k_rel_dist(const int32_t *__restrict__
         if (BetterDist()(k, best)) best = k;
     }
     best = group_best<G>(best, BetterDist());
-    if (lane == 0) {
-        dist[v] = best.u;
-        if (best.u >= 0) atomicAdd(&hubcnt[best.u], 1);
-    }
+    if (lane == 0) dist[v] = best.u;
 }
 
 // ---- cmap (resolve_conflict_12 + scan + resolve_conflict_4's copy)
@@ -658,10 +729,43 @@ __global__ void __launch_bounds__(kRowBS) k_seg_compact(const int32_t *__restric
 int64_t count_matched(Ctx &c, int32_t n, const int32_t *match, int64_t *d_cnt)
 {
     fill<int64_t>(c, d_cnt, 1, 0);
-    k_count_matched<<<grid_for(n, kBS), kBS, 0, c.stream>>>(n, match, (unsigned long long *)d_cnt);
+    unsigned blocks = (unsigned)std::min<int64_t>(((int64_t)n + kBS - 1) / kBS, 1024);
+    k_count_matched<<<blocks, kBS, 0, c.stream>>>(n, match, (unsigned long long *)d_cnt);
     HGP_LAUNCH_CHECK();
     read_scalars(c, d_cnt, sizeof(int64_t));
     return c.pinned[0];
+}
+
+// Pairs the vertices with hub[v] >= 0 that share a hub, consecutively in
+// increasing id order: compaction keeps id order and the radix sort by hub is
+// stable.  f (n bytes) is scratch.
+void pair_by_hub(Ctx &c, int32_t n, const int32_t *hub, uint8_t *f, const wgt_t *vwgt, int32_t maxvwgt,
+                 int32_t *match)
+{
+    size_t mark = c.arena.right_mark();
+    int32_t *list = c.arena.ralloc<int32_t>((size_t)n);
+    int64_t *d_nl = c.arena.ralloc<int64_t>(1);
+    compact_indices(c, f, n, list, d_nl);
+    read_scalars(c, d_nl, sizeof(int64_t));
+    int64_t nl = c.pinned[0];
+    if (nl >= 2) {
+        uint32_t *key = c.arena.ralloc<uint32_t>((size_t)nl);
+        uint32_t *val = c.arena.ralloc<uint32_t>((size_t)nl);
+        int32_t *gid = c.arena.ralloc<int32_t>((size_t)nl);
+        int32_t *start = c.arena.ralloc<int32_t>((size_t)nl);
+        unsigned g = grid_for(nl, kBS);
+        k_hub_keys<<<g, kBS, 0, c.stream>>>(d_nl, list, hub, key, val);
+        HGP_LAUNCH_CHECK();
+        radix_sort(c, key, val, nl, 0, bits_for((uint64_t)(n > 0 ? n - 1 : 0)));
+        k_hub_heads<<<g, kBS, 0, c.stream>>>(d_nl, key, gid);
+        HGP_LAUNCH_CHECK();
+        scan_inclusive(c, gid, gid, nl);
+        k_hub_start<<<g, kBS, 0, c.stream>>>(d_nl, gid, start);
+        HGP_LAUNCH_CHECK();
+        k_hub_pair<<<g, kBS, 0, c.stream>>>(d_nl, key, val, gid, start, vwgt, maxvwgt, match);
+        HGP_LAUNCH_CHECK();
+    }
+    c.arena.right_release(mark);
 }
 
 inline bool need_more(int64_t matched, int32_t n) { return 4 * matched < 3 * (int64_t)n; }
@@ -688,6 +792,7 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
     if (n <= 0) return 0;
     size_t mark = c.arena.right_mark();
     int32_t *prop = c.arena.ralloc<int32_t>((size_t)n);
+    EdgeKey *sbest = c.arena.ralloc<EdgeKey>((size_t)g.segs.nseg);
     int32_t *progress = c.arena.ralloc<int32_t>((size_t)(rounds > 0 ? rounds : 1));
     int64_t *d_cnt = c.arena.ralloc<int64_t>(2);
     fill<int32_t>(c, match, n, -1);
@@ -698,29 +803,39 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
         {
             // per level: rows (adjncy, adjwgt) + gathers of match/vwgt per entry; xadj, match, vwgt, prop per vertex
             KScope ks(c, "match_propose", 16.0 * (double)g.m + 28.0 * n);
-            for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t cnt) {
-                constexpr int G = decltype(gc)::value;
-                k_propose<G><<<grid_groups(cnt, G), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
-                                                                           g.vwgt, match, maxvwgt, seed, level,
-                                                                           prop, prev);
+            for (int cl = 0; cl < 2; cl++) {
+                int64_t cnt = g.bins.count(cl);
+                if (!cnt) continue;
+                if (cl == 0)
+                    k_propose<8><<<grid_groups(cnt, 8), kRowBS, 0, c.stream>>>(
+                        g.bins.list(0), cnt, g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, maxvwgt, seed, level, prop, prev);
+                else
+                    k_propose<64><<<grid_groups(cnt, 64), kRowBS, 0, c.stream>>>(
+                        g.bins.list(1), cnt, g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, maxvwgt, seed, level, prop, prev);
                 HGP_LAUNCH_CHECK();
-            });
+            }
+            if (g.segs.nseg) {
+                k_propose_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
+                    g.segs, g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, maxvwgt, seed, level, prop, sbest, prev);
+                HGP_LAUNCH_CHECK();
+                if (g.segs.nmulti) {
+                    k_propose_fin<<<grid_for(g.segs.nmulti, kBS), kBS, 0, c.stream>>>(g.segs, match, sbest, prop, prev);
+                    HGP_LAUNCH_CHECK();
+                }
+            }
         }
         k_accept<<<gv, kBS, 0, c.stream>>>(n, prop, match, progress + r, prev);
         HGP_LAUNCH_CHECK();
     }
     int64_t matched = count_matched(c, n, match, d_cnt);
     if (need_more(matched, n)) {
-        int32_t *cnt = prop;   // reuse
-        fill<int32_t>(c, cnt, n, 0);
-        k_leaf_count<<<gv, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, match, cnt);
+        size_t m2 = c.arena.right_mark();
+        int32_t *hub = c.arena.ralloc<int32_t>((size_t)n);
+        uint8_t *f = c.arena.ralloc<uint8_t>((size_t)n);
+        k_leaf_hub<<<gv, kBS, 0, c.stream>>>(n, g.xadj, g.adjncy, match, hub, f);
         HGP_LAUNCH_CHECK();
-        for_bins<64>(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
-            constexpr int G = decltype(gc)::value;
-            k_pair_row<G, true><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.vwgt,
-                                                                            maxvwgt, nullptr, cnt, match);
-            HGP_LAUNCH_CHECK();
-        });
+        pair_by_hub(c, n, hub, f, g.vwgt, maxvwgt, match);
+        c.arena.right_release(m2);
         matched = count_matched(c, n, match, d_cnt);
     }
     if (need_more(matched, n)) {
@@ -774,20 +889,16 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
     if (need_more(matched, n)) {
         size_t m2 = c.arena.right_mark();
         int32_t *dist = c.arena.ralloc<int32_t>((size_t)n);
-        int32_t *cnt = prop;
-        fill<int32_t>(c, cnt, n, 0);
+        uint8_t *f = c.arena.ralloc<uint8_t>((size_t)n);
         for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
             constexpr int G = decltype(gc)::value;
             k_rel_dist<G><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt, match,
-                                                                       dist, cnt);
+                                                                       dist);
             HGP_LAUNCH_CHECK();
         });
-        for_bins<64>(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
-            constexpr int G = decltype(gc)::value;
-            k_pair_row<G, false><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.vwgt,
-                                                                             maxvwgt, dist, cnt, match);
-            HGP_LAUNCH_CHECK();
-        });
+        k_hub_flag<<<gv, kBS, 0, c.stream>>>(n, dist, f);
+        HGP_LAUNCH_CHECK();
+        pair_by_hub(c, n, dist, f, g.vwgt, maxvwgt, match);
         c.arena.right_release(m2);
         matched = count_matched(c, n, match, d_cnt);
     }

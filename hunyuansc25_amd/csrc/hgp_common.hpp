@@ -26,6 +26,7 @@ constexpr int kMidBucket = 25;
 struct Error {
     int code;
     std::string msg;
+    bool overflow = false;   // an arena request did not fit (a larger arena may succeed)
 };
 
 void set_error(int code, const std::string &msg);

@@ -17,9 +17,10 @@ struct DevGraph {
     wgt_t *adjwgt = nullptr;
     wgt_t *vwgt = nullptr;
     RowBins bins;             // degree classes of the rows (set by attach_bins)
+    RowSegs segs;             // segments of the class-2 rows (set by attach_bins)
 };
 
-// Computes g.bins (ids on the left end of the arena: they live with the level).
+// Computes g.bins and g.segs (on the left end of the arena: they live with the level).
 void attach_bins(Ctx &c, DevGraph &g);
 
 struct Params {

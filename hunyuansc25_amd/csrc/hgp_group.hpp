@@ -30,6 +30,25 @@ struct RowBins {
 RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool left, int64_t t0 = kBinSmall,
                  int64_t t1 = kBinMid, int64_t t2 = INT64_MAX);
 
+// Rows of class 2 cut into segments of at most kSegLen entries, so that a hub
+// row is shared by several blocks.  A segment of a single-segment row is
+// finished by the segment kernel itself; the segments of a "multi" row leave
+// partial results that a second kernel combines per row.
+constexpr int64_t kSegLen = 4096;
+
+struct RowSegs {
+    int64_t nseg = 0;
+    int32_t *row = nullptr;    // vertex of each segment
+    int64_t *j0 = nullptr;     // first entry of each segment (CSR offset)
+    int32_t *mi = nullptr;     // multi-row index of each segment, -1 for a single-segment row
+    int32_t nmulti = 0;
+    int32_t *mrow = nullptr;   // vertex of each multi row
+    int64_t *mseg = nullptr;   // segments of multi row t: [mseg[2t], mseg[2t+1])
+};
+
+// Segments of the class-2 rows of b (arrays on the left end of the arena).
+RowSegs row_segs(Ctx &c, const eid_t *xadj, const RowBins &b);
+
 template <int G>
 __device__ inline int glane() { return (int)(threadIdx.x % G); }
 

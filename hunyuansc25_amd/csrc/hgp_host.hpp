@@ -6,6 +6,7 @@
 
 #include "hgp_common.hpp"
 
+#include <cstdint>
 #include <new>
 #include <stdexcept>
 
@@ -27,8 +28,9 @@ struct HostJob {
 void check_job(const hgp_options &o, const HostJob &j);
 // Device arena bytes the job is run with first.
 size_t job_bytes(const HostJob &j);
-// Upload, partition, download on context c (after check_job).
-void exec_job(Ctx &c, const hgp_options &o, const HostJob &j);
+// Upload, partition, download on context c (after check_job), with an arena
+// of `bytes` that may grow up to max_bytes when the run needs more.
+void exec_job(Ctx &c, const hgp_options &o, const HostJob &j, size_t bytes, size_t max_bytes = SIZE_MAX);
 
 // Runs f, turning exceptions into a status code and the thread's error text.
 template <class F>
@@ -51,9 +53,10 @@ int guarded(F f)
 }
 
 // Runs body with an arena of at least `bytes`; on arena overflow the arena
-// grows and the body is re-run (the body must be restartable).
+// doubles (up to max_bytes, three times at most) and the body is re-run (the
+// body must be restartable).
 template <class F>
-void with_arena(Ctx &c, size_t bytes, F body)
+void with_arena(Ctx &c, size_t bytes, F body, size_t max_bytes = SIZE_MAX)
 {
     for (int attempt = 0;; attempt++) {
         c.arena.reset();
@@ -66,7 +69,7 @@ void with_arena(Ctx &c, size_t bytes, F body)
         } catch (const Error &e) {
             (void)hipStreamSynchronize(c.stream);
             c.arena.reset();
-            if (e.code != -3 || attempt >= 3) throw;
+            if (!e.overflow || attempt >= 3 || bytes > max_bytes / 2) throw;
             bytes *= 2;
         }
     }

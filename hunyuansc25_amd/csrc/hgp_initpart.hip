@@ -141,8 +141,12 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
     for (int32_t i = lane; i < n; i += kWave) { M.where[i] = 1; M.moved[i] = 0; }
     fence_wave();
 
+    // GGGP: BFS from the start vertex, moving vertices to side 0 in BFS order.
+    // Every lane runs the same control flow on the same values; the rows of
+    // dequeued vertices are scanned 64 entries at a time and the unvisited
+    // neighbours appended in row order (ballot ranks), as a serial scan would.
     int64_t pw0 = 0, pw1 = tv;
-    if (lane == 0) {
+    {
         int32_t draw = 0, start;
         if (nd.T >= n) {
             start = t;
@@ -153,21 +157,34 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
             start = left + (int32_t)(trial_rng(seed, (uint32_t)nid, t, draw++) % (uint64_t)len);
         }
         int32_t first = 0, last = 1, nleft = n - 1, drain = 0;
-        M.queue[0] = start;
-        M.moved[start] = 1;
+        if (lane == 0) {
+            M.queue[0] = start;
+            M.moved[start] = 1;
+        }
+        fence_wave();
         for (;;) {
             if (first == last) {
                 if (nleft == 0 || drain) break;
+                // restart from the r-th unvisited vertex in id order
                 int32_t r = (int32_t)(trial_rng(seed, (uint32_t)nid, t, draw++) % (uint64_t)nleft);
-                int32_t v;
-                for (v = 0; v < n; v++) {
-                    if (M.moved[v] == 0) {
-                        if (r == 0) break;
-                        r--;
+                int32_t v = -1;
+                for (int32_t b0 = 0; b0 < n; b0 += kWave) {
+                    int32_t u = b0 + lane;
+                    bool un = u < n && M.moved[u] == 0;
+                    uint64_t bal = __ballot(un);
+                    int32_t cnt = __popcll(bal);
+                    if (r < cnt) {
+                        uint64_t pick = __ballot(un && __popcll(bal & lanemask_lt()) == r);
+                        v = b0 + (__ffsll((long long)pick) - 1);
+                        break;
                     }
+                    r -= cnt;
                 }
-                M.queue[0] = v;
-                M.moved[v] = 1;
+                if (lane == 0) {
+                    M.queue[0] = v;
+                    M.moved[v] = 1;
+                }
+                fence_wave();
                 first = 0;
                 last = 1;
                 nleft--;
@@ -175,19 +192,27 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
             int32_t v = M.queue[first++];
             int32_t wv = fvw[s0 + v];
             if (pw0 > 0 && pw1 - wv < onemin) { drain = 1; continue; }
-            M.where[v] = 0;
+            if (lane == 0) M.where[v] = 0;
             pw0 += wv;
             pw1 -= wv;
             if (pw1 <= onemax) break;
             drain = 0;
-            for (eid_t j = fx[s0 + v]; j < fx[s0 + v + 1]; j++) {
-                int32_t x = fadj[j] - s0;
-                if (M.moved[x] == 0) { M.queue[last++] = x; M.moved[x] = 1; nleft--; }
+            eid_t rb = fx[s0 + v], re = fx[s0 + v + 1];
+            for (eid_t j0 = rb; j0 < re; j0 += kWave) {
+                eid_t j = j0 + lane;
+                int32_t x = j < re ? fadj[j] - s0 : -1;
+                bool fresh = x >= 0 && M.moved[x] == 0;
+                uint64_t bal = __ballot(fresh);
+                if (fresh) {
+                    M.queue[last + __popcll(bal & lanemask_lt())] = x;
+                    M.moved[x] = 1;
+                }
+                last += __popcll(bal);
+                nleft -= __popcll(bal);
             }
+            fence_wave();
         }
     }
-    pw0 = __shfl(pw0, 0, kWave);
-    pw1 = __shfl(pw1, 0, kWave);
     fence_wave();
 
     // ed / id and the cut (compute_v_ed_id_bnd, :23-62)

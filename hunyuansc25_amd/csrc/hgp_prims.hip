@@ -1,5 +1,7 @@
 // hgp_prims.hip -- scans, stable radix sort, fills, reductions (wave64).
 #include "hgp_prims.hpp"
+
+#include <algorithm>
 #include "hgp_group.hpp"
 
 namespace hgp {
@@ -253,40 +255,102 @@ struct BinThr {
     __device__ int cls(int64_t d) const { return d <= t0 ? 0 : (d <= t1 ? 1 : (d <= t2 ? 2 : 3)); }
 };
 
-__global__ void k_bin_count(int32_t n, const eid_t *__restrict__ xadj, const int64_t *__restrict__ len, BinThr t,
-                            unsigned long long *cnt)
+// Class counts: per-block LDS counters, one atomic per block and class.
+__global__ void __launch_bounds__(256) k_bin_count(int32_t n, const eid_t *__restrict__ xadj,
+                                                   const int64_t *__restrict__ len, BinThr t, unsigned long long *cnt)
 {
-    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    bool valid = v < n;
-    int cls = -1;
-    if (valid) cls = t.cls(xadj ? xadj[v + 1] - xadj[v] : len[v]);
+    __shared__ uint32_t bc[4];
+    if (threadIdx.x < 4) bc[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256) {
+        int cls = t.cls(xadj ? xadj[v + 1] - xadj[v] : len[v]);
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        uint64_t b = __ballot(cls == c);
-        if (lane_id() == 0 && b) atomicAdd(&cnt[c], (unsigned long long)__popcll(b));
+        for (int c = 0; c < 4; c++) {
+            uint64_t b = __ballot(cls == c);
+            if (lane_id() == 0 && b) atomicAdd(&bc[c], (uint32_t)__popcll(b));
+        }
     }
+    __syncthreads();
+    if (threadIdx.x < 4 && bc[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], (unsigned long long)bc[threadIdx.x]);
 }
 
 struct BinBase {
     int64_t o[4];
 };
 
-__global__ void k_bin_fill(int32_t n, const eid_t *__restrict__ xadj, const int64_t *__restrict__ len, BinThr t,
-                           BinBase base, unsigned long long *cursor, int32_t *__restrict__ ids)
+// Each block takes its slots of every class with one atomic per class, then
+// hands them out to its waves through LDS.
+__global__ void __launch_bounds__(256) k_bin_fill(int32_t n, const eid_t *__restrict__ xadj,
+                                                  const int64_t *__restrict__ len, BinThr t, BinBase base,
+                                                  unsigned long long *cursor, int32_t *__restrict__ ids)
 {
-    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    bool valid = v < n;
-    int cls = -1;
-    if (valid) cls = t.cls(xadj ? xadj[v + 1] - xadj[v] : len[v]);
+    __shared__ uint32_t wc[4][256 / kWave];
+    __shared__ unsigned long long bb[4];
+    const int w = threadIdx.x / kWave;
+    for (int64_t v0 = (int64_t)blockIdx.x * 256; v0 < n; v0 += (int64_t)gridDim.x * 256) {
+        int64_t v = v0 + threadIdx.x;
+        int cls = v < n ? t.cls(xadj ? xadj[v + 1] - xadj[v] : len[v]) : -1;
+        uint64_t bal[4];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        uint64_t b = __ballot(cls == c);
-        if (!b) continue;
-        int leader = __ffsll((long long)b) - 1;
-        unsigned long long slot = 0;
-        if (lane_id() == leader) slot = atomicAdd(&cursor[c], (unsigned long long)__popcll(b));
-        slot = __shfl(slot, leader, kWave);
-        if (cls == c) ids[base.o[c] + (int64_t)slot + __popcll(b & lanemask_lt())] = v;
+        for (int c = 0; c < 4; c++) {
+            bal[c] = __ballot(cls == c);
+            if (lane_id() == 0) wc[c][w] = (uint32_t)__popcll(bal[c]);
+        }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            int c = threadIdx.x;
+            uint32_t tot = 0;
+            for (int i = 0; i < 256 / kWave; i++) {
+                uint32_t x = wc[c][i];
+                wc[c][i] = tot;
+                tot += x;
+            }
+            bb[c] = tot ? atomicAdd(&cursor[c], (unsigned long long)tot) : 0;
+        }
+        __syncthreads();
+        if (cls >= 0)
+            ids[base.o[cls] + (int64_t)bb[cls] + wc[cls][w] + __popcll(bal[cls] & lanemask_lt())] = (int32_t)v;
+        __syncthreads();
+    }
+}
+
+__global__ void k_seg_count(int64_t nl, const int32_t *__restrict__ list, const eid_t *__restrict__ xadj,
+                            int64_t *__restrict__ cnt, uint8_t *__restrict__ multi, int32_t *__restrict__ mimap)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    int32_t v = list[i];
+    int64_t c = (xadj[v + 1] - xadj[v] + kSegLen - 1) / kSegLen;
+    cnt[i] = c;
+    multi[i] = c > 1;
+    mimap[i] = -1;
+}
+
+__global__ void k_seg_multi(int64_t nm, const int64_t *__restrict__ mlist_n, const int32_t *__restrict__ mlist,
+                            const int32_t *__restrict__ list, const int64_t *__restrict__ segoff,
+                            int32_t *__restrict__ mimap, int32_t *__restrict__ mrow, int64_t *__restrict__ mseg)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nm || t >= *mlist_n) return;
+    int32_t li = mlist[t];
+    mimap[li] = (int32_t)t;
+    mrow[t] = list[li];
+    mseg[2 * t] = segoff[li];
+    mseg[2 * t + 1] = segoff[li + 1];
+}
+
+__global__ void k_seg_fill(int64_t nl, const int32_t *__restrict__ list, const eid_t *__restrict__ xadj,
+                           const int64_t *__restrict__ segoff, const int32_t *__restrict__ mimap,
+                           int32_t *__restrict__ row, int64_t *__restrict__ j0, int32_t *__restrict__ mi)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    int32_t v = list[i];
+    eid_t b = xadj[v];
+    for (int64_t s = segoff[i]; s < segoff[i + 1]; s++) {
+        row[s] = v;
+        j0[s] = b + (s - segoff[i]) * kSegLen;
+        mi[s] = mimap[i];
     }
 }
 
@@ -301,7 +365,7 @@ RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool 
     size_t mark = c.arena.right_mark();
     unsigned long long *cnt = c.arena.ralloc<unsigned long long>(8);
     fill<uint32_t>(c, (uint32_t *)cnt, 16, 0u);
-    unsigned g = grid_for(n, 256);
+    unsigned g = (unsigned)std::min<int64_t>(((int64_t)n + 255) / 256, 1024);
     BinThr thr{t0, t1, t2};
     k_bin_count<<<g, 256, 0, c.stream>>>(n, xadj, len, thr, cnt);
     HGP_LAUNCH_CHECK();
@@ -316,6 +380,45 @@ RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool 
     HGP_LAUNCH_CHECK();
     c.arena.right_release(mark);
     return b;
+}
+
+RowSegs row_segs(Ctx &c, const eid_t *xadj, const RowBins &b)
+{
+    RowSegs r;
+    int64_t nl = b.count(2);
+    if (nl <= 0) return r;
+    const int32_t *list = b.list(2);
+    int32_t *mimap = c.arena.lalloc<int32_t>((size_t)nl);
+    size_t mark = c.arena.right_mark();
+    int64_t *cnt = c.arena.ralloc<int64_t>((size_t)nl);
+    int64_t *segoff = c.arena.ralloc<int64_t>((size_t)nl + 1);
+    uint8_t *multi = c.arena.ralloc<uint8_t>((size_t)nl);
+    int32_t *mlist = c.arena.ralloc<int32_t>((size_t)nl);
+    int64_t *d_nm = c.arena.ralloc<int64_t>(1);
+    unsigned g = grid_for(nl, 256);
+    k_seg_count<<<g, 256, 0, c.stream>>>(nl, list, xadj, cnt, multi, mimap);
+    HGP_LAUNCH_CHECK();
+    offsets_from_counts(c, cnt, segoff, nl);
+    compact_indices(c, multi, nl, mlist, d_nm);
+    HGP_CHECK_HIP(hipMemcpyAsync(c.pinned, segoff + nl, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    HGP_CHECK_HIP(hipMemcpyAsync(c.pinned + 1, d_nm, sizeof(int64_t), hipMemcpyDeviceToHost, c.stream));
+    HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+    r.nseg = c.pinned[0];
+    r.nmulti = (int32_t)c.pinned[1];
+    r.row = c.arena.lalloc<int32_t>((size_t)r.nseg);
+    r.j0 = c.arena.lalloc<int64_t>((size_t)r.nseg);
+    r.mi = c.arena.lalloc<int32_t>((size_t)r.nseg);
+    r.mrow = c.arena.lalloc<int32_t>((size_t)r.nmulti);
+    r.mseg = c.arena.lalloc<int64_t>(2 * (size_t)r.nmulti);
+    if (r.nmulti > 0) {
+        k_seg_multi<<<grid_for(r.nmulti, 256), 256, 0, c.stream>>>(r.nmulti, d_nm, mlist, list, segoff, mimap, r.mrow,
+                                                                   r.mseg);
+        HGP_LAUNCH_CHECK();
+    }
+    k_seg_fill<<<g, 256, 0, c.stream>>>(nl, list, xadj, segoff, mimap, r.row, r.j0, r.mi);
+    HGP_LAUNCH_CHECK();
+    c.arena.right_release(mark);
+    return r;
 }
 
 void scan_inclusive(Ctx &c, const int32_t *in, int32_t *out, int64_t n) { scan_impl<int32_t>(c, in, out, n); }

@@ -247,8 +247,10 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_s(const int32_t *__restri
     sel_count(A, mode, ncand);
 }
 
-// Rows of 9..512 entries: one wave per row, entries cached in registers (8 per
-// lane), a k-int LDS accumulator per wave.  Waves loop over rows on their own.
+// Rows of 9..512 entries: a wave takes 64 rows of the list at a time, decides
+// the ineligible ones lane by lane and then walks the eligible ones, one row
+// at a time with its entries cached in registers (8 per lane) and a k-int LDS
+// accumulator per wave.  Waves loop on their own, without block barriers.
 __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
 {
     extern __shared__ int32_t acc[];
@@ -263,48 +265,58 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
     for (int i = lane; i < k; i += kWave) my[i] = 0;
     wave_lds_sync();
     int32_t ncand = 0;
-    for (int64_t gi = (int64_t)blockIdx.x * nw + wv; gi < cnt; gi += (int64_t)gridDim.x * nw) {
-        int32_t v = ids[gi];
-        int32_t p = A.where[v];
-        bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
-        eid_t b = A.xadj[v], e = A.xadj[v + 1];
-        int32_t q[S], w[S];
+    for (int64_t base = ((int64_t)blockIdx.x * nw + wv) * kWave; base < cnt; base += (int64_t)gridDim.x * nw * kWave) {
+        int64_t i = base + lane;
+        int32_t mv = i < cnt ? ids[i] : -1;
+        int32_t mp = mv >= 0 ? A.where[mv] : 0;
+        bool mel = mv >= 0 && sel_eligible(A, mode, maxwgt, opt, mv, mp);
+        if (mv >= 0 && !mel) sel_decide(A, C, mode, mv, mp, false, 0, PartKey{0, -1});
+        uint64_t todo = __ballot(mel);
+        while (todo) {
+            int l = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            int32_t v = __shfl(mv, l, kWave), p = __shfl(mp, l, kWave);
+            eid_t b = A.xadj[v], e = A.xadj[v + 1];
+            int32_t q[S], w[S];
 #pragma unroll
-        for (int t = 0; t < S; t++) {
-            eid_t j = b + lane + (eid_t)t * kWave;
-            q[t] = -1;
-            w[t] = 0;
-            if (eligible && j < e) {
-                q[t] = A.where[A.adjncy[j]];
-                w[t] = A.adjwgt[j];
-                atomicAdd(&my[q[t]], w[t]);
+            for (int t = 0; t < S; t++) {
+                eid_t j = b + lane + (eid_t)t * kWave;
+                q[t] = -1;
+                w[t] = 0;
+                if (j < e) {
+                    q[t] = A.where[A.adjncy[j]];
+                    w[t] = A.adjwgt[j];
+                    atomicAdd(&my[q[t]], w[t]);
+                }
             }
-        }
-        wave_lds_sync();
-        int32_t pc = eligible ? my[p] : 0;
-        PartKey best{0, -1};
+            wave_lds_sync();
+            int32_t pc = my[p];
+            PartKey best{0, -1};
 #pragma unroll
-        for (int t = 0; t < S; t++) {
-            if (q[t] >= 0 && q[t] != p && sel_target(A, mode, maxwgt, q[t])) {
-                PartKey cand{my[q[t]], q[t]};
-                if (BetterPart()(cand, best)) best = cand;
+            for (int t = 0; t < S; t++) {
+                if (q[t] >= 0 && q[t] != p && sel_target(A, mode, maxwgt, q[t])) {
+                    PartKey cand{my[q[t]], q[t]};
+                    if (BetterPart()(cand, best)) best = cand;
+                }
             }
-        }
-        best = group_best<kWave>(best, BetterPart());
-        wave_lds_sync();
+            best = group_best<kWave>(best, BetterPart());
+            wave_lds_sync();
 #pragma unroll
-        for (int t = 0; t < S; t++)
-            if (q[t] >= 0) my[q[t]] = 0;
-        wave_lds_sync();
-        if (lane == 0) ncand += sel_decide(A, C, mode, v, p, eligible, pc, best);
+            for (int t = 0; t < S; t++)
+                if (q[t] >= 0) my[q[t]] = 0;
+            wave_lds_sync();
+            if (lane == 0) ncand += sel_decide(A, C, mode, v, p, true, pc, best);
+        }
     }
     sel_count(A, mode, ncand);
 }
 
-// Rows of > 512 entries: one block per row and a k-int LDS accumulator.  The
-// arg-max and the clearing walk the k parts when the row is longer than k,
-// else the row again.
-__global__ void __launch_bounds__(kRowBS) k_jet_select_b(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
+// Rows of > 512 entries by segments (RowSegs), a k-int LDS accumulator per
+// block.  A single-segment row is decided here: the arg-max and the clearing
+// walk the k parts when the segment is longer than k, else the entries again.
+// A multi row's segments add their sums to its k-int row of conn, which
+// k_jet_select_fin reads.
+__global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A, int32_t *__restrict__ conn)
 {
     extern __shared__ int32_t acc[];
     const JetCtl &C = *A.ctl;
@@ -315,15 +327,39 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_b(const int32_t *__restri
     for (int i = threadIdx.x; i < k; i += kRowBS) acc[i] = 0;
     __syncthreads();
     int32_t ncand = 0;
-    for (int64_t gi = blockIdx.x; gi < cnt; gi += gridDim.x) {
-        int32_t v = ids[gi];
+    for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
+        int32_t v = S.row[s];
+        int32_t mi = S.mi[s];
         int32_t p = A.where[v];
         bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
-        eid_t b = A.xadj[v], e = A.xadj[v + 1];
+        eid_t b = S.j0[s], e = min(b + (eid_t)kSegLen, A.xadj[v + 1]);
         bool walk_parts = e - b > (eid_t)k;
         if (eligible)
             for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) atomicAdd(&acc[A.where[A.adjncy[j]]], A.adjwgt[j]);
         __syncthreads();
+        if (mi >= 0) {
+            // partial sums of a multi row: move them to conn and clear
+            if (eligible) {
+                int32_t *row = conn + (size_t)mi * k;
+                if (walk_parts) {
+                    for (int q = threadIdx.x; q < k; q += kRowBS) {
+                        int32_t x = acc[q];
+                        if (x) {
+                            atomicAdd(&row[q], x);
+                            acc[q] = 0;
+                        }
+                    }
+                } else {
+                    for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) {
+                        int32_t q = A.where[A.adjncy[j]];
+                        int32_t x = atomicExch(&acc[q], 0);
+                        if (x) atomicAdd(&row[q], x);
+                    }
+                }
+            }
+            __syncthreads();
+            continue;
+        }
         int32_t pc = eligible ? acc[p] : 0;
         PartKey best{0, -1};
         if (eligible) {
@@ -351,6 +387,36 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_b(const int32_t *__restri
             }
         }
         __syncthreads();
+        if (threadIdx.x == 0) ncand += sel_decide(A, C, mode, v, p, eligible, pc, best);
+    }
+    sel_count(A, mode, ncand);
+}
+
+// Multi rows: one block per row reads the summed k-int row, decides, clears it.
+__global__ void __launch_bounds__(kRowBS) k_jet_select_fin(RowSegs S, SelArgs A, int32_t *__restrict__ conn)
+{
+    const JetCtl &C = *A.ctl;
+    if (C.done) return;
+    const int32_t mode = C.mode;
+    const int64_t maxwgt = C.maxwgt, opt = C.opt;
+    const int k = A.k;
+    int32_t ncand = 0;
+    for (int32_t t = blockIdx.x; t < S.nmulti; t += gridDim.x) {
+        int32_t v = S.mrow[t];
+        int32_t p = A.where[v];
+        bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
+        int32_t *row = conn + (size_t)t * k;
+        int32_t pc = eligible ? row[p] : 0;
+        PartKey best{0, -1};
+        if (eligible)
+            for (int q = threadIdx.x; q < k; q += kRowBS) {
+                if (q == p || !sel_target(A, mode, maxwgt, q)) continue;
+                PartKey cand{row[q], q};
+                if (BetterPart()(cand, best)) best = cand;
+            }
+        best = group_best<kRowBS>(best, BetterPart());
+        if (eligible)
+            for (int q = threadIdx.x; q < k; q += kRowBS) row[q] = 0;
         if (threadIdx.x == 0) ncand += sel_decide(A, C, mode, v, p, eligible, pc, best);
     }
     sel_count(A, mode, ncand);
@@ -394,6 +460,104 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
     }
     change = group_sum<G>(change);
     if (lane == 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
+}
+
+// Rows of 9..512 entries: 64 rows per wave at a time; non-candidates are
+// settled lane by lane, candidates walked one row at a time by the wave.
+__global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restrict__ ids, int64_t cnt,
+                                                          const eid_t *__restrict__ xadj,
+                                                          const vid_t *__restrict__ adjncy,
+                                                          const wgt_t *__restrict__ adjwgt,
+                                                          const int32_t *__restrict__ where,
+                                                          const int32_t *__restrict__ dest,
+                                                          const int32_t *__restrict__ pregain, const JetCtl *ctl,
+                                                          uint8_t *__restrict__ sel)
+{
+    if (ctl->done || ctl->mode != kModeLP) return;
+    const int lane = lane_id();
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    for (int64_t base = wid * kWave; base < cnt; base += nwaves * kWave) {
+        int64_t i = base + lane;
+        int32_t mv = i < cnt ? ids[i] : -1;
+        int32_t mg = mv >= 0 ? pregain[mv] : kNoGain;
+        if (mv >= 0 && mg == kNoGain) sel[mv] = 0;
+        uint64_t todo = __ballot(mv >= 0 && mg != kNoGain);
+        while (todo) {
+            int l = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            int32_t v = __shfl(mv, l, kWave), ig = __shfl(mg, l, kWave);
+            int32_t p = where[v], best = dest[v];
+            int64_t change = 0;
+            for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += kWave) {
+                int32_t u = adjncy[j];
+                int32_t ng = pregain[u];
+                if (ng > ig || (ng == ig && u < v)) {
+                    int32_t w = adjwgt[j];
+                    int32_t nd = dest[u];
+                    if (nd == p) change -= w; else if (nd == best) change += w;
+                    int32_t nw = where[u];
+                    if (nw == p) change += w; else if (nw == best) change -= w;
+                }
+            }
+            change = wave_sum(change);
+            if (lane == 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
+        }
+    }
+}
+
+// Class-2 rows by segments: a single-segment row decides here, a multi row
+// adds its partial change to macc for k_afterburner_fin.
+__global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid_t *__restrict__ xadj,
+                                                            const vid_t *__restrict__ adjncy,
+                                                            const wgt_t *__restrict__ adjwgt,
+                                                            const int32_t *__restrict__ where,
+                                                            const int32_t *__restrict__ dest,
+                                                            const int32_t *__restrict__ pregain, const JetCtl *ctl,
+                                                            uint8_t *__restrict__ sel,
+                                                            unsigned long long *__restrict__ macc)
+{
+    if (ctl->done || ctl->mode != kModeLP) return;
+    for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
+        int32_t v = S.row[s];
+        int32_t mi = S.mi[s];
+        int32_t ig = pregain[v];
+        if (ig == kNoGain) {
+            if (threadIdx.x == 0 && mi < 0) sel[v] = 0;
+            continue;
+        }
+        int32_t p = where[v], best = dest[v];
+        int64_t change = 0;
+        eid_t j1 = min(S.j0[s] + (eid_t)kSegLen, xadj[v + 1]);
+        for (eid_t j = S.j0[s] + threadIdx.x; j < j1; j += kRowBS) {
+            int32_t u = adjncy[j];
+            int32_t ng = pregain[u];
+            if (ng > ig || (ng == ig && u < v)) {
+                int32_t w = adjwgt[j];
+                int32_t nd = dest[u];
+                if (nd == p) change -= w; else if (nd == best) change += w;
+                int32_t nw = where[u];
+                if (nw == p) change += w; else if (nw == best) change -= w;
+            }
+        }
+        change = group_sum<kRowBS>(change);
+        if (threadIdx.x == 0) {
+            if (mi < 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
+            else if (change) atomicAdd(&macc[mi], (unsigned long long)change);
+        }
+    }
+}
+
+__global__ void k_afterburner_fin(RowSegs S, const int32_t *__restrict__ pregain, const JetCtl *ctl,
+                                  uint8_t *__restrict__ sel, int64_t *__restrict__ macc)
+{
+    if (ctl->done || ctl->mode != kModeLP) return;
+    int32_t t = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= S.nmulti) return;
+    int32_t v = S.mrow[t];
+    int32_t ig = pregain[v];
+    sel[v] = (ig != kNoGain && (int64_t)ig + macc[t] >= 0) ? 1 : 0;
+    macc[t] = 0;
 }
 
 // filter_beneficial_moves + set_lock (:2180-2221): with candidates present,
@@ -579,6 +743,99 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta(const int32_t *__restrict
     if (threadIdx.x == 0 && bdelta) atomicAdd(&ctl->delta2, bdelta);
 }
 
+// Rows of 9..512 entries: 64 rows per wave at a time, only the moved ones walked.
+__global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
+                                                         const eid_t *__restrict__ xadj,
+                                                         const vid_t *__restrict__ adjncy,
+                                                         const wgt_t *__restrict__ adjwgt,
+                                                         const wgt_t *__restrict__ vwgt,
+                                                         const int32_t *__restrict__ where,
+                                                         const int32_t *__restrict__ dest, JetCtl *ctl,
+                                                         int64_t *__restrict__ pwgts)
+{
+    extern __shared__ unsigned long long pdelta[];
+    __shared__ unsigned long long bdelta;
+    if (ctl->done) return;
+    for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
+    if (threadIdx.x == 0) bdelta = 0;
+    __syncthreads();
+    const int lane = lane_id();
+    const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    for (int64_t base = wid * kWave; base < cnt; base += nwaves * kWave) {
+        int64_t i = base + lane;
+        int32_t mv = i < cnt ? ids[i] : -1;
+        int32_t mpv = mv >= 0 ? where[mv] : 0, mdv = mv >= 0 ? dest[mv] : 0;
+        uint64_t todo = __ballot(mv >= 0 && mpv != mdv);
+        while (todo) {
+            int l = __ffsll((long long)todo) - 1;
+            todo &= todo - 1;
+            int32_t v = __shfl(mv, l, kWave), pv = __shfl(mpv, l, kWave), dv = __shfl(mdv, l, kWave);
+            int64_t d2 = 0;
+            for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += kWave) {
+                int32_t u = adjncy[j];
+                int32_t pu = where[u], du = dest[u];
+                bool um = pu != du;
+                int32_t before = pv != pu, after = dv != (um ? du : pu);
+                d2 += (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
+            }
+            d2 = wave_sum(d2);
+            if (lane == 0) {
+                if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
+                atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
+                atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += kRowBS)
+        if (pdelta[i]) atomicAdd((unsigned long long *)&pwgts[i], pdelta[i]);
+    if (threadIdx.x == 0 && bdelta) atomicAdd(&ctl->delta2, bdelta);
+}
+
+// Class-2 rows by segments; the part weights move with the row's first segment.
+__global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k, const eid_t *__restrict__ xadj,
+                                                           const vid_t *__restrict__ adjncy,
+                                                           const wgt_t *__restrict__ adjwgt,
+                                                           const wgt_t *__restrict__ vwgt,
+                                                           const int32_t *__restrict__ where,
+                                                           const int32_t *__restrict__ dest, JetCtl *ctl,
+                                                           int64_t *__restrict__ pwgts)
+{
+    extern __shared__ unsigned long long pdelta[];
+    __shared__ unsigned long long bdelta;
+    if (ctl->done) return;
+    for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
+    if (threadIdx.x == 0) bdelta = 0;
+    __syncthreads();
+    for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
+        int32_t v = S.row[s];
+        int32_t pv = where[v], dv = dest[v];
+        if (pv == dv) continue;
+        eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
+        int64_t d2 = 0;
+        for (eid_t j = j0 + threadIdx.x; j < j1; j += kRowBS) {
+            int32_t u = adjncy[j];
+            int32_t pu = where[u], du = dest[u];
+            bool um = pu != du;
+            int32_t before = pv != pu, after = dv != (um ? du : pu);
+            d2 += (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
+        }
+        d2 = group_sum<kRowBS>(d2);
+        if (threadIdx.x == 0) {
+            if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
+            if (j0 == xadj[v]) {
+                atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
+                atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += kRowBS)
+        if (pdelta[i]) atomicAdd((unsigned long long *)&pwgts[i], pdelta[i]);
+    if (threadIdx.x == 0 && bdelta) atomicAdd(&ctl->delta2, bdelta);
+}
+
 __global__ void k_move_commit(int32_t n, const JetCtl *ctl, const int32_t *__restrict__ dest, int32_t *__restrict__ where)
 {
     if (ctl->done) return;
@@ -679,21 +936,47 @@ __global__ void __launch_bounds__(kRowBS) k_cut(const int32_t *__restrict__ ids,
     if (lane == 0 && s) atomicAdd(out, (unsigned long long)s);
 }
 
-template <int MaxG = 256, class F>
-void for_bins(const RowBins &b, F f)
+__global__ void __launch_bounds__(kRowBS) k_cut_seg(RowSegs S, const eid_t *__restrict__ xadj,
+                                                    const vid_t *__restrict__ adjncy,
+                                                    const wgt_t *__restrict__ adjwgt,
+                                                    const int32_t *__restrict__ where, unsigned long long *out)
+{
+    int64_t tot = 0;
+    for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
+        int32_t v = S.row[s];
+        int32_t p = where[v];
+        eid_t j1 = min(S.j0[s] + (eid_t)kSegLen, xadj[v + 1]);
+        for (eid_t j = S.j0[s] + threadIdx.x; j < j1; j += kRowBS) tot += where[adjncy[j]] != p ? adjwgt[j] : 0;
+    }
+    tot = group_sum<kRowBS>(tot);
+    if (threadIdx.x == 0 && tot) atomicAdd(out, (unsigned long long)tot);
+}
+
+// Classes 0 and 1 (class 2 goes by segments).
+template <class F>
+void for_small_bins(const RowBins &b, F f)
 {
     if (b.count(0)) f(std::integral_constant<int, 8>{}, b.list(0), b.count(0));
     if (b.count(1)) f(std::integral_constant<int, 64>{}, b.list(1), b.count(1));
-    if (b.count(2)) f(std::integral_constant<int, MaxG < 256 ? MaxG : 256>{}, b.list(2), b.count(2));
 }
+
+inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, 16384); }
 
 __global__ void k_half(int64_t *d) { *d /= 2; }
 
-__global__ void k_pwgts(int32_t n, const wgt_t *__restrict__ vwgt, const int32_t *__restrict__ where,
-                        unsigned long long *pw)
+// Persistent blocks sum their vertices' weights per part in LDS and flush one
+// atomic per block and touched part.
+__global__ void __launch_bounds__(kBS) k_pwgts(int32_t n, int32_t k, const wgt_t *__restrict__ vwgt,
+                                               const int32_t *__restrict__ where, unsigned long long *pw)
 {
-    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (v < n) atomicAdd(&pw[where[v]], (unsigned long long)(int64_t)vwgt[v]);
+    extern __shared__ unsigned long long hist[];
+    for (int i = threadIdx.x; i < k; i += kBS) hist[i] = 0;
+    __syncthreads();
+    for (int64_t v = (int64_t)blockIdx.x * kBS + threadIdx.x; v < n; v += (int64_t)gridDim.x * kBS)
+        atomicAdd(&hist[where[v]], (unsigned long long)(int64_t)vwgt[v]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < k; i += kBS)
+        if (hist[i]) atomicAdd(&pw[i], hist[i]);
 }
 
 __global__ void k_project(int32_t n, const int32_t *__restrict__ cmap, const int32_t *__restrict__ wc,
@@ -705,7 +988,7 @@ __global__ void k_project(int32_t n, const int32_t *__restrict__ cmap, const int
 
 void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, const int64_t *pwgts,
                    const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest, int32_t *pregain,
-                   uint32_t *key)
+                   uint32_t *key, int32_t *conn)
 {
     const RowBins &b = g.bins;
     SelArgs A{k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl, dest, pregain, key};
@@ -714,16 +997,19 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
         HGP_LAUNCH_CHECK();
     }
     if (b.count(1)) {
-        int64_t blocks = std::min<int64_t>((b.count(1) + 3) / 4, 4096);
+        int64_t blocks = std::min<int64_t>((b.count(1) + kRowBS - 1) / kRowBS, 4096);
         k_jet_select_w<<<(unsigned)blocks, kRowBS, sizeof(int32_t) * 4 * (size_t)k, c.stream>>>(b.list(1),
                                                                                                b.count(1), A);
         HGP_LAUNCH_CHECK();
     }
-    if (b.count(2)) {
-        int64_t blocks = std::min<int64_t>(b.count(2), 4096);
-        k_jet_select_b<<<(unsigned)blocks, kRowBS, sizeof(int32_t) * (size_t)k, c.stream>>>(b.list(2), b.count(2),
-                                                                                           A);
+    if (g.segs.nseg) {
+        k_jet_select_seg<<<seg_grid(g.segs), kRowBS, sizeof(int32_t) * (size_t)k, c.stream>>>(g.segs, A, conn);
         HGP_LAUNCH_CHECK();
+        if (g.segs.nmulti) {
+            k_jet_select_fin<<<(unsigned)std::min<int32_t>(g.segs.nmulti, 4096), kRowBS, 0, c.stream>>>(g.segs, A,
+                                                                                                      conn);
+            HGP_LAUNCH_CHECK();
+        }
     }
 }
 
@@ -733,12 +1019,17 @@ void edge_cut(Ctx &c, const DevGraph &g, const int32_t *where, int64_t *d_cut)
 {
     fill<int64_t>(c, d_cut, 1, 0);
     if (g.n <= 0) return;
-    for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
+    for_small_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
         constexpr int G = decltype(gc)::value;
         k_cut<G><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt, where,
                                                              (unsigned long long *)d_cut);
         HGP_LAUNCH_CHECK();
     });
+    if (g.segs.nseg) {
+        k_cut_seg<<<seg_grid(g.segs), kRowBS, 0, c.stream>>>(g.segs, g.xadj, g.adjncy, g.adjwgt, where,
+                                                            (unsigned long long *)d_cut);
+        HGP_LAUNCH_CHECK();
+    }
     k_half<<<1, 1, 0, c.stream>>>(d_cut);
     HGP_LAUNCH_CHECK();
 }
@@ -747,7 +1038,9 @@ void part_weights(Ctx &c, int32_t n, const wgt_t *vwgt, const int32_t *where, in
 {
     fill<int64_t>(c, pwgts, k, 0);
     if (n <= 0) return;
-    k_pwgts<<<grid_for(n, kBS), kBS, 0, c.stream>>>(n, vwgt, where, (unsigned long long *)pwgts);
+    unsigned blocks = (unsigned)std::min<int64_t>(((int64_t)n + kBS - 1) / kBS, 1024);
+    k_pwgts<<<blocks, kBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(n, k, vwgt, where,
+                                                                              (unsigned long long *)pwgts);
     HGP_LAUNCH_CHECK();
 }
 
@@ -785,6 +1078,12 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     int64_t *wsc = c.arena.ralloc<int64_t>((size_t)n);
     int32_t *elist = c.arena.ralloc<int32_t>((size_t)n);
     int64_t *bs = c.arena.ralloc<int64_t>((size_t)n + 1);
+    int32_t *conn = c.arena.ralloc<int32_t>((size_t)g.segs.nmulti * (size_t)k);
+    int64_t *macc = c.arena.ralloc<int64_t>((size_t)g.segs.nmulti);
+    if (g.segs.nmulti) {
+        fill<int32_t>(c, conn, (int64_t)g.segs.nmulti * k, 0);
+        fill<int64_t>(c, macc, g.segs.nmulti, 0);
+    }
     int64_t *d_ne = c.arena.ralloc<int64_t>(1);
     fill<uint8_t>(c, lock, n, 0);
     HGP_CHECK_HIP(hipMemcpyAsync(best, where, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
@@ -803,18 +1102,33 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             // RW/RS: the same per-vertex traffic; rows only of overloaded parts (not counted).
             bool lp = h.mode == kModeLP;
             KScope ks(c, lp ? "jet_select_lp" : "jet_select_rb", (lp ? 12.0 * (double)g.m : 0.0) + 21.0 * n);
-            launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key);
+            launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn);
         }
         if (h.mode == kModeLP) {
             {
                 KScope ks(c, "jet_afterburner", 5.0 * n);
-                for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
-                    constexpr int G = decltype(gc)::value;
-                    k_afterburner<G><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy,
-                                                                                 g.adjwgt, where, dest, pregain,
-                                                                                 ctl, sel);
+                if (int64_t nb = g.bins.count(0)) {
+                    k_afterburner<8><<<grid_groups(nb, 8), kRowBS, 0, c.stream>>>(
+                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl, sel);
                     HGP_LAUNCH_CHECK();
-                });
+                }
+                if (int64_t nb = g.bins.count(1)) {
+                    unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 4096);
+                    k_afterburner_w<<<blocks, kRowBS, 0, c.stream>>>(g.bins.list(1), nb, g.xadj, g.adjncy,
+                                                                     g.adjwgt, where, dest, pregain, ctl, sel);
+                    HGP_LAUNCH_CHECK();
+                }
+                if (g.segs.nseg) {
+                    k_afterburner_seg<<<seg_grid(g.segs), kRowBS, 0, c.stream>>>(
+                        g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl, sel,
+                        (unsigned long long *)macc);
+                    HGP_LAUNCH_CHECK();
+                    if (g.segs.nmulti) {
+                        k_afterburner_fin<<<grid_for(g.segs.nmulti, kBS), kBS, 0, c.stream>>>(g.segs, pregain, ctl,
+                                                                                             sel, macc);
+                        HGP_LAUNCH_CHECK();
+                    }
+                }
             }
             k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock);
             HGP_LAUNCH_CHECK();
@@ -842,13 +1156,24 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         }
         {
             KScope ks(c, "jet_move_delta", 8.0 * n);
-            for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
-                constexpr int G = decltype(gc)::value;
-                int64_t blocks = std::min<int64_t>((nb + kRowBS / G - 1) / (kRowBS / G), 2048);
-                k_move_delta<G><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                    ids, nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts);
+            if (int64_t nb = g.bins.count(0)) {
+                int64_t blocks = std::min<int64_t>((nb + kRowBS / 8 - 1) / (kRowBS / 8), 2048);
+                k_move_delta<8><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                    g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts);
                 HGP_LAUNCH_CHECK();
-            });
+            }
+            if (int64_t nb = g.bins.count(1)) {
+                unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
+                k_move_delta_w<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                    g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts);
+                HGP_LAUNCH_CHECK();
+            }
+            if (g.segs.nseg) {
+                unsigned blocks = (unsigned)std::min<int64_t>(g.segs.nseg, 2048);
+                k_move_delta_seg<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                    g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts);
+                HGP_LAUNCH_CHECK();
+            }
         }
         k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where);
         HGP_LAUNCH_CHECK();

@@ -63,7 +63,7 @@ void *Arena::left(size_t bytes)
 {
     bytes = round_up(bytes);
     if (lo_ + bytes > hi_)
-        throw Error{-3, "arena: left allocation of " + std::to_string(bytes) + " bytes does not fit"};
+        throw Error{-3, "arena: left allocation of " + std::to_string(bytes) + " bytes does not fit", true};
     void *p = base_ + lo_;
     lo_ += bytes;
     if (lo_ + (cap_ - hi_) > peak_) peak_ = lo_ + (cap_ - hi_);
@@ -74,7 +74,7 @@ void *Arena::right(size_t bytes)
 {
     bytes = round_up(bytes);
     if (lo_ + bytes > hi_)
-        throw Error{-3, "arena: right allocation of " + std::to_string(bytes) + " bytes does not fit"};
+        throw Error{-3, "arena: right allocation of " + std::to_string(bytes) + " bytes does not fit", true};
     hi_ -= bytes;
     if (lo_ + (cap_ - hi_) > peak_) peak_ = lo_ + (cap_ - hi_);
     return base_ + hi_;

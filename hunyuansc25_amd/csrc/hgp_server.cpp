@@ -36,10 +36,14 @@ struct hgp_server {
     size_t peak = 0;
     std::vector<std::thread> workers;
 
+    std::vector<size_t> claim;   // bytes a busy slot may hold (its arena never exceeds it)
+
+    size_t held(size_t i) const { return busy[i] ? claim[i] : ctx[i]->arena.capacity(); }
+
     size_t reserved() const
     {
         size_t r = 0;
-        for (auto &c : ctx) r += c->arena.capacity();
+        for (size_t i = 0; i < ctx.size(); i++) r += held(i);
         return r;
     }
 
@@ -50,7 +54,7 @@ struct hgp_server {
         auto others = [&] {
             size_t r = 0;
             for (size_t i = 0; i < ctx.size(); i++)
-                if ((int)i != w) r += ctx[i]->arena.capacity();
+                if ((int)i != w) r += held(i);
             return r;
         };
         size_t own = ctx[w]->arena.capacity();
@@ -76,20 +80,39 @@ struct hgp_server {
             int64_t t = queue.front();
             queue.pop_front();
             Job *job = jobs[t].get();
+            size_t need = job->bytes;
             busy[w] = 1;
+            claim[w] = std::max(ctx[w]->arena.capacity(), need);
+            peak = std::max(peak, reserved());
             lk.unlock();
-            int rc = hgp::guarded([&] { hgp::exec_job(*ctx[w], job->o, job->j); });
+            // The arena is capped at the claim; a run that needs more goes back
+            // to the head of the queue with twice the claim (up to the budget).
+            bool again = false;
+            int rc = hgp::guarded([&] {
+                try {
+                    hgp::exec_job(*ctx[w], job->o, job->j, need, claim[w]);
+                } catch (const hgp::Error &e) {
+                    if (e.overflow && need < budget) {
+                        again = true;
+                        return;
+                    }
+                    throw;
+                }
+            });
             std::string msg = rc ? hgp::last_error_text() : "";
             lk.lock();
-            job->status = rc;
-            job->msg = msg;
-            job->t_done = Clock::now();
-            job->done = true;
-            completed++;
             busy[w] = 0;
-            peak = std::max(peak, reserved());
-            if (reserved() > budget) ctx[w]->arena.free_block();   // an arena retry grew past the budget
-            cv_done.notify_all();
+            if (again) {
+                job->bytes = std::min(2 * need, budget);
+                queue.push_front(t);
+            } else {
+                job->status = rc;
+                job->msg = msg;
+                job->t_done = Clock::now();
+                job->done = true;
+                completed++;
+                cv_done.notify_all();
+            }
             cv_work.notify_all();
         }
     }
@@ -110,6 +133,7 @@ int hgp_server_create(const hgp_options *opt, int32_t slots, int64_t budget_byte
         s->budget = (size_t)budget_bytes;
         for (int i = 0; i < slots; i++) s->ctx.push_back(hgp::create_context(s->device));
         s->busy.assign((size_t)slots, 0);
+        s->claim.assign((size_t)slots, 0);
         for (int i = 0; i < slots; i++) s->workers.emplace_back([p = s.get(), i] {
             (void)hipSetDevice(p->device);
             p->worker(i);

@@ -89,26 +89,24 @@ static inline int need_more(int64_t matched, int32_t n)
     return 4 * matched < 3 * (int64_t)n;
 }
 
-/* Leaf matches (hunyuangraph_GPU_match.h:1374-1423): unmatched degree-1
- * vertices hanging off the same hub are paired in the hub's row order. */
+/* Leaf matches (hunyuangraph_GPU_match.h:1374-1423): the unmatched degree-1
+ * vertices hanging off the same hub are paired consecutively in increasing id
+ * order (the reference pairs them in atomics order); a pair whose weight is
+ * not below maxvwgt stays unmatched. */
 static void fb_leaf(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
                     int32_t maxvwgt, int32_t *match)
 {
-    int32_t *cnt = (int32_t *)calloc((size_t)n, sizeof(int32_t));
-    for (int32_t v = 0; v < n; v++)
-        if (match[v] == -1 && deg_of(xadj, v) == 1) cnt[adjncy[xadj[v]]]++;
-    for (int32_t h = 0; h < n; h++) {
-        if (cnt[h] <= 1) continue;
-        int32_t flag = -1;
-        for (int64_t j = xadj[h]; j < xadj[h + 1]; j++) {
-            int32_t u = adjncy[j];
-            if (match[u] != -1 || deg_of(xadj, u) != 1) continue;
-            if (flag == -1) { flag = u; continue; }
-            if ((int64_t)vwgt[flag] + vwgt[u] < maxvwgt) { match[flag] = u; match[u] = flag; }
-            flag = -1;
-        }
+    int32_t *flag = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int32_t v = 0; v < n; v++) flag[v] = -1;
+    for (int32_t v = 0; v < n; v++) {
+        if (match[v] != -1 || deg_of(xadj, v) != 1) continue;
+        int32_t h = adjncy[xadj[v]];
+        if (flag[h] == -1) { flag[h] = v; continue; }
+        int32_t u = flag[h];
+        if ((int64_t)vwgt[u] + vwgt[v] < maxvwgt) { match[u] = v; match[v] = u; }
+        flag[h] = -1;
     }
-    free(cnt);
+    free(flag);
 }
 
 /* Isolate matches (hunyuangraph_GPU_match.h:1577-1594): degree-0 vertices in
@@ -174,18 +172,20 @@ static void fb_twin(int32_t n, const int64_t *xadj, const int32_t *adjncy, const
 
 /* Relative matches (hunyuangraph_GPU_match.h:1597-1680): every unmatched
  * vertex with neighbours picks dist = the neighbour of largest degree, ties to
- * the larger edge weight, then to the first in row order.  Hubs with two or
- * more such vertices pair them consecutively in row order.  The reference
- * pairs every marked neighbour of the hub and settles double claims with
- * reset_match; here a hub pairs only the vertices whose dist is that hub. */
+ * the larger edge weight, then to the first in row order.  The vertices that
+ * picked the same hub are paired consecutively in increasing id order.  The
+ * reference pairs every marked neighbour of the hub and settles double claims
+ * with reset_match; here a hub pairs only the vertices whose dist is that
+ * hub. */
 static void fb_relative(int32_t n, const int64_t *xadj, const int32_t *adjncy,
                         const int32_t *adjwgt, const int32_t *vwgt, int32_t maxvwgt,
                         int32_t *match)
 {
     int32_t *dist = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
-    int32_t *cnt = (int32_t *)calloc((size_t)(n > 0 ? n : 1), sizeof(int32_t));
+    int32_t *flag = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
     for (int32_t v = 0; v < n; v++) {
         dist[v] = -1;
+        flag[v] = -1;
         if (match[v] != -1) continue;
         int64_t bl = -1;
         int32_t bw = 0;
@@ -196,21 +196,17 @@ static void fb_relative(int32_t n, const int64_t *xadj, const int32_t *adjncy,
                 dist[v] = u; bl = l; bw = adjwgt[j];
             }
         }
-        if (dist[v] != -1) cnt[dist[v]]++;
     }
-    for (int32_t h = 0; h < n; h++) {
-        if (cnt[h] <= 1) continue;
-        int32_t flag = -1;
-        for (int64_t j = xadj[h]; j < xadj[h + 1]; j++) {
-            int32_t u = adjncy[j];
-            if (dist[u] != h || match[u] != -1 || u == flag) continue;
-            if (flag == -1) { flag = u; continue; }
-            if ((int64_t)vwgt[flag] + vwgt[u] < maxvwgt) { match[flag] = u; match[u] = flag; }
-            flag = -1;
-        }
+    for (int32_t v = 0; v < n; v++) {
+        int32_t h = dist[v];
+        if (h < 0) continue;
+        if (flag[h] == -1) { flag[h] = v; continue; }
+        int32_t u = flag[h];
+        if ((int64_t)vwgt[u] + vwgt[v] < maxvwgt) { match[u] = v; match[v] = u; }
+        flag[h] = -1;
     }
     free(dist);
-    free(cnt);
+    free(flag);
 }
 
 /* Forced matches (hunyuangraph_GPU_match.h:1682-1716): unmatched vertices in
