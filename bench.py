@@ -42,13 +42,15 @@ def host_cut_and_maxpart(xadj, adjncy, part, k):
 
 
 def cpu_baseline(scale, k, seed, hgp):
-    """Reference CPU path restated in C (oracle), single thread, bounded sample."""
+    """Reference CPU path restated in C (oracle), single thread, bounded sample.
+    The same sample is partitioned on the GPU too, for the cut comparison."""
     import oracle as O
     x, a, vw, aw = hgp.gen_rmat(scale, 16 << scale, seed=seed)
     E = len(a) // 2
     t0 = time.perf_counter()
     part, cut = O.cpu_rb_partition(x, a, k, vw, aw, seed=seed)
     dt = time.perf_counter() - t0
+    gpart, gst = hgp.partition(x, a, k, vw, aw, seed=seed)
     return {
         "value": round(E / dt / 1e6, 4),
         "unit": "M edges/s",
@@ -57,6 +59,145 @@ def cpu_baseline(scale, k, seed, hgp):
         "sample": f"rmat scale {scale} ({E} edges), k={k}, one partition in {dt:.2f} s",
         "cut": int(cut),
         "cut_fraction": round(cut / E, 4),
+        "gpu_cut_same_sample": int(gst["cut"]),
+        "gpu_cut_ratio": round(gst["cut"] / max(cut, 1), 4),
+    }
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary
+    of this workload (two rocprofv3 --pmc passes, FETCH_SIZE and WRITE_SIZE,
+    scaled by a 1 GiB copy), or None when there is none for it."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_pmc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[0] == kernel:
+            try:
+                return int(float(cells[5]))
+            except ValueError:
+                return None
+    return None
+
+
+def batch_graphs(hgp, count, seed):
+    """The batched workload: sizes log-uniform in [1k, 64k] vertices, cycling
+    2-D grids, 3-D random geometric graphs (mean degree 12) and banded
+    FEM-like patterns (bandwidth 32, 10% off-band entries); unit weights;
+    k drawn from {2, 4, 8, 16, 32, 64}."""
+    rng = np.random.default_rng(seed)
+    gs, ks = [], []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(1000), np.log(65536))))
+        kind = i % 3
+        if kind == 0:
+            side = max(2, int(round(np.sqrt(n))))
+            g = hgp.gen_grid2d(side, max(2, n // side))
+        elif kind == 1:
+            g = hgp.gen_rgg3d(n, hgp.rgg_radius(n, 12.0), seed=seed + i)
+        else:
+            g = hgp.gen_banded(n, 32, 0.1, seed=seed + i)
+        gs.append(g)
+        ks.append(int(rng.choice([2, 4, 8, 16, 32, 64])))
+    return gs, ks
+
+
+def bench_batch(hgp, count, lanes, seed):
+    """Graphs/s of the plain per-graph path (one hgp_partition call after
+    another) and of hgp_partition_batch on the same graphs; both from host
+    arrays.  The partitions must be identical."""
+    gs, ks = batch_graphs(hgp, count, seed)
+    E = sum(len(g[1]) // 2 for g in gs)
+    hgp.partition_batch(gs[: 2 * lanes], ks[: 2 * lanes], lanes=lanes, seed=seed)   # lanes' arenas warm
+    hgp.partition(gs[0][0], gs[0][1], ks[0], seed=seed)
+    t0 = time.perf_counter()
+    plain = [hgp.partition(g[0], g[1], k, g[2], g[3], seed=seed)[0] for g, k in zip(gs, ks)]
+    tp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    parts, _ = hgp.partition_batch(gs, ks, lanes=lanes, seed=seed)
+    tb = time.perf_counter() - t0
+    same = all(np.array_equal(p, q) for p, q in zip(parts, plain))
+    return {
+        "workload": f"batch_{count}_graphs_1k-64k",
+        "graphs": count,
+        "edges": int(E),
+        "lanes": lanes,
+        "plain_graphs_per_s": round(count / tp, 2),
+        "batched_graphs_per_s": round(count / tb, 2),
+        "speedup": round(tp / tb, 3),
+        "identical_to_plain": bool(same),
+    }
+
+
+def serve_requests(hgp, count, seed):
+    """Serving requests: sizes log-uniform in [10k, 4M] vertices, alternating
+    R-MAT (edge factor 16, power-of-two vertex counts) and 3-D random
+    geometric graphs (mean degree 12); k log-uniform over {8, ..., 512}."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        n = float(np.exp(rng.uniform(np.log(1e4), np.log(4e6))))
+        k = int(2 ** rng.integers(3, 10))
+        if i % 2 == 0:
+            scale = int(round(np.log2(n)))
+            g = hgp.gen_rmat(scale, 16 << scale, seed=seed + i)
+        else:
+            n = int(n)
+            g = hgp.gen_rgg3d(n, hgp.rgg_radius(n, 12.0), seed=seed + i)
+        reqs.append((g, k))
+    return reqs
+
+
+def bench_serve(hgp, count, slots, budget, seed):
+    """Poisson arrivals at a fixed rate 25% above what the plain path sustains
+    (measured on the same requests); latency = submit to result."""
+    import threading
+    reqs = serve_requests(hgp, count, seed)
+    t0 = time.perf_counter()
+    alone = [hgp.partition(g[0], g[1], k, g[2], g[3], seed=seed + i)[0] for i, (g, k) in enumerate(reqs)]
+    t_plain = time.perf_counter() - t0
+    rate = 1.25 * count / t_plain
+    rng = np.random.default_rng(seed + 7)
+    arrive = np.cumsum(rng.exponential(1.0 / rate, size=count))
+    lat = [0.0] * count
+    same = [False] * count
+    with hgp.Server(slots=slots, budget_bytes=budget) as srv:
+        tickets = [None] * count
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for i in range(count):
+                ready.acquire()
+                part, st, ms = srv.wait(tickets[i])
+                lat[i] = ms
+                same[i] = bool(np.array_equal(part, alone[i]))
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        start = time.perf_counter()
+        for i, (g, k) in enumerate(reqs):
+            delay = start + arrive[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            tickets[i] = srv.submit(g[0], g[1], k, g[2], g[3], seed=seed + i)
+            ready.release()
+        th.join()
+        span = time.perf_counter() - start
+        info = srv.info()
+    lat_s = np.sort(np.array(lat))
+    return {
+        "workload": f"serve_{count}_requests_10k-4M",
+        "requests": count,
+        "slots": slots,
+        "budget_GB": round(budget / 2**30, 1),
+        "arrival_rate_per_s": round(rate, 3),
+        "graphs_per_s": round(count / span, 3),
+        "plain_graphs_per_s": round(count / t_plain, 3),
+        "p50_ms": round(float(np.percentile(lat_s, 50)), 2),
+        "p99_ms": round(float(np.percentile(lat_s, 99)), 2),
+        "peak_reserved_GB": round(info["peak_reserved"] / 2**30, 2),
+        "identical_to_alone": bool(all(same)),
     }
 
 
@@ -72,6 +213,13 @@ def main():
     ap.add_argument("--cpu-scale", type=int, default=19)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ktimer", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="one 1 GiB device-to-device copy before the steps (counter calibration)")
+    ap.add_argument("--batch-graphs", type=int, default=256, help="batched-mode sample (0: skip)")
+    ap.add_argument("--lanes", type=int, default=8)
+    ap.add_argument("--serve-requests", type=int, default=24, help="serving-mode sample (0: skip)")
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -104,12 +252,19 @@ def main():
         return hgp.partition_device(n, d_x.data_ptr(), d_a.data_ptr(), d_vw.data_ptr(), d_aw.data_ptr(), args.k,
                                     d_part.data_ptr(), seed=args.seed)
 
+    if args.calib_copy:
+        src = torch.ones(1 << 28, dtype=torch.int32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1073741824 bytes read, 1073741824 bytes written")
+        del src, dst
+
     for i in range(args.warmup):
         st = step()
         log(f"warmup {i}: {st['ms_total']:.1f} ms device, cut {st['cut']}")
 
-    if not args.no_ktimer:
-        hgp.ktimer_enable(True)
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     stats = []
@@ -118,8 +273,14 @@ def main():
         log(f"step {i}: {stats[-1]['ms_total']:.1f} ms device")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    kt = hgp.ktimer_report() if not args.no_ktimer else {}
-    hgp.ktimer_enable(False)
+    # one more step, outside the timed region, with the per-kernel HIP-event timer on
+    kt = {}
+    if not args.no_ktimer:
+        hgp.ktimer_enable(True)
+        st_k = step()
+        kt = hgp.ktimer_report()
+        hgp.ktimer_enable(False)
+        log(f"kernel-timer step: {st_k['ms_total']:.1f} ms device")
     ms_per_step = elapsed / args.steps * 1e3
     value = E / (elapsed / args.steps) / 1e6
 
@@ -133,7 +294,8 @@ def main():
 
     roofline = None
     if kt:
-        dom = max(kt.items(), key=lambda kv: kv[1][1])
+        # the dominant kernel: largest total time among kernels with a byte model
+        dom = max(((k_, v) for k_, v in kt.items() if v[2] > 0), key=lambda kv: kv[1][1])
         name, (launches, ms, byts) = dom
         avg_ms = ms / launches
         achieved = (byts / launches) / (avg_ms * 1e-3) / 1e9
@@ -146,9 +308,21 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(f"rmat_s{args.scale}_e{args.edges_log2}_k{args.k}", name),
+            "alg_bytes_per_launch": round(byts / launches),
         }
     pipeline_gbs = st["alg_bytes"] / (st["ms_total"] * 1e-3) / 1e9
+
+    batched = None
+    if args.batch_graphs > 0:
+        log(f"batched mode: {args.batch_graphs} graphs, {args.lanes} lanes ...")
+        batched = bench_batch(hgp, args.batch_graphs, args.lanes, args.seed)
+        log(f"batched mode: {batched}")
+    serving = None
+    if args.serve_requests > 0:
+        log(f"serving mode: {args.serve_requests} requests, {args.slots} slots ...")
+        serving = bench_serve(hgp, args.serve_requests, args.slots, int(args.budget_gb * 2**30), args.seed)
+        log(f"serving mode: {serving}")
 
     cpu = None
     if not args.no_cpu:
@@ -186,11 +360,15 @@ def main():
         "phase_ms": {"coarsen": round(st["ms_coarsen"], 3), "init": round(st["ms_init"], 3),
                      "uncoarsen": round(st["ms_uncoarsen"], 3)},
         "levels": st["levels"],
+        "arena_GB": round(st["arena_bytes"] / 2**30, 2),
+        "arena_peak_GB": round(st["arena_peak"] / 2**30, 2),
         "refine_iters": st["refine_iters"],
         "pipeline_alg_GBs": round(pipeline_gbs, 2),
         "roofline": roofline,
         "kernels": {k: [v[0], round(v[1], 3), v[2]] for k, v in kt.items()},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
     }
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)

@@ -62,6 +62,8 @@ class HgpStats(C.Structure):
         ("level_n", C.c_int64 * MAX_LEVELS),
         ("level_m", C.c_int64 * MAX_LEVELS),
         ("level_iters", C.c_int32 * MAX_LEVELS),
+        ("arena_bytes", C.c_int64),
+        ("arena_peak", C.c_int64),
     ]
 
     def to_dict(self) -> dict:
@@ -83,6 +85,8 @@ class HgpStats(C.Structure):
             "level_n": [int(self.level_n[i]) for i in range(L + 1)],
             "level_m": [int(self.level_m[i]) for i in range(L + 1)],
             "level_iters": [int(self.level_iters[i]) for i in range(L + 1)],
+            "arena_bytes": int(self.arena_bytes),
+            "arena_peak": int(self.arena_peak),
         }
 
 

@@ -46,8 +46,11 @@ double alg_bytes(const std::vector<int64_t> &ns, const std::vector<int64_t> &ms,
 
 size_t estimate_arena(int32_t n, int64_t m, int32_t k, bool host_copy)
 {
-    // + the k-int rows of hub rows cut into segments (at most m/4096 rows)
-    size_t b = (size_t)64 * (size_t)m + (size_t)192 * (size_t)n + ((size_t)64 << 20) + (size_t)k * 4096 +
+    // The level stack dominates: power-law graphs keep most of their entries
+    // for several levels (the sum over levels reaches ~10 m), plus the
+    // contraction temporaries of the largest level and the k-int rows of hub
+    // rows cut into segments (at most m/4096 rows).
+    size_t b = (size_t)112 * (size_t)m + (size_t)256 * (size_t)n + ((size_t)64 << 20) + (size_t)k * 4096 +
                ((size_t)m / 1024 + 1) * (size_t)k;
     if (host_copy) b += (size_t)8 * (size_t)(n + 1) + (size_t)8 * (size_t)m + (size_t)8 * (size_t)n;
     return b;
@@ -156,6 +159,8 @@ void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_
         }
         st->refine_iters = tot;
         st->alg_bytes = alg_bytes(ns, ms, iters);
+        st->arena_bytes = (int64_t)c.arena.capacity();
+        st->arena_peak = (int64_t)c.arena.high_water();
     }
 }
 
@@ -449,6 +454,7 @@ int hgp_ktimer_enable(int on)
         t.on = on != 0;
         t.recs.clear();
         t.used = 0;
+        t.ncnt = 0;
     });
 }
 
@@ -460,6 +466,11 @@ int64_t hgp_ktimer_report(char *buf, int64_t len)
         KTimer &t = ktimer();
         struct Agg { int64_t n = 0; double ms = 0, bytes = 0; };
         std::vector<std::pair<std::string, Agg>> agg;
+        std::vector<unsigned long long> cnt((size_t)t.ncnt);
+        if (t.ncnt > 0) {
+            HGP_CHECK_HIP(hipDeviceSynchronize());
+            HGP_CHECK_HIP(hipMemcpy(cnt.data(), t.dcnt, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost));
+        }
         for (auto &r : t.recs) {
             HGP_CHECK_HIP(hipEventSynchronize(r.b));
             float ms = 0;
@@ -468,7 +479,10 @@ int64_t hgp_ktimer_report(char *buf, int64_t len)
             if (it == agg.end()) { agg.push_back({r.name, Agg{}}); it = agg.end() - 1; }
             it->second.n++;
             it->second.ms += ms;
-            it->second.bytes += r.bytes;
+            double dyn = 0;
+            if (r.slot >= 0)
+                for (int i = 0; i < kKbLanes; i++) dyn += (double)cnt[(size_t)r.slot + i];
+            it->second.bytes += r.bytes + dyn;
         }
         char tmp[256];
         for (size_t i = 0; i < agg.size(); i++) {

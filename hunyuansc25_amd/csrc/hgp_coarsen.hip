@@ -65,33 +65,40 @@ struct BetterEdge {
 template <int G>
 __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ ids, int64_t cnt,
                                                     const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
-                                                    const wgt_t *__restrict__ adjwgt, const wgt_t *__restrict__ vwgt,
-                                                    const int32_t *__restrict__ match, int32_t maxvwgt,
+                                                    const wgt_t *__restrict__ adjwgt,
+                                                    const int32_t *__restrict__ avail, int32_t maxvwgt,
                                                     uint64_t seed, int32_t level, int32_t *__restrict__ prop,
-                                                    const int32_t *prev_progress)
+                                                    const int32_t *prev_progress, unsigned long long *kb)
 {
     if (prev_progress && *prev_progress == 0) return;
     int64_t gi = gindex<G>();
     int lane = glane<G>();
-    if (gi >= cnt) return;
-    int32_t v = ids[gi];
-    if (match[v] != -1) {
-        if (lane == 0) prop[v] = -1;
-        return;
+    uint64_t kbytes = 0;
+    if (gi < cnt) {   // whole groups take the same branches
+        int32_t v = ids[gi];
+        int32_t wv = avail[v];
+        if (wv < 0) {
+            if (lane == 0) prop[v] = -1;
+        } else {
+            EdgeKey best{-1, 0, 0, 0u};
+            eid_t beg = xadj[v], end = xadj[v + 1];
+            for (eid_t j = beg + lane; j < end; j += G) {
+                int32_t u = adjncy[j];
+                int32_t wu = avail[u];
+                if (u == v || wu < 0) continue;
+                if ((int64_t)wu + wv >= maxvwgt) continue;
+                EdgeKey e{adjwgt[j], u < v ? u : v, u < v ? v : u, 0u};
+                e.h = edge_hash(seed, level, e.lo, e.hi);
+                if (BetterEdge()(e, best)) best = e;
+            }
+            best = group_best<G>(best, BetterEdge());
+            if (lane == 0) {
+                prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
+                kbytes = 16 + 12 * (uint64_t)(end - beg);   // xadj; adjncy, adjwgt, avail[u]
+            }
+        }
     }
-    int32_t wv = vwgt[v];
-    EdgeKey best{-1, 0, 0, 0u};
-    eid_t end = xadj[v + 1];
-    for (eid_t j = xadj[v] + lane; j < end; j += G) {
-        int32_t u = adjncy[j];
-        if (u == v || match[u] != -1) continue;
-        if ((int64_t)vwgt[u] + wv >= maxvwgt) continue;
-        EdgeKey e{adjwgt[j], u < v ? u : v, u < v ? v : u, 0u};
-        e.h = edge_hash(seed, level, e.lo, e.hi);
-        if (BetterEdge()(e, best)) best = e;
-    }
-    best = group_best<G>(best, BetterEdge());
-    if (lane == 0) prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
+    kb_flush(kb, kbytes);
 }
 
 // Class-2 rows by segments (a hub row is shared by several blocks): the
@@ -100,26 +107,28 @@ __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ 
 __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *__restrict__ xadj,
                                                         const vid_t *__restrict__ adjncy,
                                                         const wgt_t *__restrict__ adjwgt,
-                                                        const wgt_t *__restrict__ vwgt,
-                                                        const int32_t *__restrict__ match, int32_t maxvwgt,
+                                                        const int32_t *__restrict__ avail, int32_t maxvwgt,
                                                         uint64_t seed, int32_t level, int32_t *__restrict__ prop,
-                                                        EdgeKey *__restrict__ sbest, const int32_t *prev_progress)
+                                                        EdgeKey *__restrict__ sbest, const int32_t *prev_progress,
+                                                        unsigned long long *kb)
 {
     if (prev_progress && *prev_progress == 0) return;
+    uint64_t kbytes = 0;
     for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
         int32_t v = S.row[s];
         int32_t mi = S.mi[s];
-        if (match[v] != -1) {
+        int32_t wv = avail[v];
+        if (wv < 0) {
             if (threadIdx.x == 0 && mi < 0) prop[v] = -1;
             continue;
         }
-        int32_t wv = vwgt[v];
         EdgeKey best{-1, 0, 0, 0u};
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
         for (eid_t j = j0 + threadIdx.x; j < j1; j += kRowBS) {
             int32_t u = adjncy[j];
-            if (u == v || match[u] != -1) continue;
-            if ((int64_t)vwgt[u] + wv >= maxvwgt) continue;
+            int32_t wu = avail[u];
+            if (u == v || wu < 0) continue;
+            if ((int64_t)wu + wv >= maxvwgt) continue;
             EdgeKey e{adjwgt[j], u < v ? u : v, u < v ? v : u, 0u};
             e.h = edge_hash(seed, level, e.lo, e.hi);
             if (BetterEdge()(e, best)) best = e;
@@ -128,8 +137,10 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
         if (threadIdx.x == 0) {
             if (mi < 0) prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
             else sbest[s] = best;
+            kbytes += 8 + 12 * (uint64_t)(j1 - j0);   // xadj[v+1]; adjncy, adjwgt, avail[u]
         }
     }
+    kb_flush(kb, kbytes);
 }
 
 __global__ void k_propose_fin(RowSegs S, const int32_t *__restrict__ match, const EdgeKey *__restrict__ sbest,
@@ -149,8 +160,10 @@ __global__ void k_propose_fin(RowSegs S, const int32_t *__restrict__ match, cons
     prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
 }
 
-__global__ void k_accept(int32_t n, const int32_t *__restrict__ prop, int32_t *__restrict__ match, int32_t *progress,
-                         const int32_t *prev_progress)
+// Mutual proposals match; avail[v] (vwgt of a free vertex, -1 once matched)
+// lets the proposals read one array per neighbour.
+__global__ void k_accept(int32_t n, const int32_t *__restrict__ prop, int32_t *__restrict__ match,
+                         int32_t *__restrict__ avail, int32_t *progress, const int32_t *prev_progress)
 {
     if (prev_progress && *prev_progress == 0) return;
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
@@ -159,6 +172,7 @@ __global__ void k_accept(int32_t n, const int32_t *__restrict__ prop, int32_t *_
         int32_t u = prop[v];
         if (u >= 0 && prop[u] == v) {
             match[v] = u;
+            avail[v] = -1;
             got = true;
         }
     }
@@ -477,19 +491,14 @@ __device__ inline int32_t group_inclusive_scan(int32_t v)
     return v;
 }
 
-// Rows of <= G entries, one entry per lane.
 template <int G>
-__global__ void __launch_bounds__(kRowBS) k_seg_reg(const int32_t *__restrict__ ids, int64_t cnt,
-                                                    const int32_t *__restrict__ rep, const int32_t *__restrict__ match,
-                                                    const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
-                                                    const wgt_t *__restrict__ adjwgt, const int32_t *__restrict__ cmap,
-                                                    const int64_t *__restrict__ toff, int32_t *__restrict__ tadj,
-                                                    int32_t *__restrict__ twgt, int32_t *__restrict__ ccnt)
+__device__ inline void seg_reg_row(int32_t c, int lane, const int32_t *__restrict__ rep,
+                                   const int32_t *__restrict__ match, const eid_t *__restrict__ xadj,
+                                   const vid_t *__restrict__ adjncy, const wgt_t *__restrict__ adjwgt,
+                                   const int32_t *__restrict__ cmap, const int64_t *__restrict__ toff,
+                                   int32_t *__restrict__ tadj, int32_t *__restrict__ twgt, int32_t *__restrict__ ccnt,
+                                   uint64_t &kbytes)
 {
-    int64_t gi = gindex<G>();
-    int lane = glane<G>();
-    if (gi >= cnt) return;   // whole groups leave together
-    int32_t c = ids[gi];
     int32_t v = rep[c], u = match[v];
     CatRow row(xadj, v, u);
     eid_t len = row.dv + (u != v ? xadj[u + 1] - row.bu : 0);
@@ -515,7 +524,29 @@ __global__ void __launch_bounds__(kRowBS) k_seg_reg(const int32_t *__restrict__ 
         tadj[o] = (int32_t)key;
         twgt[o] = incl - (below ? prev : 0);
     }
-    if (lane == 0) ccnt[c] = __popcll(tm);
+    if (lane == 0) {
+        ccnt[c] = __popcll(tm);
+        kbytes += 8 * (uint64_t)__popcll(tm);
+    }
+}
+
+// Rows of <= G entries, one entry per lane.
+template <int G>
+__global__ void __launch_bounds__(kRowBS) k_seg_reg(const int32_t *__restrict__ ids, int64_t cnt,
+                                                    const int32_t *__restrict__ rep, const int32_t *__restrict__ match,
+                                                    const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                    const wgt_t *__restrict__ adjwgt, const int32_t *__restrict__ cmap,
+                                                    const int64_t *__restrict__ toff, int32_t *__restrict__ tadj,
+                                                    int32_t *__restrict__ twgt, int32_t *__restrict__ ccnt,
+                                                    unsigned long long *kb)
+{
+    int64_t gi = gindex<G>();
+    int lane = glane<G>();
+    uint64_t kbytes = 0;
+    if (gi < cnt) {   // whole groups take the same branches
+        seg_reg_row<G>(ids[gi], lane, rep, match, xadj, adjncy, adjwgt, cmap, toff, tadj, twgt, ccnt, kbytes);
+    }
+    kb_flush(kb, kbytes);
 }
 
 // Rows of 65..kSegLds entries: one block per row, bitonic sort in LDS padded
@@ -528,9 +559,10 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
                                                     const wgt_t *__restrict__ adjwgt,
                                                     const int32_t *__restrict__ cmap, const int64_t *__restrict__ toff,
                                                     int32_t *__restrict__ tadj, int32_t *__restrict__ twgt,
-                                                    int32_t *__restrict__ ccnt)
+                                                    int32_t *__restrict__ ccnt, unsigned long long *kb)
 {
     __shared__ uint32_t sk[kSegLds];
+    uint64_t kbytes = 0;
     __shared__ int32_t sw[kSegLds];
     __shared__ int64_t sh[kSegBS / kWave + 1];
     const int tid = threadIdx.x;
@@ -615,9 +647,13 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
             tadj[o + r] = (int32_t)sk[r];
             twgt[o + r] = sw[r] - (r > 0 ? sw[r - 1] : 0);
         }
-        if (tid == 0) ccnt[c] = ntot;
+        if (tid == 0) {
+            ccnt[c] = ntot;
+            kbytes += 8 * (uint64_t)ntot;
+        }
         __syncthreads();
     }
+    kb_flush(kb, kbytes);
 }
 
 // Rows longer than kSegLds: keys (list index << 32 | coarse id) of all such
@@ -712,18 +748,22 @@ __global__ void __launch_bounds__(kRowBS) k_seg_compact(const int32_t *__restric
                                                         const eid_t *__restrict__ cxadj,
                                                         const int32_t *__restrict__ tadj,
                                                         const int32_t *__restrict__ twgt, vid_t *__restrict__ cadjncy,
-                                                        wgt_t *__restrict__ cadjwgt)
+                                                        wgt_t *__restrict__ cadjwgt, unsigned long long *kb)
 {
     int64_t gi = gindex<G>();
     int lane = glane<G>();
-    if (gi >= cnt) return;
-    int32_t c = ids[gi];
-    int64_t src = toff[c];
-    eid_t dst = cxadj[c], l = cxadj[c + 1] - dst;
-    for (eid_t j = lane; j < l; j += G) {
-        cadjncy[dst + j] = tadj[src + j];
-        cadjwgt[dst + j] = twgt[src + j];
+    uint64_t kbytes = 0;
+    if (gi < cnt) {
+        int32_t c = ids[gi];
+        int64_t src = toff[c];
+        eid_t dst = cxadj[c], l = cxadj[c + 1] - dst;
+        for (eid_t j = lane; j < l; j += G) {
+            cadjncy[dst + j] = tadj[src + j];
+            cadjwgt[dst + j] = twgt[src + j];
+        }
+        if (lane == 0) kbytes = 16 * (uint64_t)l;
     }
+    kb_flush(kb, kbytes);
 }
 
 int64_t count_matched(Ctx &c, int32_t n, const int32_t *match, int64_t *d_cnt)
@@ -792,40 +832,50 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
     if (n <= 0) return 0;
     size_t mark = c.arena.right_mark();
     int32_t *prop = c.arena.ralloc<int32_t>((size_t)n);
+    int32_t *avail = c.arena.ralloc<int32_t>((size_t)n);
     EdgeKey *sbest = c.arena.ralloc<EdgeKey>((size_t)g.segs.nseg);
     int32_t *progress = c.arena.ralloc<int32_t>((size_t)(rounds > 0 ? rounds : 1));
     int64_t *d_cnt = c.arena.ralloc<int64_t>(2);
     fill<int32_t>(c, match, n, -1);
+    HGP_CHECK_HIP(hipMemcpyAsync(avail, g.vwgt, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
     fill<int32_t>(c, progress, rounds > 0 ? rounds : 1, 0);
     unsigned gv = grid_for(n, kBS);
     for (int r = 0; r < rounds; r++) {
         const int32_t *prev = r > 0 ? progress + r - 1 : nullptr;
         {
-            // per level: rows (adjncy, adjwgt) + gathers of match/vwgt per entry; xadj, match, vwgt, prop per vertex
-            KScope ks(c, "match_propose", 16.0 * (double)g.m + 28.0 * n);
+            // fixed bytes: ids, avail[v], prop per row (24 per segment); each free row adds its own in the kernel
             for (int cl = 0; cl < 2; cl++) {
                 int64_t cnt = g.bins.count(cl);
                 if (!cnt) continue;
+                KScope ks(c, cl == 0 ? "k_propose_s" : "k_propose_w", 12.0 * (double)cnt, true);
                 if (cl == 0)
-                    k_propose<8><<<grid_groups(cnt, 8), kRowBS, 0, c.stream>>>(
-                        g.bins.list(0), cnt, g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, maxvwgt, seed, level, prop, prev);
+                    k_propose<8><<<grid_groups(cnt, 8), kRowBS, 0, c.stream>>>(g.bins.list(0), cnt, g.xadj, g.adjncy,
+                                                                               g.adjwgt, avail, maxvwgt, seed, level,
+                                                                               prop, prev, ks.dev());
                 else
-                    k_propose<64><<<grid_groups(cnt, 64), kRowBS, 0, c.stream>>>(
-                        g.bins.list(1), cnt, g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, maxvwgt, seed, level, prop, prev);
+                    k_propose<64><<<grid_groups(cnt, 64), kRowBS, 0, c.stream>>>(g.bins.list(1), cnt, g.xadj,
+                                                                                 g.adjncy, g.adjwgt, avail, maxvwgt,
+                                                                                 seed, level, prop, prev, ks.dev());
                 HGP_LAUNCH_CHECK();
             }
             if (g.segs.nseg) {
-                k_propose_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
-                    g.segs, g.xadj, g.adjncy, g.adjwgt, g.vwgt, match, maxvwgt, seed, level, prop, sbest, prev);
-                HGP_LAUNCH_CHECK();
+                {
+                    KScope ks(c, "k_propose_seg", 24.0 * (double)g.segs.nseg, true);
+                    k_propose_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
+                        g.segs, g.xadj, g.adjncy, g.adjwgt, avail, maxvwgt, seed, level, prop, sbest, prev, ks.dev());
+                    HGP_LAUNCH_CHECK();
+                }
                 if (g.segs.nmulti) {
                     k_propose_fin<<<grid_for(g.segs.nmulti, kBS), kBS, 0, c.stream>>>(g.segs, match, sbest, prop, prev);
                     HGP_LAUNCH_CHECK();
                 }
             }
         }
-        k_accept<<<gv, kBS, 0, c.stream>>>(n, prop, match, progress + r, prev);
-        HGP_LAUNCH_CHECK();
+        {
+            KScope ks(c, "k_accept", 16.0 * n);   // match, prop, prop[u]; match and avail writes
+            k_accept<<<gv, kBS, 0, c.stream>>>(n, prop, match, avail, progress + r, prev);
+            HGP_LAUNCH_CHECK();
+        }
     }
     int64_t matched = count_matched(c, n, match, d_cnt);
     if (need_more(matched, n)) {
@@ -966,22 +1016,26 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     int32_t *ccnt = c.arena.ralloc<int32_t>((size_t)cn);
     RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegLds);
     {
-        // per fine entry: adjncy, adjwgt, cmap gather, merged write (upper bound); per coarse row: rep, match, 2 xadj, toff, count
-        KScope ks(c, "contract_sort", 20.0 * (double)m + 36.0 * cn);
+        // per coarse row: ids, rep, match, 2 xadj pairs, toff, count (56 B); per fine entry: adjncy, adjwgt,
+        // cmap gather (12 B); the kernels count the merged entries they write (8 B each)
+        auto bytes = [&](int cl) { return 56.0 * (double)cb.count(cl) + 12.0 * (double)cb.ent[cl]; };
         if (cb.count(0)) {
+            KScope ks(c, "k_seg_reg8", bytes(0), true);
             k_seg_reg<8><<<grid_groups(cb.count(0), 8), kRowBS, 0, c.stream>>>(
-                cb.list(0), cb.count(0), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt);
+                cb.list(0), cb.count(0), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (cb.count(1)) {
+            KScope ks(c, "k_seg_reg64", bytes(1), true);
             k_seg_reg<64><<<grid_groups(cb.count(1), 64), kRowBS, 0, c.stream>>>(
-                cb.list(1), cb.count(1), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt);
+                cb.list(1), cb.count(1), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (cb.count(2)) {
+            KScope ks(c, "k_seg_lds", bytes(2), true);
             unsigned blocks = (unsigned)std::min<int64_t>(cb.count(2), 8192);
             k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(2), cb.count(2), rep, match, g.xadj, g.adjncy,
-                                                       g.adjwgt, cmap, toff, tadj, twgt, ccnt);
+                                                       g.adjwgt, cmap, toff, tadj, twgt, ccnt, ks.dev());
             HGP_LAUNCH_CHECK();
         }
     }
@@ -1001,7 +1055,7 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
         int32_t *wsc = c.arena.ralloc<int32_t>((size_t)tl);
         int32_t *start = c.arena.ralloc<int32_t>((size_t)tl);
         {
-            KScope ks(c, "contract_long_gather", 20.0 * (double)tl + 36.0 * nl);
+            KScope ks(c, "k_long_gather", 24.0 * (double)tl + 44.0 * nl);   // + 12 B in per entry, key and val out
             k_long_gather<<<(unsigned)std::min<int64_t>(nl, 8192), kRowBS, 0, c.stream>>>(
                 lids, nl, cn, rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, loff, key, val);
             HGP_LAUNCH_CHECK();
@@ -1027,18 +1081,21 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     cg.adjncy = c.arena.lalloc<vid_t>((size_t)(cm > 0 ? cm : 1));
     cg.adjwgt = c.arena.lalloc<wgt_t>((size_t)(cm > 0 ? cm : 1));
     {
-        KScope ks(c, "contract_compact", 16.0 * (double)cm + 20.0 * cn);
         for (int cl = 0; cl < 4; cl++) {
             if (!cb.count(cl)) continue;
+            static const char *names[4] = {"k_seg_compact8", "k_seg_compact64", "k_seg_compact256",
+                                            "k_seg_compact256"};
+            // per row: ids, toff, cxadj pair (28 B); the kernel counts the merged row read and written (16 B per entry)
+            KScope ks(c, names[cl], 28.0 * (double)cb.count(cl), true);
             if (cl == 0)
                 k_seg_compact<8><<<grid_groups(cb.count(cl), 8), kRowBS, 0, c.stream>>>(
-                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt);
+                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt, ks.dev());
             else if (cl == 1)
                 k_seg_compact<64><<<grid_groups(cb.count(cl), 64), kRowBS, 0, c.stream>>>(
-                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt);
+                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt, ks.dev());
             else
                 k_seg_compact<256><<<grid_groups(cb.count(cl), 256), kRowBS, 0, c.stream>>>(
-                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt);
+                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt, ks.dev());
             HGP_LAUNCH_CHECK();
         }
     }

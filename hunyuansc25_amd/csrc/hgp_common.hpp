@@ -91,12 +91,16 @@ void destroy_context(std::unique_ptr<Ctx> c);      // releases a private context
 
 // ------------------------------------------------------------------ kernel timer
 // Optional per-kernel timing with HIP events on the context's stream: a
-// KScope records an event pair around the launches in its scope and the
-// algorithmic HBM bytes those launches must move.  Off unless enabled.
+// KScope records an event pair around the launch in its scope and the
+// algorithmic HBM bytes the launch must move: a fixed part known on the host
+// plus, for kernels whose work depends on the data (rows of free, eligible or
+// moved vertices), a device counter the kernel adds its row bytes to.  Off
+// unless enabled.
 struct KRec {
     const char *name;
     hipEvent_t a, b;
     double bytes;
+    int64_t slot;                // device counter slot, -1 if none
 };
 
 struct KTimer {
@@ -105,15 +109,20 @@ struct KTimer {
     std::vector<KRec> recs;
     std::vector<hipEvent_t> pool;
     size_t used = 0;
+    unsigned long long *dcnt = nullptr;   // device counters (bytes)
+    int64_t ncnt = 0, cnt_cap = 0;
     hipEvent_t get();
+    int64_t counter_slot();      // -1 when the counters are exhausted
 };
 
 KTimer &ktimer();
 
 class KScope {
   public:
-    KScope(Ctx &c, const char *name, double bytes);
+    KScope(Ctx &c, const char *name, double bytes, bool counted = false);
     ~KScope();
+    // Device counter for the launch's data-dependent bytes (nullptr when off).
+    unsigned long long *dev() const { return dev_; }
 
   private:
     Ctx &c_;
@@ -121,6 +130,8 @@ class KScope {
     hipEvent_t a_ = nullptr, b_ = nullptr;
     const char *name_;
     double bytes_;
+    int64_t slot_ = -1;
+    unsigned long long *dev_ = nullptr;
 };
 
 // Copies `bytes` from device to the pinned buffer and waits.
@@ -226,6 +237,25 @@ __device__ inline T block_exclusive_scan(T v, T *sh, T *total)
     if (total) *total = sh[nw];
     __syncthreads();
     return r;
+}
+
+// Kernel-timer byte counters come in groups of kKbLanes, spread over by block
+// index, so that the timed kernels do not contend on one address.
+constexpr int kKbLanes = 64;
+
+// Adds each thread's byte count to the launch's kernel-timer counters: the
+// block sums in LDS, then one atomic per block.  Every thread of the block
+// must call it (kb may be null: timer off, then it returns at once).
+__device__ inline void kb_flush(unsigned long long *kb, uint64_t v)
+{
+    if (!kb) return;
+    __shared__ unsigned long long kb_acc;
+    if (threadIdx.x == 0) kb_acc = 0;
+    __syncthreads();
+    v = wave_sum(v);
+    if (lane_id() == 0 && v) atomicAdd(&kb_acc, (unsigned long long)v);
+    __syncthreads();
+    if (threadIdx.x == 0 && kb_acc) atomicAdd(&kb[blockIdx.x % kKbLanes], kb_acc);
 }
 
 }  // namespace hgp

@@ -20,6 +20,7 @@ constexpr int kBinG[3] = {8, 64, 256};  // lanes per row of each class
 struct RowBins {
     int32_t *ids = nullptr;   // row ids grouped by class (order inside a class is arbitrary)
     int64_t off[5] = {0, 0, 0, 0, 0};
+    int64_t ent[4] = {0, 0, 0, 0};   // sum of the row lengths of each class
     int64_t count(int c) const { return off[c + 1] - off[c]; }
     const int32_t *list(int c) const { return ids + off[c]; }
 };

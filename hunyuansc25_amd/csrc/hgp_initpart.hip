@@ -567,7 +567,7 @@ void initial_partition(Ctx &c, const DevGraph &g, int32_t k, const Params &p, in
         k_ip_offsets<<<1, 64, 0, c.stream>>>(act, nact, nodes, lds_cap, total);
         HGP_LAUNCH_CHECK();
         int64_t waves = std::min<int64_t>((int64_t)nact * p.trials, n);
-        KScope ks(c, "ip_trials", 0.0);
+        KScope ks(c, "k_ip_trials", 0.0);
         k_ip_trials<<<(unsigned)std::max<int64_t>(waves, 1), kWave, lds_bytes, c.stream>>>(
             act, nact, nodes, total, fx[cb], fadj[cb], fw[cb], fvw[cb], p.seed, p.fm_passes, lds_cap, trial_where,
             gscratch, tcut);

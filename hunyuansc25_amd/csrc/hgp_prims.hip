@@ -69,7 +69,7 @@ void scan_impl(Ctx &c, const T *in, T *out, int64_t n)
     HGP_LAUNCH_CHECK();
     scan_impl<T>(c, sums, sums, tiles);
     {
-        KScope ks(c, "scan_tiles", 2.0 * sizeof(T) * (double)n);
+        KScope ks(c, sizeof(T) == 8 ? "k_tile_scan<i64>" : "k_tile_scan<i32>", 2.0 * sizeof(T) * (double)n);
         k_tile_scan<T><<<(unsigned)tiles, kScanBS, 0, c.stream>>>(in, out, n, sums);
         HGP_LAUNCH_CHECK();
     }
@@ -213,13 +213,14 @@ void radix_impl(Ctx &c, K *keys, uint32_t *vals, int64_t n, int lo, int hi)
     int passes = 0;
     for (int shift = lo; shift < hi; shift += 8) {
         {
-            KScope ks(c, "radix_hist", (double)sizeof(K) * (double)n);
+            KScope ks(c, sizeof(K) == 8 ? "k_rs_hist<u64>" : "k_rs_hist<u32>", (double)sizeof(K) * (double)n);
             k_rs_hist<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, n, shift, ntiles, hist);
             HGP_LAUNCH_CHECK();
         }
         scan_impl<int32_t>(c, (const int32_t *)hist, (int32_t *)incl, (int64_t)ntiles * 256);
         {
-            KScope ks(c, "radix_scatter", 2.0 * (double)(sizeof(K) + 4) * (double)n);
+            KScope ks(c, sizeof(K) == 8 ? "k_rs_scatter<u64>" : "k_rs_scatter<u32>",
+                      2.0 * (double)(sizeof(K) + 4) * (double)n);
             k_rs_scatter<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, src_v, dst_k, dst_v, n, shift, ntiles, hist, incl);
             HGP_LAUNCH_CHECK();
         }
@@ -260,18 +261,31 @@ __global__ void __launch_bounds__(256) k_bin_count(int32_t n, const eid_t *__res
                                                    const int64_t *__restrict__ len, BinThr t, unsigned long long *cnt)
 {
     __shared__ uint32_t bc[4];
-    if (threadIdx.x < 4) bc[threadIdx.x] = 0;
+    __shared__ unsigned long long be[4];
+    if (threadIdx.x < 4) {
+        bc[threadIdx.x] = 0;
+        be[threadIdx.x] = 0;
+    }
     __syncthreads();
-    for (int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x; v < n; v += (int64_t)gridDim.x * 256) {
-        int cls = t.cls(xadj ? xadj[v + 1] - xadj[v] : len[v]);
+    for (int64_t v0 = (int64_t)blockIdx.x * 256; v0 < n; v0 += (int64_t)gridDim.x * 256) {
+        int64_t v = v0 + threadIdx.x;
+        int64_t d = v < n ? (xadj ? xadj[v + 1] - xadj[v] : len[v]) : 0;
+        int cls = v < n ? t.cls(d) : -1;
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             uint64_t b = __ballot(cls == c);
-            if (lane_id() == 0 && b) atomicAdd(&bc[c], (uint32_t)__popcll(b));
+            uint64_t e = wave_sum((uint64_t)(cls == c ? d : 0));
+            if (lane_id() == 0 && b) {
+                atomicAdd(&bc[c], (uint32_t)__popcll(b));
+                atomicAdd(&be[c], (unsigned long long)e);
+            }
         }
     }
     __syncthreads();
-    if (threadIdx.x < 4 && bc[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], (unsigned long long)bc[threadIdx.x]);
+    if (threadIdx.x < 4 && bc[threadIdx.x]) {
+        atomicAdd(&cnt[threadIdx.x], (unsigned long long)bc[threadIdx.x]);
+        atomicAdd(&cnt[8 + threadIdx.x], be[threadIdx.x]);
+    }
 }
 
 struct BinBase {
@@ -363,18 +377,19 @@ RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool 
     b.ids = left ? c.arena.lalloc<int32_t>((size_t)n) : c.arena.ralloc<int32_t>((size_t)n);
     if (n <= 0) return b;
     size_t mark = c.arena.right_mark();
-    unsigned long long *cnt = c.arena.ralloc<unsigned long long>(8);
-    fill<uint32_t>(c, (uint32_t *)cnt, 16, 0u);
+    unsigned long long *cnt = c.arena.ralloc<unsigned long long>(12);   // counts, fill cursors, entries
+    fill<uint32_t>(c, (uint32_t *)cnt, 24, 0u);
     unsigned g = (unsigned)std::min<int64_t>(((int64_t)n + 255) / 256, 1024);
     BinThr thr{t0, t1, t2};
     k_bin_count<<<g, 256, 0, c.stream>>>(n, xadj, len, thr, cnt);
     HGP_LAUNCH_CHECK();
-    read_scalars(c, cnt, 4 * sizeof(int64_t));
+    read_scalars(c, cnt, 12 * sizeof(int64_t));
     BinBase base;
     b.off[0] = 0;
     for (int i = 0; i < 4; i++) {
         base.o[i] = b.off[i];
         b.off[i + 1] = b.off[i] + c.pinned[i];
+        b.ent[i] = c.pinned[8 + i];
     }
     k_bin_fill<<<g, 256, 0, c.stream>>>(n, xadj, len, thr, base, cnt + 4, b.ids);
     HGP_LAUNCH_CHECK();

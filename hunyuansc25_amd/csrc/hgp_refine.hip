@@ -140,6 +140,7 @@ struct SelArgs {
     int32_t *dest;
     int32_t *pregain;
     uint32_t *key;
+    unsigned long long *kb;   // kernel-timer byte counter (may be null)
 };
 
 __device__ inline bool sel_eligible(const SelArgs &A, int32_t mode, int64_t maxwgt, int64_t opt, int32_t v,
@@ -225,6 +226,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_s(const int32_t *__restri
     int64_t gi = gindex<8>();
     int lane = glane<8>();
     int32_t ncand = 0;
+    uint64_t kbytes = 0;
     if (gi < cnt) {
         int32_t v = ids[gi];
         int32_t p = A.where[v];
@@ -242,9 +244,13 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_s(const int32_t *__restri
         PartKey best{0, -1};
         if (q >= 0 && q != p && sel_target(A, mode, maxwgt, q)) best = PartKey{conn, q};
         best = group_best<8>(best, BetterPart());
-        if (lane == 0) ncand = sel_decide(A, C, mode, v, p, eligible, pc, best);
+        if (lane == 0) {
+            ncand = sel_decide(A, C, mode, v, p, eligible, pc, best);
+            if (eligible) kbytes = 16 + 12 * (uint64_t)(A.xadj[v + 1] - A.xadj[v]);
+        }
     }
     sel_count(A, mode, ncand);
+    kb_flush(A.kb, kbytes);
 }
 
 // Rows of 9..512 entries: a wave takes 64 rows of the list at a time, decides
@@ -265,6 +271,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
     for (int i = lane; i < k; i += kWave) my[i] = 0;
     wave_lds_sync();
     int32_t ncand = 0;
+    uint64_t kbytes = 0;
     for (int64_t base = ((int64_t)blockIdx.x * nw + wv) * kWave; base < cnt; base += (int64_t)gridDim.x * nw * kWave) {
         int64_t i = base + lane;
         int32_t mv = i < cnt ? ids[i] : -1;
@@ -305,10 +312,14 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
             for (int t = 0; t < S; t++)
                 if (q[t] >= 0) my[q[t]] = 0;
             wave_lds_sync();
-            if (lane == 0) ncand += sel_decide(A, C, mode, v, p, true, pc, best);
+            if (lane == 0) {
+                ncand += sel_decide(A, C, mode, v, p, true, pc, best);
+                kbytes += 16 + 12 * (uint64_t)(e - b);
+            }
         }
     }
     sel_count(A, mode, ncand);
+    kb_flush(A.kb, kbytes);
 }
 
 // Rows of > 512 entries by segments (RowSegs), a k-int LDS accumulator per
@@ -327,12 +338,14 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
     for (int i = threadIdx.x; i < k; i += kRowBS) acc[i] = 0;
     __syncthreads();
     int32_t ncand = 0;
+    uint64_t kbytes = 0;
     for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
         int32_t v = S.row[s];
         int32_t mi = S.mi[s];
         int32_t p = A.where[v];
         bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
         eid_t b = S.j0[s], e = min(b + (eid_t)kSegLen, A.xadj[v + 1]);
+        if (threadIdx.x == 0 && eligible) kbytes += 8 + 12 * (uint64_t)(e - b);
         bool walk_parts = e - b > (eid_t)k;
         if (eligible)
             for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) atomicAdd(&acc[A.where[A.adjncy[j]]], A.adjwgt[j]);
@@ -390,6 +403,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         if (threadIdx.x == 0) ncand += sel_decide(A, C, mode, v, p, eligible, pc, best);
     }
     sel_count(A, mode, ncand);
+    kb_flush(A.kb, kbytes);
 }
 
 // Multi rows: one block per row reads the summed k-int row, decides, clears it.
@@ -433,33 +447,39 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
                                                         const int32_t *__restrict__ where,
                                                         const int32_t *__restrict__ dest,
                                                         const int32_t *__restrict__ pregain, const JetCtl *ctl,
-                                                        uint8_t *__restrict__ sel)
+                                                        uint8_t *__restrict__ sel, unsigned long long *kb)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
     int64_t gi = gindex<G>();
     int lane = glane<G>();
-    if (gi >= cnt) return;
-    int32_t v = ids[gi];
-    int32_t ig = pregain[v];
-    if (ig == kNoGain) {
-        if (lane == 0) sel[v] = 0;
-        return;
-    }
-    int32_t p = where[v], best = dest[v];
-    int64_t change = 0;
-    for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) {
-        int32_t u = adjncy[j];
-        int32_t ng = pregain[u];
-        if (ng > ig || (ng == ig && u < v)) {
-            int32_t w = adjwgt[j];
-            int32_t nd = dest[u];
-            if (nd == p) change -= w; else if (nd == best) change += w;
-            int32_t nw = where[u];
-            if (nw == p) change += w; else if (nw == best) change -= w;
+    uint64_t kbytes = 0;
+    if (gi < cnt) {   // whole groups take the same branches
+        int32_t v = ids[gi];
+        int32_t ig = pregain[v];
+        if (ig == kNoGain) {
+            if (lane == 0) sel[v] = 0;
+        } else {
+            int32_t p = where[v], best = dest[v];
+            int64_t change = 0;
+            for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) {
+                int32_t u = adjncy[j];
+                int32_t ng = pregain[u];
+                if (ng > ig || (ng == ig && u < v)) {
+                    int32_t w = adjwgt[j];
+                    int32_t nd = dest[u];
+                    if (nd == p) change -= w; else if (nd == best) change += w;
+                    int32_t nw = where[u];
+                    if (nw == p) change += w; else if (nw == best) change -= w;
+                }
+            }
+            change = group_sum<G>(change);
+            if (lane == 0) {
+                sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
+                kbytes = 24 + 20 * (uint64_t)(xadj[v + 1] - xadj[v]);
+            }
         }
     }
-    change = group_sum<G>(change);
-    if (lane == 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
+    kb_flush(kb, kbytes);
 }
 
 // Rows of 9..512 entries: 64 rows per wave at a time; non-candidates are
@@ -471,10 +491,11 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
                                                           const int32_t *__restrict__ where,
                                                           const int32_t *__restrict__ dest,
                                                           const int32_t *__restrict__ pregain, const JetCtl *ctl,
-                                                          uint8_t *__restrict__ sel)
+                                                          uint8_t *__restrict__ sel, unsigned long long *kb)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
     const int lane = lane_id();
+    uint64_t kbytes = 0;
     const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     for (int64_t base = wid * kWave; base < cnt; base += nwaves * kWave) {
@@ -501,9 +522,13 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
                 }
             }
             change = wave_sum(change);
-            if (lane == 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
+            if (lane == 0) {
+                sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
+                kbytes += 24 + 20 * (uint64_t)(xadj[v + 1] - xadj[v]);
+            }
         }
     }
+    kb_flush(kb, kbytes);
 }
 
 // Class-2 rows by segments: a single-segment row decides here, a multi row
@@ -515,9 +540,11 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
                                                             const int32_t *__restrict__ dest,
                                                             const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                             uint8_t *__restrict__ sel,
-                                                            unsigned long long *__restrict__ macc)
+                                                            unsigned long long *__restrict__ macc,
+                                                            unsigned long long *kb)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
+    uint64_t kbytes = 0;
     for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
         int32_t v = S.row[s];
         int32_t mi = S.mi[s];
@@ -544,8 +571,10 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
         if (threadIdx.x == 0) {
             if (mi < 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
             else if (change) atomicAdd(&macc[mi], (unsigned long long)change);
+            kbytes += 16 + 20 * (uint64_t)(j1 - S.j0[s]);
         }
     }
+    kb_flush(kb, kbytes);
 }
 
 __global__ void k_afterburner_fin(RowSegs S, const int32_t *__restrict__ pregain, const JetCtl *ctl,
@@ -706,7 +735,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta(const int32_t *__restrict
                                                        const wgt_t *__restrict__ vwgt,
                                                        const int32_t *__restrict__ where,
                                                        const int32_t *__restrict__ dest, JetCtl *ctl,
-                                                       int64_t *__restrict__ pwgts)
+                                                       int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
     __shared__ unsigned long long bdelta;
@@ -714,6 +743,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta(const int32_t *__restrict
     for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
     if (threadIdx.x == 0) bdelta = 0;
     __syncthreads();
+    uint64_t kbytes = 0;
     constexpr int NG = kRowBS / G;
     const int g = threadIdx.x / G, lane = glane<G>();
     for (int64_t base = (int64_t)blockIdx.x * NG; base < cnt; base += (int64_t)gridDim.x * NG) {
@@ -735,8 +765,10 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta(const int32_t *__restrict
             if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
             atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
             atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
+            kbytes += 20 + 16 * (uint64_t)(xadj[v + 1] - xadj[v]);
         }
     }
+    kb_flush(kb, kbytes);
     __syncthreads();
     for (int i = threadIdx.x; i < k; i += kRowBS)
         if (pdelta[i]) atomicAdd((unsigned long long *)&pwgts[i], pdelta[i]);
@@ -751,7 +783,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
                                                          const wgt_t *__restrict__ vwgt,
                                                          const int32_t *__restrict__ where,
                                                          const int32_t *__restrict__ dest, JetCtl *ctl,
-                                                         int64_t *__restrict__ pwgts)
+                                                         int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
     __shared__ unsigned long long bdelta;
@@ -759,6 +791,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
     for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
     if (threadIdx.x == 0) bdelta = 0;
     __syncthreads();
+    uint64_t kbytes = 0;
     const int lane = lane_id();
     const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
@@ -784,9 +817,11 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
                 if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
                 atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
                 atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
+                kbytes += 20 + 16 * (uint64_t)(xadj[v + 1] - xadj[v]);
             }
         }
     }
+    kb_flush(kb, kbytes);
     __syncthreads();
     for (int i = threadIdx.x; i < k; i += kRowBS)
         if (pdelta[i]) atomicAdd((unsigned long long *)&pwgts[i], pdelta[i]);
@@ -800,7 +835,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
                                                            const wgt_t *__restrict__ vwgt,
                                                            const int32_t *__restrict__ where,
                                                            const int32_t *__restrict__ dest, JetCtl *ctl,
-                                                           int64_t *__restrict__ pwgts)
+                                                           int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
     __shared__ unsigned long long bdelta;
@@ -808,6 +843,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
     for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
     if (threadIdx.x == 0) bdelta = 0;
     __syncthreads();
+    uint64_t kbytes = 0;
     for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
         int32_t v = S.row[s];
         int32_t pv = where[v], dv = dest[v];
@@ -823,6 +859,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
         }
         d2 = group_sum<kRowBS>(d2);
         if (threadIdx.x == 0) {
+            kbytes += 16 + 16 * (uint64_t)(j1 - j0);
             if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
             if (j0 == xadj[v]) {
                 atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
@@ -830,6 +867,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
             }
         }
     }
+    kb_flush(kb, kbytes);
     __syncthreads();
     for (int i = threadIdx.x; i < k; i += kRowBS)
         if (pdelta[i]) atomicAdd((unsigned long long *)&pwgts[i], pdelta[i]);
@@ -991,21 +1029,33 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
                    uint32_t *key, int32_t *conn)
 {
     const RowBins &b = g.bins;
-    SelArgs A{k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl, dest, pregain, key};
+    SelArgs A{k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl, dest, pregain, key, nullptr};
+    // fixed bytes: ids, where, pwgts[p], vwgt, the decision (24 per row); eligible rows add xadj and
+    // 12 B per entry (adjncy, adjwgt, where gather) in the kernel
     if (b.count(0)) {
+        KScope ks(c, "k_jet_select_s", 24.0 * (double)b.count(0), true);
+        A.kb = ks.dev();
         k_jet_select_s<<<grid_groups(b.count(0), 8), kRowBS, 0, c.stream>>>(b.list(0), b.count(0), A);
         HGP_LAUNCH_CHECK();
     }
     if (b.count(1)) {
+        KScope ks(c, "k_jet_select_w", 24.0 * (double)b.count(1), true);
+        A.kb = ks.dev();
         int64_t blocks = std::min<int64_t>((b.count(1) + kRowBS - 1) / kRowBS, 4096);
         k_jet_select_w<<<(unsigned)blocks, kRowBS, sizeof(int32_t) * 4 * (size_t)k, c.stream>>>(b.list(1),
                                                                                                b.count(1), A);
         HGP_LAUNCH_CHECK();
     }
     if (g.segs.nseg) {
-        k_jet_select_seg<<<seg_grid(g.segs), kRowBS, sizeof(int32_t) * (size_t)k, c.stream>>>(g.segs, A, conn);
-        HGP_LAUNCH_CHECK();
+        {
+            KScope ks(c, "k_jet_select_seg", 24.0 * (double)g.segs.nseg, true);
+            A.kb = ks.dev();
+            k_jet_select_seg<<<seg_grid(g.segs), kRowBS, sizeof(int32_t) * (size_t)k, c.stream>>>(g.segs, A, conn);
+            HGP_LAUNCH_CHECK();
+        }
+        A.kb = nullptr;
         if (g.segs.nmulti) {
+            KScope ks(c, "k_jet_select_fin", (8.0 * k + 24.0) * (double)g.segs.nmulti);
             k_jet_select_fin<<<(unsigned)std::min<int32_t>(g.segs.nmulti, 4096), kRowBS, 0, c.stream>>>(g.segs, A,
                                                                                                       conn);
             HGP_LAUNCH_CHECK();
@@ -1097,32 +1147,32 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         read_scalars(c, ctl, sizeof(JetCtl));
         JetCtl h = *(JetCtl *)c.pinned;
         if (h.done) break;
-        {
-            // LP: every row (adjncy, adjwgt, where gather) + xadj, where, lock, dest, pregain per vertex.
-            // RW/RS: the same per-vertex traffic; rows only of overloaded parts (not counted).
-            bool lp = h.mode == kModeLP;
-            KScope ks(c, lp ? "jet_select_lp" : "jet_select_rb", (lp ? 12.0 * (double)g.m : 0.0) + 21.0 * n);
-            launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn);
-        }
+        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn);
         if (h.mode == kModeLP) {
             {
-                KScope ks(c, "jet_afterburner", 5.0 * n);
+                // fixed: ids, pregain, sel (9 B per row); candidates add xadj, where, dest and 20 B per entry
                 if (int64_t nb = g.bins.count(0)) {
+                    KScope ks(c, "k_afterburner_s", 9.0 * (double)nb, true);
                     k_afterburner<8><<<grid_groups(nb, 8), kRowBS, 0, c.stream>>>(
-                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl, sel);
+                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.count(1)) {
+                    KScope ks(c, "k_afterburner_w", 9.0 * (double)nb, true);
                     unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 4096);
                     k_afterburner_w<<<blocks, kRowBS, 0, c.stream>>>(g.bins.list(1), nb, g.xadj, g.adjncy,
-                                                                     g.adjwgt, where, dest, pregain, ctl, sel);
+                                                                     g.adjwgt, where, dest, pregain, ctl, sel,
+                                                                     ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (g.segs.nseg) {
-                    k_afterburner_seg<<<seg_grid(g.segs), kRowBS, 0, c.stream>>>(
-                        g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl, sel,
-                        (unsigned long long *)macc);
-                    HGP_LAUNCH_CHECK();
+                    {
+                        KScope ks(c, "k_afterburner_seg", 24.0 * (double)g.segs.nseg, true);
+                        k_afterburner_seg<<<seg_grid(g.segs), kRowBS, 0, c.stream>>>(
+                            g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl, sel,
+                            (unsigned long long *)macc, ks.dev());
+                        HGP_LAUNCH_CHECK();
+                    }
                     if (g.segs.nmulti) {
                         k_afterburner_fin<<<grid_for(g.segs.nmulti, kBS), kBS, 0, c.stream>>>(g.segs, pregain, ctl,
                                                                                              sel, macc);
@@ -1154,26 +1204,27 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                 HGP_LAUNCH_CHECK();
             }
         }
-        {
-            KScope ks(c, "jet_move_delta", 8.0 * n);
-            if (int64_t nb = g.bins.count(0)) {
-                int64_t blocks = std::min<int64_t>((nb + kRowBS / 8 - 1) / (kRowBS / 8), 2048);
-                k_move_delta<8><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                    g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts);
-                HGP_LAUNCH_CHECK();
-            }
-            if (int64_t nb = g.bins.count(1)) {
-                unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
-                k_move_delta_w<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                    g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts);
-                HGP_LAUNCH_CHECK();
-            }
-            if (g.segs.nseg) {
-                unsigned blocks = (unsigned)std::min<int64_t>(g.segs.nseg, 2048);
-                k_move_delta_seg<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                    g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts);
-                HGP_LAUNCH_CHECK();
-            }
+        // fixed: ids, where, dest (12 B per row); moved rows add xadj, vwgt and 16 B per entry
+        if (int64_t nb = g.bins.count(0)) {
+            KScope ks(c, "k_move_delta_s", 12.0 * (double)nb, true);
+            int64_t blocks = std::min<int64_t>((nb + kRowBS / 8 - 1) / (kRowBS / 8), 2048);
+            k_move_delta<8><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts, ks.dev());
+            HGP_LAUNCH_CHECK();
+        }
+        if (int64_t nb = g.bins.count(1)) {
+            KScope ks(c, "k_move_delta_w", 12.0 * (double)nb, true);
+            unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
+            k_move_delta_w<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts, ks.dev());
+            HGP_LAUNCH_CHECK();
+        }
+        if (g.segs.nseg) {
+            KScope ks(c, "k_move_delta_seg", 24.0 * (double)g.segs.nseg, true);
+            unsigned blocks = (unsigned)std::min<int64_t>(g.segs.nseg, 2048);
+            k_move_delta_seg<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts, ks.dev());
+            HGP_LAUNCH_CHECK();
         }
         k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where);
         HGP_LAUNCH_CHECK();

@@ -95,6 +95,7 @@ void Arena::reset()
 {
     lo_ = 0;
     hi_ = cap_;
+    peak_ = 0;
 }
 
 // ------------------------------------------------------------------ context
@@ -156,11 +157,36 @@ hipEvent_t KTimer::get()
     return pool[used++];
 }
 
-KScope::KScope(Ctx &c, const char *name, double bytes) : c_(c), on_(ktimer().on), name_(name), bytes_(bytes)
+int64_t KTimer::counter_slot()
+{
+    std::lock_guard<std::mutex> g(mu);
+    if (!dcnt) {
+        cnt_cap = (int64_t)kKbLanes << 14;
+        if (hipMalloc((void **)&dcnt, sizeof(unsigned long long) * (size_t)cnt_cap) != hipSuccess) {
+            (void)hipGetLastError();
+            dcnt = nullptr;
+            cnt_cap = 0;
+        }
+    }
+    if (ncnt + kKbLanes > cnt_cap) return -1;
+    int64_t s = ncnt;
+    ncnt += kKbLanes;
+    return s;
+}
+
+KScope::KScope(Ctx &c, const char *name, double bytes, bool counted)
+    : c_(c), on_(ktimer().on), name_(name), bytes_(bytes)
 {
     if (!on_) return;
     a_ = ktimer().get();
     b_ = ktimer().get();
+    if (counted) {
+        slot_ = ktimer().counter_slot();
+        if (slot_ >= 0) {
+            dev_ = ktimer().dcnt + slot_;
+            HGP_CHECK_HIP(hipMemsetAsync(dev_, 0, sizeof(unsigned long long) * kKbLanes, c_.stream));
+        }
+    }
     HGP_CHECK_HIP(hipEventRecord(a_, c_.stream));
 }
 
@@ -169,7 +195,7 @@ KScope::~KScope()
     if (!on_) return;
     if (hipEventRecord(b_, c_.stream) != hipSuccess) return;
     std::lock_guard<std::mutex> g(ktimer().mu);
-    ktimer().recs.push_back({name_, a_, b_, bytes_});
+    ktimer().recs.push_back({name_, a_, b_, bytes_, slot_});
 }
 
 void upload_staged(Ctx &c, void *ddst, const void *hsrc, size_t bytes)

@@ -79,6 +79,8 @@ typedef struct hgp_stats {
     int64_t level_n[HGP_MAX_LEVELS];
     int64_t level_m[HGP_MAX_LEVELS];
     int32_t level_iters[HGP_MAX_LEVELS];
+    int64_t arena_bytes;    /* device arena the run used                         */
+    int64_t arena_peak;     /* most of it in use at once                         */
 } hgp_stats;
 
 void hgp_default_options(hgp_options *opt);

@@ -102,3 +102,24 @@ def test_host_validation_rejects_bad_csr(hgp):
         hgp.partition(x, np.array([0, 0, 2, 1], np.int32), 2)        # self-loop at vertex 0
     with pytest.raises(hgp.HgpError):
         hgp.partition(x, a, 5000)                                  # k beyond the supported range
+
+
+def test_struct_layouts_match_header(hgp, tmp_path):
+    """The ctypes mirrors of hgp_options / hgp_stats have the C layout."""
+    import ctypes as C
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "hunyuangraph.h"\n'
+        'int main(void) { printf("%zu %zu %zu %zu %zu\\n", sizeof(hgp_options), sizeof(hgp_stats),'
+        ' offsetof(hgp_stats, level_iters), offsetof(hgp_stats, arena_peak), sizeof(hgp_graph)); return 0; }\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [C.sizeof(hgp.HgpOptions), C.sizeof(hgp.HgpStats), hgp.HgpStats.level_iters.offset,
+            hgp.HgpStats.arena_peak.offset, C.sizeof(hgp.HgpGraph)]
+    assert got == want
