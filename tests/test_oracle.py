@@ -4,6 +4,8 @@ formulation, a closed form, an invariant or brute force on tiny inputs --
 chosen so that a plausible mistake (dropped term, wrong sign or index,
 transposed operand) fails at least one of them."""
 import itertools
+import json
+import os
 import math
 from fractions import Fraction
 
@@ -467,3 +469,65 @@ def test_cpu_rb_path(hgp, orc):
     assert pw.min() > 0 and pw.max() <= 1.07 * len(p) / 6
     q, _ = orc.dga_partition(x, a, 6, vw, aw, seed=1)
     assert orc.edge_cut(x, a, aw, q) <= 1.5 * cut
+
+
+# ------------------------------------------------------------------ golden fixtures
+# tests/golden/*.json are written by tests/golden/make_golden.py from the
+# definitions (dictionary contraction, enumeration, exact rationals, an
+# independent SplitMix64, constructed optima), not from the oracle.
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden(name):
+    with open(os.path.join(GOLDEN, name)) as f:
+        return json.load(f)
+
+
+def test_golden_contract(orc):
+    for g in golden("contract.json"):
+        res = orc.contract(g["xadj"], g["adjncy"], g["adjwgt"], g["vwgt"], np.array(g["match"], np.int32))
+        _, cmap, cx, ca, cw, cv = res
+        assert cmap.tolist() == g["cmap"], g["name"]
+        assert cx.tolist() == g["cxadj"], g["name"]
+        assert ca.tolist() == g["cadjncy"], g["name"]
+        assert cw.tolist() == g["cadjwgt"], g["name"]
+        assert cv.tolist() == g["cvwgt"], g["name"]
+
+
+def test_golden_bisection(orc):
+    """Exhaustive-start GGGP + FM reaches the enumerated optimum of the tiny
+    graphs, and never goes below it."""
+    hits = 0
+    cases = golden("bisection.json")
+    for g in cases:
+        x, a, aw, vw = (np.array(g[k]) for k in ("xadj", "adjncy", "adjwgt", "vwgt"))
+        tv = int(vw.sum())
+        best = None
+        for t in range(g["n"]):
+            w, cut = orc.dga_trial(x, a, vw, aw, tv, g["lo"], g["hi"], tv // 2, g["n"], t, 1, 1, 10)
+            p0 = int(vw[w == 0].sum())
+            if g["lo"] <= p0 <= g["hi"]:
+                assert cut >= g["min_cut"]
+                best = cut if best is None else min(best, cut)
+        hits += best == g["min_cut"]
+    assert hits >= len(cases) - 1
+
+
+def test_golden_gain_bucket(orc):
+    for gain, vwgt, bucket in golden("gain_bucket.json"):
+        assert orc.gain_bucket(gain, vwgt) == bucket, (gain, vwgt)
+
+
+def test_golden_edge_hash(orc):
+    for seed, level, lo, hi, h in golden("edge_hash.json"):
+        assert orc.edge_hash(seed, level, lo, hi) == h
+
+
+def test_golden_known_cut(orc):
+    """k cliques on a ring: the optimal balanced k-way cut is the k ring edges."""
+    for g in golden("known_cut.json"):
+        x, a = np.array(g["xadj"]), np.array(g["adjncy"])
+        p, st = orc.dga_partition(x, a, g["k"])
+        assert st["cut"] == g["optimal_cut"]
+        assert orc.balanced(np.ones(g["n"], np.int32), p, g["k"], 103, 100)
