@@ -143,14 +143,14 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
     kb_flush(kb, kbytes);
 }
 
-__global__ void k_propose_fin(RowSegs S, const int32_t *__restrict__ match, const EdgeKey *__restrict__ sbest,
+__global__ void k_propose_fin(RowSegs S, const int32_t *__restrict__ avail, const EdgeKey *__restrict__ sbest,
                               int32_t *__restrict__ prop, const int32_t *prev_progress)
 {
     if (prev_progress && *prev_progress == 0) return;
     int32_t t = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (t >= S.nmulti) return;
     int32_t v = S.mrow[t];
-    if (match[v] != -1) {
+    if (avail[v] < 0) {   // the segments skipped this row: no fresh best edges
         prop[v] = -1;
         return;
     }
@@ -160,8 +160,11 @@ __global__ void k_propose_fin(RowSegs S, const int32_t *__restrict__ match, cons
     prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
 }
 
-// Mutual proposals match; avail[v] (vwgt of a free vertex, -1 once matched)
-// lets the proposals read one array per neighbour.
+// Mutual proposals match.  avail[v] is vwgt of a vertex that may still be
+// matched in a later round, -1 otherwise: once matched, or once it had no
+// candidate (a free vertex without a free neighbour that fits has none in
+// later rounds either, and no neighbour can pick it), so later rounds skip
+// its row and the proposals read one array per neighbour.
 __global__ void k_accept(int32_t n, const int32_t *__restrict__ prop, int32_t *__restrict__ match,
                          int32_t *__restrict__ avail, int32_t *progress, const int32_t *prev_progress)
 {
@@ -170,14 +173,15 @@ __global__ void k_accept(int32_t n, const int32_t *__restrict__ prop, int32_t *_
     bool got = false;
     if (v < n && match[v] == -1) {
         int32_t u = prop[v];
-        if (u >= 0 && prop[u] == v) {
+        if (u < 0) {
+            avail[v] = -1;
+        } else if (prop[u] == v) {
             match[v] = u;
             avail[v] = -1;
             got = true;
         }
     }
-    uint64_t b = __ballot(got);
-    if (lane_id() == 0 && b) atomicAdd(progress, (int32_t)__popcll(b));
+    if (__syncthreads_or(got) && threadIdx.x == 0) *progress = 1;   // only "any progress" is read
 }
 
 __global__ void __launch_bounds__(kBS) k_count_matched(int32_t n, const int32_t *__restrict__ match,
@@ -442,8 +446,9 @@ __global__ void k_rep(int32_t n, const eid_t *__restrict__ xadj, const wgt_t *__
 // the device radix sort.
 
 constexpr uint32_t kSent = 0xFFFFFFFFu;   // dropped entry; sorts last
+constexpr int kSegWave = 512;   // rows of 65..kSegWave entries: one wave, 8 entries per lane
 constexpr int kSegLds = 4096;
-constexpr int kSegBS = 256;
+constexpr int kSegBS = 512;
 constexpr int kSegE = kSegLds / kSegBS;   // entries per thread in the LDS merge
 
 // Entry i of coarse row c's concatenation (fine rows of v, then of u).
@@ -549,7 +554,127 @@ __global__ void __launch_bounds__(kRowBS) k_seg_reg(const int32_t *__restrict__ 
     kb_flush(kb, kbytes);
 }
 
-// Rows of 65..kSegLds entries: one block per row, bitonic sort in LDS padded
+// Bitonic network over R*64 entries held 8 per lane (entry e = r*64 + lane):
+// partners closer than 64 are exchanged with xor-shuffles, farther ones are
+// register swaps inside the lane.
+template <int R>
+__device__ inline void wave_bitonic(uint32_t (&key)[8], int32_t (&val)[8])
+{
+    const int lane = lane_id();
+    constexpr int N = R * kWave;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= kWave) {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    int r2 = r ^ (j / kWave);
+                    if (r2 > r) {
+                        bool asc = ((r * kWave) & k) == 0;   // k >= 128: the lane bits do not matter
+                        bool sw = asc ? key[r] > key[r2] : key[r] < key[r2];
+                        if (sw) {
+                            uint32_t tk = key[r];
+                            key[r] = key[r2];
+                            key[r2] = tk;
+                            int32_t tv = val[r];
+                            val[r] = val[r2];
+                            val[r2] = tv;
+                        }
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    uint32_t ok = __shfl_xor(key[r], j, kWave);
+                    int32_t ov = __shfl_xor(val[r], j, kWave);
+                    bool asc = ((r * kWave + lane) & k) == 0;
+                    bool lower = (lane & j) == 0;
+                    bool take = (lower == asc) ? ok < key[r] : ok > key[r];
+                    if (take) {
+                        key[r] = ok;
+                        val[r] = ov;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Rows of 65..kSegWave entries: one wave per row, sorted in registers (padded
+// to 128, 256 or 512 entries); tails ranked by ballots, weights by wave scans
+// carried across the registers.
+__global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__ ids, int64_t cnt,
+                                                     const int32_t *__restrict__ rep,
+                                                     const int32_t *__restrict__ match,
+                                                     const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                     const wgt_t *__restrict__ adjwgt,
+                                                     const int32_t *__restrict__ cmap,
+                                                     const int64_t *__restrict__ toff, int32_t *__restrict__ tadj,
+                                                     int32_t *__restrict__ twgt, int32_t *__restrict__ ccnt,
+                                                     unsigned long long *kb)
+{
+    const int lane = lane_id();
+    const int64_t gi = ((int64_t)blockIdx.x * kRowBS + threadIdx.x) / kWave;
+    uint64_t kbytes = 0;
+    if (gi < cnt) {   // whole waves
+        int32_t c = ids[gi];
+        int32_t v = rep[c], u = match[v];
+        CatRow row(xadj, v, u);
+        int len = (int)(row.dv + (u != v ? xadj[u + 1] - row.bu : 0));
+        uint32_t key[8];
+        int32_t val[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            int i = r * kWave + lane;
+            key[r] = kSent;
+            val[r] = 0;
+            if (i < len) {
+                eid_t j = row.at(i);
+                int32_t cc = cmap[adjncy[j]];
+                if (cc != c) {
+                    key[r] = (uint32_t)cc;
+                    val[r] = adjwgt[j];
+                }
+            }
+        }
+        int R = len <= 128 ? 2 : (len <= 256 ? 4 : 8);
+        if (R == 2) wave_bitonic<2>(key, val);
+        else if (R == 4) wave_bitonic<4>(key, val);
+        else wave_bitonic<8>(key, val);
+        const int64_t o = toff[c];
+        int32_t carry = 0, lastcum = 0, rank = 0;
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            if (r < R) {
+                uint32_t nxt = __shfl_down(key[r], 1, kWave);
+                uint32_t head_next = __shfl(key[r + 1 < 8 ? r + 1 : r], 0, kWave);
+                if (lane == kWave - 1) nxt = r + 1 < R ? head_next : kSent;
+                bool tail = key[r] != kSent && nxt != key[r];
+                int32_t incl = wave_inclusive_scan(val[r]) + carry;
+                uint64_t tm = __ballot(tail);
+                uint64_t below = tm & lanemask_lt();
+                int32_t pv = __shfl(incl, below ? 63 - __clzll(below) : 0, kWave);
+                int32_t last = __shfl(incl, tm ? 63 - __clzll(tm) : 0, kWave);
+                if (tail) {
+                    int64_t at = o + rank + __popcll(below);
+                    tadj[at] = (int32_t)key[r];
+                    twgt[at] = incl - (below ? pv : lastcum);
+                }
+                rank += __popcll(tm);
+                if (tm) lastcum = last;
+                carry = __shfl(incl, kWave - 1, kWave);
+            }
+        }
+        if (lane == 0) {
+            ccnt[c] = rank;
+            kbytes = 8 * (uint64_t)rank;
+        }
+    }
+    kb_flush(kb, kbytes);
+}
+
+// Rows of kSegWave+1..kSegLds entries: one block per row, bitonic sort in LDS padded
 // to a power of two, then tails (last entry of each run of equal ids) are
 // ranked by a block scan and their inclusive weight prefixes differenced.
 __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ ids, int64_t cnt,
@@ -654,6 +779,208 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
         __syncthreads();
     }
     kb_flush(kb, kbytes);
+}
+
+// Rows longer than kSegLds, by coarse-id ranges: the entries of each long row
+// are counted and scattered into bins (row, id >> kBktBits); each bin is then
+// merged in a dense LDS array of kBkt weights with a presence bitmap and
+// emitted in id order.  Bins of one row are consecutive, so a row's merged
+// entries are its bins' outputs in bin order.  Rows are canonical whatever
+// the order of the atomics.
+constexpr int kBktBits = 13;
+constexpr int kBkt = 1 << kBktBits;      // coarse ids per bin
+constexpr int64_t kMaxBins = 16ll << 20; // beyond this, the device radix sort path
+
+// Long rows are cut into chunks of kLbChunk entries (one block each), so a
+// hub row is spread over many blocks.  Bin counts and bin ranges are taken
+// with one global atomic per chunk and bin, after an LDS histogram.
+constexpr int kLbChunk = 4096;
+constexpr int kLbMaxBk = 1024;   // bins per row (cn <= kLbMaxBk * kBkt) for the chunk kernels' LDS
+
+__global__ void k_lb_nchunks(int64_t nl, const int32_t *__restrict__ lids, const int64_t *__restrict__ len,
+                             int64_t *__restrict__ nch)
+{
+    int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li < nl) nch[li] = (len[lids[li]] + kLbChunk - 1) / kLbChunk;
+}
+
+__global__ void k_lb_chunks(int64_t nl, const int64_t *__restrict__ choff, int32_t *__restrict__ ch_li,
+                            int32_t *__restrict__ ch_i0)
+{
+    int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nl) return;
+    for (int64_t ch = choff[li]; ch < choff[li + 1]; ch++) {
+        ch_li[ch] = (int32_t)li;
+        ch_i0[ch] = (int32_t)((ch - choff[li]) * kLbChunk);
+    }
+}
+
+__global__ void __launch_bounds__(kRowBS) k_lb_count(int64_t nch, const int32_t *__restrict__ ch_li,
+                                                     const int32_t *__restrict__ ch_i0,
+                                                     const int32_t *__restrict__ lids, int32_t nbk,
+                                                     const int32_t *__restrict__ rep,
+                                                     const int32_t *__restrict__ match,
+                                                     const eid_t *__restrict__ xadj,
+                                                     const vid_t *__restrict__ adjncy,
+                                                     const int32_t *__restrict__ cmap, uint32_t *__restrict__ cnt)
+{
+    __shared__ uint32_t hist[kLbMaxBk];
+    const int tid = threadIdx.x;
+    for (int64_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        for (int b = tid; b < nbk; b += kRowBS) hist[b] = 0;
+        __syncthreads();
+        int64_t li = ch_li[ch];
+        int32_t c = lids[li];
+        int32_t v = rep[c], u = match[v];
+        CatRow row(xadj, v, u);
+        eid_t len = row.dv + (u != v ? xadj[u + 1] - row.bu : 0);
+        eid_t i0 = ch_i0[ch], i1 = min(i0 + (eid_t)kLbChunk, len);
+        for (eid_t i = i0 + tid; i < i1; i += kRowBS) {
+            int32_t cc = cmap[adjncy[row.at(i)]];
+            if (cc != c) atomicAdd(&hist[cc >> kBktBits], 1u);
+        }
+        __syncthreads();
+        uint32_t *bins = cnt + li * nbk;
+        for (int b = tid; b < nbk; b += kRowBS)
+            if (hist[b]) atomicAdd(&bins[b], hist[b]);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kRowBS) k_lb_scatter(int64_t nch, const int32_t *__restrict__ ch_li,
+                                                       const int32_t *__restrict__ ch_i0,
+                                                       const int32_t *__restrict__ lids, int32_t nbk,
+                                                       const int32_t *__restrict__ rep,
+                                                       const int32_t *__restrict__ match,
+                                                       const eid_t *__restrict__ xadj,
+                                                       const vid_t *__restrict__ adjncy,
+                                                       const wgt_t *__restrict__ adjwgt,
+                                                       const int32_t *__restrict__ cmap,
+                                                       const int64_t *__restrict__ bstart, uint32_t *__restrict__ cur,
+                                                       uint16_t *__restrict__ lo, int32_t *__restrict__ w)
+{
+    __shared__ uint32_t hist[kLbMaxBk];
+    __shared__ int64_t gbase[kLbMaxBk];
+    __shared__ int32_t ecc[kLbChunk];
+    __shared__ int32_t ew[kLbChunk];
+    const int tid = threadIdx.x;
+    for (int64_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+        for (int b = tid; b < nbk; b += kRowBS) hist[b] = 0;
+        __syncthreads();
+        int64_t li = ch_li[ch];
+        int32_t c = lids[li];
+        int32_t v = rep[c], u = match[v];
+        CatRow row(xadj, v, u);
+        eid_t len = row.dv + (u != v ? xadj[u + 1] - row.bu : 0);
+        int32_t i0 = ch_i0[ch];
+        int32_t ne = (int32_t)min((eid_t)kLbChunk, len - i0);
+        for (int k = tid; k < ne; k += kRowBS) {
+            eid_t j = row.at(i0 + k);
+            int32_t cc = cmap[adjncy[j]];
+            ecc[k] = cc;
+            ew[k] = adjwgt[j];
+            if (cc != c) atomicAdd(&hist[cc >> kBktBits], 1u);
+        }
+        __syncthreads();
+        const int64_t b0 = li * nbk;
+        for (int b = tid; b < nbk; b += kRowBS) {
+            uint32_t h = hist[b];
+            gbase[b] = h ? bstart[b0 + b] + (int64_t)atomicAdd(&cur[b0 + b], h) : 0;
+            hist[b] = 0;
+        }
+        __syncthreads();
+        for (int k = tid; k < ne; k += kRowBS) {
+            int32_t cc = ecc[k];
+            if (cc == c) continue;
+            int b = cc >> kBktBits;
+            int64_t pos = gbase[b] + atomicAdd(&hist[b], 1u);
+            lo[pos] = (uint16_t)(cc & (kBkt - 1));
+            w[pos] = ew[k];
+        }
+        __syncthreads();
+    }
+}
+
+// One block per non-empty bin (persistent over bins); 256 threads own the 256
+// bitmap words of a bin.  The merged entries are written at the bin's own
+// start (a bin never grows) and their number to ucnt.
+__global__ void __launch_bounds__(kRowBS) k_lb_merge(int64_t nbins, int32_t nbk, const int64_t *__restrict__ bstart,
+                                                     const uint16_t *__restrict__ lo, const int32_t *__restrict__ w,
+                                                     uint32_t *__restrict__ ucnt, int32_t *__restrict__ oadj,
+                                                     int32_t *__restrict__ ow, unsigned long long *kb)
+{
+    static_assert(kBkt / 32 == kRowBS, "one bitmap word per thread");
+    __shared__ int32_t dw[kBkt];
+    __shared__ uint32_t bm[kBkt / 32];
+    __shared__ int64_t sh[kRowBS / kWave + 1];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < kBkt; i += kRowBS) dw[i] = 0;
+    bm[tid] = 0;
+    __syncthreads();
+    uint64_t kbytes = 0;
+    for (int64_t b = blockIdx.x; b < nbins; b += gridDim.x) {
+        int64_t s = bstart[b], e = bstart[b + 1];
+        if (s == e) {
+            if (tid == 0) ucnt[b] = 0;
+            continue;
+        }
+        for (int64_t i = s + tid; i < e; i += kRowBS) {
+            uint32_t x = lo[i];
+            atomicAdd(&dw[x], w[i]);
+            atomicOr(&bm[x >> 5], 1u << (x & 31));
+        }
+        __syncthreads();
+        uint32_t word = bm[tid];
+        int64_t tot;
+        int64_t ex = block_exclusive_scan((int64_t)__popc(word), sh, &tot);
+        const int32_t base = (int32_t)((b % nbk) << kBktBits);
+        int64_t at = s + ex;
+        while (word) {
+            int bit = __ffs(word) - 1;
+            word &= word - 1;
+            int idx = tid * 32 + bit;
+            oadj[at] = base + idx;
+            ow[at] = dw[idx];
+            dw[idx] = 0;
+            at++;
+        }
+        bm[tid] = 0;
+        if (tid == 0) {
+            ucnt[b] = (uint32_t)tot;
+            kbytes += 16 + 6 * (uint64_t)(e - s) + 8 * (uint64_t)tot;
+        }
+        __syncthreads();
+    }
+    kb_flush(kb, kbytes);
+}
+
+// Bin outputs to their row's place at toff; one 64-lane group per bin.
+__global__ void __launch_bounds__(kRowBS) k_lb_copy(int64_t nbins, int32_t nbk, const int32_t *__restrict__ lids,
+                                                    const int64_t *__restrict__ bstart,
+                                                    const uint32_t *__restrict__ ucnt,
+                                                    const int64_t *__restrict__ uoff,
+                                                    const int32_t *__restrict__ oadj, const int32_t *__restrict__ ow,
+                                                    const int64_t *__restrict__ toff, int32_t *__restrict__ tadj,
+                                                    int32_t *__restrict__ twgt)
+{
+    const int64_t b = gindex<kWave>();
+    if (b >= nbins) return;
+    uint32_t nu = ucnt[b];
+    if (nu == 0) return;
+    int64_t li = b / nbk;
+    int64_t dst = toff[lids[li]] + (uoff[b] - uoff[li * nbk]);
+    int64_t src = bstart[b];
+    for (uint32_t i = lane_id(); i < nu; i += kWave) {
+        tadj[dst + i] = oadj[src + i];
+        twgt[dst + i] = ow[src + i];
+    }
+}
+
+__global__ void k_lb_rowcnt(int64_t nl, int32_t nbk, const int32_t *__restrict__ lids,
+                            const int64_t *__restrict__ uoff, int32_t *__restrict__ ccnt)
+{
+    int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (li < nl) ccnt[lids[li]] = (int32_t)(uoff[(li + 1) * nbk] - uoff[li * nbk]);
 }
 
 // Rows longer than kSegLds: keys (list index << 32 | coarse id) of all such
@@ -866,7 +1193,7 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
                     HGP_LAUNCH_CHECK();
                 }
                 if (g.segs.nmulti) {
-                    k_propose_fin<<<grid_for(g.segs.nmulti, kBS), kBS, 0, c.stream>>>(g.segs, match, sbest, prop, prev);
+                    k_propose_fin<<<grid_for(g.segs.nmulti, kBS), kBS, 0, c.stream>>>(g.segs, avail, sbest, prop, prev);
                     HGP_LAUNCH_CHECK();
                 }
             }
@@ -1014,7 +1341,7 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     int32_t *tadj = c.arena.ralloc<int32_t>((size_t)m);
     int32_t *twgt = c.arena.ralloc<int32_t>((size_t)m);
     int32_t *ccnt = c.arena.ralloc<int32_t>((size_t)cn);
-    RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegLds);
+    RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegWave, kSegLds);
     {
         // per coarse row: ids, rep, match, 2 xadj pairs, toff, count (56 B); per fine entry: adjncy, adjwgt,
         // cmap gather (12 B); the kernels count the merged entries they write (8 B each)
@@ -1032,16 +1359,79 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
             HGP_LAUNCH_CHECK();
         }
         if (cb.count(2)) {
-            KScope ks(c, "k_seg_lds", bytes(2), true);
-            unsigned blocks = (unsigned)std::min<int64_t>(cb.count(2), 8192);
-            k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(2), cb.count(2), rep, match, g.xadj, g.adjncy,
+            KScope ks(c, "k_seg_wave", bytes(2), true);
+            k_seg_wave<<<grid_groups(cb.count(2), kWave), kRowBS, 0, c.stream>>>(
+                cb.list(2), cb.count(2), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt,
+                ks.dev());
+            HGP_LAUNCH_CHECK();
+        }
+        if (cb.count(3)) {
+            KScope ks(c, "k_seg_lds", bytes(3), true);
+            unsigned blocks = (unsigned)std::min<int64_t>(cb.count(3), 8192);
+            k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(3), cb.count(3), rep, match, g.xadj, g.adjncy,
                                                        g.adjwgt, cmap, toff, tadj, twgt, ccnt, ks.dev());
             HGP_LAUNCH_CHECK();
         }
     }
-    if (int64_t nl = cb.count(3)) {
+    const int32_t nbk = (int32_t)(((int64_t)cn + kBkt - 1) >> kBktBits);
+    if (int64_t nl = cb.count(4); nl > 0 && nl * (int64_t)nbk <= kMaxBins && nbk <= kLbMaxBk) {
+        // long rows by coarse-id bins
         size_t m2 = c.arena.right_mark();
-        const int32_t *lids = cb.list(3);
+        const int32_t *lids = cb.list(4);
+        const int64_t nbins = nl * (int64_t)nbk, tl = cb.ent[4];
+        int64_t *nchr = c.arena.ralloc<int64_t>((size_t)nl);
+        int64_t *choff = c.arena.ralloc<int64_t>((size_t)nl + 1);
+        k_lb_nchunks<<<grid_for(nl, kBS), kBS, 0, c.stream>>>(nl, lids, len, nchr);
+        HGP_LAUNCH_CHECK();
+        offsets_from_counts(c, nchr, choff, nl);
+        read_scalars(c, choff + nl, sizeof(int64_t));
+        const int64_t nch = c.pinned[0];
+        int32_t *ch_li = c.arena.ralloc<int32_t>((size_t)nch);
+        int32_t *ch_i0 = c.arena.ralloc<int32_t>((size_t)nch);
+        k_lb_chunks<<<grid_for(nl, kBS), kBS, 0, c.stream>>>(nl, choff, ch_li, ch_i0);
+        HGP_LAUNCH_CHECK();
+        uint32_t *cnt = c.arena.ralloc<uint32_t>((size_t)nbins);
+        int64_t *bstart = c.arena.ralloc<int64_t>((size_t)nbins + 1);
+        int64_t *uoff = c.arena.ralloc<int64_t>((size_t)nbins + 1);
+        uint16_t *lo = c.arena.ralloc<uint16_t>((size_t)tl);
+        int32_t *w = c.arena.ralloc<int32_t>((size_t)tl);
+        int32_t *oadj = c.arena.ralloc<int32_t>((size_t)tl);
+        int32_t *ow = c.arena.ralloc<int32_t>((size_t)tl);
+        unsigned ch_grid = (unsigned)std::min<int64_t>(nch, 16384);
+        fill<uint32_t>(c, cnt, nbins, 0u);
+        {
+            // per entry: adjncy, cmap gather; per chunk and bin: one count atomic
+            KScope ks(c, "k_lb_count", 8.0 * (double)tl + 56.0 * nch + 4.0 * (double)nbins);
+            k_lb_count<<<ch_grid, kRowBS, 0, c.stream>>>(nch, ch_li, ch_i0, lids, nbk, rep, match, g.xadj, g.adjncy,
+                                                        cmap, cnt);
+            HGP_LAUNCH_CHECK();
+        }
+        offsets_from_counts(c, (const int32_t *)cnt, bstart, nbins);
+        fill<uint32_t>(c, cnt, nbins, 0u);
+        {
+            // per entry: adjncy, adjwgt, cmap gather, 6 B out; per chunk and bin: bstart and a range atomic
+            KScope ks(c, "k_lb_scatter", 18.0 * (double)tl + 56.0 * nch + 12.0 * (double)nbins);
+            k_lb_scatter<<<ch_grid, kRowBS, 0, c.stream>>>(nch, ch_li, ch_i0, lids, nbk, rep, match, g.xadj, g.adjncy,
+                                                          g.adjwgt, cmap, bstart, cnt, lo, w);
+            HGP_LAUNCH_CHECK();
+        }
+        {
+            KScope ks(c, "k_lb_merge", 0.0, true);
+            k_lb_merge<<<(unsigned)std::min<int64_t>(nbins, 8192), kRowBS, 0, c.stream>>>(
+                nbins, nbk, bstart, lo, w, cnt, oadj, ow, ks.dev());
+            HGP_LAUNCH_CHECK();
+        }
+        offsets_from_counts(c, (const int32_t *)cnt, uoff, nbins);
+        k_lb_copy<<<grid_groups(nbins, kWave), kRowBS, 0, c.stream>>>(nbins, nbk, lids, bstart, cnt, uoff, oadj, ow,
+                                                                      toff, tadj, twgt);
+        HGP_LAUNCH_CHECK();
+        k_lb_rowcnt<<<grid_for(nl, kBS), kBS, 0, c.stream>>>(nl, nbk, lids, uoff, ccnt);
+        HGP_LAUNCH_CHECK();
+        c.arena.right_release(m2);
+    } else if (nl > 0) {
+        // more bins than kMaxBins: the device radix sort over (row, coarse id) keys
+        size_t m2 = c.arena.right_mark();
+        const int32_t *lids = cb.list(4);
         int64_t *llen = c.arena.ralloc<int64_t>((size_t)nl);
         int64_t *loff = c.arena.ralloc<int64_t>((size_t)nl + 1);
         k_long_len<<<grid_for(nl, kBS), kBS, 0, c.stream>>>(nl, lids, len, llen);
@@ -1081,10 +1471,10 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     cg.adjncy = c.arena.lalloc<vid_t>((size_t)(cm > 0 ? cm : 1));
     cg.adjwgt = c.arena.lalloc<wgt_t>((size_t)(cm > 0 ? cm : 1));
     {
-        for (int cl = 0; cl < 4; cl++) {
+        for (int cl = 0; cl < 5; cl++) {
             if (!cb.count(cl)) continue;
-            static const char *names[4] = {"k_seg_compact8", "k_seg_compact64", "k_seg_compact256",
-                                            "k_seg_compact256"};
+            static const char *names[5] = {"k_seg_compact8", "k_seg_compact64", "k_seg_compact256",
+                                            "k_seg_compact256", "k_seg_compact256"};
             // per row: ids, toff, cxadj pair (28 B); the kernel counts the merged row read and written (16 B per entry)
             KScope ks(c, names[cl], 28.0 * (double)cb.count(cl), true);
             if (cl == 0)

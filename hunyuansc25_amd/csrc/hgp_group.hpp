@@ -19,17 +19,17 @@ constexpr int kBinG[3] = {8, 64, 256};  // lanes per row of each class
 
 struct RowBins {
     int32_t *ids = nullptr;   // row ids grouped by class (order inside a class is arbitrary)
-    int64_t off[5] = {0, 0, 0, 0, 0};
-    int64_t ent[4] = {0, 0, 0, 0};   // sum of the row lengths of each class
+    int64_t off[6] = {0, 0, 0, 0, 0, 0};
+    int64_t ent[5] = {0, 0, 0, 0, 0};   // sum of the row lengths of each class
     int64_t count(int c) const { return off[c + 1] - off[c]; }
     const int32_t *list(int c) const { return ids + off[c]; }
 };
 
 // Classes of the rows of xadj (or of the lengths len, if xadj is null): class
-// 0 holds lengths <= t0, 1 <= t1, 2 <= t2, 3 the rest.  ids is allocated on
-// the left end of the arena when `left`, else on the right.
+// 0 holds lengths <= t0, 1 <= t1, 2 <= t2, 3 <= t3, 4 the rest.  ids is
+// allocated on the left end of the arena when `left`, else on the right.
 RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool left, int64_t t0 = kBinSmall,
-                 int64_t t1 = kBinMid, int64_t t2 = INT64_MAX);
+                 int64_t t1 = kBinMid, int64_t t2 = INT64_MAX, int64_t t3 = INT64_MAX);
 
 // Rows of class 2 cut into segments of at most kSegLen entries, so that a hub
 // row is shared by several blocks.  A segment of a single-segment row is

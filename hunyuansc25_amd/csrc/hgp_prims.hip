@@ -251,18 +251,20 @@ __global__ void k_compact(const uint8_t *__restrict__ f, const int32_t *__restri
 
 // ------------------------------------------------------------ degree classes
 
+constexpr int kNumCls = 5;
+
 struct BinThr {
-    int64_t t0, t1, t2;
-    __device__ int cls(int64_t d) const { return d <= t0 ? 0 : (d <= t1 ? 1 : (d <= t2 ? 2 : 3)); }
+    int64_t t0, t1, t2, t3;
+    __device__ int cls(int64_t d) const { return d <= t0 ? 0 : (d <= t1 ? 1 : (d <= t2 ? 2 : (d <= t3 ? 3 : 4))); }
 };
 
 // Class counts: per-block LDS counters, one atomic per block and class.
 __global__ void __launch_bounds__(256) k_bin_count(int32_t n, const eid_t *__restrict__ xadj,
                                                    const int64_t *__restrict__ len, BinThr t, unsigned long long *cnt)
 {
-    __shared__ uint32_t bc[4];
-    __shared__ unsigned long long be[4];
-    if (threadIdx.x < 4) {
+    __shared__ uint32_t bc[kNumCls];
+    __shared__ unsigned long long be[kNumCls];
+    if (threadIdx.x < kNumCls) {
         bc[threadIdx.x] = 0;
         be[threadIdx.x] = 0;
     }
@@ -272,7 +274,7 @@ __global__ void __launch_bounds__(256) k_bin_count(int32_t n, const eid_t *__res
         int64_t d = v < n ? (xadj ? xadj[v + 1] - xadj[v] : len[v]) : 0;
         int cls = v < n ? t.cls(d) : -1;
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < kNumCls; c++) {
             uint64_t b = __ballot(cls == c);
             uint64_t e = wave_sum((uint64_t)(cls == c ? d : 0));
             if (lane_id() == 0 && b) {
@@ -282,14 +284,14 @@ __global__ void __launch_bounds__(256) k_bin_count(int32_t n, const eid_t *__res
         }
     }
     __syncthreads();
-    if (threadIdx.x < 4 && bc[threadIdx.x]) {
+    if (threadIdx.x < kNumCls && bc[threadIdx.x]) {
         atomicAdd(&cnt[threadIdx.x], (unsigned long long)bc[threadIdx.x]);
-        atomicAdd(&cnt[8 + threadIdx.x], be[threadIdx.x]);
+        atomicAdd(&cnt[2 * kNumCls + threadIdx.x], be[threadIdx.x]);
     }
 }
 
 struct BinBase {
-    int64_t o[4];
+    int64_t o[kNumCls];
 };
 
 // Each block takes its slots of every class with one atomic per class, then
@@ -298,20 +300,20 @@ __global__ void __launch_bounds__(256) k_bin_fill(int32_t n, const eid_t *__rest
                                                   const int64_t *__restrict__ len, BinThr t, BinBase base,
                                                   unsigned long long *cursor, int32_t *__restrict__ ids)
 {
-    __shared__ uint32_t wc[4][256 / kWave];
-    __shared__ unsigned long long bb[4];
+    __shared__ uint32_t wc[kNumCls][256 / kWave];
+    __shared__ unsigned long long bb[kNumCls];
     const int w = threadIdx.x / kWave;
     for (int64_t v0 = (int64_t)blockIdx.x * 256; v0 < n; v0 += (int64_t)gridDim.x * 256) {
         int64_t v = v0 + threadIdx.x;
         int cls = v < n ? t.cls(xadj ? xadj[v + 1] - xadj[v] : len[v]) : -1;
-        uint64_t bal[4];
+        uint64_t bal[kNumCls];
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
+        for (int c = 0; c < kNumCls; c++) {
             bal[c] = __ballot(cls == c);
             if (lane_id() == 0) wc[c][w] = (uint32_t)__popcll(bal[c]);
         }
         __syncthreads();
-        if (threadIdx.x < 4) {
+        if (threadIdx.x < kNumCls) {
             int c = threadIdx.x;
             uint32_t tot = 0;
             for (int i = 0; i < 256 / kWave; i++) {
@@ -371,27 +373,27 @@ __global__ void k_seg_fill(int64_t nl, const int32_t *__restrict__ list, const e
 }  // namespace
 
 RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool left, int64_t t0, int64_t t1,
-                 int64_t t2)
+                 int64_t t2, int64_t t3)
 {
     RowBins b;
     b.ids = left ? c.arena.lalloc<int32_t>((size_t)n) : c.arena.ralloc<int32_t>((size_t)n);
     if (n <= 0) return b;
     size_t mark = c.arena.right_mark();
-    unsigned long long *cnt = c.arena.ralloc<unsigned long long>(12);   // counts, fill cursors, entries
-    fill<uint32_t>(c, (uint32_t *)cnt, 24, 0u);
+    unsigned long long *cnt = c.arena.ralloc<unsigned long long>(3 * kNumCls);   // counts, cursors, entries
+    fill<uint32_t>(c, (uint32_t *)cnt, 6 * kNumCls, 0u);
     unsigned g = (unsigned)std::min<int64_t>(((int64_t)n + 255) / 256, 1024);
-    BinThr thr{t0, t1, t2};
+    BinThr thr{t0, t1, t2, t3};
     k_bin_count<<<g, 256, 0, c.stream>>>(n, xadj, len, thr, cnt);
     HGP_LAUNCH_CHECK();
-    read_scalars(c, cnt, 12 * sizeof(int64_t));
+    read_scalars(c, cnt, 3 * kNumCls * sizeof(int64_t));
     BinBase base;
     b.off[0] = 0;
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < kNumCls; i++) {
         base.o[i] = b.off[i];
         b.off[i + 1] = b.off[i] + c.pinned[i];
-        b.ent[i] = c.pinned[8 + i];
+        b.ent[i] = c.pinned[2 * kNumCls + i];
     }
-    k_bin_fill<<<g, 256, 0, c.stream>>>(n, xadj, len, thr, base, cnt + 4, b.ids);
+    k_bin_fill<<<g, 256, 0, c.stream>>>(n, xadj, len, thr, base, cnt + kNumCls, b.ids);
     HGP_LAUNCH_CHECK();
     c.arena.right_release(mark);
     return b;

@@ -26,7 +26,7 @@ struct JetCtl {
     int64_t cut, best_cut, best_max, maxwgt, opt, maxdest;
     unsigned long long delta2;      // 2 x cut change (two's complement)
     int32_t count, balance_counter, mode, done, iters, patience;
-    int32_t ncand, nunder, copy_best, filt_num;
+    int32_t ncand, nunder, copy_best, filt_num;   // ncand: 1 if an LP candidate exists
     int64_t nsorted;                // candidates of RW/RS
     int64_t nevict;
 };
@@ -200,11 +200,12 @@ __device__ inline int32_t sel_decide(const SelArgs &A, const JetCtl &C, int32_t 
     return cand;
 }
 
+// LP only needs to know whether any candidate exists: one store per block
+// that has some (every thread of the block calls this).
 __device__ inline void sel_count(const SelArgs &A, int32_t mode, int32_t ncand)
 {
     if (mode != kModeLP) return;
-    int32_t s = wave_sum(ncand);
-    if (lane_id() == 0 && s) atomicAdd(&A.ctl->ncand, s);
+    if (__syncthreads_or(ncand > 0) && threadIdx.x == 0) A.ctl->ncand = 1;
 }
 
 // Orders this wave's LDS accesses before and after the point (the wave runs
@@ -215,39 +216,66 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Rows of <= 8 entries: 8 lanes, one entry each; a lane's connectivity to its
-// entry's part is summed over the group with shuffles (no LDS).
+// Rows of <= 8 entries: each wave screens 64 rows (one per lane), settles the
+// ineligible ones lane by lane, then takes the eligible ones eight at a time,
+// 8 lanes and one entry per lane each; a lane's connectivity to its entry's
+// part is summed over the group with shuffles (no LDS).
 __global__ void __launch_bounds__(kRowBS) k_jet_select_s(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
 {
+    __shared__ int32_t slots[kRowBS];   // per wave: lanes holding eligible rows, in rank order
     const JetCtl &C = *A.ctl;
     if (C.done) return;
     const int32_t mode = C.mode;
     const int64_t maxwgt = C.maxwgt, opt = C.opt;
-    int64_t gi = gindex<8>();
-    int lane = glane<8>();
+    const int lane = lane_id(), grp = lane / 8, gl = lane % 8;
+    int32_t *sl = slots + (threadIdx.x / kWave) * kWave;
+    const int64_t wid = ((int64_t)blockIdx.x * kRowBS + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * kRowBS) / kWave;
     int32_t ncand = 0;
     uint64_t kbytes = 0;
-    if (gi < cnt) {
-        int32_t v = ids[gi];
-        int32_t p = A.where[v];
-        bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
-        eid_t j = A.xadj[v] + lane;
-        int32_t q = -1, w = 0;
-        if (eligible && j < A.xadj[v + 1]) { q = A.where[A.adjncy[j]]; w = A.adjwgt[j]; }
-        int32_t conn = w;
+    for (int64_t base = wid * kWave; base < cnt; base += nwaves * kWave) {
+        // one row per lane: screen, settle the ineligible ones
+        int64_t i = base + lane;
+        int32_t mv = i < cnt ? ids[i] : -1;
+        int32_t mp = mv >= 0 ? A.where[mv] : 0;
+        bool mel = mv >= 0 && sel_eligible(A, mode, maxwgt, opt, mv, mp);
+        if (mv >= 0 && !mel) sel_decide(A, C, mode, mv, mp, false, 0, PartKey{0, -1});
+        uint64_t bal = __ballot(mel);
+        const int ne = __popcll(bal);
+        if (mel) sl[__popcll(bal & lanemask_lt())] = lane;
+        wave_lds_sync();
+        // eligible rows, eight at a time, one per 8-lane group
+        for (int b0 = 0; b0 < ne; b0 += 8) {
+            int r = b0 + grp;
+            bool live = r < ne;
+            int src = live ? sl[r] : 0;
+            int32_t v = __shfl(mv, src, kWave), p = __shfl(mp, src, kWave);
+            int32_t q = -1, w = 0;
+            eid_t rb = 0, re = 0;
+            if (live) {
+                rb = A.xadj[v];
+                re = A.xadj[v + 1];
+                if (rb + gl < re) {
+                    q = A.where[A.adjncy[rb + gl]];
+                    w = A.adjwgt[rb + gl];
+                }
+            }
+            int32_t conn = w;
 #pragma unroll
-        for (int o = 1; o < 8; o++) {
-            int32_t qq = __shfl_xor(q, o, 8), ww = __shfl_xor(w, o, 8);
-            if (qq == q) conn += ww;
+            for (int o = 1; o < 8; o++) {
+                int32_t qq = __shfl_xor(q, o, 8), ww = __shfl_xor(w, o, 8);
+                if (qq == q) conn += ww;
+            }
+            int32_t pc = group_sum<8>(q == p ? w : 0);
+            PartKey best{0, -1};
+            if (q >= 0 && q != p && sel_target(A, mode, maxwgt, q)) best = PartKey{conn, q};
+            best = group_best<8>(best, BetterPart());
+            if (live && gl == 0) {
+                ncand += sel_decide(A, C, mode, v, p, true, pc, best);
+                kbytes += 16 + 12 * (uint64_t)(re - rb);
+            }
         }
-        int32_t pc = group_sum<8>(q == p ? w : 0);
-        PartKey best{0, -1};
-        if (q >= 0 && q != p && sel_target(A, mode, maxwgt, q)) best = PartKey{conn, q};
-        best = group_best<8>(best, BetterPart());
-        if (lane == 0) {
-            ncand = sel_decide(A, C, mode, v, p, eligible, pc, best);
-            if (eligible) kbytes = 16 + 12 * (uint64_t)(A.xadj[v + 1] - A.xadj[v]);
-        }
+        wave_lds_sync();
     }
     sel_count(A, mode, ncand);
     kb_flush(A.kb, kbytes);
@@ -284,6 +312,17 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
             todo &= todo - 1;
             int32_t v = __shfl(mv, l, kWave), p = __shfl(mp, l, kWave);
             eid_t b = A.xadj[v], e = A.xadj[v + 1];
+            if (mode == kModeRS) {
+                // the strong rebalancer needs only the connectivity to the own part
+                int32_t own = 0;
+                for (eid_t j = b + lane; j < e; j += kWave) own += A.where[A.adjncy[j]] == p ? A.adjwgt[j] : 0;
+                own = wave_sum(own);
+                if (lane == 0) {
+                    sel_decide(A, C, mode, v, p, true, own, PartKey{0, -1});
+                    kbytes += 16 + 12 * (uint64_t)(e - b);
+                }
+                continue;
+            }
             int32_t q[S], w[S];
 #pragma unroll
             for (int t = 0; t < S; t++) {
@@ -346,6 +385,18 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
         eid_t b = S.j0[s], e = min(b + (eid_t)kSegLen, A.xadj[v + 1]);
         if (threadIdx.x == 0 && eligible) kbytes += 8 + 12 * (uint64_t)(e - b);
+        if (mode == kModeRS) {
+            // only the connectivity to the own part: a block sum, no k-int row
+            int32_t own = 0;
+            if (eligible)
+                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) own += A.where[A.adjncy[j]] == p ? A.adjwgt[j] : 0;
+            own = group_sum<kRowBS>(own);
+            if (threadIdx.x == 0) {
+                if (mi < 0) sel_decide(A, C, mode, v, p, eligible, own, PartKey{0, -1});
+                else if (eligible && own) atomicAdd(&conn[(size_t)mi * k + p], own);
+            }
+            continue;
+        }
         bool walk_parts = e - b > (eid_t)k;
         if (eligible)
             for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) atomicAdd(&acc[A.where[A.adjncy[j]]], A.adjwgt[j]);
@@ -724,49 +775,68 @@ __global__ void k_gather_w(const int64_t *__restrict__ ne_ptr, const int32_t *__
 // Exact cut change of applying dest[] (edges between two moved vertices are
 // seen from both ends, so each side counts once and one-sided edges twice),
 // plus part-weight updates (perform_moves, :4034-4155).
-// Persistent blocks over the row list; the cut change and the part-weight
-// changes are summed in LDS (a k-entry histogram) and flushed with one global
-// atomic per block and touched part.
-template <int G>
-__global__ void __launch_bounds__(kRowBS) k_move_delta(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
-                                                       const eid_t *__restrict__ xadj,
-                                                       const vid_t *__restrict__ adjncy,
-                                                       const wgt_t *__restrict__ adjwgt,
-                                                       const wgt_t *__restrict__ vwgt,
-                                                       const int32_t *__restrict__ where,
-                                                       const int32_t *__restrict__ dest, JetCtl *ctl,
-                                                       int64_t *__restrict__ pwgts, unsigned long long *kb)
+// Move accounting (perform_moves, :4034-4155).  The cut change and the
+// part-weight changes are summed in LDS and flushed with one atomic per block
+// and touched part.  Rows of <= 8 entries: each wave screens 64 rows, then
+// walks the moved ones eight at a time (8 lanes, one entry per lane each).
+__global__ void __launch_bounds__(kRowBS) k_move_delta_s(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
+                                                         const eid_t *__restrict__ xadj,
+                                                         const vid_t *__restrict__ adjncy,
+                                                         const wgt_t *__restrict__ adjwgt,
+                                                         const wgt_t *__restrict__ vwgt,
+                                                         const int32_t *__restrict__ where,
+                                                         const int32_t *__restrict__ dest, JetCtl *ctl,
+                                                         int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
     __shared__ unsigned long long bdelta;
+    __shared__ int32_t slots[kRowBS];
     if (ctl->done) return;
     for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
     if (threadIdx.x == 0) bdelta = 0;
     __syncthreads();
+    const int lane = lane_id(), grp = lane / 8, gl = lane % 8;
+    int32_t *sl = slots + (threadIdx.x / kWave) * kWave;
+    const int64_t wid = ((int64_t)blockIdx.x * kRowBS + threadIdx.x) / kWave;
+    const int64_t nwaves = ((int64_t)gridDim.x * kRowBS) / kWave;
     uint64_t kbytes = 0;
-    constexpr int NG = kRowBS / G;
-    const int g = threadIdx.x / G, lane = glane<G>();
-    for (int64_t base = (int64_t)blockIdx.x * NG; base < cnt; base += (int64_t)gridDim.x * NG) {
-        int64_t gi = base + g;
-        int32_t v = gi < cnt ? ids[gi] : 0;
-        int32_t pv = gi < cnt ? where[v] : 0, dv = gi < cnt ? dest[v] : 0;
-        int64_t d2 = 0;
-        if (pv != dv) {
-            for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) {
-                int32_t u = adjncy[j];
-                int32_t pu = where[u], du = dest[u];
-                bool um = pu != du;
-                int32_t before = pv != pu, after = dv != (um ? du : pu);
-                d2 += (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
+    for (int64_t base = wid * kWave; base < cnt; base += nwaves * kWave) {
+        int64_t i = base + lane;
+        int32_t mv = i < cnt ? ids[i] : -1;
+        int32_t mpv = mv >= 0 ? where[mv] : 0, mdv = mv >= 0 ? dest[mv] : 0;
+        bool moved = mv >= 0 && mpv != mdv;
+        uint64_t bal = __ballot(moved);
+        const int nm = __popcll(bal);
+        if (moved) sl[__popcll(bal & lanemask_lt())] = lane;
+        wave_lds_sync();
+        for (int b0 = 0; b0 < nm; b0 += 8) {
+            int r = b0 + grp;
+            bool live = r < nm;
+            int src = live ? sl[r] : 0;
+            int32_t v = __shfl(mv, src, kWave), pv = __shfl(mpv, src, kWave), dv = __shfl(mdv, src, kWave);
+            int64_t d2 = 0;
+            eid_t rb = 0, re = 0;
+            if (live) {
+                rb = xadj[v];
+                re = xadj[v + 1];
+                eid_t j = rb + gl;
+                if (j < re) {
+                    int32_t u = adjncy[j];
+                    int32_t pu = where[u], du = dest[u];
+                    bool um = pu != du;
+                    int32_t before = pv != pu, after = dv != (um ? du : pu);
+                    d2 = (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
+                }
+            }
+            d2 = group_sum<8>(d2);
+            if (live && gl == 0) {
+                if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
+                atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
+                atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
+                kbytes += 20 + 16 * (uint64_t)(re - rb);
             }
         }
-        d2 = group_sum<G>(d2);
-        if (lane == 0 && pv != dv) {
-            if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
-            atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
-            atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
-            kbytes += 20 + 16 * (uint64_t)(xadj[v + 1] - xadj[v]);
-        }
+        wave_lds_sync();
     }
     kb_flush(kb, kbytes);
     __syncthreads();
@@ -1035,7 +1105,8 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
     if (b.count(0)) {
         KScope ks(c, "k_jet_select_s", 24.0 * (double)b.count(0), true);
         A.kb = ks.dev();
-        k_jet_select_s<<<grid_groups(b.count(0), 8), kRowBS, 0, c.stream>>>(b.list(0), b.count(0), A);
+        unsigned blocks = (unsigned)std::min<int64_t>((b.count(0) + kRowBS - 1) / kRowBS, 16384);
+        k_jet_select_s<<<blocks, kRowBS, 0, c.stream>>>(b.list(0), b.count(0), A);
         HGP_LAUNCH_CHECK();
     }
     if (b.count(1)) {
@@ -1207,8 +1278,8 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         // fixed: ids, where, dest (12 B per row); moved rows add xadj, vwgt and 16 B per entry
         if (int64_t nb = g.bins.count(0)) {
             KScope ks(c, "k_move_delta_s", 12.0 * (double)nb, true);
-            int64_t blocks = std::min<int64_t>((nb + kRowBS / 8 - 1) / (kRowBS / 8), 2048);
-            k_move_delta<8><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+            int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
+            k_move_delta_s<<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
                 g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }

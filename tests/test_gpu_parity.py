@@ -207,3 +207,63 @@ def test_invalid_inputs_raise(hgp):
         hgp.partition(x, a, 0)
     with pytest.raises(hgp.HgpError):
         hgp.partition(x, a, 2, vwgt=np.array([1, 0, 1, 1], np.int32))
+
+
+def _hub_graph(hgp):
+    """R-MAT with hubs of > 8k entries: its coarse rows reach every contraction
+    class (registers, wave registers, LDS, long-row bins) and its hub rows are
+    cut into several segments."""
+    x, a, vw, aw = hgp.gen_rmat(16, 1 << 20, seed=12)
+    deg = np.diff(x)
+    assert deg.max() > 2 * 4096
+    return x, a, vw, aw
+
+
+def test_contract_hub_rows(hgp, orc):
+    x, a, vw, aw = _hub_graph(hgp)
+    tv = int(vw.sum())
+    for lvl, k in ((0, 64), (2, 8)):
+        mv = orc.maxvwgt(tv, orc.coarsen_threshold(k))
+        m_g, _ = hgp.match(x, a, vw, aw, mv, 3, lvl, 8)
+        m_o, _ = orc.dga_match(x, a, vw, aw, mv, 3, lvl, 8)
+        np.testing.assert_array_equal(m_g, m_o)
+        got = hgp.contract(x, a, aw, vw, m_o)
+        exp = orc.contract(x, a, aw, vw, m_o)
+        for g_arr, e_arr, what in zip(got, exp, ["match", "cmap", "cxadj", "cadjncy", "cadjwgt", "cvwgt"]):
+            np.testing.assert_array_equal(g_arr, e_arr, err_msg=what)
+    # the same rows with weights: merged weights are sums
+    rng = np.random.default_rng(1)
+    aw2 = np.zeros_like(aw)
+    n = len(x) - 1
+    rows = np.repeat(np.arange(n), np.diff(x))
+    key = np.minimum(rows, a).astype(np.int64) * n + np.maximum(rows, a)
+    aw2 = (key % 13 + 1).astype(np.int32)     # symmetric per undirected edge
+    mt, _ = orc.dga_match(x, a, vw, aw2, orc.maxvwgt(tv, 512), 4, 0, 8)
+    for g_arr, e_arr in zip(hgp.contract(x, a, aw2, vw, mt), orc.contract(x, a, aw2, vw, mt)):
+        np.testing.assert_array_equal(g_arr, e_arr)
+
+
+def test_pipeline_hub_graph(hgp, orc):
+    x, a, vw, aw = _hub_graph(hgp)
+    p_g, st_g = hgp.partition(x, a, 64, vw, aw, seed=2)
+    p_o, st_o = orc.dga_partition(x, a, 64, vw, aw, seed=2)
+    np.testing.assert_array_equal(p_g, p_o)
+    assert st_g["cut"] == st_o["cut"] == orc.edge_cut(x, a, aw, p_g)
+    assert orc.balanced(vw, p_g, 64, 103, 100)
+
+
+def test_golden_fixtures_on_gpu(hgp):
+    """The committed fixtures (tests/golden, derived from the definitions)
+    hold for the GPU path too."""
+    import json
+    import os
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    for g in json.load(open(os.path.join(gdir, "contract.json"))):
+        got = hgp.contract(np.array(g["xadj"], np.int64), np.array(g["adjncy"], np.int32),
+                           np.array(g["adjwgt"], np.int32), np.array(g["vwgt"], np.int32),
+                           np.array(g["match"], np.int32))
+        for arr, key in zip(got[1:], ["cmap", "cxadj", "cadjncy", "cadjwgt", "cvwgt"]):
+            assert arr.tolist() == g[key], (g["name"], key)
+    for g in json.load(open(os.path.join(gdir, "known_cut.json"))):
+        p, st = hgp.partition(np.array(g["xadj"], np.int64), np.array(g["adjncy"], np.int32), g["k"])
+        assert st["cut"] == g["optimal_cut"]
