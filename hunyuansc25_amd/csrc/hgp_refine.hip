@@ -31,6 +31,24 @@ struct JetCtl {
     int64_t nevict;
 };
 
+// Part ids of neighbours are gathered from a narrow copy of where/dest (uint8
+// for k <= 256, else uint16): 1-2 bytes per vertex keep the gathered array
+// cache-resident where the int32 original would not be.
+struct NarrowParts {
+    const uint8_t *p8;
+    const uint16_t *p16;
+    __device__ int32_t operator()(int32_t u) const { return p8 ? (int32_t)p8[u] : (int32_t)p16[u]; }
+};
+
+__global__ void k_narrow(int32_t n, const int32_t *__restrict__ src, uint8_t *__restrict__ d8,
+                         uint16_t *__restrict__ d16)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= n) return;
+    if (d8) d8[v] = (uint8_t)src[v];
+    else d16[v] = (uint16_t)src[v];
+}
+
 // Integer gain_bucket (hunyuangraph_GPU_krefine.h:2477-2513); identical to the
 // oracle's restatement.
 __device__ inline int32_t gain_bucket(int64_t gain, int64_t vwgt)
@@ -141,6 +159,7 @@ struct SelArgs {
     int32_t *pregain;
     uint32_t *key;
     unsigned long long *kb;   // kernel-timer byte counter (may be null)
+    NarrowParts wq;           // where of the neighbours
 };
 
 __device__ inline bool sel_eligible(const SelArgs &A, int32_t mode, int64_t maxwgt, int64_t opt, int32_t v,
@@ -256,7 +275,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_s(const int32_t *__restri
                 rb = A.xadj[v];
                 re = A.xadj[v + 1];
                 if (rb + gl < re) {
-                    q = A.where[A.adjncy[rb + gl]];
+                    q = A.wq(A.adjncy[rb + gl]);
                     w = A.adjwgt[rb + gl];
                 }
             }
@@ -315,7 +334,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
             if (mode == kModeRS) {
                 // the strong rebalancer needs only the connectivity to the own part
                 int32_t own = 0;
-                for (eid_t j = b + lane; j < e; j += kWave) own += A.where[A.adjncy[j]] == p ? A.adjwgt[j] : 0;
+                for (eid_t j = b + lane; j < e; j += kWave) own += A.wq(A.adjncy[j]) == p ? A.adjwgt[j] : 0;
                 own = wave_sum(own);
                 if (lane == 0) {
                     sel_decide(A, C, mode, v, p, true, own, PartKey{0, -1});
@@ -330,7 +349,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
                 q[t] = -1;
                 w[t] = 0;
                 if (j < e) {
-                    q[t] = A.where[A.adjncy[j]];
+                    q[t] = A.wq(A.adjncy[j]);
                     w[t] = A.adjwgt[j];
                     atomicAdd(&my[q[t]], w[t]);
                 }
@@ -389,7 +408,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
             // only the connectivity to the own part: a block sum, no k-int row
             int32_t own = 0;
             if (eligible)
-                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) own += A.where[A.adjncy[j]] == p ? A.adjwgt[j] : 0;
+                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) own += A.wq(A.adjncy[j]) == p ? A.adjwgt[j] : 0;
             own = group_sum<kRowBS>(own);
             if (threadIdx.x == 0) {
                 if (mi < 0) sel_decide(A, C, mode, v, p, eligible, own, PartKey{0, -1});
@@ -399,7 +418,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         }
         bool walk_parts = e - b > (eid_t)k;
         if (eligible)
-            for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) atomicAdd(&acc[A.where[A.adjncy[j]]], A.adjwgt[j]);
+            for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) atomicAdd(&acc[A.wq(A.adjncy[j])], A.adjwgt[j]);
         __syncthreads();
         if (mi >= 0) {
             // partial sums of a multi row: move them to conn and clear
@@ -415,7 +434,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
                     }
                 } else {
                     for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) {
-                        int32_t q = A.where[A.adjncy[j]];
+                        int32_t q = A.wq(A.adjncy[j]);
                         int32_t x = atomicExch(&acc[q], 0);
                         if (x) atomicAdd(&row[q], x);
                     }
@@ -435,7 +454,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
                 }
             } else {
                 for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) {
-                    int32_t q = A.where[A.adjncy[j]];
+                    int32_t q = A.wq(A.adjncy[j]);
                     if (q == p || !sel_target(A, mode, maxwgt, q)) continue;
                     PartKey cand{acc[q], q};
                     if (BetterPart()(cand, best)) best = cand;
@@ -447,7 +466,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
             if (walk_parts) {
                 for (int q = threadIdx.x; q < k; q += kRowBS) acc[q] = 0;
             } else {
-                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) acc[A.where[A.adjncy[j]]] = 0;
+                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) acc[A.wq(A.adjncy[j])] = 0;
             }
         }
         __syncthreads();
@@ -496,7 +515,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
                                                         const vid_t *__restrict__ adjncy,
                                                         const wgt_t *__restrict__ adjwgt,
                                                         const int32_t *__restrict__ where,
-                                                        const int32_t *__restrict__ dest,
+                                                        const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq,
                                                         const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                         uint8_t *__restrict__ sel, unsigned long long *kb)
 {
@@ -517,9 +536,9 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
                 int32_t ng = pregain[u];
                 if (ng > ig || (ng == ig && u < v)) {
                     int32_t w = adjwgt[j];
-                    int32_t nd = dest[u];
+                    int32_t nd = dq(u);
                     if (nd == p) change -= w; else if (nd == best) change += w;
-                    int32_t nw = where[u];
+                    int32_t nw = wq(u);
                     if (nw == p) change += w; else if (nw == best) change -= w;
                 }
             }
@@ -540,7 +559,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
                                                           const vid_t *__restrict__ adjncy,
                                                           const wgt_t *__restrict__ adjwgt,
                                                           const int32_t *__restrict__ where,
-                                                          const int32_t *__restrict__ dest,
+                                                          const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq,
                                                           const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                           uint8_t *__restrict__ sel, unsigned long long *kb)
 {
@@ -566,9 +585,9 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
                 int32_t ng = pregain[u];
                 if (ng > ig || (ng == ig && u < v)) {
                     int32_t w = adjwgt[j];
-                    int32_t nd = dest[u];
+                    int32_t nd = dq(u);
                     if (nd == p) change -= w; else if (nd == best) change += w;
-                    int32_t nw = where[u];
+                    int32_t nw = wq(u);
                     if (nw == p) change += w; else if (nw == best) change -= w;
                 }
             }
@@ -588,7 +607,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
                                                             const vid_t *__restrict__ adjncy,
                                                             const wgt_t *__restrict__ adjwgt,
                                                             const int32_t *__restrict__ where,
-                                                            const int32_t *__restrict__ dest,
+                                                            const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq,
                                                             const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                             uint8_t *__restrict__ sel,
                                                             unsigned long long *__restrict__ macc,
@@ -612,9 +631,9 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
             int32_t ng = pregain[u];
             if (ng > ig || (ng == ig && u < v)) {
                 int32_t w = adjwgt[j];
-                int32_t nd = dest[u];
+                int32_t nd = dq(u);
                 if (nd == p) change -= w; else if (nd == best) change += w;
-                int32_t nw = where[u];
+                int32_t nw = wq(u);
                 if (nw == p) change += w; else if (nw == best) change -= w;
             }
         }
@@ -785,7 +804,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_s(const int32_t *__restri
                                                          const wgt_t *__restrict__ adjwgt,
                                                          const wgt_t *__restrict__ vwgt,
                                                          const int32_t *__restrict__ where,
-                                                         const int32_t *__restrict__ dest, JetCtl *ctl,
+                                                         const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq, JetCtl *ctl,
                                                          int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
@@ -822,7 +841,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_s(const int32_t *__restri
                 eid_t j = rb + gl;
                 if (j < re) {
                     int32_t u = adjncy[j];
-                    int32_t pu = where[u], du = dest[u];
+                    int32_t pu = wq(u), du = dq(u);
                     bool um = pu != du;
                     int32_t before = pv != pu, after = dv != (um ? du : pu);
                     d2 = (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
@@ -852,7 +871,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
                                                          const wgt_t *__restrict__ adjwgt,
                                                          const wgt_t *__restrict__ vwgt,
                                                          const int32_t *__restrict__ where,
-                                                         const int32_t *__restrict__ dest, JetCtl *ctl,
+                                                         const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq, JetCtl *ctl,
                                                          int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
@@ -877,7 +896,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
             int64_t d2 = 0;
             for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += kWave) {
                 int32_t u = adjncy[j];
-                int32_t pu = where[u], du = dest[u];
+                int32_t pu = wq(u), du = dq(u);
                 bool um = pu != du;
                 int32_t before = pv != pu, after = dv != (um ? du : pu);
                 d2 += (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
@@ -904,7 +923,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
                                                            const wgt_t *__restrict__ adjwgt,
                                                            const wgt_t *__restrict__ vwgt,
                                                            const int32_t *__restrict__ where,
-                                                           const int32_t *__restrict__ dest, JetCtl *ctl,
+                                                           const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq, JetCtl *ctl,
                                                            int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
@@ -922,7 +941,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
         int64_t d2 = 0;
         for (eid_t j = j0 + threadIdx.x; j < j1; j += kRowBS) {
             int32_t u = adjncy[j];
-            int32_t pu = where[u], du = dest[u];
+            int32_t pu = wq(u), du = dq(u);
             bool um = pu != du;
             int32_t before = pv != pu, after = dv != (um ? du : pu);
             d2 += (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
@@ -1096,10 +1115,10 @@ __global__ void k_project(int32_t n, const int32_t *__restrict__ cmap, const int
 
 void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, const int64_t *pwgts,
                    const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest, int32_t *pregain,
-                   uint32_t *key, int32_t *conn)
+                   uint32_t *key, int32_t *conn, NarrowParts wq)
 {
     const RowBins &b = g.bins;
-    SelArgs A{k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl, dest, pregain, key, nullptr};
+    SelArgs A{k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl, dest, pregain, key, nullptr, wq};
     // fixed bytes: ids, where, pwgts[p], vwgt, the decision (24 per row); eligible rows add xadj and
     // 12 B per entry (adjncy, adjwgt, where gather) in the kernel
     if (b.count(0)) {
@@ -1199,6 +1218,17 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     int64_t *wsc = c.arena.ralloc<int64_t>((size_t)n);
     int32_t *elist = c.arena.ralloc<int32_t>((size_t)n);
     int64_t *bs = c.arena.ralloc<int64_t>((size_t)n + 1);
+    // narrow copies of where and dest for the neighbour gathers
+    const bool byte_parts = k <= 256;
+    uint8_t *w8 = byte_parts ? c.arena.ralloc<uint8_t>((size_t)n) : nullptr;
+    uint8_t *d8 = byte_parts ? c.arena.ralloc<uint8_t>((size_t)n) : nullptr;
+    uint16_t *w16 = byte_parts ? nullptr : c.arena.ralloc<uint16_t>((size_t)n);
+    uint16_t *d16 = byte_parts ? nullptr : c.arena.ralloc<uint16_t>((size_t)n);
+    const NarrowParts wq{w8, w16}, dq{d8, d16};
+    auto narrow = [&](const int32_t *src, uint8_t *o8, uint16_t *o16) {
+        k_narrow<<<grid_for(n, kBS), kBS, 0, c.stream>>>(n, src, o8, o16);
+        HGP_LAUNCH_CHECK();
+    };
     int32_t *conn = c.arena.ralloc<int32_t>((size_t)g.segs.nmulti * (size_t)k);
     int64_t *macc = c.arena.ralloc<int64_t>((size_t)g.segs.nmulti);
     if (g.segs.nmulti) {
@@ -1210,6 +1240,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     HGP_CHECK_HIP(hipMemcpyAsync(best, where, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
     part_weights(c, n, g.vwgt, where, k, pwgts);
     k_jet_init<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, d_cut, maxwgt, opt, p.patience, finest ? 1 : 3);
+    narrow(where, w8, w16);   // where changes only at the commit below
     HGP_LAUNCH_CHECK();
     unsigned gv = grid_for(n, kBS);
     for (;;) {
@@ -1218,29 +1249,30 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         read_scalars(c, ctl, sizeof(JetCtl));
         JetCtl h = *(JetCtl *)c.pinned;
         if (h.done) break;
-        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn);
+        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq);
         if (h.mode == kModeLP) {
+            narrow(dest, d8, d16);   // destinations chosen by the selection
             {
                 // fixed: ids, pregain, sel (9 B per row); candidates add xadj, where, dest and 20 B per entry
                 if (int64_t nb = g.bins.count(0)) {
                     KScope ks(c, "k_afterburner_s", 9.0 * (double)nb, true);
                     k_afterburner<8><<<grid_groups(nb, 8), kRowBS, 0, c.stream>>>(
-                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl, sel, ks.dev());
+                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, wq, dq, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.count(1)) {
                     KScope ks(c, "k_afterburner_w", 9.0 * (double)nb, true);
                     unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 4096);
                     k_afterburner_w<<<blocks, kRowBS, 0, c.stream>>>(g.bins.list(1), nb, g.xadj, g.adjncy,
-                                                                     g.adjwgt, where, dest, pregain, ctl, sel,
-                                                                     ks.dev());
+                                                                     g.adjwgt, where, dest, wq, dq, pregain, ctl,
+                                                                     sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (g.segs.nseg) {
                     {
                         KScope ks(c, "k_afterburner_seg", 24.0 * (double)g.segs.nseg, true);
                         k_afterburner_seg<<<seg_grid(g.segs), kRowBS, 0, c.stream>>>(
-                            g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, pregain, ctl, sel,
+                            g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, wq, dq, pregain, ctl, sel,
                             (unsigned long long *)macc, ks.dev());
                         HGP_LAUNCH_CHECK();
                     }
@@ -1253,6 +1285,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             }
             k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock);
             HGP_LAUNCH_CHECK();
+            narrow(dest, d8, d16);   // unselected candidates stay
         } else {
             k_iota_u32<<<gv, kBS, 0, c.stream>>>(n, val);
             HGP_LAUNCH_CHECK();
@@ -1275,30 +1308,32 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                 HGP_LAUNCH_CHECK();
             }
         }
+        if (h.mode != kModeLP) narrow(dest, d8, d16);   // the rebalancers' evictions
         // fixed: ids, where, dest (12 B per row); moved rows add xadj, vwgt and 16 B per entry
         if (int64_t nb = g.bins.count(0)) {
             KScope ks(c, "k_move_delta_s", 12.0 * (double)nb, true);
             int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_s<<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts, ks.dev());
+                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (int64_t nb = g.bins.count(1)) {
             KScope ks(c, "k_move_delta_w", 12.0 * (double)nb, true);
             unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_w<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts, ks.dev());
+                g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (g.segs.nseg) {
             KScope ks(c, "k_move_delta_seg", 24.0 * (double)g.segs.nseg, true);
             unsigned blocks = (unsigned)std::min<int64_t>(g.segs.nseg, 2048);
             k_move_delta_seg<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, ctl, pwgts, ks.dev());
+                g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where);
         HGP_LAUNCH_CHECK();
+        narrow(where, w8, w16);
         k_jet_end<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts);
         HGP_LAUNCH_CHECK();
         k_copy_best<<<gv, kBS, 0, c.stream>>>(n, ctl, where, best);
