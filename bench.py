@@ -81,6 +81,30 @@ def measured_traffic(workload, kernel):
     return None
 
 
+def bench_other(hgp, torch, name, graph, k, seed, steps=3):
+    """Throughput of one more configuration, device-resident as the headline."""
+    x, a, vw, aw = graph
+    n, E = len(x) - 1, len(a) // 2
+    dev = torch.device("cuda:0")
+    d = [torch.from_numpy(arr).to(dev) for arr in (x, a, vw, aw)]
+    d_part = torch.zeros(n, dtype=torch.int32, device=dev)
+    run = lambda: hgp.partition_device(n, d[0].data_ptr(), d[1].data_ptr(), d[2].data_ptr(), d[3].data_ptr(), k,
+                                       d_part.data_ptr(), seed=seed)
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = run()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    part = d_part.cpu().numpy()
+    cut, maxp = host_cut_and_maxpart(x, a, part, k) if int(aw.min(initial=1)) == 1 == int(aw.max(initial=1)) \
+        else (st["cut"], st["max_pwgt"])
+    tv = int(vw.sum())
+    return {"workload": name, "n": n, "edges": E, "k": k, "M_edges_per_s": round(E / dt / 1e6, 3),
+            "ms": round(dt * 1e3, 3), "cut": int(st["cut"]), "balanced_1.03": bool(100 * k * st["max_pwgt"] <= 103 * tv)}
+
+
 def batch_graphs(hgp, count, seed):
     """The batched workload: sizes log-uniform in [1k, 64k] vertices, cycling
     2-D grids, 3-D random geometric graphs (mean degree 12) and banded
@@ -213,6 +237,7 @@ def main():
     ap.add_argument("--cpu-scale", type=int, default=19)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ktimer", action="store_true")
+    ap.add_argument("--no-others", action="store_true", help="skip the grid and RGG configurations")
     ap.add_argument("--calib-copy", action="store_true",
                     help="one 1 GiB device-to-device copy before the steps (counter calibration)")
     ap.add_argument("--batch-graphs", type=int, default=256, help="batched-mode sample (0: skip)")
@@ -313,6 +338,15 @@ def main():
         }
     pipeline_gbs = st["alg_bytes"] / (st["ms_total"] * 1e-3) / 1e9
 
+    others = []
+    if not args.no_others:
+        log("other configurations: grid 256x256 k=8, 3-D RGG 2^20 weighted k=64 ...")
+        others.append(bench_other(hgp, torch, "grid2d_256x256_k8", hgp.gen_grid2d(256, 256), 8, args.seed))
+        n2 = 1 << 20
+        rgg = hgp.gen_rgg3d(n2, hgp.rgg_radius(n2, 12.0), seed=args.seed, vw=(1, 4), ew=(1, 16))
+        others.append(bench_other(hgp, torch, "rgg3d_2^20_w_k64", rgg, 64, args.seed))
+        log(f"other configurations: {others}")
+
     batched = None
     if args.batch_graphs > 0:
         log(f"batched mode: {args.batch_graphs} graphs, {args.lanes} lanes ...")
@@ -367,6 +401,7 @@ def main():
         "roofline": roofline,
         "kernels": {k: [v[0], round(v[1], 3), v[2]] for k, v in kt.items()},
         "cpu_baseline": cpu,
+        "other_configs": others,
         "batched": batched,
         "serving": serving,
     }

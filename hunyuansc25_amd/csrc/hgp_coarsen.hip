@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ 
                                                     const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
                                                     const wgt_t *__restrict__ adjwgt,
                                                     const int32_t *__restrict__ avail, int32_t maxvwgt,
-                                                    uint64_t seed, int32_t level, int32_t *__restrict__ prop,
+                                                    uint64_t s1, int32_t *__restrict__ prop,
                                                     const int32_t *prev_progress, unsigned long long *kb)
 {
     if (prev_progress && *prev_progress == 0) return;
@@ -82,13 +82,14 @@ __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ 
         } else {
             EdgeKey best{-1, 0, 0, 0u};
             eid_t beg = xadj[v], end = xadj[v + 1];
+            const uint64_t tv = mix64(s1 ^ (uint64_t)(uint32_t)v);
             for (eid_t j = beg + lane; j < end; j += G) {
                 int32_t u = adjncy[j];
                 int32_t wu = avail[u];
                 if (u == v || wu < 0) continue;
                 if ((int64_t)wu + wv >= maxvwgt) continue;
                 EdgeKey e{adjwgt[j], u < v ? u : v, u < v ? v : u, 0u};
-                e.h = edge_hash(seed, level, e.lo, e.hi);
+                e.h = edge_hash_row(s1, tv, v, u);
                 if (BetterEdge()(e, best)) best = e;
             }
             best = group_best<G>(best, BetterEdge());
@@ -108,7 +109,7 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
                                                         const vid_t *__restrict__ adjncy,
                                                         const wgt_t *__restrict__ adjwgt,
                                                         const int32_t *__restrict__ avail, int32_t maxvwgt,
-                                                        uint64_t seed, int32_t level, int32_t *__restrict__ prop,
+                                                        uint64_t s1, int32_t *__restrict__ prop,
                                                         EdgeKey *__restrict__ sbest, const int32_t *prev_progress,
                                                         unsigned long long *kb)
 {
@@ -124,13 +125,14 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
         }
         EdgeKey best{-1, 0, 0, 0u};
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
+        const uint64_t tv = mix64(s1 ^ (uint64_t)(uint32_t)v);
         for (eid_t j = j0 + threadIdx.x; j < j1; j += kRowBS) {
             int32_t u = adjncy[j];
             int32_t wu = avail[u];
             if (u == v || wu < 0) continue;
             if ((int64_t)wu + wv >= maxvwgt) continue;
             EdgeKey e{adjwgt[j], u < v ? u : v, u < v ? v : u, 0u};
-            e.h = edge_hash(seed, level, e.lo, e.hi);
+            e.h = edge_hash_row(s1, tv, v, u);
             if (BetterEdge()(e, best)) best = e;
         }
         best = group_best<kRowBS>(best, BetterEdge());
@@ -1167,6 +1169,7 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
     HGP_CHECK_HIP(hipMemcpyAsync(avail, g.vwgt, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
     fill<int32_t>(c, progress, rounds > 0 ? rounds : 1, 0);
     unsigned gv = grid_for(n, kBS);
+    const uint64_t s1 = edge_key(seed, level);
     for (int r = 0; r < rounds; r++) {
         const int32_t *prev = r > 0 ? progress + r - 1 : nullptr;
         {
@@ -1177,19 +1180,19 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
                 KScope ks(c, cl == 0 ? "k_propose_s" : "k_propose_w", 12.0 * (double)cnt, true);
                 if (cl == 0)
                     k_propose<8><<<grid_groups(cnt, 8), kRowBS, 0, c.stream>>>(g.bins.list(0), cnt, g.xadj, g.adjncy,
-                                                                               g.adjwgt, avail, maxvwgt, seed, level,
-                                                                               prop, prev, ks.dev());
+                                                                               g.adjwgt, avail, maxvwgt, s1, prop,
+                                                                               prev, ks.dev());
                 else
                     k_propose<64><<<grid_groups(cnt, 64), kRowBS, 0, c.stream>>>(g.bins.list(1), cnt, g.xadj,
                                                                                  g.adjncy, g.adjwgt, avail, maxvwgt,
-                                                                                 seed, level, prop, prev, ks.dev());
+                                                                                 s1, prop, prev, ks.dev());
                 HGP_LAUNCH_CHECK();
             }
             if (g.segs.nseg) {
                 {
                     KScope ks(c, "k_propose_seg", 24.0 * (double)g.segs.nseg, true);
                     k_propose_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
-                        g.segs, g.xadj, g.adjncy, g.adjwgt, avail, maxvwgt, seed, level, prop, sbest, prev, ks.dev());
+                        g.segs, g.xadj, g.adjncy, g.adjwgt, avail, maxvwgt, s1, prop, sbest, prev, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (g.segs.nmulti) {

@@ -163,6 +163,20 @@ __host__ __device__ inline uint32_t edge_hash(uint64_t seed, int32_t level, int3
                            (uint64_t)(uint32_t)hi) >> 32);
 }
 
+// edge_hash split for the proposal loops: the level's key s1 is computed once
+// per launch and mix64(s1 ^ v) once per row, so an edge (v, u) costs one mix64
+// when u > v and two when u < v (the same value as edge_hash).
+__host__ __device__ inline uint64_t edge_key(uint64_t seed, int32_t level)
+{
+    return mix64(seed ^ (0x100000000ull + (uint64_t)(uint32_t)level));
+}
+
+__device__ inline uint32_t edge_hash_row(uint64_t s1, uint64_t tv, int32_t v, int32_t u)
+{
+    uint64_t z = u > v ? mix64(tv ^ (uint64_t)(uint32_t)u) : mix64(mix64(s1 ^ (uint64_t)(uint32_t)u) ^ (uint64_t)(uint32_t)v);
+    return (uint32_t)(z >> 32);
+}
+
 __host__ __device__ inline uint64_t trial_rng(uint64_t seed, uint32_t node, int32_t trial, int32_t draw)
 {
     return rng3(seed, 0x200000000ull + node, (uint64_t)(uint32_t)trial, (uint64_t)(uint32_t)draw);

@@ -267,3 +267,27 @@ def test_golden_fixtures_on_gpu(hgp):
     for g in json.load(open(os.path.join(gdir, "known_cut.json"))):
         p, st = hgp.partition(np.array(g["xadj"], np.int64), np.array(g["adjncy"], np.int32), g["k"])
         assert st["cut"] == g["optimal_cut"]
+
+
+def test_pipeline_weighted_rgg_k64(hgp, orc):
+    """The 3-D RGG config at 2^18 points (vertex weights 1..4, edge weights 1..16, k = 64)."""
+    n = 1 << 18
+    x, a, vw, aw = hgp.gen_rgg3d(n, hgp.rgg_radius(n, 12.0), seed=1, vw=(1, 4), ew=(1, 16))
+    p_g, st_g = hgp.partition(x, a, 64, vw, aw, seed=1)
+    p_o, st_o = orc.dga_partition(x, a, 64, vw, aw, seed=1)
+    np.testing.assert_array_equal(p_g, p_o)
+    assert st_g["cut"] == st_o["cut"] == orc.edge_cut(x, a, aw, p_g)
+    assert orc.balanced(vw, p_g, 64, 103, 100)
+
+
+def test_large_rmat_properties(hgp, orc):
+    """R-MAT 2^20 / 2^24 at k = 256 (too large for the oracle in a test): the
+    part ids, the exact balance bound, the cut against an exact recount and a
+    repeat run that must be identical."""
+    x, a, vw, aw = hgp.gen_rmat(20, 1 << 24, seed=3)
+    p1, s1 = hgp.partition(x, a, 256, vw, aw, seed=4)
+    assert p1.min() >= 0 and p1.max() < 256
+    assert orc.balanced(vw, p1, 256, 103, 100)
+    assert s1["cut"] == orc.edge_cut(x, a, aw, p1)
+    p2, s2 = hgp.partition(x, a, 256, vw, aw, seed=4)
+    np.testing.assert_array_equal(p1, p2)
