@@ -83,14 +83,19 @@ __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ 
             EdgeKey best{-1, 0, 0, 0u};
             eid_t beg = xadj[v], end = xadj[v + 1];
             const uint64_t tv = mix64(s1 ^ (uint64_t)(uint32_t)v);
-            for (eid_t j = beg + lane; j < end; j += G) {
-                int32_t u = adjncy[j];
-                int32_t wu = avail[u];
-                if (u == v || wu < 0) continue;
-                if ((int64_t)wu + wv >= maxvwgt) continue;
-                EdgeKey e{adjwgt[j], u < v ? u : v, u < v ? v : u, 0u};
-                e.h = edge_hash_row(s1, tv, v, u);
-                if (BetterEdge()(e, best)) best = e;
+            constexpr int T = G == kWave ? (int)(kBinMid / kWave) : 1;   // entries per lane and phase
+            for (eid_t c0 = beg; c0 < end; c0 += (eid_t)T * G) {
+                int32_t u[T], w[T], wu[T];
+                load_entries<T, G>(adjncy, adjwgt, c0, end, lane, u, w);
+                gather_slots<T>(avail, u, wu, -1);
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    if (u[t] < 0 || u[t] == v || wu[t] < 0) continue;
+                    if ((int64_t)wu[t] + wv >= maxvwgt) continue;
+                    EdgeKey e{w[t], u[t] < v ? u[t] : v, u[t] < v ? v : u[t], 0u};
+                    e.h = edge_hash_row(s1, tv, v, u[t]);
+                    if (BetterEdge()(e, best)) best = e;
+                }
             }
             best = group_best<G>(best, BetterEdge());
             if (lane == 0) {
@@ -126,14 +131,19 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
         EdgeKey best{-1, 0, 0, 0u};
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
         const uint64_t tv = mix64(s1 ^ (uint64_t)(uint32_t)v);
-        for (eid_t j = j0 + threadIdx.x; j < j1; j += kRowBS) {
-            int32_t u = adjncy[j];
-            int32_t wu = avail[u];
-            if (u == v || wu < 0) continue;
-            if ((int64_t)wu + wv >= maxvwgt) continue;
-            EdgeKey e{adjwgt[j], u < v ? u : v, u < v ? v : u, 0u};
-            e.h = edge_hash_row(s1, tv, v, u);
-            if (BetterEdge()(e, best)) best = e;
+        constexpr int T = 8;   // entries per thread and phase
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+            int32_t u[T], w[T], wu[T];
+            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+            gather_slots<T>(avail, u, wu, -1);
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                if (u[t] < 0 || u[t] == v || wu[t] < 0) continue;
+                if ((int64_t)wu[t] + wv >= maxvwgt) continue;
+                EdgeKey e{w[t], u[t] < v ? u[t] : v, u[t] < v ? v : u[t], 0u};
+                e.h = edge_hash_row(s1, tv, v, u[t]);
+                if (BetterEdge()(e, best)) best = e;
+            }
         }
         best = group_best<kRowBS>(best, BetterEdge());
         if (threadIdx.x == 0) {
@@ -448,7 +458,9 @@ __global__ void k_rep(int32_t n, const eid_t *__restrict__ xadj, const wgt_t *__
 // the device radix sort.
 
 constexpr uint32_t kSent = 0xFFFFFFFFu;   // dropped entry; sorts last
-constexpr int kSegWave = 512;   // rows of 65..kSegWave entries: one wave, 8 entries per lane
+constexpr int kSegWave = 1024;  // rows of 65..kSegWave entries: one wave, up to 16 entries per lane
+constexpr int kSegWaveHalf = 512;   // rows of 65..512 entries take the 8-register variant
+constexpr int kClsSegLong = 5;      // contraction class of the rows longer than kSegLds
 constexpr int kSegLds = 4096;
 constexpr int kSegBS = 512;
 constexpr int kSegE = kSegLds / kSegBS;   // entries per thread in the LDS merge
@@ -556,11 +568,11 @@ __global__ void __launch_bounds__(kRowBS) k_seg_reg(const int32_t *__restrict__ 
     kb_flush(kb, kbytes);
 }
 
-// Bitonic network over R*64 entries held 8 per lane (entry e = r*64 + lane):
+// Bitonic network over R*64 entries held R per lane (entry e = r*64 + lane):
 // partners closer than 64 are exchanged with xor-shuffles, farther ones are
 // register swaps inside the lane.
-template <int R>
-__device__ inline void wave_bitonic(uint32_t (&key)[8], int32_t (&val)[8])
+template <int R, int KR>
+__device__ inline void wave_bitonic(uint32_t (&key)[KR], int32_t (&val)[KR])
 {
     const int lane = lane_id();
     constexpr int N = R * kWave;
@@ -603,9 +615,11 @@ __device__ inline void wave_bitonic(uint32_t (&key)[8], int32_t (&val)[8])
     }
 }
 
-// Rows of 65..kSegWave entries: one wave per row, sorted in registers (padded
-// to 128, 256 or 512 entries); tails ranked by ballots, weights by wave scans
-// carried across the registers.
+// Rows of 65..KR*64 entries: one wave per row, sorted in registers (padded
+// to 128, 256, 512 or 1024 entries); tails ranked by ballots, weights by wave
+// scans carried across the registers.  KR = 8 serves rows of <= 512 entries
+// at twice the occupancy of KR = 16.
+template <int KR>
 __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__ ids, int64_t cnt,
                                                      const int32_t *__restrict__ rep,
                                                      const int32_t *__restrict__ match,
@@ -624,10 +638,10 @@ __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__
         int32_t v = rep[c], u = match[v];
         CatRow row(xadj, v, u);
         int len = (int)(row.dv + (u != v ? xadj[u + 1] - row.bu : 0));
-        uint32_t key[8];
-        int32_t val[8];
+        uint32_t key[KR];
+        int32_t val[KR];
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
+        for (int r = 0; r < KR; r++) {
             int i = r * kWave + lane;
             key[r] = kSent;
             val[r] = 0;
@@ -640,17 +654,18 @@ __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__
                 }
             }
         }
-        int R = len <= 128 ? 2 : (len <= 256 ? 4 : 8);
+        int R = len <= 128 ? 2 : (len <= 256 ? 4 : (len <= 512 ? 8 : 16));
         if (R == 2) wave_bitonic<2>(key, val);
         else if (R == 4) wave_bitonic<4>(key, val);
-        else wave_bitonic<8>(key, val);
+        else if (KR == 8 || R == 8) wave_bitonic<8>(key, val);
+        else if constexpr (KR >= 16) wave_bitonic<16>(key, val);
         const int64_t o = toff[c];
         int32_t carry = 0, lastcum = 0, rank = 0;
 #pragma unroll
-        for (int r = 0; r < 8; r++) {
+        for (int r = 0; r < KR; r++) {
             if (r < R) {
                 uint32_t nxt = __shfl_down(key[r], 1, kWave);
-                uint32_t head_next = __shfl(key[r + 1 < 8 ? r + 1 : r], 0, kWave);
+                uint32_t head_next = __shfl(key[r + 1 < KR ? r + 1 : r], 0, kWave);
                 if (lane == kWave - 1) nxt = r + 1 < R ? head_next : kSent;
                 bool tail = key[r] != kSent && nxt != key[r];
                 int32_t incl = wave_inclusive_scan(val[r]) + carry;
@@ -1145,8 +1160,9 @@ template <int MaxG = 256, class F>
 void for_bins(const RowBins &b, F f)
 {
     if (b.count(0)) f(std::integral_constant<int, 8>{}, b.list(0), b.count(0));
-    if (b.count(1)) f(std::integral_constant<int, 64>{}, b.list(1), b.count(1));
-    if (b.count(2)) f(std::integral_constant<int, MaxG < 256 ? MaxG : 256>{}, b.list(2), b.count(2));
+    if (b.count(1)) f(std::integral_constant<int, 32>{}, b.list(1), b.count(1));
+    if (b.count(2)) f(std::integral_constant<int, 64>{}, b.list(2), b.count(2));
+    if (b.count(3)) f(std::integral_constant<int, MaxG < 256 ? MaxG : 256>{}, b.list(3), b.count(3));
 }
 
 }  // namespace
@@ -1174,18 +1190,24 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
         const int32_t *prev = r > 0 ? progress + r - 1 : nullptr;
         {
             // fixed bytes: ids, avail[v], prop per row (24 per segment); each free row adds its own in the kernel
-            for (int cl = 0; cl < 2; cl++) {
+            static const char *names[3] = {"k_propose_s", "k_propose_m", "k_propose_w"};
+            for (int cl = 0; cl < 3; cl++) {
                 int64_t cnt = g.bins.count(cl);
                 if (!cnt) continue;
-                KScope ks(c, cl == 0 ? "k_propose_s" : "k_propose_w", 12.0 * (double)cnt, true);
+                KScope ks(c, names[cl], 12.0 * (double)cnt, true);
+                const int32_t *ids = g.bins.list(cl);
                 if (cl == 0)
-                    k_propose<8><<<grid_groups(cnt, 8), kRowBS, 0, c.stream>>>(g.bins.list(0), cnt, g.xadj, g.adjncy,
-                                                                               g.adjwgt, avail, maxvwgt, s1, prop,
-                                                                               prev, ks.dev());
+                    k_propose<8><<<grid_groups(cnt, 8), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
+                                                                               avail, maxvwgt, s1, prop, prev,
+                                                                               ks.dev());
+                else if (cl == 1)
+                    k_propose<32><<<grid_groups(cnt, 32), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
+                                                                                 avail, maxvwgt, s1, prop, prev,
+                                                                                 ks.dev());
                 else
-                    k_propose<64><<<grid_groups(cnt, 64), kRowBS, 0, c.stream>>>(g.bins.list(1), cnt, g.xadj,
-                                                                                 g.adjncy, g.adjwgt, avail, maxvwgt,
-                                                                                 s1, prop, prev, ks.dev());
+                    k_propose<64><<<grid_groups(cnt, 64), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
+                                                                                 avail, maxvwgt, s1, prop, prev,
+                                                                                 ks.dev());
                 HGP_LAUNCH_CHECK();
             }
             if (g.segs.nseg) {
@@ -1344,7 +1366,7 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     int32_t *tadj = c.arena.ralloc<int32_t>((size_t)m);
     int32_t *twgt = c.arena.ralloc<int32_t>((size_t)m);
     int32_t *ccnt = c.arena.ralloc<int32_t>((size_t)cn);
-    RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegWave, kSegLds);
+    RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegWaveHalf, kSegWave, kSegLds);
     {
         // per coarse row: ids, rep, match, 2 xadj pairs, toff, count (56 B); per fine entry: adjncy, adjwgt,
         // cmap gather (12 B); the kernels count the merged entries they write (8 B each)
@@ -1362,26 +1384,33 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
             HGP_LAUNCH_CHECK();
         }
         if (cb.count(2)) {
-            KScope ks(c, "k_seg_wave", bytes(2), true);
-            k_seg_wave<<<grid_groups(cb.count(2), kWave), kRowBS, 0, c.stream>>>(
+            KScope ks(c, "k_seg_wave8", bytes(2), true);
+            k_seg_wave<8><<<grid_groups(cb.count(2), kWave), kRowBS, 0, c.stream>>>(
                 cb.list(2), cb.count(2), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt,
                 ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (cb.count(3)) {
-            KScope ks(c, "k_seg_lds", bytes(3), true);
-            unsigned blocks = (unsigned)std::min<int64_t>(cb.count(3), 8192);
-            k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(3), cb.count(3), rep, match, g.xadj, g.adjncy,
+            KScope ks(c, "k_seg_wave16", bytes(3), true);
+            k_seg_wave<16><<<grid_groups(cb.count(3), kWave), kRowBS, 0, c.stream>>>(
+                cb.list(3), cb.count(3), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt,
+                ks.dev());
+            HGP_LAUNCH_CHECK();
+        }
+        if (cb.count(4)) {
+            KScope ks(c, "k_seg_lds", bytes(4), true);
+            unsigned blocks = (unsigned)std::min<int64_t>(cb.count(4), 8192);
+            k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(4), cb.count(4), rep, match, g.xadj, g.adjncy,
                                                        g.adjwgt, cmap, toff, tadj, twgt, ccnt, ks.dev());
             HGP_LAUNCH_CHECK();
         }
     }
     const int32_t nbk = (int32_t)(((int64_t)cn + kBkt - 1) >> kBktBits);
-    if (int64_t nl = cb.count(4); nl > 0 && nl * (int64_t)nbk <= kMaxBins && nbk <= kLbMaxBk) {
+    if (int64_t nl = cb.count(kClsSegLong); nl > 0 && nl * (int64_t)nbk <= kMaxBins && nbk <= kLbMaxBk) {
         // long rows by coarse-id bins
         size_t m2 = c.arena.right_mark();
-        const int32_t *lids = cb.list(4);
-        const int64_t nbins = nl * (int64_t)nbk, tl = cb.ent[4];
+        const int32_t *lids = cb.list(kClsSegLong);
+        const int64_t nbins = nl * (int64_t)nbk, tl = cb.ent[kClsSegLong];
         int64_t *nchr = c.arena.ralloc<int64_t>((size_t)nl);
         int64_t *choff = c.arena.ralloc<int64_t>((size_t)nl + 1);
         k_lb_nchunks<<<grid_for(nl, kBS), kBS, 0, c.stream>>>(nl, lids, len, nchr);
@@ -1434,7 +1463,7 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     } else if (nl > 0) {
         // more bins than kMaxBins: the device radix sort over (row, coarse id) keys
         size_t m2 = c.arena.right_mark();
-        const int32_t *lids = cb.list(4);
+        const int32_t *lids = cb.list(kClsSegLong);
         int64_t *llen = c.arena.ralloc<int64_t>((size_t)nl);
         int64_t *loff = c.arena.ralloc<int64_t>((size_t)nl + 1);
         k_long_len<<<grid_for(nl, kBS), kBS, 0, c.stream>>>(nl, lids, len, llen);
@@ -1474,21 +1503,23 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     cg.adjncy = c.arena.lalloc<vid_t>((size_t)(cm > 0 ? cm : 1));
     cg.adjwgt = c.arena.lalloc<wgt_t>((size_t)(cm > 0 ? cm : 1));
     {
-        for (int cl = 0; cl < 5; cl++) {
-            if (!cb.count(cl)) continue;
-            static const char *names[5] = {"k_seg_compact8", "k_seg_compact64", "k_seg_compact256",
-                                            "k_seg_compact256", "k_seg_compact256"};
+        // classes 2..5 are consecutive in the id list: one launch of 256-lane groups
+        const int64_t cnt[3] = {cb.count(0), cb.count(1), cb.off[kClsSegLong + 1] - cb.off[2]};
+        const int32_t *lst[3] = {cb.list(0), cb.list(1), cb.list(2)};
+        static const char *names[3] = {"k_seg_compact8", "k_seg_compact64", "k_seg_compact256"};
+        for (int cl = 0; cl < 3; cl++) {
+            if (!cnt[cl]) continue;
             // per row: ids, toff, cxadj pair (28 B); the kernel counts the merged row read and written (16 B per entry)
-            KScope ks(c, names[cl], 28.0 * (double)cb.count(cl), true);
+            KScope ks(c, names[cl], 28.0 * (double)cnt[cl], true);
             if (cl == 0)
-                k_seg_compact<8><<<grid_groups(cb.count(cl), 8), kRowBS, 0, c.stream>>>(
-                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt, ks.dev());
+                k_seg_compact<8><<<grid_groups(cnt[cl], 8), kRowBS, 0, c.stream>>>(
+                    lst[cl], cnt[cl], toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt, ks.dev());
             else if (cl == 1)
-                k_seg_compact<64><<<grid_groups(cb.count(cl), 64), kRowBS, 0, c.stream>>>(
-                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt, ks.dev());
+                k_seg_compact<64><<<grid_groups(cnt[cl], 64), kRowBS, 0, c.stream>>>(
+                    lst[cl], cnt[cl], toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt, ks.dev());
             else
-                k_seg_compact<256><<<grid_groups(cb.count(cl), 256), kRowBS, 0, c.stream>>>(
-                    cb.list(cl), cb.count(cl), toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt, ks.dev());
+                k_seg_compact<256><<<grid_groups(cnt[cl], 256), kRowBS, 0, c.stream>>>(
+                    lst[cl], cnt[cl], toff, cg.xadj, tadj, twgt, cg.adjncy, cg.adjwgt, ks.dev());
             HGP_LAUNCH_CHECK();
         }
     }

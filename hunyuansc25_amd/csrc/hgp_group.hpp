@@ -13,25 +13,29 @@
 namespace hgp {
 
 constexpr int kRowBS = 256;             // block size of row-parallel kernels
-constexpr int64_t kBinSmall = 8;        // rows <= 8 entries
-constexpr int64_t kBinMid = 512;        // rows <= 512 entries
-constexpr int kBinG[3] = {8, 64, 256};  // lanes per row of each class
+// Degree classes of the rows of a level: 0 = <= 8 entries (8 lanes per row),
+// 1 = <= 32 (32 lanes), 2 = <= 512 (one wave), 3 = longer (segments of
+// kSegLen entries, a block each).
+constexpr int64_t kBinSmall = 8;
+constexpr int64_t kBinHalf = 32;
+constexpr int64_t kBinMid = 512;
+constexpr int kClsLong = 3;
 
 struct RowBins {
     int32_t *ids = nullptr;   // row ids grouped by class (order inside a class is arbitrary)
-    int64_t off[6] = {0, 0, 0, 0, 0, 0};
-    int64_t ent[5] = {0, 0, 0, 0, 0};   // sum of the row lengths of each class
+    int64_t off[7] = {0, 0, 0, 0, 0, 0, 0};
+    int64_t ent[6] = {0, 0, 0, 0, 0, 0};   // sum of the row lengths of each class
     int64_t count(int c) const { return off[c + 1] - off[c]; }
     const int32_t *list(int c) const { return ids + off[c]; }
 };
 
 // Classes of the rows of xadj (or of the lengths len, if xadj is null): class
-// 0 holds lengths <= t0, 1 <= t1, 2 <= t2, 3 <= t3, 4 the rest.  ids is
-// allocated on the left end of the arena when `left`, else on the right.
+// 0 holds lengths <= t0, 1 <= t1, 2 <= t2, 3 <= t3, 4 <= t4, 5 the rest.  ids
+// is allocated on the left end of the arena when `left`, else on the right.
 RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool left, int64_t t0 = kBinSmall,
-                 int64_t t1 = kBinMid, int64_t t2 = INT64_MAX, int64_t t3 = INT64_MAX);
+                 int64_t t1 = kBinHalf, int64_t t2 = kBinMid, int64_t t3 = INT64_MAX, int64_t t4 = INT64_MAX);
 
-// Rows of class 2 cut into segments of at most kSegLen entries, so that a hub
+// Rows of class kClsLong cut into segments of at most kSegLen entries, so that a hub
 // row is shared by several blocks.  A segment of a single-segment row is
 // finished by the segment kernel itself; the segments of a "multi" row leave
 // partial results that a second kernel combines per row.
@@ -47,7 +51,7 @@ struct RowSegs {
     int64_t *mseg = nullptr;   // segments of multi row t: [mseg[2t], mseg[2t+1])
 };
 
-// Segments of the class-2 rows of b (arrays on the left end of the arena).
+// Segments of the class-kClsLong rows of b (arrays on the left end of the arena).
 RowSegs row_segs(Ctx &c, const eid_t *xadj, const RowBins &b);
 
 template <int G>
@@ -134,6 +138,36 @@ __device__ inline uint64_t group_lanemask_lt()
 {
     int l = glane<G>();
     return l == 0 ? 0ull : (~0ull >> (64 - l));
+}
+
+// Phased row walks.  A group of L lanes takes T entries per lane of the row
+// range [b, e): j = b + lane + t*L.  Every index and weight load is issued
+// before the first one is used, and the gathers through the indices follow
+// back to back, so a lane keeps T loads in flight where an entry-by-entry
+// loop waits out two memory round trips per entry.  Slots past e hold
+// neighbour -1 and weight 0.
+template <int T, int L>
+__device__ inline void load_entries(const vid_t *__restrict__ adjncy, const wgt_t *__restrict__ adjwgt, eid_t b,
+                                    eid_t e, int lane, int32_t (&u)[T], int32_t (&w)[T])
+{
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        eid_t j = b + lane + (eid_t)t * L;
+        u[t] = -1;
+        w[t] = 0;
+        if (j < e) {
+            u[t] = adjncy[j];
+            w[t] = adjwgt[j];
+        }
+    }
+}
+
+// x[t] = a[u[t]] for the live slots (u[t] >= 0), dflt for the others.
+template <int T, class A>
+__device__ inline void gather_slots(const A *__restrict__ a, const int32_t (&u)[T], int32_t (&x)[T], int32_t dflt)
+{
+#pragma unroll
+    for (int t = 0; t < T; t++) x[t] = u[t] >= 0 ? (int32_t)a[u[t]] : dflt;
 }
 
 inline unsigned grid_groups(int64_t groups, int G)

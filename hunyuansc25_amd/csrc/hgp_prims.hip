@@ -251,11 +251,14 @@ __global__ void k_compact(const uint8_t *__restrict__ f, const int32_t *__restri
 
 // ------------------------------------------------------------ degree classes
 
-constexpr int kNumCls = 5;
+constexpr int kNumCls = 6;
 
 struct BinThr {
-    int64_t t0, t1, t2, t3;
-    __device__ int cls(int64_t d) const { return d <= t0 ? 0 : (d <= t1 ? 1 : (d <= t2 ? 2 : (d <= t3 ? 3 : 4))); }
+    int64_t t0, t1, t2, t3, t4;
+    __device__ int cls(int64_t d) const
+    {
+        return d <= t0 ? 0 : (d <= t1 ? 1 : (d <= t2 ? 2 : (d <= t3 ? 3 : (d <= t4 ? 4 : 5))));
+    }
 };
 
 // Class counts: per-block LDS counters, one atomic per block and class.
@@ -373,7 +376,7 @@ __global__ void k_seg_fill(int64_t nl, const int32_t *__restrict__ list, const e
 }  // namespace
 
 RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool left, int64_t t0, int64_t t1,
-                 int64_t t2, int64_t t3)
+                 int64_t t2, int64_t t3, int64_t t4)
 {
     RowBins b;
     b.ids = left ? c.arena.lalloc<int32_t>((size_t)n) : c.arena.ralloc<int32_t>((size_t)n);
@@ -382,7 +385,7 @@ RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool 
     unsigned long long *cnt = c.arena.ralloc<unsigned long long>(3 * kNumCls);   // counts, cursors, entries
     fill<uint32_t>(c, (uint32_t *)cnt, 6 * kNumCls, 0u);
     unsigned g = (unsigned)std::min<int64_t>(((int64_t)n + 255) / 256, 1024);
-    BinThr thr{t0, t1, t2, t3};
+    BinThr thr{t0, t1, t2, t3, t4};
     k_bin_count<<<g, 256, 0, c.stream>>>(n, xadj, len, thr, cnt);
     HGP_LAUNCH_CHECK();
     read_scalars(c, cnt, 3 * kNumCls * sizeof(int64_t));
@@ -402,9 +405,9 @@ RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool 
 RowSegs row_segs(Ctx &c, const eid_t *xadj, const RowBins &b)
 {
     RowSegs r;
-    int64_t nl = b.count(2);
+    int64_t nl = b.count(kClsLong);
     if (nl <= 0) return r;
-    const int32_t *list = b.list(2);
+    const int32_t *list = b.list(kClsLong);
     int32_t *mimap = c.arena.lalloc<int32_t>((size_t)nl);
     size_t mark = c.arena.right_mark();
     int64_t *cnt = c.arena.ralloc<int64_t>((size_t)nl);

@@ -38,6 +38,13 @@ struct NarrowParts {
     const uint8_t *p8;
     const uint16_t *p16;
     __device__ int32_t operator()(int32_t u) const { return p8 ? (int32_t)p8[u] : (int32_t)p16[u]; }
+    // q[t] = part of u[t] for the live slots, -1 for the others
+    template <int T>
+    __device__ void gather(const int32_t (&u)[T], int32_t (&q)[T]) const
+    {
+        if (p8) gather_slots<T>(p8, u, q, -1);
+        else gather_slots<T>(p16, u, q, -1);
+    }
 };
 
 __global__ void k_narrow(int32_t n, const int32_t *__restrict__ src, uint8_t *__restrict__ d8,
@@ -235,63 +242,96 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Rows of <= 8 entries: each wave screens 64 rows (one per lane), settles the
-// ineligible ones lane by lane, then takes the eligible ones eight at a time,
-// 8 lanes and one entry per lane each; a lane's connectivity to its entry's
-// part is summed over the group with shuffles (no LDS).
-__global__ void __launch_bounds__(kRowBS) k_jet_select_s(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
+// Part flags of the iteration for the block: tg[q] = 1 if q may receive a
+// vertex (sel_target), read once per block instead of once per entry.
+__device__ inline void load_targets(const SelArgs &A, int32_t mode, int64_t maxwgt, uint8_t *tg)
 {
+    for (int q = threadIdx.x; q < A.k; q += blockDim.x) tg[q] = sel_target(A, mode, maxwgt, q) ? 1 : 0;
+    __syncthreads();
+}
+
+// Rows of <= G entries (G = 8): each wave screens 64 rows (one per lane),
+// settles the ineligible ones lane by lane, then takes the eligible ones 64/G
+// at a time, G lanes and one entry per lane each.  The entry loads of all
+// passes are issued before the first pass is reduced.  A lane's connectivity
+// to its entry's part is summed over the group with shuffles (no LDS).
+template <int G>
+__global__ void __launch_bounds__(kRowBS) k_jet_select_g(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
+{
+    extern __shared__ uint8_t tg[];     // k part flags
     __shared__ int32_t slots[kRowBS];   // per wave: lanes holding eligible rows, in rank order
     const JetCtl &C = *A.ctl;
     if (C.done) return;
     const int32_t mode = C.mode;
     const int64_t maxwgt = C.maxwgt, opt = C.opt;
-    const int lane = lane_id(), grp = lane / 8, gl = lane % 8;
+    load_targets(A, mode, maxwgt, tg);
+    constexpr int RP = kWave / G;   // rows per pass
+    constexpr int NP = kWave / RP;  // passes that cover 64 rows
+    static_assert(NP <= 8, "the passes' entries are held in registers at once");
+    const int lane = lane_id(), grp = lane / G, gl = lane % G;
     int32_t *sl = slots + (threadIdx.x / kWave) * kWave;
     const int64_t wid = ((int64_t)blockIdx.x * kRowBS + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * kRowBS) / kWave;
     int32_t ncand = 0;
     uint64_t kbytes = 0;
     for (int64_t base = wid * kWave; base < cnt; base += nwaves * kWave) {
-        // one row per lane: screen, settle the ineligible ones
+        // one row per lane: screen, settle the ineligible ones, fetch the bounds of the others
         int64_t i = base + lane;
         int32_t mv = i < cnt ? ids[i] : -1;
         int32_t mp = mv >= 0 ? A.where[mv] : 0;
         bool mel = mv >= 0 && sel_eligible(A, mode, maxwgt, opt, mv, mp);
         if (mv >= 0 && !mel) sel_decide(A, C, mode, mv, mp, false, 0, PartKey{0, -1});
+        eid_t mb = 0, me = 0;
+        if (mel) {
+            mb = A.xadj[mv];
+            me = A.xadj[mv + 1];
+        }
         uint64_t bal = __ballot(mel);
         const int ne = __popcll(bal);
         if (mel) sl[__popcll(bal & lanemask_lt())] = lane;
         wave_lds_sync();
-        // eligible rows, eight at a time, one per 8-lane group
-        for (int b0 = 0; b0 < ne; b0 += 8) {
-            int r = b0 + grp;
-            bool live = r < ne;
-            int src = live ? sl[r] : 0;
-            int32_t v = __shfl(mv, src, kWave), p = __shfl(mp, src, kWave);
-            int32_t q = -1, w = 0;
-            eid_t rb = 0, re = 0;
-            if (live) {
-                rb = A.xadj[v];
-                re = A.xadj[v + 1];
-                if (rb + gl < re) {
-                    q = A.wq(A.adjncy[rb + gl]);
-                    w = A.adjwgt[rb + gl];
+        const int np = (ne + RP - 1) / RP;
+        int32_t src[NP], rl[NP], u[NP], w[NP], q[NP];
+#pragma unroll
+        for (int t = 0; t < NP; t++) {
+            u[t] = -1;
+            w[t] = 0;
+            src[t] = -1;
+            rl[t] = 0;
+            if (t < np) {   // wave-uniform
+                int r = t * RP + grp;
+                int s = r < ne ? sl[r] : 0;
+                eid_t b = __shfl(mb, s, kWave), e = __shfl(me, s, kWave);
+                if (r < ne) {
+                    src[t] = s;
+                    rl[t] = (int32_t)(e - b);
+                }
+                if (r < ne && b + gl < e) {
+                    u[t] = A.adjncy[b + gl];
+                    w[t] = A.adjwgt[b + gl];
                 }
             }
-            int32_t conn = w;
+        }
+        A.wq.gather(u, q);
 #pragma unroll
-            for (int o = 1; o < 8; o++) {
-                int32_t qq = __shfl_xor(q, o, 8), ww = __shfl_xor(w, o, 8);
-                if (qq == q) conn += ww;
-            }
-            int32_t pc = group_sum<8>(q == p ? w : 0);
-            PartKey best{0, -1};
-            if (q >= 0 && q != p && sel_target(A, mode, maxwgt, q)) best = PartKey{conn, q};
-            best = group_best<8>(best, BetterPart());
-            if (live && gl == 0) {
-                ncand += sel_decide(A, C, mode, v, p, true, pc, best);
-                kbytes += 16 + 12 * (uint64_t)(re - rb);
+        for (int t = 0; t < NP; t++) {
+            if (t < np) {
+                int s = src[t] >= 0 ? src[t] : 0;
+                int32_t v = __shfl(mv, s, kWave), p = __shfl(mp, s, kWave);
+                int32_t conn = w[t];
+#pragma unroll
+                for (int o = 1; o < G; o++) {
+                    int32_t qq = __shfl_xor(q[t], o, G), ww = __shfl_xor(w[t], o, G);
+                    if (qq == q[t]) conn += ww;
+                }
+                int32_t pc = group_sum<G>(q[t] == p ? w[t] : 0);
+                PartKey best{0, -1};
+                if (q[t] >= 0 && q[t] != p && tg[q[t]]) best = PartKey{conn, q[t]};
+                best = group_best<G>(best, BetterPart());
+                if (src[t] >= 0 && gl == 0) {
+                    ncand += sel_decide(A, C, mode, v, p, true, pc, best);
+                    kbytes += 16 + 12 * (uint64_t)rl[t];
+                }
             }
         }
         wave_lds_sync();
@@ -300,13 +340,14 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_s(const int32_t *__restri
     kb_flush(A.kb, kbytes);
 }
 
-// Rows of 9..512 entries: a wave takes 64 rows of the list at a time, decides
-// the ineligible ones lane by lane and then walks the eligible ones, one row
-// at a time with its entries cached in registers (8 per lane) and a k-int LDS
-// accumulator per wave.  Waves loop on their own, without block barriers.
+// Rows of 9..512 entries (classes 1 and 2): a wave takes 64 rows of the list
+// at a time, decides the ineligible ones lane by lane and then walks the
+// eligible ones, one row at a time with its entries in registers (8 per
+// lane, loaded in phases) and a k-int LDS accumulator per wave.  Waves loop
+// on their own, without block barriers.
 __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
 {
-    extern __shared__ int32_t acc[];
+    extern __shared__ int32_t acc[];   // k ints per wave, then k part flags
     constexpr int S = (int)(kBinMid / kWave);
     const JetCtl &C = *A.ctl;
     if (C.done) return;
@@ -315,8 +356,9 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
     const int k = A.k, lane = lane_id();
     const int wv = threadIdx.x / kWave, nw = kRowBS / kWave;
     int32_t *my = acc + (size_t)wv * k;
+    uint8_t *tg = (uint8_t *)(acc + (size_t)nw * k);
     for (int i = lane; i < k; i += kWave) my[i] = 0;
-    wave_lds_sync();
+    load_targets(A, mode, maxwgt, tg);
     int32_t ncand = 0;
     uint64_t kbytes = 0;
     for (int64_t base = ((int64_t)blockIdx.x * nw + wv) * kWave; base < cnt; base += (int64_t)gridDim.x * nw * kWave) {
@@ -325,16 +367,25 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
         int32_t mp = mv >= 0 ? A.where[mv] : 0;
         bool mel = mv >= 0 && sel_eligible(A, mode, maxwgt, opt, mv, mp);
         if (mv >= 0 && !mel) sel_decide(A, C, mode, mv, mp, false, 0, PartKey{0, -1});
+        eid_t mb = 0, me = 0;
+        if (mel) {
+            mb = A.xadj[mv];
+            me = A.xadj[mv + 1];
+        }
         uint64_t todo = __ballot(mel);
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
             todo &= todo - 1;
             int32_t v = __shfl(mv, l, kWave), p = __shfl(mp, l, kWave);
-            eid_t b = A.xadj[v], e = A.xadj[v + 1];
+            eid_t b = __shfl(mb, l, kWave), e = __shfl(me, l, kWave);
+            int32_t u[S], w[S], q[S];
+            load_entries<S, kWave>(A.adjncy, A.adjwgt, b, e, lane, u, w);
+            A.wq.gather(u, q);
             if (mode == kModeRS) {
                 // the strong rebalancer needs only the connectivity to the own part
                 int32_t own = 0;
-                for (eid_t j = b + lane; j < e; j += kWave) own += A.wq(A.adjncy[j]) == p ? A.adjwgt[j] : 0;
+#pragma unroll
+                for (int t = 0; t < S; t++) own += q[t] == p ? w[t] : 0;
                 own = wave_sum(own);
                 if (lane == 0) {
                     sel_decide(A, C, mode, v, p, true, own, PartKey{0, -1});
@@ -342,24 +393,15 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
                 }
                 continue;
             }
-            int32_t q[S], w[S];
 #pragma unroll
-            for (int t = 0; t < S; t++) {
-                eid_t j = b + lane + (eid_t)t * kWave;
-                q[t] = -1;
-                w[t] = 0;
-                if (j < e) {
-                    q[t] = A.wq(A.adjncy[j]);
-                    w[t] = A.adjwgt[j];
-                    atomicAdd(&my[q[t]], w[t]);
-                }
-            }
+            for (int t = 0; t < S; t++)
+                if (q[t] >= 0) atomicAdd(&my[q[t]], w[t]);
             wave_lds_sync();
             int32_t pc = my[p];
             PartKey best{0, -1};
 #pragma unroll
             for (int t = 0; t < S; t++) {
-                if (q[t] >= 0 && q[t] != p && sel_target(A, mode, maxwgt, q[t])) {
+                if (q[t] >= 0 && q[t] != p && tg[q[t]]) {
                     PartKey cand{my[q[t]], q[t]};
                     if (BetterPart()(cand, best)) best = cand;
                 }
@@ -381,20 +423,22 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
 }
 
 // Rows of > 512 entries by segments (RowSegs), a k-int LDS accumulator per
-// block.  A single-segment row is decided here: the arg-max and the clearing
-// walk the k parts when the segment is longer than k, else the entries again.
-// A multi row's segments add their sums to its k-int row of conn, which
-// k_jet_select_fin reads.
+// block.  The segment's entries stay in registers (kSegLen / kRowBS per
+// thread, loaded in phases) for the accumulation, the arg-max and the
+// clearing.  A single-segment row is decided here; a multi row's segments add
+// their sums to its k-int row of conn, which k_jet_select_fin reads.
 __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A, int32_t *__restrict__ conn)
 {
-    extern __shared__ int32_t acc[];
+    extern __shared__ int32_t acc[];   // k ints, then k part flags
+    constexpr int T = (int)(kSegLen / kRowBS);
     const JetCtl &C = *A.ctl;
     if (C.done) return;
     const int32_t mode = C.mode;
     const int64_t maxwgt = C.maxwgt, opt = C.opt;
     const int k = A.k;
+    uint8_t *tg = (uint8_t *)(acc + k);
     for (int i = threadIdx.x; i < k; i += kRowBS) acc[i] = 0;
-    __syncthreads();
+    load_targets(A, mode, maxwgt, tg);
     int32_t ncand = 0;
     uint64_t kbytes = 0;
     for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
@@ -404,11 +448,14 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
         eid_t b = S.j0[s], e = min(b + (eid_t)kSegLen, A.xadj[v + 1]);
         if (threadIdx.x == 0 && eligible) kbytes += 8 + 12 * (uint64_t)(e - b);
+        int32_t u[T], w[T], q[T];
+        load_entries<T, kRowBS>(A.adjncy, A.adjwgt, b, eligible ? e : b, threadIdx.x, u, w);
+        A.wq.gather(u, q);
         if (mode == kModeRS) {
             // only the connectivity to the own part: a block sum, no k-int row
             int32_t own = 0;
-            if (eligible)
-                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) own += A.wq(A.adjncy[j]) == p ? A.adjwgt[j] : 0;
+#pragma unroll
+            for (int t = 0; t < T; t++) own += q[t] == p ? w[t] : 0;
             own = group_sum<kRowBS>(own);
             if (threadIdx.x == 0) {
                 if (mi < 0) sel_decide(A, C, mode, v, p, eligible, own, PartKey{0, -1});
@@ -417,26 +464,28 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
             continue;
         }
         bool walk_parts = e - b > (eid_t)k;
-        if (eligible)
-            for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) atomicAdd(&acc[A.wq(A.adjncy[j])], A.adjwgt[j]);
+#pragma unroll
+        for (int t = 0; t < T; t++)
+            if (q[t] >= 0) atomicAdd(&acc[q[t]], w[t]);
         __syncthreads();
         if (mi >= 0) {
             // partial sums of a multi row: move them to conn and clear
             if (eligible) {
                 int32_t *row = conn + (size_t)mi * k;
                 if (walk_parts) {
-                    for (int q = threadIdx.x; q < k; q += kRowBS) {
-                        int32_t x = acc[q];
+                    for (int r = threadIdx.x; r < k; r += kRowBS) {
+                        int32_t x = acc[r];
                         if (x) {
-                            atomicAdd(&row[q], x);
-                            acc[q] = 0;
+                            atomicAdd(&row[r], x);
+                            acc[r] = 0;
                         }
                     }
                 } else {
-                    for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) {
-                        int32_t q = A.wq(A.adjncy[j]);
-                        int32_t x = atomicExch(&acc[q], 0);
-                        if (x) atomicAdd(&row[q], x);
+#pragma unroll
+                    for (int t = 0; t < T; t++) {
+                        if (q[t] < 0) continue;
+                        int32_t x = atomicExch(&acc[q[t]], 0);
+                        if (x) atomicAdd(&row[q[t]], x);
                     }
                 }
             }
@@ -447,16 +496,16 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         PartKey best{0, -1};
         if (eligible) {
             if (walk_parts) {
-                for (int q = threadIdx.x; q < k; q += kRowBS) {
-                    if (q == p || !sel_target(A, mode, maxwgt, q)) continue;
-                    PartKey cand{acc[q], q};
+                for (int r = threadIdx.x; r < k; r += kRowBS) {
+                    if (r == p || !tg[r]) continue;
+                    PartKey cand{acc[r], r};
                     if (BetterPart()(cand, best)) best = cand;
                 }
             } else {
-                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) {
-                    int32_t q = A.wq(A.adjncy[j]);
-                    if (q == p || !sel_target(A, mode, maxwgt, q)) continue;
-                    PartKey cand{acc[q], q};
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    if (q[t] < 0 || q[t] == p || !tg[q[t]]) continue;
+                    PartKey cand{acc[q[t]], q[t]};
                     if (BetterPart()(cand, best)) best = cand;
                 }
             }
@@ -464,9 +513,11 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         best = group_best<kRowBS>(best, BetterPart());
         if (eligible) {
             if (walk_parts) {
-                for (int q = threadIdx.x; q < k; q += kRowBS) acc[q] = 0;
+                for (int r = threadIdx.x; r < k; r += kRowBS) acc[r] = 0;
             } else {
-                for (eid_t j = b + threadIdx.x; j < e; j += kRowBS) acc[A.wq(A.adjncy[j])] = 0;
+#pragma unroll
+                for (int t = 0; t < T; t++)
+                    if (q[t] >= 0) acc[q[t]] = 0;
             }
         }
         __syncthreads();
@@ -532,13 +583,11 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
             int32_t p = where[v], best = dest[v];
             int64_t change = 0;
             for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) {
-                int32_t u = adjncy[j];
-                int32_t ng = pregain[u];
+                // the three gathers go out together
+                int32_t u = adjncy[j], w = adjwgt[j];
+                int32_t ng = pregain[u], nd = dq(u), nw = wq(u);
                 if (ng > ig || (ng == ig && u < v)) {
-                    int32_t w = adjwgt[j];
-                    int32_t nd = dq(u);
                     if (nd == p) change -= w; else if (nd == best) change += w;
-                    int32_t nw = wq(u);
                     if (nw == p) change += w; else if (nw == best) change -= w;
                 }
             }
@@ -552,8 +601,9 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
     kb_flush(kb, kbytes);
 }
 
-// Rows of 9..512 entries: 64 rows per wave at a time; non-candidates are
-// settled lane by lane, candidates walked one row at a time by the wave.
+// Rows of 33..512 entries: 64 rows per wave at a time; non-candidates are
+// settled lane by lane, candidates walked one row at a time by the wave with
+// the row's entries loaded in phases (8 per lane).
 __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restrict__ ids, int64_t cnt,
                                                           const eid_t *__restrict__ xadj,
                                                           const vid_t *__restrict__ adjncy,
@@ -564,6 +614,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
                                                           uint8_t *__restrict__ sel, unsigned long long *kb)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
+    constexpr int S = (int)(kBinMid / kWave);
     const int lane = lane_id();
     uint64_t kbytes = 0;
     const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -572,37 +623,49 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
         int64_t i = base + lane;
         int32_t mv = i < cnt ? ids[i] : -1;
         int32_t mg = mv >= 0 ? pregain[mv] : kNoGain;
-        if (mv >= 0 && mg == kNoGain) sel[mv] = 0;
-        uint64_t todo = __ballot(mv >= 0 && mg != kNoGain);
+        bool live = mv >= 0 && mg != kNoGain;
+        if (mv >= 0 && !live) sel[mv] = 0;
+        eid_t mb = 0, me = 0;
+        int32_t mp = 0, md = 0;
+        if (live) {
+            mb = xadj[mv];
+            me = xadj[mv + 1];
+            mp = where[mv];
+            md = dest[mv];
+        }
+        uint64_t todo = __ballot(live);
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
             todo &= todo - 1;
             int32_t v = __shfl(mv, l, kWave), ig = __shfl(mg, l, kWave);
-            int32_t p = where[v], best = dest[v];
+            int32_t p = __shfl(mp, l, kWave), best = __shfl(md, l, kWave);
+            eid_t b = __shfl(mb, l, kWave), e = __shfl(me, l, kWave);
+            int32_t u[S], w[S], ng[S], nd[S], nw[S];
+            load_entries<S, kWave>(adjncy, adjwgt, b, e, lane, u, w);
+            gather_slots<S>(pregain, u, ng, kNoGain);
+            dq.gather(u, nd);
+            wq.gather(u, nw);
             int64_t change = 0;
-            for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += kWave) {
-                int32_t u = adjncy[j];
-                int32_t ng = pregain[u];
-                if (ng > ig || (ng == ig && u < v)) {
-                    int32_t w = adjwgt[j];
-                    int32_t nd = dq(u);
-                    if (nd == p) change -= w; else if (nd == best) change += w;
-                    int32_t nw = wq(u);
-                    if (nw == p) change += w; else if (nw == best) change -= w;
+#pragma unroll
+            for (int t = 0; t < S; t++) {
+                if (u[t] >= 0 && (ng[t] > ig || (ng[t] == ig && u[t] < v))) {
+                    if (nd[t] == p) change -= w[t]; else if (nd[t] == best) change += w[t];
+                    if (nw[t] == p) change += w[t]; else if (nw[t] == best) change -= w[t];
                 }
             }
             change = wave_sum(change);
             if (lane == 0) {
                 sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
-                kbytes += 24 + 20 * (uint64_t)(xadj[v + 1] - xadj[v]);
+                kbytes += 24 + 20 * (uint64_t)(e - b);
             }
         }
     }
     kb_flush(kb, kbytes);
 }
 
-// Class-2 rows by segments: a single-segment row decides here, a multi row
-// adds its partial change to macc for k_afterburner_fin.
+// Long rows by segments: a single-segment row decides here, a multi row adds
+// its partial change to macc for k_afterburner_fin.  Entries in phases of 8
+// per thread.
 __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid_t *__restrict__ xadj,
                                                             const vid_t *__restrict__ adjncy,
                                                             const wgt_t *__restrict__ adjwgt,
@@ -614,6 +677,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
                                                             unsigned long long *kb)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
+    constexpr int T = 8;
     uint64_t kbytes = 0;
     for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
         int32_t v = S.row[s];
@@ -625,23 +689,26 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
         }
         int32_t p = where[v], best = dest[v];
         int64_t change = 0;
-        eid_t j1 = min(S.j0[s] + (eid_t)kSegLen, xadj[v + 1]);
-        for (eid_t j = S.j0[s] + threadIdx.x; j < j1; j += kRowBS) {
-            int32_t u = adjncy[j];
-            int32_t ng = pregain[u];
-            if (ng > ig || (ng == ig && u < v)) {
-                int32_t w = adjwgt[j];
-                int32_t nd = dq(u);
-                if (nd == p) change -= w; else if (nd == best) change += w;
-                int32_t nw = wq(u);
-                if (nw == p) change += w; else if (nw == best) change -= w;
+        eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+            int32_t u[T], w[T], ng[T], nd[T], nw[T];
+            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+            gather_slots<T>(pregain, u, ng, kNoGain);
+            dq.gather(u, nd);
+            wq.gather(u, nw);
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                if (u[t] >= 0 && (ng[t] > ig || (ng[t] == ig && u[t] < v))) {
+                    if (nd[t] == p) change -= w[t]; else if (nd[t] == best) change += w[t];
+                    if (nw[t] == p) change += w[t]; else if (nw[t] == best) change -= w[t];
+                }
             }
         }
         change = group_sum<kRowBS>(change);
         if (threadIdx.x == 0) {
             if (mi < 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
             else if (change) atomicAdd(&macc[mi], (unsigned long long)change);
-            kbytes += 16 + 20 * (uint64_t)(j1 - S.j0[s]);
+            kbytes += 16 + 20 * (uint64_t)(j1 - j0);
         }
     }
     kb_flush(kb, kbytes);
@@ -791,14 +858,22 @@ __global__ void k_gather_w(const int64_t *__restrict__ ne_ptr, const int32_t *__
 
 // ---------------------------------------------------------------- moves
 
-// Exact cut change of applying dest[] (edges between two moved vertices are
-// seen from both ends, so each side counts once and one-sided edges twice),
-// plus part-weight updates (perform_moves, :4034-4155).
-// Move accounting (perform_moves, :4034-4155).  The cut change and the
-// part-weight changes are summed in LDS and flushed with one atomic per block
-// and touched part.  Rows of <= 8 entries: each wave screens 64 rows, then
-// walks the moved ones eight at a time (8 lanes, one entry per lane each).
-__global__ void __launch_bounds__(kRowBS) k_move_delta_s(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
+// Move accounting (perform_moves, :4034-4155): the exact cut change of
+// applying dest[] (edges between two moved vertices are seen from both ends,
+// so each side counts once and one-sided edges twice) and the part-weight
+// updates.  Both are summed in LDS and flushed with one atomic per block and
+// touched part.  Rows of <= G entries (G = 8 or 32): each wave screens 64
+// rows, then walks the moved ones 64/G at a time (G lanes, one entry per lane
+// each), the entry loads of all passes issued before the first is used.
+__device__ inline int64_t move_d2(int32_t pv, int32_t dv, int32_t pu, int32_t du, int32_t w)
+{
+    bool um = pu != du;
+    int32_t before = pv != pu, after = dv != (um ? du : pu);
+    return (int64_t)(after - before) * w * (um ? 1 : 2);
+}
+
+template <int G>
+__global__ void __launch_bounds__(kRowBS) k_move_delta_g(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
                                                          const eid_t *__restrict__ xadj,
                                                          const vid_t *__restrict__ adjncy,
                                                          const wgt_t *__restrict__ adjwgt,
@@ -814,7 +889,9 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_s(const int32_t *__restri
     for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
     if (threadIdx.x == 0) bdelta = 0;
     __syncthreads();
-    const int lane = lane_id(), grp = lane / 8, gl = lane % 8;
+    constexpr int RP = kWave / G;                       // rows per pass
+    constexpr int PB = kWave / RP < 8 ? kWave / RP : 8;   // passes whose loads go out together
+    const int lane = lane_id(), grp = lane / G, gl = lane % G;
     int32_t *sl = slots + (threadIdx.x / kWave) * kWave;
     const int64_t wid = ((int64_t)blockIdx.x * kRowBS + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * kRowBS) / kWave;
@@ -824,37 +901,51 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_s(const int32_t *__restri
         int32_t mv = i < cnt ? ids[i] : -1;
         int32_t mpv = mv >= 0 ? where[mv] : 0, mdv = mv >= 0 ? dest[mv] : 0;
         bool moved = mv >= 0 && mpv != mdv;
+        eid_t mb = 0, me = 0;
+        if (moved) {
+            mb = xadj[mv];
+            me = xadj[mv + 1];
+            int32_t wv = vwgt[mv];
+            atomicAdd(&pdelta[mpv], (unsigned long long)(-(int64_t)wv));
+            atomicAdd(&pdelta[mdv], (unsigned long long)(int64_t)wv);
+            kbytes += 20 + 16 * (uint64_t)(me - mb);
+        }
         uint64_t bal = __ballot(moved);
         const int nm = __popcll(bal);
         if (moved) sl[__popcll(bal & lanemask_lt())] = lane;
         wave_lds_sync();
-        for (int b0 = 0; b0 < nm; b0 += 8) {
-            int r = b0 + grp;
-            bool live = r < nm;
-            int src = live ? sl[r] : 0;
-            int32_t v = __shfl(mv, src, kWave), pv = __shfl(mpv, src, kWave), dv = __shfl(mdv, src, kWave);
-            int64_t d2 = 0;
-            eid_t rb = 0, re = 0;
-            if (live) {
-                rb = xadj[v];
-                re = xadj[v + 1];
-                eid_t j = rb + gl;
-                if (j < re) {
-                    int32_t u = adjncy[j];
-                    int32_t pu = wq(u), du = dq(u);
-                    bool um = pu != du;
-                    int32_t before = pv != pu, after = dv != (um ? du : pu);
-                    d2 = (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
+        const int np = (nm + RP - 1) / RP;
+        int64_t d2 = 0;
+        for (int t0 = 0; t0 < np; t0 += PB) {
+            int32_t src[PB], u[PB], w[PB], pu[PB], du[PB];
+#pragma unroll
+            for (int t = 0; t < PB; t++) {
+                u[t] = -1;
+                w[t] = 0;
+                src[t] = 0;
+                if (t0 + t < np) {   // wave-uniform
+                    int r = (t0 + t) * RP + grp;
+                    int s = r < nm ? sl[r] : 0;
+                    eid_t b = __shfl(mb, s, kWave), e = __shfl(me, s, kWave);
+                    src[t] = s;
+                    if (r < nm && b + gl < e) {
+                        u[t] = adjncy[b + gl];
+                        w[t] = adjwgt[b + gl];
+                    }
                 }
             }
-            d2 = group_sum<8>(d2);
-            if (live && gl == 0) {
-                if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
-                atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
-                atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
-                kbytes += 20 + 16 * (uint64_t)(re - rb);
+            wq.gather(u, pu);
+            dq.gather(u, du);
+#pragma unroll
+            for (int t = 0; t < PB; t++) {
+                if (t0 + t < np) {
+                    int32_t pv = __shfl(mpv, src[t], kWave), dv = __shfl(mdv, src[t], kWave);
+                    if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
+                }
             }
         }
+        d2 = wave_sum(d2);
+        if (lane == 0 && d2) atomicAdd(&bdelta, (unsigned long long)d2);
         wave_lds_sync();
     }
     kb_flush(kb, kbytes);
@@ -864,7 +955,8 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_s(const int32_t *__restri
     if (threadIdx.x == 0 && bdelta) atomicAdd(&ctl->delta2, bdelta);
 }
 
-// Rows of 9..512 entries: 64 rows per wave at a time, only the moved ones walked.
+// Rows of 33..512 entries: 64 rows per wave at a time, only the moved ones
+// walked, each with its entries loaded in phases (8 per lane).
 __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
                                                          const eid_t *__restrict__ xadj,
                                                          const vid_t *__restrict__ adjncy,
@@ -877,6 +969,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
     extern __shared__ unsigned long long pdelta[];
     __shared__ unsigned long long bdelta;
     if (ctl->done) return;
+    constexpr int S = (int)(kBinMid / kWave);
     for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
     if (threadIdx.x == 0) bdelta = 0;
     __syncthreads();
@@ -884,32 +977,38 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
     const int lane = lane_id();
     const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
+    int64_t d2 = 0;
     for (int64_t base = wid * kWave; base < cnt; base += nwaves * kWave) {
         int64_t i = base + lane;
         int32_t mv = i < cnt ? ids[i] : -1;
         int32_t mpv = mv >= 0 ? where[mv] : 0, mdv = mv >= 0 ? dest[mv] : 0;
-        uint64_t todo = __ballot(mv >= 0 && mpv != mdv);
+        bool moved = mv >= 0 && mpv != mdv;
+        eid_t mb = 0, me = 0;
+        if (moved) {
+            mb = xadj[mv];
+            me = xadj[mv + 1];
+            int32_t wv = vwgt[mv];
+            atomicAdd(&pdelta[mpv], (unsigned long long)(-(int64_t)wv));
+            atomicAdd(&pdelta[mdv], (unsigned long long)(int64_t)wv);
+            kbytes += 20 + 16 * (uint64_t)(me - mb);
+        }
+        uint64_t todo = __ballot(moved);
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
             todo &= todo - 1;
-            int32_t v = __shfl(mv, l, kWave), pv = __shfl(mpv, l, kWave), dv = __shfl(mdv, l, kWave);
-            int64_t d2 = 0;
-            for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += kWave) {
-                int32_t u = adjncy[j];
-                int32_t pu = wq(u), du = dq(u);
-                bool um = pu != du;
-                int32_t before = pv != pu, after = dv != (um ? du : pu);
-                d2 += (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
-            }
-            d2 = wave_sum(d2);
-            if (lane == 0) {
-                if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
-                atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
-                atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
-                kbytes += 20 + 16 * (uint64_t)(xadj[v + 1] - xadj[v]);
-            }
+            int32_t pv = __shfl(mpv, l, kWave), dv = __shfl(mdv, l, kWave);
+            eid_t b = __shfl(mb, l, kWave), e = __shfl(me, l, kWave);
+            int32_t u[S], w[S], pu[S], du[S];
+            load_entries<S, kWave>(adjncy, adjwgt, b, e, lane, u, w);
+            wq.gather(u, pu);
+            dq.gather(u, du);
+#pragma unroll
+            for (int t = 0; t < S; t++)
+                if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
         }
     }
+    d2 = wave_sum(d2);
+    if (lane_id() == 0 && d2) atomicAdd(&bdelta, (unsigned long long)d2);
     kb_flush(kb, kbytes);
     __syncthreads();
     for (int i = threadIdx.x; i < k; i += kRowBS)
@@ -917,7 +1016,8 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
     if (threadIdx.x == 0 && bdelta) atomicAdd(&ctl->delta2, bdelta);
 }
 
-// Class-2 rows by segments; the part weights move with the row's first segment.
+// Long rows by segments; the part weights move with the row's first segment.
+// Entries in phases of 8 per thread.
 __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k, const eid_t *__restrict__ xadj,
                                                            const vid_t *__restrict__ adjncy,
                                                            const wgt_t *__restrict__ adjwgt,
@@ -929,33 +1029,36 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
     extern __shared__ unsigned long long pdelta[];
     __shared__ unsigned long long bdelta;
     if (ctl->done) return;
+    constexpr int T = 8;
     for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
     if (threadIdx.x == 0) bdelta = 0;
     __syncthreads();
     uint64_t kbytes = 0;
+    int64_t d2 = 0;
     for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
         int32_t v = S.row[s];
         int32_t pv = where[v], dv = dest[v];
         if (pv == dv) continue;
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
-        int64_t d2 = 0;
-        for (eid_t j = j0 + threadIdx.x; j < j1; j += kRowBS) {
-            int32_t u = adjncy[j];
-            int32_t pu = wq(u), du = dq(u);
-            bool um = pu != du;
-            int32_t before = pv != pu, after = dv != (um ? du : pu);
-            d2 += (int64_t)(after - before) * adjwgt[j] * (um ? 1 : 2);
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+            int32_t u[T], w[T], pu[T], du[T];
+            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+            wq.gather(u, pu);
+            dq.gather(u, du);
+#pragma unroll
+            for (int t = 0; t < T; t++)
+                if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
         }
-        d2 = group_sum<kRowBS>(d2);
         if (threadIdx.x == 0) {
             kbytes += 16 + 16 * (uint64_t)(j1 - j0);
-            if (d2) atomicAdd(&bdelta, (unsigned long long)d2);
             if (j0 == xadj[v]) {
                 atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)vwgt[v]));
                 atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)vwgt[v]);
             }
         }
     }
+    d2 = group_sum<kRowBS>(d2);
+    if (threadIdx.x == 0 && d2) atomicAdd(&bdelta, (unsigned long long)d2);
     kb_flush(kb, kbytes);
     __syncthreads();
     for (int i = threadIdx.x; i < k; i += kRowBS)
@@ -1079,12 +1182,13 @@ __global__ void __launch_bounds__(kRowBS) k_cut_seg(RowSegs S, const eid_t *__re
     if (threadIdx.x == 0 && tot) atomicAdd(out, (unsigned long long)tot);
 }
 
-// Classes 0 and 1 (class 2 goes by segments).
+// Classes 0-2 (the long rows go by segments).
 template <class F>
 void for_small_bins(const RowBins &b, F f)
 {
     if (b.count(0)) f(std::integral_constant<int, 8>{}, b.list(0), b.count(0));
-    if (b.count(1)) f(std::integral_constant<int, 64>{}, b.list(1), b.count(1));
+    if (b.count(1)) f(std::integral_constant<int, 32>{}, b.list(1), b.count(1));
+    if (b.count(2)) f(std::integral_constant<int, 64>{}, b.list(2), b.count(2));
 }
 
 inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, 16384); }
@@ -1119,28 +1223,32 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
 {
     const RowBins &b = g.bins;
     SelArgs A{k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl, dest, pregain, key, nullptr, wq};
+    const size_t tg_bytes = ((size_t)k + 3) & ~(size_t)3;   // part flags of the selection kernels
     // fixed bytes: ids, where, pwgts[p], vwgt, the decision (24 per row); eligible rows add xadj and
     // 12 B per entry (adjncy, adjwgt, where gather) in the kernel
     if (b.count(0)) {
         KScope ks(c, "k_jet_select_s", 24.0 * (double)b.count(0), true);
         A.kb = ks.dev();
         unsigned blocks = (unsigned)std::min<int64_t>((b.count(0) + kRowBS - 1) / kRowBS, 16384);
-        k_jet_select_s<<<blocks, kRowBS, 0, c.stream>>>(b.list(0), b.count(0), A);
+        k_jet_select_g<8><<<blocks, kRowBS, tg_bytes, c.stream>>>(b.list(0), b.count(0), A);
         HGP_LAUNCH_CHECK();
     }
-    if (b.count(1)) {
-        KScope ks(c, "k_jet_select_w", 24.0 * (double)b.count(1), true);
+    // classes 1 and 2 are consecutive in the id list: one wave-per-row launch
+    // over both (32-lane groups for class 1 measured slower here)
+    if (int64_t nw = b.count(1) + b.count(2)) {
+        KScope ks(c, "k_jet_select_w", 24.0 * (double)nw, true);
         A.kb = ks.dev();
-        int64_t blocks = std::min<int64_t>((b.count(1) + kRowBS - 1) / kRowBS, 4096);
-        k_jet_select_w<<<(unsigned)blocks, kRowBS, sizeof(int32_t) * 4 * (size_t)k, c.stream>>>(b.list(1),
-                                                                                               b.count(1), A);
+        int64_t blocks = std::min<int64_t>((nw + kRowBS - 1) / kRowBS, 4096);
+        k_jet_select_w<<<(unsigned)blocks, kRowBS, sizeof(int32_t) * 4 * (size_t)k + tg_bytes, c.stream>>>(
+            b.list(1), nw, A);
         HGP_LAUNCH_CHECK();
     }
     if (g.segs.nseg) {
         {
             KScope ks(c, "k_jet_select_seg", 24.0 * (double)g.segs.nseg, true);
             A.kb = ks.dev();
-            k_jet_select_seg<<<seg_grid(g.segs), kRowBS, sizeof(int32_t) * (size_t)k, c.stream>>>(g.segs, A, conn);
+            k_jet_select_seg<<<seg_grid(g.segs), kRowBS, sizeof(int32_t) * (size_t)k + tg_bytes, c.stream>>>(
+                g.segs, A, conn);
             HGP_LAUNCH_CHECK();
         }
         A.kb = nullptr;
@@ -1261,9 +1369,15 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.count(1)) {
+                    KScope ks(c, "k_afterburner_m", 9.0 * (double)nb, true);
+                    k_afterburner<32><<<grid_groups(nb, 32), kRowBS, 0, c.stream>>>(
+                        g.bins.list(1), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, wq, dq, pregain, ctl, sel, ks.dev());
+                    HGP_LAUNCH_CHECK();
+                }
+                if (int64_t nb = g.bins.count(2)) {
                     KScope ks(c, "k_afterburner_w", 9.0 * (double)nb, true);
                     unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 4096);
-                    k_afterburner_w<<<blocks, kRowBS, 0, c.stream>>>(g.bins.list(1), nb, g.xadj, g.adjncy,
+                    k_afterburner_w<<<blocks, kRowBS, 0, c.stream>>>(g.bins.list(2), nb, g.xadj, g.adjncy,
                                                                      g.adjwgt, where, dest, wq, dq, pregain, ctl,
                                                                      sel, ks.dev());
                     HGP_LAUNCH_CHECK();
@@ -1313,15 +1427,22 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         if (int64_t nb = g.bins.count(0)) {
             KScope ks(c, "k_move_delta_s", 12.0 * (double)nb, true);
             int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
-            k_move_delta_s<<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+            k_move_delta_g<8><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
                 g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (int64_t nb = g.bins.count(1)) {
+            KScope ks(c, "k_move_delta_m", 12.0 * (double)nb, true);
+            int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
+            k_move_delta_g<32><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
+            HGP_LAUNCH_CHECK();
+        }
+        if (int64_t nb = g.bins.count(2)) {
             KScope ks(c, "k_move_delta_w", 12.0 * (double)nb, true);
             unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_w<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
+                g.bins.list(2), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (g.segs.nseg) {
