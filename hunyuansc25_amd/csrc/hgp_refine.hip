@@ -169,13 +169,32 @@ struct SelArgs {
     NarrowParts wq;           // where of the neighbours
 };
 
-__device__ inline bool sel_eligible(const SelArgs &A, int32_t mode, int64_t maxwgt, int64_t opt, int32_t v,
-                                    int32_t p)
+// The per-vertex inputs of a decision besides its connectivity: part, vertex
+// weight, lock (LP) and the weight of the own part (RW, RS).  Loaded once
+// per vertex when its row is screened.
+struct SelRow {
+    int32_t p = 0, vw = 0, lk = 0;
+    int64_t pwp = 0;
+};
+
+__device__ inline SelRow sel_row(const SelArgs &A, int32_t mode, int32_t v)
+{
+    SelRow r;
+    r.p = A.where[v];
+    if (mode == kModeLP) {
+        r.lk = A.lock[v];
+    } else {
+        r.vw = A.vwgt[v];
+        r.pwp = A.pwgts[r.p];
+    }
+    return r;
+}
+
+__device__ inline bool sel_eligible(int32_t mode, int64_t maxwgt, int64_t opt, const SelRow &r)
 {
     if (mode == kModeLP) return true;
-    int64_t pw = A.pwgts[p];
-    if (mode == kModeRW) return pw > maxwgt && 2 * (int64_t)A.vwgt[v] < 3 * (pw - opt);
-    return pw > maxwgt;
+    if (mode == kModeRW) return r.pwp > maxwgt && 2 * (int64_t)r.vw < 3 * (r.pwp - opt);
+    return r.pwp > maxwgt;
 }
 
 // Whether part q may receive v in this mode (q != p).
@@ -185,45 +204,78 @@ __device__ inline bool sel_target(const SelArgs &A, int32_t mode, int64_t maxwgt
     return mode != kModeRW || A.pwgts[q] < maxwgt;
 }
 
-// Writes dest/pregain/key of v from (pc, best); returns 1 for an LP candidate.
-__device__ inline int32_t sel_decide(const SelArgs &A, const JetCtl &C, int32_t mode, int32_t v, int32_t p,
+// A decision: destination, pregain (LP) or rebalancing key (RW, RS, as its
+// bit pattern), and whether v is an LP candidate.
+struct SelOut {
+    int32_t d, val, cand;
+};
+
+__device__ inline SelOut sel_compute(const SelArgs &A, const JetCtl &C, int32_t mode, int32_t v, const SelRow &r,
                                      bool eligible, int32_t pc, PartKey best)
 {
     const uint32_t nokey = (uint32_t)A.k * kMaxBuckets;
+    const int32_t p = r.p;
     int32_t bc = best.c, bq = best.q;
-    int32_t d = p, cand = 0;
+    SelOut o{p, 0, 0};
     if (mode == kModeLP) {
         int32_t gain = kNoGain;
         if (bq >= 0 && bc > 0) {
             int32_t fn = C.filt_num;
             if (bc >= pc || (int64_t)(pc - bc) < ((int64_t)fn * pc) / 4) {
-                d = bq;
-                if (!A.lock[v]) { gain = bc - pc; cand = 1; }
+                o.d = bq;
+                if (!r.lk) { gain = bc - pc; o.cand = 1; }
             }
         }
-        A.pregain[v] = gain;
+        o.val = gain;
     } else if (mode == kModeRW) {
         uint32_t kk = nokey;
         if (eligible) {
             int64_t gain = 0;
             bool mv = false;
+            int32_t d = p;
             if (bq >= 0 && bc > 0) { d = bq; gain = (int64_t)bc - pc; mv = true; }
             else if (C.nunder > 0) { d = A.under[v % C.nunder]; gain = -(int64_t)pc; mv = true; }
-            if (mv && d != p) kk = (uint32_t)p * kMaxBuckets + (uint32_t)gain_bucket(gain, A.vwgt[v]);
-            else d = p;
+            if (mv && d != p) {
+                kk = (uint32_t)p * kMaxBuckets + (uint32_t)gain_bucket(gain, r.vw);
+                o.d = d;
+            }
         }
-        A.key[v] = kk;
+        o.val = (int32_t)kk;
     } else {
         uint32_t kk = nokey;
         if (eligible) {
-            int32_t bkt = gain_bucket(-(int64_t)pc, A.vwgt[v]);
-            if (bkt < kMaxBuckets && (int64_t)A.vwgt[v] < 2 * (A.pwgts[p] - C.opt))
-                kk = (uint32_t)p * kMaxBuckets + (uint32_t)bkt;
+            int32_t bkt = gain_bucket(-(int64_t)pc, r.vw);
+            if (bkt < kMaxBuckets && (int64_t)r.vw < 2 * (r.pwp - C.opt)) kk = (uint32_t)p * kMaxBuckets + (uint32_t)bkt;
         }
-        A.key[v] = kk;
+        o.val = (int32_t)kk;
     }
-    A.dest[v] = d;
-    return cand;
+    return o;
+}
+
+__device__ inline void sel_store(const SelArgs &A, int32_t mode, int32_t v, const SelOut &o)
+{
+    A.dest[v] = o.d;
+    if (mode == kModeLP) A.pregain[v] = o.val;
+    else A.key[v] = (uint32_t)o.val;
+}
+
+// Writes dest/pregain/key of v from (pc, best); returns 1 for an LP candidate.
+__device__ inline int32_t sel_decide(const SelArgs &A, const JetCtl &C, int32_t mode, int32_t v, const SelRow &r,
+                                     bool eligible, int32_t pc, PartKey best)
+{
+    SelOut o = sel_compute(A, C, mode, v, r, eligible, pc, best);
+    sel_store(A, mode, v, o);
+    return o.cand;
+}
+
+__device__ inline SelRow shfl_row(const SelRow &r, int src)
+{
+    SelRow o;
+    o.p = __shfl(r.p, src, kWave);
+    o.vw = __shfl(r.vw, src, kWave);
+    o.lk = __shfl(r.lk, src, kWave);
+    o.pwp = __shfl(r.pwp, src, kWave);
+    return o;
 }
 
 // LP only needs to know whether any candidate exists: one store per block
@@ -278,9 +330,10 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_g(const int32_t *__restri
         // one row per lane: screen, settle the ineligible ones, fetch the bounds of the others
         int64_t i = base + lane;
         int32_t mv = i < cnt ? ids[i] : -1;
-        int32_t mp = mv >= 0 ? A.where[mv] : 0;
-        bool mel = mv >= 0 && sel_eligible(A, mode, maxwgt, opt, mv, mp);
-        if (mv >= 0 && !mel) sel_decide(A, C, mode, mv, mp, false, 0, PartKey{0, -1});
+        SelRow mr;
+        if (mv >= 0) mr = sel_row(A, mode, mv);
+        bool mel = mv >= 0 && sel_eligible(mode, maxwgt, opt, mr);
+        if (mv >= 0 && !mel) sel_decide(A, C, mode, mv, mr, false, 0, PartKey{0, -1});
         eid_t mb = 0, me = 0;
         if (mel) {
             mb = A.xadj[mv];
@@ -317,7 +370,9 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_g(const int32_t *__restri
         for (int t = 0; t < NP; t++) {
             if (t < np) {
                 int s = src[t] >= 0 ? src[t] : 0;
-                int32_t v = __shfl(mv, s, kWave), p = __shfl(mp, s, kWave);
+                int32_t v = __shfl(mv, s, kWave);
+                const SelRow rr = shfl_row(mr, s);
+                const int32_t p = rr.p;
                 int32_t conn = w[t];
 #pragma unroll
                 for (int o = 1; o < G; o++) {
@@ -329,7 +384,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_g(const int32_t *__restri
                 if (q[t] >= 0 && q[t] != p && tg[q[t]]) best = PartKey{conn, q[t]};
                 best = group_best<G>(best, BetterPart());
                 if (src[t] >= 0 && gl == 0) {
-                    ncand += sel_decide(A, C, mode, v, p, true, pc, best);
+                    ncand += sel_decide(A, C, mode, v, rr, true, pc, best);
                     kbytes += 16 + 12 * (uint64_t)rl[t];
                 }
             }
@@ -364,9 +419,10 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
     for (int64_t base = ((int64_t)blockIdx.x * nw + wv) * kWave; base < cnt; base += (int64_t)gridDim.x * nw * kWave) {
         int64_t i = base + lane;
         int32_t mv = i < cnt ? ids[i] : -1;
-        int32_t mp = mv >= 0 ? A.where[mv] : 0;
-        bool mel = mv >= 0 && sel_eligible(A, mode, maxwgt, opt, mv, mp);
-        if (mv >= 0 && !mel) sel_decide(A, C, mode, mv, mp, false, 0, PartKey{0, -1});
+        SelRow mr;
+        if (mv >= 0) mr = sel_row(A, mode, mv);
+        bool mel = mv >= 0 && sel_eligible(mode, maxwgt, opt, mr);
+        if (mv >= 0 && !mel) sel_decide(A, C, mode, mv, mr, false, 0, PartKey{0, -1});
         eid_t mb = 0, me = 0;
         if (mel) {
             mb = A.xadj[mv];
@@ -376,7 +432,9 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
             todo &= todo - 1;
-            int32_t v = __shfl(mv, l, kWave), p = __shfl(mp, l, kWave);
+            int32_t v = __shfl(mv, l, kWave);
+            const SelRow rr = shfl_row(mr, l);
+            const int32_t p = rr.p;
             eid_t b = __shfl(mb, l, kWave), e = __shfl(me, l, kWave);
             int32_t u[S], w[S], q[S];
             load_entries<S, kWave>(A.adjncy, A.adjwgt, b, e, lane, u, w);
@@ -388,7 +446,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
                 for (int t = 0; t < S; t++) own += q[t] == p ? w[t] : 0;
                 own = wave_sum(own);
                 if (lane == 0) {
-                    sel_decide(A, C, mode, v, p, true, own, PartKey{0, -1});
+                    sel_decide(A, C, mode, v, rr, true, own, PartKey{0, -1});
                     kbytes += 16 + 12 * (uint64_t)(e - b);
                 }
                 continue;
@@ -413,7 +471,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
                 if (q[t] >= 0) my[q[t]] = 0;
             wave_lds_sync();
             if (lane == 0) {
-                ncand += sel_decide(A, C, mode, v, p, true, pc, best);
+                ncand += sel_decide(A, C, mode, v, rr, true, pc, best);
                 kbytes += 16 + 12 * (uint64_t)(e - b);
             }
         }
@@ -444,8 +502,9 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
     for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
         int32_t v = S.row[s];
         int32_t mi = S.mi[s];
-        int32_t p = A.where[v];
-        bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
+        const SelRow rr = sel_row(A, mode, v);
+        const int32_t p = rr.p;
+        bool eligible = sel_eligible(mode, maxwgt, opt, rr);
         eid_t b = S.j0[s], e = min(b + (eid_t)kSegLen, A.xadj[v + 1]);
         if (threadIdx.x == 0 && eligible) kbytes += 8 + 12 * (uint64_t)(e - b);
         int32_t u[T], w[T], q[T];
@@ -458,7 +517,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
             for (int t = 0; t < T; t++) own += q[t] == p ? w[t] : 0;
             own = group_sum<kRowBS>(own);
             if (threadIdx.x == 0) {
-                if (mi < 0) sel_decide(A, C, mode, v, p, eligible, own, PartKey{0, -1});
+                if (mi < 0) sel_decide(A, C, mode, v, rr, eligible, own, PartKey{0, -1});
                 else if (eligible && own) atomicAdd(&conn[(size_t)mi * k + p], own);
             }
             continue;
@@ -521,7 +580,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
             }
         }
         __syncthreads();
-        if (threadIdx.x == 0) ncand += sel_decide(A, C, mode, v, p, eligible, pc, best);
+        if (threadIdx.x == 0) ncand += sel_decide(A, C, mode, v, rr, eligible, pc, best);
     }
     sel_count(A, mode, ncand);
     kb_flush(A.kb, kbytes);
@@ -538,8 +597,9 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_fin(RowSegs S, SelArgs A,
     int32_t ncand = 0;
     for (int32_t t = blockIdx.x; t < S.nmulti; t += gridDim.x) {
         int32_t v = S.mrow[t];
-        int32_t p = A.where[v];
-        bool eligible = sel_eligible(A, mode, maxwgt, opt, v, p);
+        const SelRow rr = sel_row(A, mode, v);
+        const int32_t p = rr.p;
+        bool eligible = sel_eligible(mode, maxwgt, opt, rr);
         int32_t *row = conn + (size_t)t * k;
         int32_t pc = eligible ? row[p] : 0;
         PartKey best{0, -1};
@@ -552,7 +612,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_fin(RowSegs S, SelArgs A,
         best = group_best<kRowBS>(best, BetterPart());
         if (eligible)
             for (int q = threadIdx.x; q < k; q += kRowBS) row[q] = 0;
-        if (threadIdx.x == 0) ncand += sel_decide(A, C, mode, v, p, eligible, pc, best);
+        if (threadIdx.x == 0) ncand += sel_decide(A, C, mode, v, rr, eligible, pc, best);
     }
     sel_count(A, mode, ncand);
 }
@@ -691,14 +751,23 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
         int64_t change = 0;
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
         for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
-            int32_t u[T], w[T], ng[T], nd[T], nw[T];
+            int32_t u[T], w[T], ng[T], ua[T], nd[T], nw[T];
             load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
             gather_slots<T>(pregain, u, ng, kNoGain);
-            dq.gather(u, nd);
-            wq.gather(u, nw);
+            // only the neighbours ahead of v in priority need their parts.  ua is
+            // a separate array filled by selects: a short-circuit test that
+            // conditionally overwrote u[t] in place was compiled into code that
+            // kept the old ids
 #pragma unroll
             for (int t = 0; t < T; t++) {
-                if (u[t] >= 0 && (ng[t] > ig || (ng[t] == ig && u[t] < v))) {
+                bool ahead = (u[t] >= 0) & ((ng[t] > ig) | ((ng[t] == ig) & (u[t] < v)));
+                ua[t] = ahead ? u[t] : -1;
+            }
+            dq.gather(ua, nd);
+            wq.gather(ua, nw);
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                if (ua[t] >= 0) {
                     if (nd[t] == p) change -= w[t]; else if (nd[t] == best) change += w[t];
                     if (nw[t] == p) change += w[t]; else if (nw[t] == best) change -= w[t];
                 }
