@@ -35,7 +35,6 @@ int32_t coarsen_maxvwgt(int64_t tvwgt, int32_t thr)
 namespace {
 
 constexpr int kBS = 256;
-constexpr int kG = 16;          // lanes per vertex in row-scanning kernels
 
 // Edge order of the handshake: weight desc, hash desc, lo asc, hi asc.
 __device__ inline bool better(int32_t w, uint32_t h, int32_t lo, int32_t hi, int32_t bw, uint32_t bh, int32_t blo,
@@ -303,25 +302,65 @@ __global__ void k_pair_list(int64_t cap, const int64_t *__restrict__ count, cons
 
 // ---- twin matches (hunyuangraph_GPU_match.h:1519-1574), restated as exact
 // grouping by (degree, row sum) with consecutive pairing in id order.
-__global__ void __launch_bounds__(kBS) k_rowsum(int64_t cap, const int64_t *__restrict__ count,
-                                                const int32_t *__restrict__ list, const eid_t *__restrict__ xadj,
-                                                const vid_t *__restrict__ adjncy, uint64_t *__restrict__ key,
-                                                uint32_t *__restrict__ val, int64_t *__restrict__ rsum)
+// Row sums of adjncy for the unmatched rows (the twin key), by degree class:
+// G lanes per row for rows of <= 512 entries, segments for longer rows
+// (adding into rsum, which starts at zero).
+template <int G>
+__global__ void __launch_bounds__(kRowBS) k_rsum_g(const int32_t *__restrict__ ids, int64_t cnt,
+                                                   const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
+                                                   const wgt_t *__restrict__ adjwgt, const int32_t *__restrict__ match,
+                                                   int64_t *__restrict__ rsum)
 {
-    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t i = gt / kG;
-    int lane = (int)(threadIdx.x % kG);
+    int64_t gi = gindex<G>();
+    int lane = glane<G>();
+    if (gi >= cnt) return;   // whole groups
+    int32_t v = ids[gi];
+    if (match[v] != -1) return;
+    constexpr int T = G == kWave ? (int)(kBinMid / kWave) : 1;   // entries per lane and phase
+    int64_t sum = 0;
+    eid_t b = xadj[v], e = xadj[v + 1];
+    for (eid_t c0 = b; c0 < e; c0 += (eid_t)T * G) {
+        int32_t u[T], w[T];
+        load_entries<T, G>(adjncy, adjwgt, c0, e, lane, u, w);
+#pragma unroll
+        for (int t = 0; t < T; t++) sum += u[t] >= 0 ? u[t] : 0;
+    }
+    sum = group_sum<G>(sum);
+    if (lane == 0) rsum[v] = sum;
+}
+
+__global__ void __launch_bounds__(kRowBS) k_rsum_seg(RowSegs S, const eid_t *__restrict__ xadj,
+                                                     const vid_t *__restrict__ adjncy,
+                                                     const wgt_t *__restrict__ adjwgt,
+                                                     const int32_t *__restrict__ match,
+                                                     unsigned long long *__restrict__ rsum)
+{
+    constexpr int T = 8;
+    for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
+        int32_t v = S.row[s];
+        if (match[v] != -1) continue;   // block-uniform
+        eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
+        int64_t sum = 0;
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+            int32_t u[T], w[T];
+            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+#pragma unroll
+            for (int t = 0; t < T; t++) sum += u[t] >= 0 ? u[t] : 0;
+        }
+        sum = group_sum<kRowBS>(sum);
+        if (threadIdx.x == 0 && sum) atomicAdd(&rsum[v], (unsigned long long)sum);
+    }
+}
+
+// Twin sort keys of the unmatched list: the row sum, with the vertex id.
+__global__ void k_twin_keys(int64_t cap, const int64_t *__restrict__ count, const int32_t *__restrict__ list,
+                            const int64_t *__restrict__ rsum, uint64_t *__restrict__ key, uint32_t *__restrict__ val)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= cap || i >= *count) return;
     int32_t v = list[i];
-    int64_t s = 0;
-    for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += kG) s += adjncy[j];
-#pragma unroll
-    for (int o = kG / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, kG);
-    if (lane == 0) {
-        key[i] = (uint64_t)s;
-        val[i] = (uint32_t)v;
-        rsum[v] = s;
-    }
+    key[i] = (uint64_t)rsum[v];
+    val[i] = (uint32_t)v;
 }
 
 __global__ void k_degkey(int64_t cap, const int64_t *__restrict__ count, const uint32_t *__restrict__ val,
@@ -400,15 +439,87 @@ __global__ void __launch_bounds__(kRowBS) k_rel_dist(const int32_t *__restrict__
         if (lane == 0) dist[v] = -1;
         return;
     }
+    constexpr int T = G == kWave ? (int)(kBinMid / kWave) : 1;   // entries per lane and phase
     DistKey best{-1, 0, 0, -1};
     eid_t b = xadj[v], e = xadj[v + 1];
-    for (eid_t j = b + lane; j < e; j += G) {
-        int32_t u = adjncy[j];
-        DistKey k{(int32_t)(xadj[u + 1] - xadj[u]), adjwgt[j], (int32_t)(j - b), u};
-        if (BetterDist()(k, best)) best = k;
+    for (eid_t c0 = b; c0 < e; c0 += (eid_t)T * G) {
+        int32_t u[T], w[T];
+        int64_t d0[T], d1[T];
+        load_entries<T, G>(adjncy, adjwgt, c0, e, lane, u, w);
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            d0[t] = u[t] >= 0 ? xadj[u[t]] : 0;
+            d1[t] = u[t] >= 0 ? xadj[u[t] + 1] : 0;
+        }
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            if (u[t] < 0) continue;
+            DistKey k{(int32_t)(d1[t] - d0[t]), w[t], (int32_t)(c0 + lane + (eid_t)t * G - b), u[t]};
+            if (BetterDist()(k, best)) best = k;
+        }
     }
     best = group_best<G>(best, BetterDist());
     if (lane == 0) dist[v] = best.u;
+}
+
+// Long rows by segments: a single-segment row writes dist, a multi row leaves
+// the best of each segment in dbest for k_rel_dist_fin.  (Degrees are
+// differences of xadj entries, exact in 32 bits for rows of < 2^31 entries.)
+__global__ void __launch_bounds__(kRowBS) k_rel_dist_seg(RowSegs S, const eid_t *__restrict__ xadj,
+                                                         const vid_t *__restrict__ adjncy,
+                                                         const wgt_t *__restrict__ adjwgt,
+                                                         const int32_t *__restrict__ match,
+                                                         int32_t *__restrict__ dist, DistKey *__restrict__ dbest)
+{
+    constexpr int T = 8;
+    for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
+        int32_t v = S.row[s];
+        int32_t mi = S.mi[s];
+        if (match[v] != -1) {   // block-uniform
+            if (threadIdx.x == 0 && mi < 0) dist[v] = -1;
+            continue;
+        }
+        const eid_t b = xadj[v];
+        eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
+        DistKey best{-1, 0, 0, -1};
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+            int32_t u[T], w[T];
+            int64_t d0[T], d1[T];
+            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                d0[t] = u[t] >= 0 ? xadj[u[t]] : 0;
+                d1[t] = u[t] >= 0 ? xadj[u[t] + 1] : 0;
+            }
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                if (u[t] < 0) continue;
+                DistKey k{(int32_t)(d1[t] - d0[t]), w[t], (int32_t)(c0 + threadIdx.x + (eid_t)t * kRowBS - b), u[t]};
+                if (BetterDist()(k, best)) best = k;
+            }
+        }
+        best = group_best<kRowBS>(best, BetterDist());
+        if (threadIdx.x == 0) {
+            if (mi < 0) dist[v] = best.u;
+            else dbest[s] = best;
+        }
+    }
+}
+
+__global__ void k_rel_dist_fin(RowSegs S, const int32_t *__restrict__ match, const DistKey *__restrict__ dbest,
+                               int32_t *__restrict__ dist)
+{
+    int32_t t = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= S.nmulti) return;
+    int32_t v = S.mrow[t];
+    if (match[v] != -1) {
+        dist[v] = -1;
+        return;
+    }
+    DistKey best{-1, 0, 0, -1};
+    for (int64_t s = S.mseg[2 * t]; s < S.mseg[2 * t + 1]; s++)
+        if (BetterDist()(dbest[s], best)) best = dbest[s];
+    dist[v] = best.u;
 }
 
 // ---- cmap (resolve_conflict_12 + scan + resolve_conflict_4's copy)
@@ -1154,17 +1265,6 @@ void pair_by_hub(Ctx &c, int32_t n, const int32_t *hub, uint8_t *f, const wgt_t 
 
 inline bool need_more(int64_t matched, int32_t n) { return 4 * matched < 3 * (int64_t)n; }
 
-// Calls f(integral_constant<G>, ids, count) for every non-empty degree class;
-// MaxG caps the group size (ballot-based kernels need G <= 64).
-template <int MaxG = 256, class F>
-void for_bins(const RowBins &b, F f)
-{
-    if (b.count(0)) f(std::integral_constant<int, 8>{}, b.list(0), b.count(0));
-    if (b.count(1)) f(std::integral_constant<int, 32>{}, b.list(1), b.count(1));
-    if (b.count(2)) f(std::integral_constant<int, 64>{}, b.list(2), b.count(2));
-    if (b.count(3)) f(std::integral_constant<int, MaxG < 256 ? MaxG : 256>{}, b.list(3), b.count(3));
-}
-
 }  // namespace
 
 // hunyuangraph_gpu_match (hunyuangraph_GPU_match.h:1846-3214) restated with
@@ -1269,8 +1369,28 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
         int64_t nc = c.pinned[0];
         if (nc > 1) {
             unsigned gc = grid_for(nc, kBS);
-            k_rowsum<<<grid_for(nc * kG, kBS), kBS, 0, c.stream>>>(nc, d_cnt + 1, list, g.xadj, g.adjncy, key, val,
-                                                                   rsum);
+            fill<int64_t>(c, rsum, n, 0);
+            for (int cl = 0; cl < 3; cl++) {
+                int64_t nb = g.bins.count(cl);
+                if (!nb) continue;
+                const int32_t *ids = g.bins.list(cl);
+                if (cl == 0)
+                    k_rsum_g<8><<<grid_groups(nb, 8), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt,
+                                                                            match, rsum);
+                else if (cl == 1)
+                    k_rsum_g<32><<<grid_groups(nb, 32), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt,
+                                                                              match, rsum);
+                else
+                    k_rsum_g<64><<<grid_groups(nb, 64), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt,
+                                                                              match, rsum);
+                HGP_LAUNCH_CHECK();
+            }
+            if (g.segs.nseg) {
+                k_rsum_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
+                    g.segs, g.xadj, g.adjncy, g.adjwgt, match, (unsigned long long *)rsum);
+                HGP_LAUNCH_CHECK();
+            }
+            k_twin_keys<<<gc, kBS, 0, c.stream>>>(nc, d_cnt + 1, list, rsum, key, val);
             HGP_LAUNCH_CHECK();
             radix_sort(c, key, val, nc, 0, bits_for((uint64_t)(n > 0 ? n - 1 : 0) * (uint64_t)(g.m > 0 ? g.m : 1)));
             k_degkey<<<gc, kBS, 0, c.stream>>>(nc, d_cnt + 1, val, g.xadj, key32);
@@ -1292,12 +1412,31 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
         size_t m2 = c.arena.right_mark();
         int32_t *dist = c.arena.ralloc<int32_t>((size_t)n);
         uint8_t *f = c.arena.ralloc<uint8_t>((size_t)n);
-        for_bins(g.bins, [&](auto gc, const int32_t *ids, int64_t nb) {
-            constexpr int G = decltype(gc)::value;
-            k_rel_dist<G><<<grid_groups(nb, G), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt, match,
-                                                                       dist);
+        for (int cl = 0; cl < 3; cl++) {
+            int64_t nb = g.bins.count(cl);
+            if (!nb) continue;
+            const int32_t *ids = g.bins.list(cl);
+            if (cl == 0)
+                k_rel_dist<8><<<grid_groups(nb, 8), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt, match,
+                                                                          dist);
+            else if (cl == 1)
+                k_rel_dist<32><<<grid_groups(nb, 32), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt,
+                                                                            match, dist);
+            else
+                k_rel_dist<64><<<grid_groups(nb, 64), kRowBS, 0, c.stream>>>(ids, nb, g.xadj, g.adjncy, g.adjwgt,
+                                                                            match, dist);
             HGP_LAUNCH_CHECK();
-        });
+        }
+        if (g.segs.nseg) {
+            DistKey *dbest = c.arena.ralloc<DistKey>((size_t)g.segs.nseg);
+            k_rel_dist_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
+                g.segs, g.xadj, g.adjncy, g.adjwgt, match, dist, dbest);
+            HGP_LAUNCH_CHECK();
+            if (g.segs.nmulti) {
+                k_rel_dist_fin<<<grid_for(g.segs.nmulti, kBS), kBS, 0, c.stream>>>(g.segs, match, dbest, dist);
+                HGP_LAUNCH_CHECK();
+            }
+        }
         k_hub_flag<<<gv, kBS, 0, c.stream>>>(n, dist, f);
         HGP_LAUNCH_CHECK();
         pair_by_hub(c, n, dist, f, g.vwgt, maxvwgt, match);

@@ -884,16 +884,24 @@ __global__ void k_rs_cookie(int32_t k, const int64_t *__restrict__ bs, const int
     for (int32_t p = 0; p < k; p++) {
         es[p] = start;
         int64_t cap = maxwgt > pwgts[p] ? maxwgt - pwgts[p] : 0;
-        int64_t find = bs[start] + cap;
-        int32_t lo = start, hi = ne;
-        while (lo + 1 < hi) {
-            int32_t mid = (lo + hi) / 2;
-            if (bs[mid] >= find) hi = mid; else lo = mid;
+        int32_t end = start;
+        // Vertex weights are >= 1, so bs is strictly increasing: a slice of
+        // capacity 0 (or an exhausted list) ends where it starts, and the
+        // first prefix reaching bs[start] + cap lies within cap entries of
+        // start, which bounds the search (the binary search then ends on the
+        // same pair lo, hi as over [start, ne)).
+        if (cap > 0 && start < ne) {
+            int64_t find = bs[start] + cap;
+            int32_t lo = start, hi = (int32_t)min((int64_t)ne, (int64_t)start + cap);
+            while (lo + 1 < hi) {
+                int32_t mid = (lo + hi) / 2;
+                if (bs[mid] >= find) hi = mid; else lo = mid;
+            }
+            int64_t dh = bs[hi] - find, dl = bs[lo] - find;
+            dh = dh < 0 ? -dh : dh;
+            dl = dl < 0 ? -dl : dl;
+            end = dh < dl ? hi : lo;
         }
-        int64_t dh = bs[hi] - find, dl = bs[lo] - find;
-        dh = dh < 0 ? -dh : dh;
-        dl = dl < 0 ? -dl : dl;
-        int32_t end = dh < dl ? hi : lo;
         ee[p] = end;
         start = end;
     }
