@@ -56,6 +56,75 @@ __global__ void k_narrow(int32_t n, const int32_t *__restrict__ src, uint8_t *__
     else d16[v] = (uint16_t)src[v];
 }
 
+// where and dest of a vertex packed into one word (8 bits each for k <= 256,
+// else 16), so that the move accounting gathers one cache line per
+// neighbour instead of two.
+struct PairParts {
+    const uint16_t *p16;
+    const uint32_t *p32;
+    __device__ void operator()(int32_t u, int32_t &w, int32_t &d) const
+    {
+        if (p16) {
+            uint32_t x = p16[u];
+            w = (int32_t)(x & 0xFFu);
+            d = (int32_t)(x >> 8);
+        } else {
+            uint32_t x = p32[u];
+            w = (int32_t)(x & 0xFFFFu);
+            d = (int32_t)(x >> 16);
+        }
+    }
+    // (where, dest) of u[t] for the live slots, (-1, -1) for the others
+    template <int T>
+    __device__ void gather(const int32_t (&u)[T], int32_t (&w)[T], int32_t (&d)[T]) const
+    {
+        uint32_t x[T];
+        if (p16) {
+#pragma unroll
+            for (int t = 0; t < T; t++) x[t] = u[t] >= 0 ? (uint32_t)p16[u[t]] : 0u;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                w[t] = u[t] >= 0 ? (int32_t)(x[t] & 0xFFu) : -1;
+                d[t] = u[t] >= 0 ? (int32_t)(x[t] >> 8) : -1;
+            }
+        } else {
+#pragma unroll
+            for (int t = 0; t < T; t++) x[t] = u[t] >= 0 ? p32[u[t]] : 0u;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                w[t] = u[t] >= 0 ? (int32_t)(x[t] & 0xFFFFu) : -1;
+                d[t] = u[t] >= 0 ? (int32_t)(x[t] >> 16) : -1;
+            }
+        }
+    }
+};
+
+__global__ void k_pack_pair(int32_t n, const int32_t *__restrict__ where, const int32_t *__restrict__ dest,
+                            uint16_t *__restrict__ p16, uint32_t *__restrict__ p32)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= n) return;
+    if (p16) p16[v] = (uint16_t)((uint32_t)where[v] | ((uint32_t)dest[v] << 8));
+    else p32[v] = (uint32_t)where[v] | ((uint32_t)dest[v] << 16);
+}
+
+// The afterburner's view of a neighbour: pregain (low 32 bits), where and
+// dest (16 bits each above it), one 8-byte gather per neighbour.
+__device__ inline void ab_unpack(uint64_t x, int32_t &g, int32_t &w, int32_t &d)
+{
+    g = (int32_t)(uint32_t)x;
+    w = (int32_t)((x >> 32) & 0xFFFFu);
+    d = (int32_t)(x >> 48);
+}
+
+__global__ void k_pack_ab(int32_t n, const int32_t *__restrict__ pregain, const int32_t *__restrict__ where,
+                          const int32_t *__restrict__ dest, uint64_t *__restrict__ ab)
+{
+    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (v >= n) return;
+    ab[v] = (uint64_t)(uint32_t)pregain[v] | ((uint64_t)(uint32_t)where[v] << 32) | ((uint64_t)(uint32_t)dest[v] << 48);
+}
+
 // Integer gain_bucket (hunyuangraph_GPU_krefine.h:2477-2513); identical to the
 // oracle's restatement.
 __device__ inline int32_t gain_bucket(int64_t gain, int64_t vwgt)
@@ -619,14 +688,27 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_fin(RowSegs S, SelArgs A,
 
 // Afterburner (afterburner_heuristic, :2137-2178): recompute a candidate's
 // gain assuming every higher-priority neighbour (greater gain, ties to the
-// smaller id) has already moved.
+// smaller id) has already moved.  A neighbour's pregain, where and dest come
+// from one 8-byte record (ab).
+__device__ inline int64_t ab_change(int32_t v, int32_t ig, int32_t p, int32_t best, int32_t u, int32_t w, uint64_t x)
+{
+    int32_t ng, nw, nd;
+    ab_unpack(x, ng, nw, nd);
+    int64_t ch = 0;
+    if (ng > ig || (ng == ig && u < v)) {
+        if (nd == p) ch -= w; else if (nd == best) ch += w;
+        if (nw == p) ch += w; else if (nw == best) ch -= w;
+    }
+    return ch;
+}
+
 template <int G>
 __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restrict__ ids, int64_t cnt,
                                                         const eid_t *__restrict__ xadj,
                                                         const vid_t *__restrict__ adjncy,
                                                         const wgt_t *__restrict__ adjwgt,
                                                         const int32_t *__restrict__ where,
-                                                        const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq,
+                                                        const int32_t *__restrict__ dest, const uint64_t *__restrict__ ab,
                                                         const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                         uint8_t *__restrict__ sel, unsigned long long *kb)
 {
@@ -643,18 +725,13 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
             int32_t p = where[v], best = dest[v];
             int64_t change = 0;
             for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) {
-                // the three gathers go out together
                 int32_t u = adjncy[j], w = adjwgt[j];
-                int32_t ng = pregain[u], nd = dq(u), nw = wq(u);
-                if (ng > ig || (ng == ig && u < v)) {
-                    if (nd == p) change -= w; else if (nd == best) change += w;
-                    if (nw == p) change += w; else if (nw == best) change -= w;
-                }
+                change += ab_change(v, ig, p, best, u, w, ab[u]);
             }
             change = group_sum<G>(change);
             if (lane == 0) {
                 sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
-                kbytes = 24 + 20 * (uint64_t)(xadj[v + 1] - xadj[v]);
+                kbytes = 24 + 16 * (uint64_t)(xadj[v + 1] - xadj[v]);
             }
         }
     }
@@ -669,7 +746,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
                                                           const vid_t *__restrict__ adjncy,
                                                           const wgt_t *__restrict__ adjwgt,
                                                           const int32_t *__restrict__ where,
-                                                          const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq,
+                                                          const int32_t *__restrict__ dest, const uint64_t *__restrict__ ab,
                                                           const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                           uint8_t *__restrict__ sel, unsigned long long *kb)
 {
@@ -700,23 +777,19 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
             int32_t v = __shfl(mv, l, kWave), ig = __shfl(mg, l, kWave);
             int32_t p = __shfl(mp, l, kWave), best = __shfl(md, l, kWave);
             eid_t b = __shfl(mb, l, kWave), e = __shfl(me, l, kWave);
-            int32_t u[S], w[S], ng[S], nd[S], nw[S];
+            int32_t u[S], w[S];
+            uint64_t x[S];
             load_entries<S, kWave>(adjncy, adjwgt, b, e, lane, u, w);
-            gather_slots<S>(pregain, u, ng, kNoGain);
-            dq.gather(u, nd);
-            wq.gather(u, nw);
+#pragma unroll
+            for (int t = 0; t < S; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
             int64_t change = 0;
 #pragma unroll
-            for (int t = 0; t < S; t++) {
-                if (u[t] >= 0 && (ng[t] > ig || (ng[t] == ig && u[t] < v))) {
-                    if (nd[t] == p) change -= w[t]; else if (nd[t] == best) change += w[t];
-                    if (nw[t] == p) change += w[t]; else if (nw[t] == best) change -= w[t];
-                }
-            }
+            for (int t = 0; t < S; t++)
+                if (u[t] >= 0) change += ab_change(v, ig, p, best, u[t], w[t], x[t]);
             change = wave_sum(change);
             if (lane == 0) {
                 sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
-                kbytes += 24 + 20 * (uint64_t)(e - b);
+                kbytes += 24 + 16 * (uint64_t)(e - b);
             }
         }
     }
@@ -730,7 +803,8 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
                                                             const vid_t *__restrict__ adjncy,
                                                             const wgt_t *__restrict__ adjwgt,
                                                             const int32_t *__restrict__ where,
-                                                            const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq,
+                                                            const int32_t *__restrict__ dest,
+                                                            const uint64_t *__restrict__ ab,
                                                             const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                             uint8_t *__restrict__ sel,
                                                             unsigned long long *__restrict__ macc,
@@ -751,33 +825,20 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
         int64_t change = 0;
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
         for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
-            int32_t u[T], w[T], ng[T], ua[T], nd[T], nw[T];
+            int32_t u[T], w[T];
+            uint64_t x[T];
             load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
-            gather_slots<T>(pregain, u, ng, kNoGain);
-            // only the neighbours ahead of v in priority need their parts.  ua is
-            // a separate array filled by selects: a short-circuit test that
-            // conditionally overwrote u[t] in place was compiled into code that
-            // kept the old ids
 #pragma unroll
-            for (int t = 0; t < T; t++) {
-                bool ahead = (u[t] >= 0) & ((ng[t] > ig) | ((ng[t] == ig) & (u[t] < v)));
-                ua[t] = ahead ? u[t] : -1;
-            }
-            dq.gather(ua, nd);
-            wq.gather(ua, nw);
+            for (int t = 0; t < T; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
 #pragma unroll
-            for (int t = 0; t < T; t++) {
-                if (ua[t] >= 0) {
-                    if (nd[t] == p) change -= w[t]; else if (nd[t] == best) change += w[t];
-                    if (nw[t] == p) change += w[t]; else if (nw[t] == best) change -= w[t];
-                }
-            }
+            for (int t = 0; t < T; t++)
+                if (u[t] >= 0) change += ab_change(v, ig, p, best, u[t], w[t], x[t]);
         }
         change = group_sum<kRowBS>(change);
         if (threadIdx.x == 0) {
             if (mi < 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
             else if (change) atomicAdd(&macc[mi], (unsigned long long)change);
-            kbytes += 16 + 20 * (uint64_t)(j1 - j0);
+            kbytes += 16 + 16 * (uint64_t)(j1 - j0);
         }
     }
     kb_flush(kb, kbytes);
@@ -956,7 +1017,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_g(const int32_t *__restri
                                                          const wgt_t *__restrict__ adjwgt,
                                                          const wgt_t *__restrict__ vwgt,
                                                          const int32_t *__restrict__ where,
-                                                         const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq, JetCtl *ctl,
+                                                         const int32_t *__restrict__ dest, PairParts pq, JetCtl *ctl,
                                                          int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
@@ -1011,8 +1072,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_g(const int32_t *__restri
                     }
                 }
             }
-            wq.gather(u, pu);
-            dq.gather(u, du);
+            pq.gather(u, pu, du);
 #pragma unroll
             for (int t = 0; t < PB; t++) {
                 if (t0 + t < np) {
@@ -1040,7 +1100,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
                                                          const wgt_t *__restrict__ adjwgt,
                                                          const wgt_t *__restrict__ vwgt,
                                                          const int32_t *__restrict__ where,
-                                                         const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq, JetCtl *ctl,
+                                                         const int32_t *__restrict__ dest, PairParts pq, JetCtl *ctl,
                                                          int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
@@ -1077,8 +1137,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
             eid_t b = __shfl(mb, l, kWave), e = __shfl(me, l, kWave);
             int32_t u[S], w[S], pu[S], du[S];
             load_entries<S, kWave>(adjncy, adjwgt, b, e, lane, u, w);
-            wq.gather(u, pu);
-            dq.gather(u, du);
+            pq.gather(u, pu, du);
 #pragma unroll
             for (int t = 0; t < S; t++)
                 if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
@@ -1100,7 +1159,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
                                                            const wgt_t *__restrict__ adjwgt,
                                                            const wgt_t *__restrict__ vwgt,
                                                            const int32_t *__restrict__ where,
-                                                           const int32_t *__restrict__ dest, NarrowParts wq, NarrowParts dq, JetCtl *ctl,
+                                                           const int32_t *__restrict__ dest, PairParts pq, JetCtl *ctl,
                                                            int64_t *__restrict__ pwgts, unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
@@ -1120,8 +1179,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
         for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
             int32_t u[T], w[T], pu[T], du[T];
             load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
-            wq.gather(u, pu);
-            dq.gather(u, du);
+            pq.gather(u, pu, du);
 #pragma unroll
             for (int t = 0; t < T; t++)
                 if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
@@ -1143,11 +1201,17 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
     if (threadIdx.x == 0 && bdelta) atomicAdd(&ctl->delta2, bdelta);
 }
 
-__global__ void k_move_commit(int32_t n, const JetCtl *ctl, const int32_t *__restrict__ dest, int32_t *__restrict__ where)
+// Applies dest and refreshes the narrow copy of where.
+__global__ void k_move_commit(int32_t n, const JetCtl *ctl, const int32_t *__restrict__ dest, int32_t *__restrict__ where,
+                              uint8_t *__restrict__ w8, uint16_t *__restrict__ w16)
 {
     if (ctl->done) return;
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (v < n) where[v] = dest[v];
+    if (v >= n) return;
+    int32_t d = dest[v];
+    where[v] = d;
+    if (w8) w8[v] = (uint8_t)d;
+    else w16[v] = (uint16_t)d;
 }
 
 // Best-partition bookkeeping (:4805-4839) on the heaviest part weight.
@@ -1403,17 +1467,17 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     int64_t *wsc = c.arena.ralloc<int64_t>((size_t)n);
     int32_t *elist = c.arena.ralloc<int32_t>((size_t)n);
     int64_t *bs = c.arena.ralloc<int64_t>((size_t)n + 1);
-    // narrow copies of where and dest for the neighbour gathers
+    // compact copies for the neighbour gathers: where alone (selection),
+    // where and dest in one word (move accounting), pregain, where and dest
+    // in 8 bytes (afterburner)
     const bool byte_parts = k <= 256;
     uint8_t *w8 = byte_parts ? c.arena.ralloc<uint8_t>((size_t)n) : nullptr;
-    uint8_t *d8 = byte_parts ? c.arena.ralloc<uint8_t>((size_t)n) : nullptr;
     uint16_t *w16 = byte_parts ? nullptr : c.arena.ralloc<uint16_t>((size_t)n);
-    uint16_t *d16 = byte_parts ? nullptr : c.arena.ralloc<uint16_t>((size_t)n);
-    const NarrowParts wq{w8, w16}, dq{d8, d16};
-    auto narrow = [&](const int32_t *src, uint8_t *o8, uint16_t *o16) {
-        k_narrow<<<grid_for(n, kBS), kBS, 0, c.stream>>>(n, src, o8, o16);
-        HGP_LAUNCH_CHECK();
-    };
+    uint16_t *pd16 = byte_parts ? c.arena.ralloc<uint16_t>((size_t)n) : nullptr;
+    uint32_t *pd32 = byte_parts ? nullptr : c.arena.ralloc<uint32_t>((size_t)n);
+    uint64_t *ab = c.arena.ralloc<uint64_t>((size_t)n);
+    const NarrowParts wq{w8, w16};
+    const PairParts pq{pd16, pd32};
     int32_t *conn = c.arena.ralloc<int32_t>((size_t)g.segs.nmulti * (size_t)k);
     int64_t *macc = c.arena.ralloc<int64_t>((size_t)g.segs.nmulti);
     if (g.segs.nmulti) {
@@ -1425,7 +1489,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     HGP_CHECK_HIP(hipMemcpyAsync(best, where, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
     part_weights(c, n, g.vwgt, where, k, pwgts);
     k_jet_init<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, d_cut, maxwgt, opt, p.patience, finest ? 1 : 3);
-    narrow(where, w8, w16);   // where changes only at the commit below
+    k_narrow<<<grid_for(n, kBS), kBS, 0, c.stream>>>(n, where, w8, w16);   // refreshed by each commit
     HGP_LAUNCH_CHECK();
     unsigned gv = grid_for(n, kBS);
     for (;;) {
@@ -1436,26 +1500,27 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         if (h.done) break;
         launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq);
         if (h.mode == kModeLP) {
-            narrow(dest, d8, d16);   // destinations chosen by the selection
+            k_pack_ab<<<gv, kBS, 0, c.stream>>>(n, pregain, where, dest, ab);   // the selection's pregain and dest
+            HGP_LAUNCH_CHECK();
             {
-                // fixed: ids, pregain, sel (9 B per row); candidates add xadj, where, dest and 20 B per entry
+                // fixed: ids, pregain, sel (9 B per row); candidates add xadj, where, dest and 16 B per entry
                 if (int64_t nb = g.bins.count(0)) {
                     KScope ks(c, "k_afterburner_s", 9.0 * (double)nb, true);
                     k_afterburner<8><<<grid_groups(nb, 8), kRowBS, 0, c.stream>>>(
-                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, wq, dq, pregain, ctl, sel, ks.dev());
+                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.count(1)) {
                     KScope ks(c, "k_afterburner_m", 9.0 * (double)nb, true);
                     k_afterburner<32><<<grid_groups(nb, 32), kRowBS, 0, c.stream>>>(
-                        g.bins.list(1), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, wq, dq, pregain, ctl, sel, ks.dev());
+                        g.bins.list(1), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.count(2)) {
                     KScope ks(c, "k_afterburner_w", 9.0 * (double)nb, true);
                     unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 4096);
                     k_afterburner_w<<<blocks, kRowBS, 0, c.stream>>>(g.bins.list(2), nb, g.xadj, g.adjncy,
-                                                                     g.adjwgt, where, dest, wq, dq, pregain, ctl,
+                                                                     g.adjwgt, where, dest, ab, pregain, ctl,
                                                                      sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
@@ -1463,7 +1528,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                     {
                         KScope ks(c, "k_afterburner_seg", 24.0 * (double)g.segs.nseg, true);
                         k_afterburner_seg<<<seg_grid(g.segs), kRowBS, 0, c.stream>>>(
-                            g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, wq, dq, pregain, ctl, sel,
+                            g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel,
                             (unsigned long long *)macc, ks.dev());
                         HGP_LAUNCH_CHECK();
                     }
@@ -1476,7 +1541,6 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             }
             k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock);
             HGP_LAUNCH_CHECK();
-            narrow(dest, d8, d16);   // unselected candidates stay
         } else {
             k_iota_u32<<<gv, kBS, 0, c.stream>>>(n, val);
             HGP_LAUNCH_CHECK();
@@ -1499,39 +1563,39 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                 HGP_LAUNCH_CHECK();
             }
         }
-        if (h.mode != kModeLP) narrow(dest, d8, d16);   // the rebalancers' evictions
+        k_pack_pair<<<gv, kBS, 0, c.stream>>>(n, where, dest, pd16, pd32);
+        HGP_LAUNCH_CHECK();
         // fixed: ids, where, dest (12 B per row); moved rows add xadj, vwgt and 16 B per entry
         if (int64_t nb = g.bins.count(0)) {
             KScope ks(c, "k_move_delta_s", 12.0 * (double)nb, true);
             int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_g<8><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
+                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (int64_t nb = g.bins.count(1)) {
             KScope ks(c, "k_move_delta_m", 12.0 * (double)nb, true);
             int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_g<32><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
+                g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (int64_t nb = g.bins.count(2)) {
             KScope ks(c, "k_move_delta_w", 12.0 * (double)nb, true);
             unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_w<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(2), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
+                g.bins.list(2), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (g.segs.nseg) {
             KScope ks(c, "k_move_delta_seg", 24.0 * (double)g.segs.nseg, true);
             unsigned blocks = (unsigned)std::min<int64_t>(g.segs.nseg, 2048);
             k_move_delta_seg<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, wq, dq, ctl, pwgts, ks.dev());
+                g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
-        k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where);
+        k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where, w8, w16);
         HGP_LAUNCH_CHECK();
-        narrow(where, w8, w16);
         k_jet_end<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts);
         HGP_LAUNCH_CHECK();
         k_copy_best<<<gv, kBS, 0, c.stream>>>(n, ctl, where, best);
