@@ -215,17 +215,20 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
     }
     fence_wave();
 
-    // ed / id and the cut (compute_v_ed_id_bnd, :23-62)
+    // ed / id and the cut (compute_v_ed_id_bnd, :23-62).  From here on
+    // moved[v] holds bit 0 = moved in this pass, bit 1 = v has no edges, so
+    // the FM arg-max scan reads only this trial's arrays.
     int64_t cutacc = 0;
     for (int32_t v = lane; v < n; v += kWave) {
         int32_t e = 0, in = 0;
         uint8_t me = M.where[v];
-        for (eid_t j = fx[s0 + v]; j < fx[s0 + v + 1]; j++) {
+        eid_t rb = fx[s0 + v], re = fx[s0 + v + 1];
+        for (eid_t j = rb; j < re; j++) {
             if (M.where[fadj[j] - s0] != me) e += fw[j]; else in += fw[j];
         }
         M.ed[v] = e;
         M.id[v] = in;
-        M.moved[v] = 0;
+        M.moved[v] = rb == re ? 2 : 0;
         cutacc += e;
     }
     int64_t cut = wave_sum(cutacc) / 2;
@@ -249,9 +252,10 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
             int to = from ^ 1;
             int32_t bv = -1, bg = 0;
             for (int32_t u = lane; u < n; u += kWave) {
-                if (M.where[u] != from || M.moved[u]) continue;
+                uint8_t st = M.moved[u];
+                if (M.where[u] != from || (st & 1)) continue;
                 int32_t e = M.ed[u];
-                bool bnd = e > 0 || fx[s0 + u] == fx[s0 + u + 1];
+                bool bnd = e > 0 || (st & 2);
                 if (!bnd) continue;
                 int32_t g = e - M.id[u];
                 if (bv == -1 || g > bg) { bv = u; bg = g; }
@@ -279,7 +283,7 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
             pw[from] -= wv;
             if (lane == 0) {
                 M.where[bv] = (uint8_t)to;
-                M.moved[bv] = 1;
+                M.moved[bv] |= 1;
                 swaps[nswaps] = bv;
                 int32_t tmp = M.ed[bv];
                 M.ed[bv] = M.id[bv];
@@ -294,7 +298,7 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
             }
             fence_wave();
         }
-        for (int32_t i = lane; i < nswaps; i += kWave) M.moved[swaps[i]] = 0;
+        for (int32_t i = lane; i < nswaps; i += kWave) M.moved[swaps[i]] &= 2;
         fence_wave();
         for (nswaps--; nswaps > mincutorder; nswaps--) {
             int32_t v = swaps[nswaps];
