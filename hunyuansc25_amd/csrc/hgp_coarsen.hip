@@ -58,6 +58,18 @@ struct BetterEdge {
     }
 };
 
+// Candidates of a free vertex only drop out over the rounds of a level (a
+// neighbour that is matched, dead or too heavy for v stays so while v is
+// free), so if last round's choice of v is still free it is still v's best:
+// v proposes to it again without reading its row.
+__device__ inline bool keeps_proposal(int32_t first, int32_t v, const int32_t *__restrict__ avail,
+                                      const int32_t *__restrict__ prop)
+{
+    if (first) return false;
+    int32_t u = prop[v];
+    return u >= 0 && avail[u] >= 0;
+}
+
 // Handshake proposal (replaces random_match_conflict, hunyuangraph_GPU_match.h:454-510,
 // and set_receive_send_topk_one, :800-861): every free vertex proposes to its
 // best free neighbour u with vwgt[u]+vwgt[v] < maxvwgt.  G lanes per row.
@@ -67,7 +79,8 @@ __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ 
                                                     const wgt_t *__restrict__ adjwgt,
                                                     const int32_t *__restrict__ avail, int32_t maxvwgt,
                                                     uint64_t s1, int32_t *__restrict__ prop,
-                                                    const int32_t *prev_progress, unsigned long long *kb)
+                                                    const int32_t *prev_progress, int32_t first,
+                                                    unsigned long long *kb)
 {
     if (prev_progress && *prev_progress == 0) return;
     int64_t gi = gindex<G>();
@@ -78,6 +91,8 @@ __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ 
         int32_t wv = avail[v];
         if (wv < 0) {
             if (lane == 0) prop[v] = -1;
+        } else if (keeps_proposal(first, v, avail, prop)) {
+            kbytes = 0;
         } else {
             EdgeKey best{-1, 0, 0, 0u};
             eid_t beg = xadj[v], end = xadj[v + 1];
@@ -115,7 +130,7 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
                                                         const int32_t *__restrict__ avail, int32_t maxvwgt,
                                                         uint64_t s1, int32_t *__restrict__ prop,
                                                         EdgeKey *__restrict__ sbest, const int32_t *prev_progress,
-                                                        unsigned long long *kb)
+                                                        int32_t first, unsigned long long *kb)
 {
     if (prev_progress && *prev_progress == 0) return;
     uint64_t kbytes = 0;
@@ -127,6 +142,7 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
             if (threadIdx.x == 0 && mi < 0) prop[v] = -1;
             continue;
         }
+        if (keeps_proposal(first, v, avail, prop)) continue;   // (k_propose_fin makes the same test)
         EdgeKey best{-1, 0, 0, 0u};
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
         const uint64_t tv = mix64(s1 ^ (uint64_t)(uint32_t)v);
@@ -155,7 +171,7 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
 }
 
 __global__ void k_propose_fin(RowSegs S, const int32_t *__restrict__ avail, const EdgeKey *__restrict__ sbest,
-                              int32_t *__restrict__ prop, const int32_t *prev_progress)
+                              int32_t *__restrict__ prop, const int32_t *prev_progress, int32_t first)
 {
     if (prev_progress && *prev_progress == 0) return;
     int32_t t = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
@@ -165,6 +181,7 @@ __global__ void k_propose_fin(RowSegs S, const int32_t *__restrict__ avail, cons
         prop[v] = -1;
         return;
     }
+    if (keeps_proposal(first, v, avail, prop)) return;   // skipped by the segments too
     EdgeKey best{-1, 0, 0, 0u};
     for (int64_t s = S.mseg[2 * t]; s < S.mseg[2 * t + 1]; s++)
         if (BetterEdge()(sbest[s], best)) best = sbest[s];
@@ -1299,26 +1316,27 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
                 if (cl == 0)
                     k_propose<8><<<grid_groups(cnt, 8), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
                                                                                avail, maxvwgt, s1, prop, prev,
-                                                                               ks.dev());
+                                                                               r == 0, ks.dev());
                 else if (cl == 1)
                     k_propose<32><<<grid_groups(cnt, 32), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
                                                                                  avail, maxvwgt, s1, prop, prev,
-                                                                                 ks.dev());
+                                                                                 r == 0, ks.dev());
                 else
                     k_propose<64><<<grid_groups(cnt, 64), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
                                                                                  avail, maxvwgt, s1, prop, prev,
-                                                                                 ks.dev());
+                                                                                 r == 0, ks.dev());
                 HGP_LAUNCH_CHECK();
             }
             if (g.segs.nseg) {
                 {
                     KScope ks(c, "k_propose_seg", 24.0 * (double)g.segs.nseg, true);
                     k_propose_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
-                        g.segs, g.xadj, g.adjncy, g.adjwgt, avail, maxvwgt, s1, prop, sbest, prev, ks.dev());
+                        g.segs, g.xadj, g.adjncy, g.adjwgt, avail, maxvwgt, s1, prop, sbest, prev, r == 0, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (g.segs.nmulti) {
-                    k_propose_fin<<<grid_for(g.segs.nmulti, kBS), kBS, 0, c.stream>>>(g.segs, avail, sbest, prop, prev);
+                    k_propose_fin<<<grid_for(g.segs.nmulti, kBS), kBS, 0, c.stream>>>(g.segs, avail, sbest, prop, prev,
+                                                                                     r == 0);
                     HGP_LAUNCH_CHECK();
                 }
             }
