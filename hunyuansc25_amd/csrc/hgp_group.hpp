@@ -155,9 +155,9 @@ __device__ inline void load_entries(const vid_t *__restrict__ adjncy, const wgt_
         eid_t j = b + lane + (eid_t)t * L;
         u[t] = -1;
         w[t] = 0;
-        if (j < e) {
-            u[t] = adjncy[j];
-            w[t] = adjwgt[j];
+        if (j < e) {   // streamed once: kept out of the way of the gathered arrays in L2
+            u[t] = __builtin_nontemporal_load(adjncy + j);
+            w[t] = __builtin_nontemporal_load(adjwgt + j);
         }
     }
 }
