@@ -25,6 +25,7 @@ ERRORS = {
     -3: "out of device memory",
     -4: "unsupported",
     -5: "no HIP device",
+    -6: "file error",
 }
 
 MAX_LEVELS = 64
@@ -133,6 +134,10 @@ SIGNATURES = {
     "hgp_gen_rgg3d": (C.c_int, [_I32, _F64, _U64, _I32, _I32, _I32, _I32, _P]),
     "hgp_gen_rmat": (C.c_int, [_I32, _I64, _F64, _F64, _F64, _U64, _P]),
     "hgp_gen_banded": (C.c_int, [_I32, _I32, _F64, _U64, _P]),
+    "hgp_read_graph": (C.c_int, [C.c_char_p, _P]),
+    "hgp_write_partition": (C.c_int, [C.c_char_p, _P, _I32, _I32]),
+    "hgp_read_mtx": (C.c_int, [C.c_char_p, _P]),
+    "hgp_write_graph": (C.c_int, [C.c_char_p, _P, _I32]),
 }
 
 _lib_handle = None
@@ -454,6 +459,49 @@ def gen_banded(n, bw=32, frac=0.1, seed=1):
     g = HgpGraph()
     _check(lib().hgp_gen_banded(int(n), int(bw), float(frac), int(seed), C.byref(g)))
     return _take_graph(g)
+
+
+# ------------------------------------------------------------------ files
+
+def read_graph(path):
+    """METIS .graph file -> (xadj, adjncy, vwgt, adjwgt), as hunyuangraph_readgraph
+    reads it (hunyuangraph_io.h:25-186); raises HgpError on a malformed file."""
+    g = HgpGraph()
+    _check(lib().hgp_read_graph(os.fsencode(path), C.byref(g)))
+    return _take_graph(g)
+
+
+def read_mtx(path):
+    """Matrix Market matrix -> graph of pattern(A + A^T) without the diagonal
+    (matrix_to_graph/matrix_to_graph.c:117-281)."""
+    g = HgpGraph()
+    _check(lib().hgp_read_mtx(os.fsencode(path), C.byref(g)))
+    return _take_graph(g)
+
+
+def write_graph(path, xadj, adjncy, vwgt=None, adjwgt=None):
+    """Writes a METIS .graph file; weights are written when given."""
+    xadj, adjncy = _i64(xadj), _i32(adjncy)
+    vw = None if vwgt is None else _i32(vwgt)
+    ew = None if adjwgt is None else _i32(adjwgt)
+    g = HgpGraph()
+    g.n, g.m = len(xadj) - 1, int(xadj[-1])
+    g.xadj = xadj.ctypes.data_as(C.POINTER(C.c_int64))
+    g.adjncy = adjncy.ctypes.data_as(C.POINTER(C.c_int32))
+    if ew is not None:
+        g.adjwgt = ew.ctypes.data_as(C.POINTER(C.c_int32))
+    if vw is not None:
+        g.vwgt = vw.ctypes.data_as(C.POINTER(C.c_int32))
+    weights = (10 if vw is not None else 0) + (1 if ew is not None else 0)
+    _check(lib().hgp_write_graph(os.fsencode(path), C.byref(g), weights))
+
+
+def write_partition(fname, part, nparts):
+    """Writes part one value per line to "<fname>.part.<nparts>" (hunyuangraph_io.h:189-205);
+    returns that path."""
+    p = _i32(part)
+    _check(lib().hgp_write_partition(os.fsencode(fname), _ptr(p), len(p), int(nparts)))
+    return f"{os.fsdecode(fname)}.part.{int(nparts)}"
 
 
 def rgg_radius(n: int, degree: float = 12.0) -> float:

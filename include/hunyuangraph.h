@@ -38,7 +38,8 @@ enum {
     HGP_ERR_DEVICE = -2,      /* HIP runtime error                                 */
     HGP_ERR_OOM = -3,         /* device memory or HBM budget exhausted             */
     HGP_ERR_UNSUPPORTED = -4, /* outside the supported range (e.g. k too large)    */
-    HGP_ERR_NODEVICE = -5     /* no HIP device present                              */
+    HGP_ERR_NODEVICE = -5,    /* no HIP device present                              */
+    HGP_ERR_IO = -6           /* a file could not be opened, read or written        */
 };
 
 /* Text of the last error on the calling thread ("" if none). */
@@ -223,6 +224,40 @@ int hgp_gen_rmat(int32_t scale, int64_t nedges, double a, double b, double c, ui
 /* Banded FEM-like pattern: |i-j| <= bw plus frac*n*bw random symmetric
  * off-band entries. */
 int hgp_gen_banded(int32_t n, int32_t bw, double frac, uint64_t seed, hgp_graph *g);
+
+/* ------------------------------------------------------------ files
+ * Host code; the library allocates the graph arrays (release with
+ * hgp_graph_free; on error the graph is left empty). */
+
+/* Replaces hunyuangraph_readgraph (hunyuangraph_io.h:25-186).  METIS text
+ * format: lines whose first character is '%' are skipped; header "n m [fmt
+ * [ncon]]" with m counting each undirected edge once; fmt digits (zero-padded
+ * to three) flag vertex sizes, vertex weights and edge weights; then one line
+ * per vertex with 1-based neighbours.  Vertex weights must be >= 0, edge
+ * weights > 0, neighbours in 1..n, and the rows must hold exactly 2m entries.
+ * Differences from the reference: errors return HGP_ERR_INVALID with the
+ * reference's reason in hgp_last_error() (it prints and exits), a vertex size
+ * is read and ignored (the reference takes it for a neighbour), negative fmt
+ * and ncon > 1 are refused. */
+int hgp_read_graph(const char *path, hgp_graph *g);
+
+/* Replaces hunyuangraph_writetofile (hunyuangraph_io.h:189-205): writes
+ * part[0..n) one value per line to "<fname>.part.<nparts>". */
+int hgp_write_partition(const char *fname, const int32_t *part, int32_t n, int32_t nparts);
+
+/* Replaces the converter matrix_to_graph (matrix_to_graph/matrix_to_graph.c:
+ * 117-281 with mmio_highlevel.h): a square Matrix Market coordinate matrix A
+ * (pattern, real, integer or complex; general, symmetric or hermitian)
+ * becomes the graph of pattern(A + A^T) without the diagonal, neighbours
+ * ascending, unit vertex and edge weights.  Unlike the reference, duplicate
+ * entries are merged whatever the entry order, and a non-square matrix is
+ * refused. */
+int hgp_read_mtx(const char *path, hgp_graph *g);
+
+/* Writes g as a METIS .graph file readable by hgp_read_graph and by the
+ * reference.  weights: 0 none, 1 edge weights, 10 vertex weights, 11 both
+ * (the header's fmt field). */
+int hgp_write_graph(const char *path, const hgp_graph *g, int32_t weights);
 
 #ifdef __cplusplus
 }

@@ -291,3 +291,23 @@ def test_large_rmat_properties(hgp, orc):
     assert s1["cut"] == orc.edge_cut(x, a, aw, p1)
     p2, s2 = hgp.partition(x, a, 256, vw, aw, seed=4)
     np.testing.assert_array_equal(p1, p2)
+
+
+def test_files_end_to_end(hgp, orc, tmp_path):
+    """Graph file -> partition -> partition file, as the reference's command
+    line does it (read, partition, write <graph>.part.<k>)."""
+    import os
+    gdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "io")
+    cases = [("metis_manual.graph", 2), ("weighted_011.graph", 3), ("symmetric_pattern.mtx", 2)]
+    x, a, vw, aw = hgp.gen_rgg3d(5000, hgp.rgg_radius(5000, 12), seed=21, vw=(1, 3), ew=(1, 9))
+    big = str(tmp_path / "rgg5k.graph")
+    hgp.write_graph(big, x, a, vw, aw)
+    cases.append((big, 16))
+    for name, k in cases:
+        path = os.path.join(gdir, name)
+        x, a, vw, aw = hgp.read_mtx(path) if name.endswith(".mtx") else hgp.read_graph(path)
+        p_g, st = hgp.partition(x, a, k, vw, aw, seed=1)
+        p_o, _ = orc.dga_partition(x, a, k, vw, aw, seed=1)
+        np.testing.assert_array_equal(p_g, p_o, err_msg=name)
+        out = hgp.write_partition(str(tmp_path / os.path.basename(name)), p_g, k)
+        assert np.loadtxt(out, dtype=np.int32, ndmin=1).tolist() == p_g.tolist()
