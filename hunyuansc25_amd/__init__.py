@@ -42,6 +42,7 @@ class HgpOptions(C.Structure):
         ("patience", C.c_int32),
         ("validate", C.c_int32),
         ("device", C.c_int32),
+        ("level_stats", C.c_int32),
     ]
 
 
@@ -65,6 +66,8 @@ class HgpStats(C.Structure):
         ("level_iters", C.c_int32 * MAX_LEVELS),
         ("arena_bytes", C.c_int64),
         ("arena_peak", C.c_int64),
+        ("level_adjwgt", C.c_int64 * MAX_LEVELS),
+        ("level_ms", C.c_double * MAX_LEVELS),
     ]
 
     def to_dict(self) -> dict:
@@ -88,6 +91,8 @@ class HgpStats(C.Structure):
             "level_iters": [int(self.level_iters[i]) for i in range(L + 1)],
             "arena_bytes": int(self.arena_bytes),
             "arena_peak": int(self.arena_peak),
+            "level_adjwgt": [int(self.level_adjwgt[i]) for i in range(L + 1)],
+            "level_ms": [float(self.level_ms[i]) for i in range(L + 1)],
         }
 
 
@@ -138,6 +143,7 @@ SIGNATURES = {
     "hgp_write_partition": (C.c_int, [C.c_char_p, _P, _I32, _I32]),
     "hgp_read_mtx": (C.c_int, [C.c_char_p, _P]),
     "hgp_write_graph": (C.c_int, [C.c_char_p, _P, _I32]),
+    "hgp_coarsen": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
 }
 
 _lib_handle = None
@@ -231,6 +237,22 @@ def partition(xadj, adjncy, k, vwgt=None, adjwgt=None, **opts):
     _check(lib().hgp_partition(n, _ptr(xadj), _ptr(adjncy), _ptr(vw), _ptr(aw), int(k), C.byref(o), _ptr(part),
                                C.byref(st)))
     return part, st.to_dict()
+
+
+def coarsen(xadj, adjncy, k, vwgt=None, adjwgt=None, **opts):
+    """Coarsening phase of partition() alone; returns ((xadj, adjncy, vwgt,
+    adjwgt) of the coarsest graph, stats dict)."""
+    xadj = _i64(xadj)
+    adjncy = _i32(adjncy)
+    n = len(xadj) - 1
+    vw = None if vwgt is None else _i32(vwgt)
+    aw = None if adjwgt is None else _i32(adjwgt)
+    o = options(**opts)
+    st = HgpStats()
+    g = HgpGraph()
+    _check(lib().hgp_coarsen(n, _ptr(xadj), _ptr(adjncy), _ptr(vw), _ptr(aw), int(k), C.byref(o), C.byref(g),
+                             C.byref(st)))
+    return _take_graph(g), st.to_dict()
 
 
 def partition_batch(graphs, ks, lanes=0, **opts):

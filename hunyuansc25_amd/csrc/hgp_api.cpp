@@ -56,33 +56,59 @@ size_t estimate_arena(int32_t n, int64_t m, int32_t k, bool host_copy)
     return b;
 }
 
-// The multilevel pipeline on a device-resident level-0 graph
-// (hunyuangraph_kway_partition, hunyuangraph_partitiongraph.h:13-149).
-void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_t *d_part, hgp_stats *st)
-{
-    int32_t n = g0.n;
-    int64_t *dsc = c.arena.lalloc<int64_t>(8);
-    HGP_CHECK_HIP(hipEventRecord(c.ev[0], c.stream));
-    reduce_sum(c, g0.vwgt, n, dsc);
-    read_scalars(c, dsc, sizeof(int64_t));
-    int64_t tvwgt = c.pinned[0];
-    if (k == 1) {
-        fill<int32_t>(c, d_part, n, 0);
-        HGP_CHECK_HIP(hipEventRecord(c.ev[1], c.stream));
-        HGP_CHECK_HIP(hipEventRecord(c.ev[2], c.stream));
-        HGP_CHECK_HIP(hipEventRecord(c.ev[3], c.stream));
-        HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
-        if (st) { st->tvwgt = tvwgt; st->max_pwgt = tvwgt; }
-        return;
+// Per-level probes of the coarsening loop (Params::level_stats): an event
+// after each level is built, and the total edge weight of every level, taken
+// after the loop so that the reductions stay out of the level times.
+struct LevelProbe {
+    bool on = false;
+    std::vector<hipEvent_t> ev;
+    int64_t *wsum = nullptr;   // device, one per level
+    explicit LevelProbe(bool enabled) : on(enabled) {}
+    ~LevelProbe()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     }
+    void mark(Ctx &c)
+    {
+        if (!on) return;
+        hipEvent_t e;
+        HGP_CHECK_HIP(hipEventCreate(&e));
+        ev.push_back(e);
+        HGP_CHECK_HIP(hipEventRecord(e, c.stream));
+    }
+    void weigh(Ctx &c, const std::vector<DevGraph> &lv)
+    {
+        if (!on) return;
+        wsum = c.arena.lalloc<int64_t>(lv.size());
+        for (size_t l = 0; l < lv.size(); l++) reduce_sum(c, lv[l].adjwgt, lv[l].m, wsum + l);
+    }
+    // after the stream has passed the last mark
+    void fill(Ctx &c, hgp_stats *st, size_t levels)
+    {
+        if (!on || !st) return;
+        std::vector<int64_t> w(levels);
+        HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+        HGP_CHECK_HIP(hipMemcpy(w.data(), wsum, sizeof(int64_t) * levels, hipMemcpyDeviceToHost));
+        for (size_t l = 0; l < levels && l < HGP_MAX_LEVELS; l++) {
+            st->level_adjwgt[l] = w[l];
+            float ms = 0;
+            if (l + 1 < ev.size()) HGP_CHECK_HIP(hipEventElapsedTime(&ms, ev[l], ev[l + 1]));
+            st->level_ms[l] = ms;
+        }
+    }
+};
+
+// hunyuangarph_coarsen (hunyuangraph_GPU_coarsen.h:163-253): do-while with
+// the stop rule n > thr && n < 0.85 n_finer && m > n/2 (exact integers).  lv
+// holds level 0 on entry and every level on return; cmaps[l] maps level l to
+// level l+1.
+void coarsen_levels(Ctx &c, int32_t k, const Params &p, int64_t tvwgt, std::vector<DevGraph> &lv,
+                    std::vector<int32_t *> &cmaps, LevelProbe &probe)
+{
     int32_t thr = coarsen_threshold(k);
     int32_t maxvwgt = coarsen_maxvwgt(tvwgt, thr);
-
-    std::vector<DevGraph> lv{g0};
+    probe.mark(c);
     attach_bins(c, lv[0]);
-    std::vector<int32_t *> cmaps;
-    // hunyuangarph_coarsen (hunyuangraph_GPU_coarsen.h:163-253): do-while with
-    // the stop rule n > thr && n < 0.85 n_finer && m > n/2 (exact integers).
     for (;;) {
         const DevGraph f = lv.back();   // a copy: lv.push_back below may reallocate
         int32_t *cmap = c.arena.lalloc<int32_t>((size_t)f.n);
@@ -94,12 +120,72 @@ void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_
         DevGraph cg = contract(c, f, match, cmap, cn);
         c.arena.right_release(rmark);
         attach_bins(c, cg);
+        probe.mark(c);
         cmaps.push_back(cmap);
         lv.push_back(cg);
         int32_t nf = f.n;
         bool go = cg.n > thr && 100 * (int64_t)cg.n < 85 * (int64_t)nf && cg.m > cg.n / 2;
         if (!go || (int)lv.size() >= HGP_MAX_LEVELS - 1) break;
     }
+}
+
+int64_t total_vwgt(Ctx &c, const DevGraph &g, int64_t *dsc)
+{
+    reduce_sum(c, g.vwgt, g.n, dsc);
+    read_scalars(c, dsc, sizeof(int64_t));
+    return c.pinned[0];
+}
+
+// Copies the coarsest level to the host graph h (arrays allocated by the
+// caller) and fills the coarsening statistics.
+void copy_coarsest(Ctx &c, const DevGraph &cg, hgp_graph &h, const std::vector<DevGraph> &lv, int64_t tvwgt,
+                   LevelProbe &probe, hgp_stats *stats)
+{
+    probe.weigh(c, lv);
+    HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+    HGP_CHECK_HIP(hipMemcpy(h.xadj, cg.xadj, sizeof(int64_t) * ((size_t)cg.n + 1), hipMemcpyDeviceToHost));
+    HGP_CHECK_HIP(hipMemcpy(h.adjncy, cg.adjncy, sizeof(int32_t) * (size_t)cg.m, hipMemcpyDeviceToHost));
+    HGP_CHECK_HIP(hipMemcpy(h.adjwgt, cg.adjwgt, sizeof(int32_t) * (size_t)cg.m, hipMemcpyDeviceToHost));
+    HGP_CHECK_HIP(hipMemcpy(h.vwgt, cg.vwgt, sizeof(int32_t) * (size_t)cg.n, hipMemcpyDeviceToHost));
+    if (!stats) return;
+    float t01 = 0;
+    HGP_CHECK_HIP(hipEventElapsedTime(&t01, c.ev[0], c.ev[1]));
+    stats->tvwgt = tvwgt;
+    stats->levels = (int32_t)lv.size() - 1;
+    stats->coarsest_n = cg.n;
+    stats->coarsest_m = cg.m;
+    stats->ms_coarsen = t01;
+    stats->ms_total = t01;
+    for (size_t l = 0; l < lv.size() && l < HGP_MAX_LEVELS; l++) {
+        stats->level_n[l] = lv[l].n;
+        stats->level_m[l] = lv[l].m;
+    }
+    stats->arena_bytes = (int64_t)c.arena.capacity();
+    stats->arena_peak = (int64_t)c.arena.high_water();
+    probe.fill(c, stats, lv.size());
+}
+
+// The multilevel pipeline on a device-resident level-0 graph
+// (hunyuangraph_kway_partition, hunyuangraph_partitiongraph.h:13-149).
+void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_t *d_part, hgp_stats *st)
+{
+    int32_t n = g0.n;
+    int64_t *dsc = c.arena.lalloc<int64_t>(8);
+    HGP_CHECK_HIP(hipEventRecord(c.ev[0], c.stream));
+    int64_t tvwgt = total_vwgt(c, g0, dsc);
+    if (k == 1) {
+        fill<int32_t>(c, d_part, n, 0);
+        HGP_CHECK_HIP(hipEventRecord(c.ev[1], c.stream));
+        HGP_CHECK_HIP(hipEventRecord(c.ev[2], c.stream));
+        HGP_CHECK_HIP(hipEventRecord(c.ev[3], c.stream));
+        HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+        if (st) { st->tvwgt = tvwgt; st->max_pwgt = tvwgt; }
+        return;
+    }
+    std::vector<DevGraph> lv{g0};
+    std::vector<int32_t *> cmaps;
+    LevelProbe probe(p.level_stats && st);
+    coarsen_levels(c, k, p, tvwgt, lv, cmaps, probe);
     HGP_CHECK_HIP(hipEventRecord(c.ev[1], c.stream));
     int32_t L = (int32_t)lv.size() - 1;
     const DevGraph &cg = lv[L];
@@ -162,6 +248,10 @@ void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_
         st->arena_bytes = (int64_t)c.arena.capacity();
         st->arena_peak = (int64_t)c.arena.high_water();
     }
+    if (probe.on) {
+        probe.weigh(c, lv);
+        probe.fill(c, st, lv.size());
+    }
 }
 
 Params params_from(const hgp_options *o)
@@ -175,6 +265,7 @@ Params params_from(const hgp_options *o)
     p.trials = o->trials;
     p.fm_passes = o->fm_passes;
     p.patience = o->patience;
+    p.level_stats = o->level_stats != 0;
     return p;
 }
 
@@ -328,6 +419,7 @@ void hgp_default_options(hgp_options *o)
     o->patience = 6;
     o->validate = 0;
     o->device = 0;
+    o->level_stats = 0;
 }
 
 int hgp_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
@@ -420,6 +512,52 @@ int hgp_partition_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjn
             g.adjwgt = const_cast<wgt_t *>(d_adjwgt);
             g.vwgt = const_cast<wgt_t *>(d_vwgt);
             run_pipeline(c, g, k, p, d_part, stats);
+        });
+    });
+}
+
+int hgp_coarsen(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt, const int32_t *adjwgt,
+                int32_t k, const hgp_options *opt, hgp_graph *coarsest, hgp_stats *stats)
+{
+    return guarded([&] {
+        hgp_options o;
+        if (opt) o = *opt; else hgp_default_options(&o);
+        Params p = params_from(&o);
+        check_params(p, k);
+        if (!coarsest) throw Error{-1, "coarsest is NULL"};
+        memset(coarsest, 0, sizeof(*coarsest));
+        if (stats) memset(stats, 0, sizeof(*stats));
+        validate_host(n, xadj, adjncy, vwgt, adjwgt, o.validate != 0);
+        if (n == 0) throw Error{-1, "the graph has no vertices"};
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(o.device);
+        HGP_CHECK_HIP(hipSetDevice(c.device));
+        with_arena(c, estimate_arena(n, xadj[n], k, true), [&] {
+            DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
+            int64_t *dsc = c.arena.lalloc<int64_t>(8);
+            HGP_CHECK_HIP(hipEventRecord(c.ev[0], c.stream));
+            int64_t tvwgt = total_vwgt(c, g, dsc);
+            std::vector<DevGraph> lv{g};
+            std::vector<int32_t *> cmaps;
+            LevelProbe probe(p.level_stats && stats);
+            coarsen_levels(c, k, p, tvwgt, lv, cmaps, probe);
+            HGP_CHECK_HIP(hipEventRecord(c.ev[1], c.stream));
+            const DevGraph &cg = lv.back();
+            hgp_graph h{};
+            h.n = cg.n;
+            h.m = cg.m;
+            h.xadj = (int64_t *)malloc(sizeof(int64_t) * ((size_t)cg.n + 1));
+            h.adjncy = (int32_t *)malloc(sizeof(int32_t) * (size_t)(cg.m > 0 ? cg.m : 1));
+            h.adjwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(cg.m > 0 ? cg.m : 1));
+            h.vwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)cg.n);
+            try {
+                if (!h.xadj || !h.adjncy || !h.adjwgt || !h.vwgt) throw Error{-3, "host out of memory"};
+                copy_coarsest(c, cg, h, lv, tvwgt, probe, stats);
+            } catch (...) {
+                hgp_graph_free(&h);
+                throw;
+            }
+            *coarsest = h;
         });
     });
 }

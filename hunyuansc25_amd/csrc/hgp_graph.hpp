@@ -30,6 +30,7 @@ struct Params {
     int32_t trials = 1024;
     int32_t fm_passes = 10;
     int32_t patience = 6;
+    bool level_stats = false;   // per-level edge-weight sums and build times
 };
 
 // ---------------------------------------------------------------- coarsening

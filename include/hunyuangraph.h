@@ -61,6 +61,7 @@ typedef struct hgp_options {
     int32_t patience;       /* Jet non-improving rounds (default 6)              */
     int32_t validate;       /* 1: check CSR symmetry on the host before running  */
     int32_t device;         /* HIP device ordinal (default 0)                    */
+    int32_t level_stats;    /* 1: fill level_adjwgt and level_ms of hgp_stats    */
 } hgp_options;
 
 #define HGP_MAX_LEVELS 64
@@ -82,6 +83,11 @@ typedef struct hgp_stats {
     int32_t level_iters[HGP_MAX_LEVELS];
     int64_t arena_bytes;    /* device arena the run used                         */
     int64_t arena_peak;     /* most of it in use at once                         */
+    /* with level_stats: total edge weight of each level (sum of adjwgt over
+     * the directed entries) and the device time of building level l+1 from
+     * level l (level_ms[levels] = 0) */
+    int64_t level_adjwgt[HGP_MAX_LEVELS];
+    double level_ms[HGP_MAX_LEVELS];
 } hgp_stats;
 
 void hgp_default_options(hgp_options *opt);
@@ -258,6 +264,20 @@ int hgp_read_mtx(const char *path, hgp_graph *g);
  * reference.  weights: 0 none, 1 edge weights, 10 vertex weights, 11 both
  * (the header's fmt field). */
 int hgp_write_graph(const char *path, const hgp_graph *g, int32_t weights);
+
+/* ------------------------------------------------------------ coarsening only */
+
+/* The coarsening phase of hgp_partition alone (the same hierarchy for the
+ * same graph, k and options), for the reference's study builds that stop
+ * after coarsening: -DFIGURE9_SUM / -DFIGURE9_TIME print per-level sizes,
+ * edge-weight sums and times, -DFIGURE10_CGRAPH dumps the coarsest graph
+ * (hunyuangraph_partitiongraph.h:46-80, hunyuangraph_GPU_coarsen.h:198-242).
+ * *coarsest receives the coarsest graph (library-allocated; release with
+ * hgp_graph_free) and *stats (optional) levels, level_n, level_m,
+ * ms_coarsen and, with opt->level_stats, level_adjwgt and level_ms. */
+int hgp_coarsen(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                const int32_t *adjwgt, int32_t k, const hgp_options *opt, hgp_graph *coarsest,
+                hgp_stats *stats);
 
 #ifdef __cplusplus
 }

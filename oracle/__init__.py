@@ -42,6 +42,7 @@ class OracleStats(C.Structure):
         ("level_n", C.c_int64 * MAX_LEVELS),
         ("level_m", C.c_int64 * MAX_LEVELS),
         ("level_iters", C.c_int32 * MAX_LEVELS),
+        ("level_adjwgt", C.c_int64 * MAX_LEVELS),
     ]
 
     def to_dict(self):
@@ -57,6 +58,7 @@ class OracleStats(C.Structure):
             "level_n": [int(self.level_n[i]) for i in range(L + 1)],
             "level_m": [int(self.level_m[i]) for i in range(L + 1)],
             "level_iters": [int(self.level_iters[i]) for i in range(L + 1)],
+            "level_adjwgt": [int(self.level_adjwgt[i]) for i in range(L + 1)],
         }
 
 
@@ -72,6 +74,7 @@ _SIG = {
     "oracle_contract": (_I64, [_I32, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P, _P]),
     "oracle_default_opts": (None, [_P]),
     "oracle_dga_partition": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
+    "oracle_dga_coarsen": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "oracle_coarsen_threshold": (_I32, [_I32]),
     "oracle_maxvwgt": (_I32, [_I64, _I32]),
     "oracle_dga_match": (_I64, [_I32, _P, _P, _P, _P, _I32, _U64, _I32, _I32, _P]),
@@ -211,6 +214,28 @@ def dga_partition(xadj, adjncy, k, vwgt=None, adjwgt=None, **kw):
     if rc != 0:
         raise ValueError(f"oracle_dga_partition failed: {rc}")
     return part, st.to_dict()
+
+
+def dga_coarsen(xadj, adjncy, k, vwgt=None, adjwgt=None, **kw):
+    """Coarsening phase alone: ((cxadj, cadjncy, cvwgt, cadjwgt), stats)."""
+    xadj, adjncy = _i64(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    m = len(adjncy)
+    vwgt = np.ones(n, np.int32) if vwgt is None else _i32(vwgt)
+    adjwgt = np.ones(m, np.int32) if adjwgt is None else _i32(adjwgt)
+    o = opts(**kw)
+    st = OracleStats()
+    cn, cm = C.c_int32(0), C.c_int64(0)
+    cx = np.zeros(n + 1, np.int64)
+    ca = np.zeros(max(m, 1), np.int32)
+    cw = np.zeros(max(m, 1), np.int32)
+    cv = np.zeros(n, np.int32)
+    rc = lib().oracle_dga_coarsen(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(k), C.byref(o), C.byref(cn),
+                                  C.byref(cm), _p(cx), _p(ca), _p(cw), _p(cv), C.byref(st))
+    if rc != 0:
+        raise ValueError(f"oracle_dga_coarsen failed: {rc}")
+    c, e = cn.value, cm.value
+    return (cx[:c + 1].copy(), ca[:e].copy(), cv[:c].copy(), cw[:e].copy()), st.to_dict()
 
 
 def dga_initpart(xadj, adjncy, vwgt, adjwgt, k, **kw):

@@ -102,6 +102,7 @@ typedef struct oracle_stats {
     int64_t level_n[64];
     int64_t level_m[64];
     int32_t level_iters[64];    /* Jet iterations per level (index = level)      */
+    int64_t level_adjwgt[64];   /* total edge weight of each level               */
 } oracle_stats;
 
 void oracle_default_opts(oracle_opts *o);
@@ -111,6 +112,14 @@ void oracle_default_opts(oracle_opts *o);
 int oracle_dga_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy,
                          const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
                          const oracle_opts *opts, int32_t *part, oracle_stats *st);
+
+/* The coarsening phase of oracle_dga_partition alone: the coarsest graph
+ * (cxadj n+1, cadjncy/cadjwgt xadj[n], cvwgt n entries, caller-allocated) and
+ * the level statistics.  Returns 0, or -1 for n < 1 or k < 1. */
+int oracle_dga_coarsen(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                       const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
+                       const oracle_opts *opts, int32_t *cn, int64_t *cm, int64_t *cxadj,
+                       int32_t *cadjncy, int32_t *cadjwgt, int32_t *cvwgt, oracle_stats *st);
 
 /* Stage entry points of the DGA, exposed for stage-by-stage parity tests. */
 

@@ -937,43 +937,36 @@ int32_t oracle_dga_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
  * hunyuangraph_kway_partition (:13-149): coarsen (hunyuangraph_GPU_coarsen.h:163-253),
  * initial partition (hunyuangraph_GPU_initialpartition.h:4405-4496), uncoarsen
  * (hunyuangraph_GPU_uncoarsen.h:555-587). */
-int oracle_dga_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy,
-                         const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
-                         const oracle_opts *opts, int32_t *part, oracle_stats *st)
+/* Level 0 as a private copy of the input graph. */
+static void level0_copy(olevel *l0, int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                        const int32_t *vwgt, const int32_t *adjwgt)
 {
-    oracle_opts o;
-    if (opts) o = *opts; else oracle_default_opts(&o);
-    if (st) memset(st, 0, sizeof(*st));
-    if (n < 0 || k < 1) return -1;
-    if (n == 0) return 0;
-    if (k == 1) {
-        for (int32_t v = 0; v < n; v++) part[v] = 0;
-        return 0;
-    }
-    int64_t tvwgt = 0;
-    for (int32_t v = 0; v < n; v++) tvwgt += vwgt[v];
+    int64_t m0 = xadj[n];
+    l0->n = n; l0->m = m0;
+    l0->xadj = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    l0->adjncy = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m0 + 1));
+    l0->adjwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m0 + 1));
+    l0->vwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    memcpy(l0->xadj, xadj, sizeof(int64_t) * (size_t)(n + 1));
+    memcpy(l0->adjncy, adjncy, sizeof(int32_t) * (size_t)m0);
+    memcpy(l0->adjwgt, adjwgt, sizeof(int32_t) * (size_t)m0);
+    memcpy(l0->vwgt, vwgt, sizeof(int32_t) * (size_t)n);
+}
+
+/* Coarsening loop of hunyuangarph_coarsen (hunyuangraph_GPU_coarsen.h:163-253):
+ * a do-while over levels with the stop rule n > thr && n < 0.85 n_finer &&
+ * m > n/2 (hunyuangraph_GPU_coarsen.h:244-247), in exact integers.  lv[0]
+ * holds the input; returns the index L of the coarsest level. */
+static int32_t coarsen_levels(olevel *lv, int32_t k, int64_t tvwgt, const oracle_opts *o)
+{
     int32_t thr = oracle_coarsen_threshold(k);
     int32_t maxvwgt = oracle_maxvwgt(tvwgt, thr);
-
-    olevel lv[64];
-    memset(lv, 0, sizeof(lv));
-    int64_t m0 = xadj[n];
-    lv[0].n = n; lv[0].m = m0;
-    lv[0].xadj = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
-    lv[0].adjncy = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m0 + 1));
-    lv[0].adjwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m0 + 1));
-    lv[0].vwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
-    memcpy(lv[0].xadj, xadj, sizeof(int64_t) * (size_t)(n + 1));
-    memcpy(lv[0].adjncy, adjncy, sizeof(int32_t) * (size_t)m0);
-    memcpy(lv[0].adjwgt, adjwgt, sizeof(int32_t) * (size_t)m0);
-    memcpy(lv[0].vwgt, vwgt, sizeof(int32_t) * (size_t)n);
-
     int32_t L = 0;
     for (;;) {
         olevel *f = &lv[L];
         int32_t *match = (int32_t *)malloc(sizeof(int32_t) * (size_t)f->n);
-        oracle_dga_match(f->n, f->xadj, f->adjncy, f->vwgt, f->adjwgt, maxvwgt, o.seed, L,
-                         o.match_rounds, match);
+        oracle_dga_match(f->n, f->xadj, f->adjncy, f->vwgt, f->adjwgt, maxvwgt, o->seed, L,
+                         o->match_rounds, match);
         f->cmap = (int32_t *)malloc(sizeof(int32_t) * (size_t)f->n);
         int32_t cn = oracle_cmap(f->n, match, f->cmap);
         olevel *c = &lv[L + 1];
@@ -989,17 +982,80 @@ int oracle_dga_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy,
         if (!(c->n > thr && 100 * (int64_t)c->n < 85 * (int64_t)f->n && c->m > c->n / 2) || L >= 62)
             break;
     }
+    return L;
+}
+
+/* Level sizes and total edge weights (the per-level figures of the
+ * reference's -DFIGURE9_SUM build, hunyuangraph_GPU_coarsen.h:240-242). */
+static void level_stats(const olevel *lv, int32_t L, oracle_stats *st)
+{
+    st->levels = L;
+    st->coarsest_n = lv[L].n;
+    st->coarsest_m = lv[L].m;
+    for (int32_t l = 0; l <= L && l < 64; l++) {
+        st->level_n[l] = lv[l].n;
+        st->level_m[l] = lv[l].m;
+        int64_t w = 0;
+        for (int64_t j = 0; j < lv[l].m; j++) w += lv[l].adjwgt[j];
+        st->level_adjwgt[l] = w;
+    }
+}
+
+int oracle_dga_coarsen(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                       const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
+                       const oracle_opts *opts, int32_t *cn, int64_t *cm, int64_t *cxadj,
+                       int32_t *cadjncy, int32_t *cadjwgt, int32_t *cvwgt, oracle_stats *st)
+{
+    oracle_opts o;
+    if (opts) o = *opts; else oracle_default_opts(&o);
+    if (st) memset(st, 0, sizeof(*st));
+    if (n < 1 || k < 1) return -1;
+    int64_t tvwgt = 0;
+    for (int32_t v = 0; v < n; v++) tvwgt += vwgt[v];
+    olevel lv[64];
+    memset(lv, 0, sizeof(lv));
+    level0_copy(&lv[0], n, xadj, adjncy, vwgt, adjwgt);
+    int32_t L = coarsen_levels(lv, k, tvwgt, &o);
+    if (st) level_stats(lv, L, st);
+    const olevel *c = &lv[L];
+    *cn = c->n;
+    *cm = c->m;
+    memcpy(cxadj, c->xadj, sizeof(int64_t) * (size_t)(c->n + 1));
+    memcpy(cadjncy, c->adjncy, sizeof(int32_t) * (size_t)c->m);
+    memcpy(cadjwgt, c->adjwgt, sizeof(int32_t) * (size_t)c->m);
+    memcpy(cvwgt, c->vwgt, sizeof(int32_t) * (size_t)c->n);
+    for (int32_t l = 0; l <= L; l++) level_free(&lv[l]);
+    return 0;
+}
+
+int oracle_dga_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                         const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
+                         const oracle_opts *opts, int32_t *part, oracle_stats *st)
+{
+    oracle_opts o;
+    if (opts) o = *opts; else oracle_default_opts(&o);
+    if (st) memset(st, 0, sizeof(*st));
+    if (n < 0 || k < 1) return -1;
+    if (n == 0) return 0;
+    if (k == 1) {
+        for (int32_t v = 0; v < n; v++) part[v] = 0;
+        return 0;
+    }
+    int64_t tvwgt = 0;
+    for (int32_t v = 0; v < n; v++) tvwgt += vwgt[v];
+
+    olevel lv[64];
+    memset(lv, 0, sizeof(lv));
+    level0_copy(&lv[0], n, xadj, adjncy, vwgt, adjwgt);
+    int32_t L = coarsen_levels(lv, k, tvwgt, &o);
 
     olevel *cg = &lv[L];
     cg->where = (int32_t *)malloc(sizeof(int32_t) * (size_t)(cg->n > 0 ? cg->n : 1));
     oracle_dga_initpart(cg->n, cg->xadj, cg->adjncy, cg->vwgt, cg->adjwgt, k, &o, cg->where);
     int64_t cut = oracle_edge_cut(cg->n, cg->xadj, cg->adjncy, cg->adjwgt, cg->where);
     if (st) {
-        st->levels = L;
-        st->coarsest_n = cg->n;
-        st->coarsest_m = cg->m;
+        level_stats(lv, L, st);
         st->init_cut = cut;
-        for (int32_t l = 0; l <= L && l < 64; l++) { st->level_n[l] = lv[l].n; st->level_m[l] = lv[l].m; }
     }
     for (int32_t l = L; l >= 0; l--) {
         olevel *g = &lv[l];

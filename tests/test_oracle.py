@@ -461,6 +461,59 @@ def test_dga_partition_weighted_rgg(hgp, orc):
     assert orc.balanced(vw, p, 16)
 
 
+@pytest.mark.parametrize("k", [2, 16])
+def test_dga_coarsen_matches_partition_hierarchy(hgp, orc, k):
+    """The coarsening phase alone builds the hierarchy the full pipeline
+    builds, and its coarsest graph is the one the initial partition runs on."""
+    n = 4000
+    x, a, vw, aw = hgp.gen_rgg3d(n, hgp.rgg_radius(n), seed=5, vw=(1, 3), ew=(1, 7))
+    (cx, ca, cv, cw), st = orc.dga_coarsen(x, a, k, vw, aw, seed=2)
+    p, st2 = orc.dga_partition(x, a, k, vw, aw, seed=2)
+    for key in ["levels", "coarsest_n", "coarsest_m", "level_n", "level_m", "level_adjwgt"]:
+        assert st[key] == st2[key], key
+    w = orc.dga_initpart(cx, ca, cv, cw, k, seed=2)
+    assert orc.edge_cut(cx, ca, cw, w) == st2["init_cut"]
+    # vertex weight is conserved; edge weight only leaves through contracted edges
+    assert cv.sum() == vw.sum()
+    lw = st["level_adjwgt"]
+    assert lw[0] == aw.sum() and lw[-1] == cw.sum()
+    assert all(b < a_ for a_, b in zip(lw, lw[1:]))
+    assert st["level_n"][-1] == len(cx) - 1 and st["level_m"][-1] == len(ca)
+    rows = np.repeat(np.arange(len(cx) - 1), np.diff(cx))
+    assert not (rows == ca).any()
+    fwd = sorted(zip(rows.tolist(), ca.tolist(), cw.tolist()))
+    assert fwd == sorted(zip(ca.tolist(), rows.tolist(), cw.tolist()))
+    # the loop stopped at the first level that fails the continue rule
+    thr = orc.coarsen_threshold(k)
+    ns, ms = st["level_n"], st["level_m"]
+    go = [ns[l] > thr and 100 * ns[l] < 85 * ns[l - 1] and ms[l] > ns[l] // 2 for l in range(1, len(ns))]
+    assert all(go[:-1]) and not go[-1]
+
+
+def test_dga_coarsen_level_weights_brute_force(orc):
+    """Total edge weight per level against contraction done by hand: with the
+    matching of each level from dga_match, the coarse edge weight is the fine
+    one minus twice the weight of matched pairs' own edges."""
+    from conftest import random_graph
+    x, a, vw, aw = random_graph(400, 0.02, seed=4, wmax=9, vmax=3)
+    k = 2
+    (cx, ca, cv, cw), st = orc.dga_coarsen(x, a, k, vw, aw, seed=6)
+    tv = int(vw.sum())
+    mx = orc.maxvwgt(tv, orc.coarsen_threshold(k))
+    gx, ga, gv, gw = x, a, vw, aw
+    for l in range(st["levels"]):
+        m, _ = orc.dga_match(gx, ga, gv, gw, mx, 6, l, 8)
+        rows = np.repeat(np.arange(len(gx) - 1), np.diff(gx))
+        inner = int(gw[m[rows] == ga].sum())   # both directions of each matched pair's edge
+        assert st["level_adjwgt"][l] == int(gw.sum())
+        *_, gx, ga, gw, gv = orc.contract(gx, ga, gw, gv, m)
+        assert int(gw.sum()) == st["level_adjwgt"][l] - inner
+    np.testing.assert_array_equal(gx, cx)
+    np.testing.assert_array_equal(ga, ca)
+    np.testing.assert_array_equal(gw, cw)
+    np.testing.assert_array_equal(gv, cv)
+
+
 def test_cpu_rb_path(hgp, orc):
     x, a, vw, aw = hgp.gen_grid2d(48, 48)
     p, cut = orc.cpu_rb_partition(x, a, 6, vw, aw, seed=1)
