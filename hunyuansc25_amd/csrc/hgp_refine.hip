@@ -469,7 +469,8 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_g(const int32_t *__restri
 // eligible ones, one row at a time with its entries in registers (8 per
 // lane, loaded in phases) and a k-int LDS accumulator per wave.  Waves loop
 // on their own, without block barriers.
-__global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
+__global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restrict__ ids, int64_t cnt, int32_t rpw,
+                                                         SelArgs A)
 {
     extern __shared__ int32_t acc[];   // k ints per wave, then k part flags
     constexpr int S = (int)(kBinMid / kWave);
@@ -485,9 +486,9 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
     load_targets(A, mode, maxwgt, tg);
     int32_t ncand = 0;
     uint64_t kbytes = 0;
-    for (int64_t base = ((int64_t)blockIdx.x * nw + wv) * kWave; base < cnt; base += (int64_t)gridDim.x * nw * kWave) {
+    for (int64_t base = ((int64_t)blockIdx.x * nw + wv) * rpw; base < cnt; base += (int64_t)gridDim.x * nw * rpw) {
         int64_t i = base + lane;
-        int32_t mv = i < cnt ? ids[i] : -1;
+        int32_t mv = lane < rpw && i < cnt ? ids[i] : -1;
         SelRow mr;
         if (mv >= 0) mr = sel_row(A, mode, mv);
         bool mel = mv >= 0 && sel_eligible(mode, maxwgt, opt, mr);
@@ -748,7 +749,8 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
                                                           const int32_t *__restrict__ where,
                                                           const int32_t *__restrict__ dest, const uint64_t *__restrict__ ab,
                                                           const int32_t *__restrict__ pregain, const JetCtl *ctl,
-                                                          uint8_t *__restrict__ sel, unsigned long long *kb)
+                                                          uint8_t *__restrict__ sel, int32_t rpw,
+                                                          unsigned long long *kb)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
     constexpr int S = (int)(kBinMid / kWave);
@@ -756,9 +758,9 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
     uint64_t kbytes = 0;
     const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
-    for (int64_t base = wid * kWave; base < cnt; base += nwaves * kWave) {
+    for (int64_t base = wid * rpw; base < cnt; base += nwaves * rpw) {
         int64_t i = base + lane;
-        int32_t mv = i < cnt ? ids[i] : -1;
+        int32_t mv = lane < rpw && i < cnt ? ids[i] : -1;
         int32_t mg = mv >= 0 ? pregain[mv] : kNoGain;
         bool live = mv >= 0 && mg != kNoGain;
         if (mv >= 0 && !live) sel[mv] = 0;
@@ -1101,7 +1103,8 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
                                                          const wgt_t *__restrict__ vwgt,
                                                          const int32_t *__restrict__ where,
                                                          const int32_t *__restrict__ dest, PairParts pq, JetCtl *ctl,
-                                                         int64_t *__restrict__ pwgts, unsigned long long *kb)
+                                                         int64_t *__restrict__ pwgts, int32_t rpw,
+                                                         unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
     __shared__ unsigned long long bdelta;
@@ -1115,9 +1118,9 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
     const int64_t wid = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) / kWave;
     int64_t d2 = 0;
-    for (int64_t base = wid * kWave; base < cnt; base += nwaves * kWave) {
+    for (int64_t base = wid * rpw; base < cnt; base += nwaves * rpw) {
         int64_t i = base + lane;
-        int32_t mv = i < cnt ? ids[i] : -1;
+        int32_t mv = lane < rpw && i < cnt ? ids[i] : -1;
         int32_t mpv = mv >= 0 ? where[mv] : 0, mdv = mv >= 0 ? dest[mv] : 0;
         bool moved = mv >= 0 && mpv != mdv;
         eid_t mb = 0, me = 0;
@@ -1358,6 +1361,26 @@ __global__ void k_project(int32_t n, const int32_t *__restrict__ cmap, const int
     if (v < n) wf[v] = wc[cmap[v]];
 }
 
+// Rows per wave chunk of the wave-per-row kernels.  A wave screens a chunk
+// of rows and walks the live ones one after another, so on small levels a
+// 64-row chunk is a long serial chain on few waves; the chunk shrinks until
+// about kFillWaves waves share the rows (64 rows once the level is large).
+constexpr int64_t kFillWaves = 8192;
+
+inline int32_t rows_per_wave(int64_t rows)
+{
+    int64_t r = (rows + kFillWaves - 1) / kFillWaves;
+    return (int32_t)std::max<int64_t>(1, std::min<int64_t>(r, kWave));
+}
+
+// Blocks of kRowBS threads that give each chunk of rpw rows its own wave, at most cap.
+inline unsigned wave_blocks(int64_t rows, int32_t rpw, int64_t cap)
+{
+    int64_t waves = (rows + rpw - 1) / rpw;
+    int64_t b = (waves + kRowBS / kWave - 1) / (kRowBS / kWave);
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cap));
+}
+
 void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, const int64_t *pwgts,
                    const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest, int32_t *pregain,
                    uint32_t *key, int32_t *conn, NarrowParts wq)
@@ -1379,9 +1402,9 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
     if (int64_t nw = b.count(1) + b.count(2)) {
         KScope ks(c, "k_jet_select_w", 24.0 * (double)nw, true);
         A.kb = ks.dev();
-        int64_t blocks = std::min<int64_t>((nw + kRowBS - 1) / kRowBS, 4096);
-        k_jet_select_w<<<(unsigned)blocks, kRowBS, sizeof(int32_t) * 4 * (size_t)k + tg_bytes, c.stream>>>(
-            b.list(1), nw, A);
+        int32_t rpw = rows_per_wave(nw);
+        k_jet_select_w<<<wave_blocks(nw, rpw, 4096), kRowBS, sizeof(int32_t) * 4 * (size_t)k + tg_bytes, c.stream>>>(
+            b.list(1), nw, rpw, A);
         HGP_LAUNCH_CHECK();
     }
     if (g.segs.nseg) {
@@ -1518,10 +1541,10 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                 }
                 if (int64_t nb = g.bins.count(2)) {
                     KScope ks(c, "k_afterburner_w", 9.0 * (double)nb, true);
-                    unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 4096);
-                    k_afterburner_w<<<blocks, kRowBS, 0, c.stream>>>(g.bins.list(2), nb, g.xadj, g.adjncy,
-                                                                     g.adjwgt, where, dest, ab, pregain, ctl,
-                                                                     sel, ks.dev());
+                    int32_t rpw = rows_per_wave(nb);
+                    k_afterburner_w<<<wave_blocks(nb, rpw, 4096), kRowBS, 0, c.stream>>>(
+                        g.bins.list(2), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, rpw,
+                        ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (g.segs.nseg) {
@@ -1582,9 +1605,10 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         }
         if (int64_t nb = g.bins.count(2)) {
             KScope ks(c, "k_move_delta_w", 12.0 * (double)nb, true);
-            unsigned blocks = (unsigned)std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
-            k_move_delta_w<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(2), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
+            int32_t rpw = rows_per_wave(nb);
+            k_move_delta_w<<<wave_blocks(nb, rpw, 2048), kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                g.bins.list(2), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, rpw,
+                ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (g.segs.nseg) {
