@@ -138,10 +138,57 @@ __global__ void __launch_bounds__(kRsBS) k_rs_hist(const K *__restrict__ keys, i
     hist[(int64_t)threadIdx.x * ntiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Stable scatter: items of a tile are ranked in index order; ranks inside a
-// wave come from an 8-ballot digit match, ranks across waves and rounds from
-// LDS counters.  incl = inclusive scan of hist (digit-major), so the tile's
-// first slot for digit d is incl[d*ntiles+tile] - hist[d*ntiles+tile].
+// One round (kRsBS items from `first`) of the stable scatter: items are
+// ranked in index order; ranks inside a wave come from an 8-ballot digit
+// match, ranks across waves from the LDS counters wcnt.  base_d[d] holds the
+// next output slot of digit d and is advanced past the round's items.
+template <class K>
+__device__ inline void scatter_round(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
+                                     K *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n, int shift,
+                                     int64_t first, uint32_t *base_d, uint32_t (*wcnt)[256])
+{
+    int t = threadIdx.x, w = t / kWave;
+    int64_t i = first + t;
+    bool valid = i < n;
+    K key = valid ? kin[i] : (K)0;
+    uint32_t val = valid ? vin[i] : 0u;
+    uint32_t d = (uint32_t)(key >> shift) & 255u;
+#pragma unroll
+    for (int ww = 0; ww < kRsWaves; ww++) wcnt[ww][t] = 0;
+    __syncthreads();
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+        bool bit = (d >> b) & 1u;
+        uint64_t bb = __ballot(bit);
+        m &= bit ? bb : ~bb;
+    }
+    uint64_t lower = m & lanemask_lt();
+    uint32_t rank = (uint32_t)__popcll(lower);
+    if (valid && lower == 0) wcnt[w][d] = (uint32_t)__popcll(m);
+    __syncthreads();
+    {
+        uint32_t acc = base_d[t];
+#pragma unroll
+        for (int ww = 0; ww < kRsWaves; ww++) {
+            uint32_t c = wcnt[ww][t];
+            wcnt[ww][t] = acc;
+            acc += c;
+        }
+        base_d[t] = acc;
+    }
+    __syncthreads();
+    if (valid) {
+        uint32_t pos = wcnt[w][d] + rank;
+        kout[pos] = key;
+        vout[pos] = val;
+    }
+    __syncthreads();
+}
+
+// Stable scatter of one tile.  incl = inclusive scan of hist (digit-major),
+// so the tile's first slot for digit d is incl[d*ntiles+tile] -
+// hist[d*ntiles+tile].
 template <class K>
 __global__ void __launch_bounds__(kRsBS) k_rs_scatter(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                       K *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
@@ -150,7 +197,7 @@ __global__ void __launch_bounds__(kRsBS) k_rs_scatter(const K *__restrict__ kin,
 {
     __shared__ uint32_t base_d[256];
     __shared__ uint32_t wcnt[kRsWaves][256];
-    int t = threadIdx.x, w = t / kWave;
+    int t = threadIdx.x;
     {
         int64_t h = (int64_t)t * ntiles + blockIdx.x;
         base_d[t] = incl[h] - hist[h];
@@ -158,42 +205,42 @@ __global__ void __launch_bounds__(kRsBS) k_rs_scatter(const K *__restrict__ kin,
     int64_t tbase = (int64_t)blockIdx.x * kRsTile;
     for (int r = 0; r < kRsIPT; r++) {
         if (tbase + (int64_t)r * kRsBS >= n) break;      // uniform across the block
-        int64_t i = tbase + (int64_t)r * kRsBS + t;
-        bool valid = i < n;
-        K key = valid ? kin[i] : (K)0;
-        uint32_t val = valid ? vin[i] : 0u;
-        uint32_t d = (uint32_t)(key >> shift) & 255u;
-#pragma unroll
-        for (int ww = 0; ww < kRsWaves; ww++) wcnt[ww][t] = 0;
+        scatter_round<K>(kin, vin, kout, vout, n, shift, tbase + (int64_t)r * kRsBS, base_d, wcnt);
+    }
+}
+
+// All passes of a sort of at most one tile in a single block: per pass a
+// digit histogram in LDS, its exclusive scan and the stable scatter into the
+// other buffer; the sorted items end in keys/vals.  The passes read what the
+// block wrote in the pass before (ordered by the barriers at block scope).
+template <class K>
+__global__ void __launch_bounds__(kRsBS) k_rs_small(K *keys, uint32_t *vals, K *k2, uint32_t *v2, int32_t n,
+                                                    int lo, int hi)
+{
+    __shared__ uint32_t cnt[256];
+    __shared__ uint32_t base_d[256];
+    __shared__ uint32_t wcnt[kRsWaves][256];
+    __shared__ uint32_t wsh[kRsWaves + 1];
+    int t = threadIdx.x;
+    K *src_k = keys, *dst_k = k2;
+    uint32_t *src_v = vals, *dst_v = v2;
+    for (int shift = lo; shift < hi; shift += 8) {
+        cnt[t] = 0;
         __syncthreads();
-        uint64_t m = __ballot(valid);
-#pragma unroll
-        for (int b = 0; b < 8; b++) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bb = __ballot(bit);
-            m &= bit ? bb : ~bb;
+        for (int32_t i = t; i < n; i += kRsBS) atomicAdd(&cnt[(uint32_t)(src_k[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        base_d[t] = block_exclusive_scan<uint32_t>(cnt[t], wsh, nullptr);
+        __syncthreads();
+        for (int32_t first = 0; first < n; first += kRsBS)
+            scatter_round<K>(src_k, src_v, dst_k, dst_v, n, shift, first, base_d, wcnt);
+        K *tk = src_k; src_k = dst_k; dst_k = tk;
+        uint32_t *tv = src_v; src_v = dst_v; dst_v = tv;
+    }
+    if (src_k != keys) {
+        for (int32_t i = t; i < n; i += kRsBS) {
+            keys[i] = src_k[i];
+            vals[i] = src_v[i];
         }
-        uint64_t lower = m & lanemask_lt();
-        uint32_t rank = (uint32_t)__popcll(lower);
-        if (valid && lower == 0) wcnt[w][d] = (uint32_t)__popcll(m);
-        __syncthreads();
-        {
-            uint32_t acc = base_d[t];
-#pragma unroll
-            for (int ww = 0; ww < kRsWaves; ww++) {
-                uint32_t c = wcnt[ww][t];
-                wcnt[ww][t] = acc;
-                acc += c;
-            }
-            base_d[t] = acc;
-        }
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = wcnt[w][d] + rank;
-            kout[pos] = key;
-            vout[pos] = val;
-        }
-        __syncthreads();
     }
 }
 
@@ -206,6 +253,14 @@ void radix_impl(Ctx &c, K *keys, uint32_t *vals, int64_t n, int lo, int hi)
     int ntiles = (int)((n + kRsTile - 1) / kRsTile);
     K *k2 = c.arena.ralloc<K>((size_t)n);
     uint32_t *v2 = c.arena.ralloc<uint32_t>((size_t)n);
+    if (ntiles == 1) {
+        KScope ks(c, sizeof(K) == 8 ? "k_rs_small<u64>" : "k_rs_small<u32>",
+                  (double)((hi - lo + 7) / 8) * 3.0 * (double)(sizeof(K) + 4) * (double)n);
+        k_rs_small<K><<<1, kRsBS, 0, c.stream>>>(keys, vals, k2, v2, (int32_t)n, lo, hi);
+        HGP_LAUNCH_CHECK();
+        c.arena.right_release(mark);
+        return;
+    }
     uint32_t *hist = c.arena.ralloc<uint32_t>((size_t)ntiles * 256);
     uint32_t *incl = c.arena.ralloc<uint32_t>((size_t)ntiles * 256);
     K *src_k = keys, *dst_k = k2;

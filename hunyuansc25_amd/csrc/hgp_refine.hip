@@ -99,13 +99,11 @@ struct PairParts {
     }
 };
 
-__global__ void k_pack_pair(int32_t n, const int32_t *__restrict__ where, const int32_t *__restrict__ dest,
-                            uint16_t *__restrict__ p16, uint32_t *__restrict__ p32)
+// Writes the (where, dest) word of v.
+__device__ inline void store_pair(uint16_t *p16, uint32_t *p32, int32_t v, int32_t w, int32_t d)
 {
-    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (v >= n) return;
-    if (p16) p16[v] = (uint16_t)((uint32_t)where[v] | ((uint32_t)dest[v] << 8));
-    else p32[v] = (uint32_t)where[v] | ((uint32_t)dest[v] << 16);
+    if (p16) p16[v] = (uint16_t)((uint32_t)w | ((uint32_t)d << 8));
+    else p32[v] = (uint32_t)w | ((uint32_t)d << 16);
 }
 
 // The afterburner's view of a neighbour: pregain (low 32 bits), where and
@@ -117,12 +115,9 @@ __device__ inline void ab_unpack(uint64_t x, int32_t &g, int32_t &w, int32_t &d)
     d = (int32_t)(x >> 48);
 }
 
-__global__ void k_pack_ab(int32_t n, const int32_t *__restrict__ pregain, const int32_t *__restrict__ where,
-                          const int32_t *__restrict__ dest, uint64_t *__restrict__ ab)
+__device__ inline uint64_t ab_pack(int32_t g, int32_t w, int32_t d)
 {
-    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (v >= n) return;
-    ab[v] = (uint64_t)(uint32_t)pregain[v] | ((uint64_t)(uint32_t)where[v] << 32) | ((uint64_t)(uint32_t)dest[v] << 48);
+    return (uint64_t)(uint32_t)g | ((uint64_t)(uint32_t)w << 32) | ((uint64_t)(uint32_t)d << 48);
 }
 
 // Integer gain_bucket (hunyuangraph_GPU_krefine.h:2477-2513); identical to the
@@ -151,13 +146,19 @@ __device__ inline int32_t gain_bucket(int64_t gain, int64_t vwgt)
 
 // ---------------------------------------------------------------- control
 
-// Start of an iteration: patience check, mode choice (:4661-4700) and, for
-// the weak rebalancer, the undersized list in part order (:3136-3163).
-__global__ void __launch_bounds__(kBS) k_jet_begin(JetCtl *ctl, int32_t k, const int64_t *__restrict__ pwgts,
-                                                   int32_t *__restrict__ under)
+// The end of one Jet iteration and the start of the next in one launch
+// (k_refine's loop, hunyuangraph_GPU_krefine.h:4658-4842).  End (not on the
+// first call): apply the cut change, count the iteration and keep the best
+// partition by the acceptance rule (:4805-4839) on the heaviest part weight.
+// Start: stop after `patience` iterations without a > 0.1 % improvement,
+// else pick the mode (LP when balanced, RW twice, then RS) and list the parts
+// below maxdest for RW.
+__global__ void __launch_bounds__(kBS) k_jet_next(JetCtl *ctl, int32_t k, const int64_t *__restrict__ pwgts,
+                                                  int32_t *__restrict__ under, int32_t first)
 {
     __shared__ int64_t smax[kBS];
     __shared__ int32_t sh[kBS / kWave + 1];
+    __shared__ int32_t s_count, s_bc;
     JetCtl &C = *ctl;
     if (C.done) return;
     int64_t mx = INT64_MIN;
@@ -168,9 +169,31 @@ __global__ void __launch_bounds__(kBS) k_jet_begin(JetCtl *ctl, int32_t k, const
         if ((int)threadIdx.x < s && smax[threadIdx.x + s] > smax[threadIdx.x]) smax[threadIdx.x] = smax[threadIdx.x + s];
         __syncthreads();
     }
-    bool stop = C.count >= C.patience;
-    bool balanced = smax[0] <= C.maxwgt;
-    int32_t mode = balanced ? kModeLP : (C.balance_counter < 2 ? kModeRW : kModeRS);
+    const int64_t curmax = smax[0];
+    if (threadIdx.x == 0) {
+        if (!first) {
+            C.cut += (int64_t)C.delta2 / 2;
+            C.iters++;
+            C.copy_best = 0;
+            if (C.best_max > C.maxwgt && curmax < C.best_max) {
+                C.best_max = curmax;
+                C.best_cut = C.cut;
+                C.copy_best = 1;
+                C.count = 0;
+            } else if (C.cut < C.best_cut && (curmax <= C.maxwgt || curmax <= C.best_max)) {
+                if (1000 * C.cut < 999 * C.best_cut) C.count = 0;
+                C.best_max = curmax;
+                C.best_cut = C.cut;
+                C.copy_best = 1;
+            }
+        }
+        s_count = C.count;
+        s_bc = C.balance_counter;
+    }
+    __syncthreads();
+    bool stop = s_count >= C.patience;
+    bool balanced = curmax <= C.maxwgt;
+    int32_t mode = balanced ? kModeLP : (s_bc < 2 ? kModeRW : kModeRS);
     if (!stop && mode == kModeRW) {
         int32_t base = 0;
         for (int q0 = 0; q0 < k; q0 += kBS) {
@@ -191,7 +214,6 @@ __global__ void __launch_bounds__(kBS) k_jet_begin(JetCtl *ctl, int32_t k, const
         if (balanced) C.balance_counter = 0; else C.balance_counter++;
         C.ncand = 0;
         C.delta2 = 0;
-        C.copy_best = 0;
         C.nsorted = 0;
         C.nevict = 0;
     }
@@ -236,6 +258,9 @@ struct SelArgs {
     uint32_t *key;
     unsigned long long *kb;   // kernel-timer byte counter (may be null)
     NarrowParts wq;           // where of the neighbours
+    uint64_t *ab;             // LP: the afterburner's (pregain, where, dest) records
+    uint16_t *pd16;           // RW/RS: (where, dest) words of the move accounting
+    uint32_t *pd32;
 };
 
 // The per-vertex inputs of a decision besides its connectivity: part, vertex
@@ -321,11 +346,20 @@ __device__ inline SelOut sel_compute(const SelArgs &A, const JetCtl &C, int32_t 
     return o;
 }
 
-__device__ inline void sel_store(const SelArgs &A, int32_t mode, int32_t v, const SelOut &o)
+// Stores the decision, and the packed copies the next kernels gather: the
+// afterburner record in LP (the final LP dest is settled by k_lp_finish), the
+// (where, dest) word in RW/RS (changed later only for the vertices the
+// eviction keeps or the cookie cutter moves).
+__device__ inline void sel_store(const SelArgs &A, int32_t mode, int32_t v, int32_t p, const SelOut &o)
 {
     A.dest[v] = o.d;
-    if (mode == kModeLP) A.pregain[v] = o.val;
-    else A.key[v] = (uint32_t)o.val;
+    if (mode == kModeLP) {
+        A.pregain[v] = o.val;
+        A.ab[v] = ab_pack(o.val, p, o.d);
+    } else {
+        A.key[v] = (uint32_t)o.val;
+        store_pair(A.pd16, A.pd32, v, p, o.d);
+    }
 }
 
 // Writes dest/pregain/key of v from (pc, best); returns 1 for an LP candidate.
@@ -333,7 +367,7 @@ __device__ inline int32_t sel_decide(const SelArgs &A, const JetCtl &C, int32_t 
                                      bool eligible, int32_t pc, PartKey best)
 {
     SelOut o = sel_compute(A, C, mode, v, r, eligible, pc, best);
-    sel_store(A, mode, v, o);
+    sel_store(A, mode, v, r.p, o);
     return o.cand;
 }
 
@@ -862,18 +896,22 @@ __global__ void k_afterburner_fin(RowSegs S, const int32_t *__restrict__ pregain
 // locks become exactly the selected set; without any, nothing changes.
 __global__ void k_lp_finish(int32_t n, const JetCtl *ctl, const uint8_t *__restrict__ sel,
                             const int32_t *__restrict__ where, int32_t *__restrict__ dest,
-                            uint8_t *__restrict__ lock)
+                            uint8_t *__restrict__ lock, uint16_t *__restrict__ pd16, uint32_t *__restrict__ pd32)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (v >= n) return;
-    if (ctl->ncand == 0) {
-        dest[v] = where[v];
-        return;
+    const int32_t w = where[v];
+    int32_t d = w;
+    if (ctl->ncand != 0) {
+        uint8_t s = sel[v];
+        lock[v] = s;
+        if (s) d = dest[v];
+        else dest[v] = w;
+    } else {
+        dest[v] = w;
     }
-    uint8_t s = sel[v];
-    lock[v] = s;
-    if (!s) dest[v] = where[v];
+    store_pair(pd16, pd32, v, w, d);
 }
 
 // ---------------------------------------------------------------- rebalancers
@@ -912,7 +950,8 @@ __global__ void k_rb_pstart(int32_t n, int32_t k, const uint32_t *__restrict__ k
 __global__ void k_rb_evict(int32_t n, int32_t k, const uint32_t *__restrict__ key, const uint32_t *__restrict__ val,
                            const int64_t *__restrict__ wsc, const wgt_t *__restrict__ vwgt,
                            const int32_t *__restrict__ pstart, const int64_t *__restrict__ pwgts, const JetCtl *ctl,
-                           const int32_t *__restrict__ where, int32_t *__restrict__ dest, uint8_t *__restrict__ ev)
+                           const int32_t *__restrict__ where, int32_t *__restrict__ dest, uint8_t *__restrict__ ev,
+                           uint16_t *__restrict__ pd16, uint32_t *__restrict__ pd32)
 {
     int32_t i = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
@@ -927,10 +966,10 @@ __global__ void k_rb_evict(int32_t n, int32_t k, const uint32_t *__restrict__ ke
     int64_t before = wsc[i] - vwgt[v] - (s > 0 ? wsc[s - 1] : 0);
     bool evict = before < pwgts[p] - ctl->maxwgt;
     ev[i] = evict ? 1 : 0;
-    if (ctl->mode == kModeRW) {
-        if (!evict) dest[v] = where[v];
-    } else {
-        dest[v] = where[v];
+    if (ctl->mode != kModeRW || !evict) {
+        int32_t w = where[v];
+        dest[v] = w;
+        store_pair(pd16, pd32, v, w, w);
     }
 }
 
@@ -974,7 +1013,7 @@ __global__ void k_rs_cookie(int32_t k, const int64_t *__restrict__ bs, const int
 __global__ void k_rs_assign(int32_t k, const int64_t *__restrict__ ne_ptr, const int32_t *__restrict__ elist,
                             const uint32_t *__restrict__ val, const int32_t *__restrict__ es,
                             const int32_t *__restrict__ ee, const int32_t *__restrict__ where,
-                            int32_t *__restrict__ dest)
+                            int32_t *__restrict__ dest, uint16_t *__restrict__ pd16, uint32_t *__restrict__ pd32)
 {
     int32_t i = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= *ne_ptr) return;
@@ -984,7 +1023,11 @@ __global__ void k_rs_assign(int32_t k, const int64_t *__restrict__ ne_ptr, const
         if (es[mid] <= i) lo = mid; else hi = mid - 1;
     }
     int32_t v = (int32_t)val[elist[i]];
-    if (i < ee[lo] && lo != where[v]) dest[v] = lo;
+    int32_t w = where[v];
+    if (i < ee[lo] && lo != w) {
+        dest[v] = lo;
+        store_pair(pd16, pd32, v, w, lo);
+    }
 }
 
 __global__ void k_gather_w(const int64_t *__restrict__ ne_ptr, const int32_t *__restrict__ elist,
@@ -1217,40 +1260,11 @@ __global__ void k_move_commit(int32_t n, const JetCtl *ctl, const int32_t *__res
     else w16[v] = (uint16_t)d;
 }
 
-// Best-partition bookkeeping (:4805-4839) on the heaviest part weight.
-__global__ void __launch_bounds__(kBS) k_jet_end(JetCtl *ctl, int32_t k, const int64_t *__restrict__ pwgts)
-{
-    __shared__ int64_t smax[kBS];
-    JetCtl &C = *ctl;
-    if (C.done) return;
-    int64_t mx = INT64_MIN;
-    for (int q = threadIdx.x; q < k; q += kBS) mx = pwgts[q] > mx ? pwgts[q] : mx;
-    smax[threadIdx.x] = mx;
-    __syncthreads();
-    for (int s = kBS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s && smax[threadIdx.x + s] > smax[threadIdx.x]) smax[threadIdx.x] = smax[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    int64_t curmax = smax[0];
-    C.cut += (int64_t)C.delta2 / 2;
-    C.iters++;
-    if (C.best_max > C.maxwgt && curmax < C.best_max) {
-        C.best_max = curmax;
-        C.best_cut = C.cut;
-        C.copy_best = 1;
-        C.count = 0;
-    } else if (C.cut < C.best_cut && (curmax <= C.maxwgt || curmax <= C.best_max)) {
-        if (1000 * C.cut < 999 * C.best_cut) C.count = 0;
-        C.best_max = curmax;
-        C.best_cut = C.cut;
-        C.copy_best = 1;
-    }
-}
-
+// Keeps the iteration's partition when k_jet_next found it better (also when
+// that kernel ended the level: the last iteration's result still counts).
 __global__ void k_copy_best(int32_t n, const JetCtl *ctl, const int32_t *__restrict__ where, int32_t *__restrict__ best)
 {
-    if (ctl->done || !ctl->copy_best) return;
+    if (!ctl->copy_best) return;
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (v < n) best[v] = where[v];
 }
@@ -1383,10 +1397,11 @@ inline unsigned wave_blocks(int64_t rows, int32_t rpw, int64_t cap)
 
 void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, const int64_t *pwgts,
                    const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest, int32_t *pregain,
-                   uint32_t *key, int32_t *conn, NarrowParts wq)
+                   uint32_t *key, int32_t *conn, NarrowParts wq, uint64_t *ab, uint16_t *pd16, uint32_t *pd32)
 {
     const RowBins &b = g.bins;
-    SelArgs A{k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl, dest, pregain, key, nullptr, wq};
+    SelArgs A{k,   g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl,
+              dest, pregain, key, nullptr, wq, ab, pd16, pd32};
     const size_t tg_bytes = ((size_t)k + 3) & ~(size_t)3;   // part flags of the selection kernels
     // fixed bytes: ids, where, pwgts[p], vwgt, the decision (24 per row); eligible rows add xadj and
     // 12 B per entry (adjncy, adjwgt, where gather) in the kernel
@@ -1515,16 +1530,14 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     k_narrow<<<grid_for(n, kBS), kBS, 0, c.stream>>>(n, where, w8, w16);   // refreshed by each commit
     HGP_LAUNCH_CHECK();
     unsigned gv = grid_for(n, kBS);
+    k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 1);
+    HGP_LAUNCH_CHECK();
     for (;;) {
-        k_jet_begin<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under);
-        HGP_LAUNCH_CHECK();
         read_scalars(c, ctl, sizeof(JetCtl));
         JetCtl h = *(JetCtl *)c.pinned;
         if (h.done) break;
-        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq);
+        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32);
         if (h.mode == kModeLP) {
-            k_pack_ab<<<gv, kBS, 0, c.stream>>>(n, pregain, where, dest, ab);   // the selection's pregain and dest
-            HGP_LAUNCH_CHECK();
             {
                 // fixed: ids, pregain, sel (9 B per row); candidates add xadj, where, dest and 16 B per entry
                 if (int64_t nb = g.bins.count(0)) {
@@ -1562,7 +1575,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                     }
                 }
             }
-            k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock);
+            k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock, pd16, pd32);
             HGP_LAUNCH_CHECK();
         } else {
             k_iota_u32<<<gv, kBS, 0, c.stream>>>(n, val);
@@ -1573,7 +1586,8 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             scan_inclusive(c, wsc, wsc, n);
             k_rb_pstart<<<gv, kBS, 0, c.stream>>>(n, k, key, pstart);
             HGP_LAUNCH_CHECK();
-            k_rb_evict<<<gv, kBS, 0, c.stream>>>(n, k, key, val, wsc, g.vwgt, pstart, pwgts, ctl, where, dest, sel);
+            k_rb_evict<<<gv, kBS, 0, c.stream>>>(n, k, key, val, wsc, g.vwgt, pstart, pwgts, ctl, where, dest, sel,
+                                                 pd16, pd32);
             HGP_LAUNCH_CHECK();
             if (h.mode == kModeRS) {
                 compact_indices(c, sel, n, elist, d_ne);
@@ -1582,12 +1596,10 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                 scan_inclusive(c, bs + 1, bs + 1, n);
                 k_rs_cookie<<<1, 64, 0, c.stream>>>(k, bs, d_ne, pwgts, ctl, es, ee);
                 HGP_LAUNCH_CHECK();
-                k_rs_assign<<<gv, kBS, 0, c.stream>>>(k, d_ne, elist, val, es, ee, where, dest);
+                k_rs_assign<<<gv, kBS, 0, c.stream>>>(k, d_ne, elist, val, es, ee, where, dest, pd16, pd32);
                 HGP_LAUNCH_CHECK();
             }
         }
-        k_pack_pair<<<gv, kBS, 0, c.stream>>>(n, where, dest, pd16, pd32);
-        HGP_LAUNCH_CHECK();
         // fixed: ids, where, dest (12 B per row); moved rows add xadj, vwgt and 16 B per entry
         if (int64_t nb = g.bins.count(0)) {
             KScope ks(c, "k_move_delta_s", 12.0 * (double)nb, true);
@@ -1620,7 +1632,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         }
         k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where, w8, w16);
         HGP_LAUNCH_CHECK();
-        k_jet_end<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts);
+        k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 0);
         HGP_LAUNCH_CHECK();
         k_copy_best<<<gv, kBS, 0, c.stream>>>(n, ctl, where, best);
         HGP_LAUNCH_CHECK();
