@@ -1308,7 +1308,15 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
         {
             // fixed bytes: ids, avail[v], prop per row (24 per segment); each free row adds its own in the kernel
             static const char *names[3] = {"k_propose_s", "k_propose_m", "k_propose_w"};
-            for (int cl = 0; cl < 3; cl++) {
+            if (g.bins.few() && g.bins.short_rows() > 0) {
+                int64_t cnt = g.bins.short_rows();
+                KScope ks(c, "k_propose_few", 12.0 * (double)cnt, true);
+                k_propose<64><<<grid_groups(cnt, 64), kRowBS, 0, c.stream>>>(g.bins.list(0), cnt, g.xadj, g.adjncy,
+                                                                             g.adjwgt, avail, maxvwgt, s1, prop, prev,
+                                                                             r == 0, ks.dev());
+                HGP_LAUNCH_CHECK();
+            }
+            for (int cl = 0; cl < 3 && !g.bins.few(); cl++) {
                 int64_t cnt = g.bins.count(cl);
                 if (!cnt) continue;
                 KScope ks(c, names[cl], 12.0 * (double)cnt, true);

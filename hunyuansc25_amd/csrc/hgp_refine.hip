@@ -1405,7 +1405,17 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
     const size_t tg_bytes = ((size_t)k + 3) & ~(size_t)3;   // part flags of the selection kernels
     // fixed bytes: ids, where, pwgts[p], vwgt, the decision (24 per row); eligible rows add xadj and
     // 12 B per entry (adjncy, adjwgt, where gather) in the kernel
-    if (b.count(0)) {
+    if (b.few()) {
+        if (int64_t nw = b.short_rows()) {
+            KScope ks(c, "k_jet_select_few", 24.0 * (double)nw, true);
+            A.kb = ks.dev();
+            int32_t rpw = rows_per_wave(nw);
+            k_jet_select_w<<<wave_blocks(nw, rpw, 4096), kRowBS, sizeof(int32_t) * 4 * (size_t)k + tg_bytes,
+                             c.stream>>>(b.list(0), nw, rpw, A);
+            HGP_LAUNCH_CHECK();
+        }
+    }
+    if (b.count(0) && !b.few()) {
         KScope ks(c, "k_jet_select_s", 24.0 * (double)b.count(0), true);
         A.kb = ks.dev();
         unsigned blocks = (unsigned)std::min<int64_t>((b.count(0) + kRowBS - 1) / kRowBS, 16384);
@@ -1414,7 +1424,7 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
     }
     // classes 1 and 2 are consecutive in the id list: one wave-per-row launch
     // over both (32-lane groups for class 1 measured slower here)
-    if (int64_t nw = b.count(1) + b.count(2)) {
+    if (int64_t nw = b.few() ? 0 : b.count(1) + b.count(2)) {
         KScope ks(c, "k_jet_select_w", 24.0 * (double)nw, true);
         A.kb = ks.dev();
         int32_t rpw = rows_per_wave(nw);
@@ -1540,19 +1550,27 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         if (h.mode == kModeLP) {
             {
                 // fixed: ids, pregain, sel (9 B per row); candidates add xadj, where, dest and 16 B per entry
-                if (int64_t nb = g.bins.count(0)) {
+                if (int64_t nb = g.bins.few() ? g.bins.short_rows() : 0) {
+                    KScope ks(c, "k_afterburner_few", 9.0 * (double)nb, true);
+                    int32_t rpw = rows_per_wave(nb);
+                    k_afterburner_w<<<wave_blocks(nb, rpw, 4096), kRowBS, 0, c.stream>>>(
+                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, rpw,
+                        ks.dev());
+                    HGP_LAUNCH_CHECK();
+                }
+                if (int64_t nb = g.bins.few() ? 0 : g.bins.count(0)) {
                     KScope ks(c, "k_afterburner_s", 9.0 * (double)nb, true);
                     k_afterburner<8><<<grid_groups(nb, 8), kRowBS, 0, c.stream>>>(
                         g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
-                if (int64_t nb = g.bins.count(1)) {
+                if (int64_t nb = g.bins.few() ? 0 : g.bins.count(1)) {
                     KScope ks(c, "k_afterburner_m", 9.0 * (double)nb, true);
                     k_afterburner<32><<<grid_groups(nb, 32), kRowBS, 0, c.stream>>>(
                         g.bins.list(1), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
-                if (int64_t nb = g.bins.count(2)) {
+                if (int64_t nb = g.bins.few() ? 0 : g.bins.count(2)) {
                     KScope ks(c, "k_afterburner_w", 9.0 * (double)nb, true);
                     int32_t rpw = rows_per_wave(nb);
                     k_afterburner_w<<<wave_blocks(nb, rpw, 4096), kRowBS, 0, c.stream>>>(
@@ -1601,21 +1619,29 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             }
         }
         // fixed: ids, where, dest (12 B per row); moved rows add xadj, vwgt and 16 B per entry
-        if (int64_t nb = g.bins.count(0)) {
+        if (int64_t nb = g.bins.few() ? g.bins.short_rows() : 0) {
+            KScope ks(c, "k_move_delta_few", 12.0 * (double)nb, true);
+            int32_t rpw = rows_per_wave(nb);
+            k_move_delta_w<<<wave_blocks(nb, rpw, 2048), kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, rpw,
+                ks.dev());
+            HGP_LAUNCH_CHECK();
+        }
+        if (int64_t nb = g.bins.few() ? 0 : g.bins.count(0)) {
             KScope ks(c, "k_move_delta_s", 12.0 * (double)nb, true);
             int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_g<8><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
                 g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
-        if (int64_t nb = g.bins.count(1)) {
+        if (int64_t nb = g.bins.few() ? 0 : g.bins.count(1)) {
             KScope ks(c, "k_move_delta_m", 12.0 * (double)nb, true);
             int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_g<32><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
                 g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
-        if (int64_t nb = g.bins.count(2)) {
+        if (int64_t nb = g.bins.few() ? 0 : g.bins.count(2)) {
             KScope ks(c, "k_move_delta_w", 12.0 * (double)nb, true);
             int32_t rpw = rows_per_wave(nb);
             k_move_delta_w<<<wave_blocks(nb, rpw, 2048), kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
