@@ -18,6 +18,7 @@ namespace hgp {
 namespace {
 
 constexpr int kBS = 256;
+constexpr int kMaxWaves = 1024 / kWave;   // waves of the largest block here
 constexpr int32_t kNoGain = INT32_MIN;
 
 enum Mode : int32_t { kModeLP = 0, kModeRW = 1, kModeRS = 2 };
@@ -153,23 +154,25 @@ __device__ inline int32_t gain_bucket(int64_t gain, int64_t vwgt)
 // Start: stop after `patience` iterations without a > 0.1 % improvement,
 // else pick the mode (LP when balanced, RW twice, then RS) and list the parts
 // below maxdest for RW.
-__global__ void __launch_bounds__(kBS) k_jet_next(JetCtl *ctl, int32_t k, const int64_t *__restrict__ pwgts,
-                                                  int32_t *__restrict__ under, int32_t first)
+struct NextScratch {
+    int64_t wmax[kMaxWaves];
+    int32_t sh[kMaxWaves + 1];
+    int32_t count, bc;
+};
+
+// Block-wide body (any block of whole waves, up to kMaxWaves): every thread
+// of the block calls it.
+__device__ inline void jet_next(JetCtl &C, int32_t k, const int64_t *__restrict__ pwgts, int32_t *__restrict__ under,
+                                int32_t first, NextScratch &S)
 {
-    __shared__ int64_t smax[kBS];
-    __shared__ int32_t sh[kBS / kWave + 1];
-    __shared__ int32_t s_count, s_bc;
-    JetCtl &C = *ctl;
-    if (C.done) return;
+    const int nw = blockDim.x / kWave;
     int64_t mx = INT64_MIN;
-    for (int q = threadIdx.x; q < k; q += kBS) mx = pwgts[q] > mx ? pwgts[q] : mx;
-    smax[threadIdx.x] = mx;
+    for (int q = threadIdx.x; q < k; q += blockDim.x) mx = pwgts[q] > mx ? pwgts[q] : mx;
+    mx = wave_max(mx);
+    if (lane_id() == 0) S.wmax[threadIdx.x / kWave] = mx;
     __syncthreads();
-    for (int s = kBS / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s && smax[threadIdx.x + s] > smax[threadIdx.x]) smax[threadIdx.x] = smax[threadIdx.x + s];
-        __syncthreads();
-    }
-    const int64_t curmax = smax[0];
+    int64_t curmax = S.wmax[0];
+    for (int w = 1; w < nw; w++) curmax = S.wmax[w] > curmax ? S.wmax[w] : curmax;
     if (threadIdx.x == 0) {
         if (!first) {
             C.cut += (int64_t)C.delta2 / 2;
@@ -187,20 +190,20 @@ __global__ void __launch_bounds__(kBS) k_jet_next(JetCtl *ctl, int32_t k, const 
                 C.copy_best = 1;
             }
         }
-        s_count = C.count;
-        s_bc = C.balance_counter;
+        S.count = C.count;
+        S.bc = C.balance_counter;
     }
     __syncthreads();
-    bool stop = s_count >= C.patience;
+    bool stop = S.count >= C.patience;
     bool balanced = curmax <= C.maxwgt;
-    int32_t mode = balanced ? kModeLP : (s_bc < 2 ? kModeRW : kModeRS);
+    int32_t mode = balanced ? kModeLP : (S.bc < 2 ? kModeRW : kModeRS);
     if (!stop && mode == kModeRW) {
         int32_t base = 0;
-        for (int q0 = 0; q0 < k; q0 += kBS) {
+        for (int q0 = 0; q0 < k; q0 += blockDim.x) {
             int q = q0 + threadIdx.x;
             int32_t f = (q < k && pwgts[q] < C.maxdest) ? 1 : 0;
             int32_t tot;
-            int32_t ex = block_exclusive_scan(f, sh, &tot);
+            int32_t ex = block_exclusive_scan(f, S.sh, &tot);
             if (f) under[base + ex] = q;
             base += tot;
         }
@@ -208,15 +211,27 @@ __global__ void __launch_bounds__(kBS) k_jet_next(JetCtl *ctl, int32_t k, const 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        if (stop) { C.done = 1; return; }
-        C.count++;
-        C.mode = mode;
-        if (balanced) C.balance_counter = 0; else C.balance_counter++;
-        C.ncand = 0;
-        C.delta2 = 0;
-        C.nsorted = 0;
-        C.nevict = 0;
+        if (stop) {
+            C.done = 1;
+        } else {
+            C.count++;
+            C.mode = mode;
+            if (balanced) C.balance_counter = 0; else C.balance_counter++;
+            C.ncand = 0;
+            C.delta2 = 0;
+            C.nsorted = 0;
+            C.nevict = 0;
+        }
     }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBS) k_jet_next(JetCtl *ctl, int32_t k, const int64_t *__restrict__ pwgts,
+                                                  int32_t *__restrict__ under, int32_t first)
+{
+    __shared__ NextScratch S;
+    if (ctl->done) return;
+    jet_next(*ctl, k, pwgts, under, first, S);
 }
 
 // ---------------------------------------------------------------- selection
@@ -1488,6 +1503,403 @@ void project(Ctx &c, int32_t n, const int32_t *cmap, const int32_t *where_c, int
     HGP_LAUNCH_CHECK();
 }
 
+// ---------------------------------------------------------------- tiny levels
+
+// A level of at most kTinyUse vertices (k <= kTinyK) runs its whole Jet loop
+// in one block of kTinyBS threads: the same phases as the kernels above
+// (selection, afterburner, LP filter, RW/RS eviction with its sort and cookie
+// cutter, move accounting, commit, bookkeeping) separated by block barriers
+// instead of launches, and no host round trip per iteration.  Rows are
+// walked one per wave in chunks of 512 entries; per-vertex phases stride the
+// block.  The block's 16 waves walk rows one after another, so the path
+// pays off only while a level's rows are few: measured on grids, 3-D RGGs
+// and banded graphs, 512 vertices beat both 256 and 1024.
+constexpr int kTinyBS = 1024;
+constexpr int kTinyWaves = kTinyBS / kWave;
+constexpr int32_t kTinyN = 4096;   // capacity of the block-local scans
+constexpr int32_t kTinyK = 256;
+constexpr int32_t kTinyUse = 512;   // levels of at most this many vertices take the block path
+constexpr int kTinyChunk = (int)(kBinMid / kWave);   // entries per lane and chunk
+
+struct TinyArgs {
+    SelArgs A;                   // selection inputs and outputs (dest, pregain, key, ab, pairs, under, ctl)
+    int32_t n, sort_bits;
+    int32_t *where, *under, *best, *pstart, *elist, *es, *ee;
+    uint8_t *sel, *lock, *w8;
+    uint16_t *w16;
+    uint32_t *val, *k2, *v2;
+    int64_t *pwgts, *wsc, *bs;
+};
+
+struct TinyShared {
+    NextScratch nx;
+    int32_t ncand, nsort, ne;
+    uint32_t cnt[256], base_d[256];
+    uint32_t wcnt[kTinyWaves][256];
+    uint32_t ssh[kTinyWaves + 1];
+    int64_t lsh[kTinyWaves + 1];
+    unsigned long long d2[kTinyWaves];
+    int32_t done, mode;
+};
+
+// Inclusive scan of a[0..n) in place (n <= kTinyBS * 4), one block.
+__device__ inline void tiny_scan_i64(int64_t *a, int32_t n, TinyShared &S)
+{
+    constexpr int E = kTinyN / kTinyBS;
+    int32_t b = threadIdx.x * E;
+    int64_t x[E], run = 0;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        x[e] = b + e < n ? a[b + e] : 0;
+        run += x[e];
+    }
+    int64_t off = block_exclusive_scan<int64_t>(run, S.lsh, nullptr);
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        off += x[e];
+        if (b + e < n) a[b + e] = off;
+    }
+    __syncthreads();
+}
+
+// Stable LSD radix sort of (key, val)[0..n) over bits [0, hi) in 8-bit
+// digits, one block; the result ends in key/val.
+__device__ inline void tiny_sort(uint32_t *key, uint32_t *val, uint32_t *k2, uint32_t *v2, int32_t n, int hi,
+                                 TinyShared &S)
+{
+    const int t = threadIdx.x, w = t / kWave;
+    uint32_t *sk = key, *sv = val, *dk = k2, *dv = v2;
+    for (int shift = 0; shift < hi; shift += 8) {
+        if (t < 256) S.cnt[t] = 0;
+        __syncthreads();
+        for (int32_t i = t; i < n; i += kTinyBS) atomicAdd(&S.cnt[(sk[i] >> shift) & 255u], 1u);
+        __syncthreads();
+        uint32_t ex = block_exclusive_scan<uint32_t>(t < 256 ? S.cnt[t] : 0u, S.ssh, nullptr);
+        if (t < 256) S.base_d[t] = ex;
+        __syncthreads();
+        for (int32_t first = 0; first < n; first += kTinyBS) {
+            for (int i = t; i < kTinyWaves * 256; i += kTinyBS) (&S.wcnt[0][0])[i] = 0;
+            __syncthreads();
+            int32_t i = first + t;
+            bool valid = i < n;
+            uint32_t kk = valid ? sk[i] : 0u, vv = valid ? sv[i] : 0u;
+            uint32_t d = (kk >> shift) & 255u;
+            uint64_t m = __ballot(valid);
+#pragma unroll
+            for (int bit = 0; bit < 8; bit++) {
+                bool on = (d >> bit) & 1u;
+                uint64_t bb = __ballot(on);
+                m &= on ? bb : ~bb;
+            }
+            uint64_t lower = m & lanemask_lt();
+            uint32_t rank = (uint32_t)__popcll(lower);
+            if (valid && lower == 0) S.wcnt[w][d] = (uint32_t)__popcll(m);
+            __syncthreads();
+            if (t < 256) {
+                uint32_t acc = S.base_d[t];
+                for (int ww = 0; ww < kTinyWaves; ww++) {
+                    uint32_t c = S.wcnt[ww][t];
+                    S.wcnt[ww][t] = acc;
+                    acc += c;
+                }
+                S.base_d[t] = acc;
+            }
+            __syncthreads();
+            if (valid) {
+                uint32_t pos = S.wcnt[w][d] + rank;
+                dk[pos] = kk;
+                dv[pos] = vv;
+            }
+            __syncthreads();
+        }
+        uint32_t *tk = sk; sk = dk; dk = tk;
+        uint32_t *tv = sv; sv = dv; dv = tv;
+    }
+    if (sk != key) {
+        for (int32_t i = t; i < n; i += kTinyBS) {
+            key[i] = sk[i];
+            val[i] = sv[i];
+        }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kTinyBS) k_jet_tiny(TinyArgs T)
+{
+    extern __shared__ int32_t dyn[];   // per-wave k-int conn rows | k part flags (8-byte padded) | k int64 deltas
+    __shared__ TinyShared S;
+    const SelArgs &A = T.A;
+    JetCtl &C = *A.ctl;
+    const int32_t n = T.n, k = A.k;
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid / kWave;
+    int32_t *my = dyn + (size_t)wv * k;
+    uint8_t *tg = (uint8_t *)(dyn + (size_t)kTinyWaves * k);
+    unsigned long long *pdelta = (unsigned long long *)(dyn + (size_t)kTinyWaves * k + (((size_t)k + 7) / 8) * 2);
+    const PairParts pq{A.pd16, A.pd32};
+    const uint32_t nokey = (uint32_t)k * kMaxBuckets;
+    for (int i = tid; i < kTinyWaves * k; i += kTinyBS) dyn[i] = 0;
+    __syncthreads();
+    jet_next(C, k, T.pwgts, T.under, 1, S.nx);
+    if (tid == 0) { S.done = C.done; S.mode = C.mode; }
+    __syncthreads();
+    while (!S.done) {
+        const int32_t mode = S.mode;
+        const int64_t maxwgt = C.maxwgt, opt = C.opt;
+
+        // selection (k_jet_select_*): one row per wave
+        for (int q = tid; q < k; q += kTinyBS) tg[q] = sel_target(A, mode, maxwgt, q) ? 1 : 0;
+        __syncthreads();
+        int32_t ncand = 0;
+        for (int32_t v = wv; v < n; v += kTinyWaves) {
+            const SelRow r = sel_row(A, mode, v);
+            const int32_t p = r.p;
+            if (!sel_eligible(mode, maxwgt, opt, r)) {
+                if (lane == 0) sel_decide(A, C, mode, v, r, false, 0, PartKey{0, -1});
+                continue;
+            }
+            const eid_t b = A.xadj[v], e = A.xadj[v + 1];
+            int32_t u[kTinyChunk], w[kTinyChunk], q[kTinyChunk];
+#pragma unroll
+            for (int t = 0; t < kTinyChunk; t++) q[t] = -1;
+            if (mode == kModeRS) {
+                int32_t own = 0;
+                for (eid_t c0 = b; c0 < e; c0 += kBinMid) {
+                    load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + (eid_t)kBinMid, e), lane, u, w);
+                    A.wq.gather(u, q);
+#pragma unroll
+                    for (int t = 0; t < kTinyChunk; t++) own += q[t] == p ? w[t] : 0;
+                }
+                own = wave_sum(own);
+                if (lane == 0) sel_decide(A, C, mode, v, r, true, own, PartKey{0, -1});
+                continue;
+            }
+            const bool one = e - b <= (eid_t)kBinMid;
+            for (eid_t c0 = b; c0 < e; c0 += kBinMid) {
+                load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + (eid_t)kBinMid, e), lane, u, w);
+                A.wq.gather(u, q);
+#pragma unroll
+                for (int t = 0; t < kTinyChunk; t++)
+                    if (q[t] >= 0) atomicAdd(&my[q[t]], w[t]);
+            }
+            wave_lds_sync();
+            const int32_t pc = my[p];
+            PartKey best{0, -1};
+            if (one) {
+#pragma unroll
+                for (int t = 0; t < kTinyChunk; t++) {
+                    if (q[t] >= 0 && q[t] != p && tg[q[t]]) {
+                        PartKey cand{my[q[t]], q[t]};
+                        if (BetterPart()(cand, best)) best = cand;
+                    }
+                }
+            } else {
+                for (int qq = lane; qq < k; qq += kWave) {
+                    if (qq == p || !tg[qq]) continue;
+                    PartKey cand{my[qq], qq};
+                    if (BetterPart()(cand, best)) best = cand;
+                }
+            }
+            best = group_best<kWave>(best, BetterPart());
+            wave_lds_sync();
+            if (one) {
+#pragma unroll
+                for (int t = 0; t < kTinyChunk; t++)
+                    if (q[t] >= 0) my[q[t]] = 0;
+            } else {
+                for (int qq = lane; qq < k; qq += kWave) my[qq] = 0;
+            }
+            wave_lds_sync();
+            if (lane == 0) ncand += sel_decide(A, C, mode, v, r, true, pc, best);
+        }
+        const int any = __syncthreads_or(ncand > 0);
+        if (mode == kModeLP && any && tid == 0) C.ncand = 1;
+
+        if (mode == kModeLP) {
+            // afterburner (k_afterburner*)
+            for (int32_t v = wv; v < n; v += kTinyWaves) {
+                const int32_t ig = A.pregain[v];
+                if (ig == kNoGain) {
+                    if (lane == 0) T.sel[v] = 0;
+                    continue;
+                }
+                const int32_t p = T.where[v], bd = A.dest[v];
+                const eid_t b = A.xadj[v], e = A.xadj[v + 1];
+                int64_t change = 0;
+                for (eid_t c0 = b; c0 < e; c0 += kBinMid) {
+                    int32_t u[kTinyChunk], w[kTinyChunk];
+                    uint64_t x[kTinyChunk];
+                    load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + (eid_t)kBinMid, e), lane, u, w);
+#pragma unroll
+                    for (int t = 0; t < kTinyChunk; t++) x[t] = u[t] >= 0 ? A.ab[u[t]] : 0ull;
+#pragma unroll
+                    for (int t = 0; t < kTinyChunk; t++)
+                        if (u[t] >= 0) change += ab_change(v, ig, p, bd, u[t], w[t], x[t]);
+                }
+                change = wave_sum(change);
+                if (lane == 0) T.sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
+            }
+            __syncthreads();
+            // filter and locks (k_lp_finish)
+            for (int32_t v = tid; v < n; v += kTinyBS) {
+                const int32_t w = T.where[v];
+                int32_t d = w;
+                if (any) {
+                    const uint8_t s8 = T.sel[v];
+                    T.lock[v] = s8;
+                    if (s8) d = A.dest[v];
+                    else A.dest[v] = w;
+                } else {
+                    A.dest[v] = w;
+                }
+                store_pair(A.pd16, A.pd32, v, w, d);
+            }
+        } else {
+            // RW / RS eviction (k_rb_*, k_rs_*)
+            for (int32_t v = tid; v < n; v += kTinyBS) T.val[v] = (uint32_t)v;
+            __syncthreads();
+            tiny_sort(A.key, T.val, T.k2, T.v2, n, T.sort_bits, S);
+            for (int32_t i = tid; i < n; i += kTinyBS) T.wsc[i] = A.key[i] < nokey ? A.vwgt[T.val[i]] : 0;
+            __syncthreads();
+            tiny_scan_i64(T.wsc, n, S);
+            for (int32_t i = tid; i < n; i += kTinyBS) {
+                uint32_t kk = A.key[i];
+                if (kk >= nokey) continue;
+                uint32_t p = kk / kMaxBuckets;
+                if (i == 0 || A.key[i - 1] / kMaxBuckets != p) T.pstart[p] = i;
+            }
+            __syncthreads();
+            for (int32_t i = tid; i < n; i += kTinyBS) {
+                uint32_t kk = A.key[i];
+                int32_t v = (int32_t)T.val[i];
+                if (kk >= nokey) {
+                    T.sel[i] = 0;
+                    continue;
+                }
+                int32_t p = (int32_t)(kk / kMaxBuckets);
+                int32_t s0 = T.pstart[p];
+                int64_t before = T.wsc[i] - A.vwgt[v] - (s0 > 0 ? T.wsc[s0 - 1] : 0);
+                bool evict = before < T.pwgts[p] - maxwgt;
+                T.sel[i] = evict ? 1 : 0;
+                if (mode != kModeRW || !evict) {
+                    int32_t w = T.where[v];
+                    A.dest[v] = w;
+                    store_pair(A.pd16, A.pd32, v, w, w);
+                }
+            }
+            __syncthreads();
+            if (mode == kModeRS) {
+                // the evicted list in sorted order, its weight prefix, the cookie cutter
+                constexpr int E = kTinyN / kTinyBS;
+                const int32_t b0 = tid * E;
+                int32_t f[E], cnt = 0;
+#pragma unroll
+                for (int e = 0; e < E; e++) {
+                    f[e] = (b0 + e < n && T.sel[b0 + e]) ? 1 : 0;
+                    cnt += f[e];
+                }
+                uint32_t tot;
+                uint32_t off = block_exclusive_scan<uint32_t>((uint32_t)cnt, S.ssh, &tot);
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                    if (f[e]) T.elist[off++] = b0 + e;
+                const int32_t ne = (int32_t)tot;
+                __syncthreads();
+                if (tid == 0) T.bs[0] = 0;
+                for (int32_t i = tid; i < ne; i += kTinyBS) T.bs[i + 1] = A.vwgt[T.val[T.elist[i]]];
+                __syncthreads();
+                tiny_scan_i64(T.bs + 1, ne, S);
+                if (tid == 0) {
+                    int32_t start = 0;
+                    for (int32_t p = 0; p < k; p++) {
+                        T.es[p] = start;
+                        int64_t cap = maxwgt > T.pwgts[p] ? maxwgt - T.pwgts[p] : 0;
+                        int32_t end = start;
+                        if (cap > 0 && start < ne) {
+                            int64_t find = T.bs[start] + cap;
+                            int32_t lo = start, hi = (int32_t)min((int64_t)ne, (int64_t)start + cap);
+                            while (lo + 1 < hi) {
+                                int32_t mid = (lo + hi) / 2;
+                                if (T.bs[mid] >= find) hi = mid; else lo = mid;
+                            }
+                            int64_t dh = T.bs[hi] - find, dl = T.bs[lo] - find;
+                            dh = dh < 0 ? -dh : dh;
+                            dl = dl < 0 ? -dl : dl;
+                            end = dh < dl ? hi : lo;
+                        }
+                        T.ee[p] = end;
+                        start = end;
+                    }
+                }
+                __syncthreads();
+                for (int32_t i = tid; i < ne; i += kTinyBS) {
+                    int32_t lo = 0, hi = k - 1;
+                    while (lo < hi) {
+                        int32_t mid = (lo + hi + 1) >> 1;
+                        if (T.es[mid] <= i) lo = mid; else hi = mid - 1;
+                    }
+                    int32_t v = (int32_t)T.val[T.elist[i]];
+                    int32_t w = T.where[v];
+                    if (i < T.ee[lo] && lo != w) {
+                        A.dest[v] = lo;
+                        store_pair(A.pd16, A.pd32, v, w, lo);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+
+        // move accounting (k_move_delta_*): part weights and twice the cut change
+        for (int q = tid; q < k; q += kTinyBS) pdelta[q] = 0;
+        __syncthreads();
+        int64_t d2 = 0;
+        for (int32_t v = wv; v < n; v += kTinyWaves) {
+            const int32_t pv = T.where[v], dv = A.dest[v];
+            if (pv == dv) continue;
+            if (lane == 0) {
+                const int32_t wvv = A.vwgt[v];
+                atomicAdd(&pdelta[pv], (unsigned long long)(-(int64_t)wvv));
+                atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)wvv);
+            }
+            const eid_t b = A.xadj[v], e = A.xadj[v + 1];
+            for (eid_t c0 = b; c0 < e; c0 += kBinMid) {
+                int32_t u[kTinyChunk], w[kTinyChunk], pu[kTinyChunk], du[kTinyChunk];
+                load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + (eid_t)kBinMid, e), lane, u, w);
+                pq.gather(u, pu, du);
+#pragma unroll
+                for (int t = 0; t < kTinyChunk; t++)
+                    if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
+            }
+        }
+        d2 = wave_sum(d2);
+        if (lane == 0) S.d2[wv] = (unsigned long long)d2;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long tot = 0;
+            for (int ww = 0; ww < kTinyWaves; ww++) tot += S.d2[ww];
+            C.delta2 += tot;
+        }
+        for (int q = tid; q < k; q += kTinyBS)
+            if (pdelta[q]) T.pwgts[q] += (int64_t)pdelta[q];
+        __syncthreads();
+
+        // commit (k_move_commit), bookkeeping and the next mode (k_jet_next), best copy (k_copy_best)
+        for (int32_t v = tid; v < n; v += kTinyBS) {
+            const int32_t d = A.dest[v];
+            T.where[v] = d;
+            if (T.w8) T.w8[v] = (uint8_t)d;
+            else T.w16[v] = (uint16_t)d;
+        }
+        __syncthreads();
+        jet_next(C, k, T.pwgts, T.under, 0, S.nx);
+        if (tid == 0) { S.done = C.done; S.mode = C.mode; }
+        const int32_t copy = C.copy_best;
+        __syncthreads();
+        if (copy)
+            for (int32_t v = tid; v < n; v += kTinyBS) T.best[v] = T.where[v];
+        __syncthreads();
+    }
+}
+
 // hunyuangraph_malloc_krefine (hunyuangraph_GPU_uncoarsen.h:428-508) + k_refine.
 RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bool finest, const Params &p,
                          int32_t *where, int64_t *d_cut)
@@ -1540,9 +1952,45 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     k_narrow<<<grid_for(n, kBS), kBS, 0, c.stream>>>(n, where, w8, w16);   // refreshed by each commit
     HGP_LAUNCH_CHECK();
     unsigned gv = grid_for(n, kBS);
-    k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 1);
-    HGP_LAUNCH_CHECK();
-    for (;;) {
+    if (n <= kTinyUse && k <= kTinyK) {
+        // the whole loop in one block
+        uint32_t *k2 = c.arena.ralloc<uint32_t>((size_t)n);
+        uint32_t *v2 = c.arena.ralloc<uint32_t>((size_t)n);
+        TinyArgs T;
+        T.A = SelArgs{k,    g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl,
+                      dest, pregain, key, nullptr, wq, ab, pd16, pd32};
+        T.n = n;
+        T.sort_bits = bits_for((uint64_t)k * kMaxBuckets);
+        T.where = where;
+        T.under = under;
+        T.best = best;
+        T.pstart = pstart;
+        T.elist = elist;
+        T.es = es;
+        T.ee = ee;
+        T.sel = sel;
+        T.lock = lock;
+        T.w8 = w8;
+        T.w16 = w16;
+        T.val = val;
+        T.k2 = k2;
+        T.v2 = v2;
+        T.pwgts = pwgts;
+        T.wsc = wsc;
+        T.bs = bs;
+        size_t lds = sizeof(int32_t) * (size_t)kTinyWaves * k + (((size_t)k + 7) / 8) * 8 + sizeof(int64_t) * (size_t)k;
+        {
+            KScope ks(c, "k_jet_tiny", 0.0);
+            k_jet_tiny<<<1, kTinyBS, lds, c.stream>>>(T);
+            HGP_LAUNCH_CHECK();
+        }
+        read_scalars(c, ctl, sizeof(JetCtl));
+    }
+    if (!(n <= kTinyUse && k <= kTinyK)) {
+        k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 1);
+        HGP_LAUNCH_CHECK();
+    }
+    for (; !(n <= kTinyUse && k <= kTinyK);) {
         read_scalars(c, ctl, sizeof(JetCtl));
         JetCtl h = *(JetCtl *)c.pinned;
         if (h.done) break;
