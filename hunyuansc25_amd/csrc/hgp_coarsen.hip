@@ -143,6 +143,12 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
             continue;
         }
         if (keeps_proposal(first, v, avail, prop)) continue;   // (k_propose_fin makes the same test)
+        if (mi >= 0 && !first) {
+            // The same argument per segment: a segment whose best edge still
+            // leads to a free vertex (or that had none) keeps it in sbest.
+            const EdgeKey sb = sbest[s];
+            if (sb.w < 0 || avail[sb.lo == v ? sb.hi : sb.lo] >= 0) continue;
+        }
         EdgeKey best{-1, 0, 0, 0u};
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
         const uint64_t tv = mix64(s1 ^ (uint64_t)(uint32_t)v);
