@@ -749,6 +749,89 @@ __device__ inline void wave_bitonic(uint32_t (&key)[KR], int32_t (&val)[KR])
     }
 }
 
+// One compare-exchange stage at distance j (j < 64: xor-shuffles, else
+// register swaps) of a bitonic network whose entry e = base + r*64 + lane
+// sorts ascending where (e & k) == 0.
+template <int R>
+__device__ inline void bitonic_stage(uint32_t (&key)[R], int32_t (&val)[R], int base, int k, int j)
+{
+    const int lane = lane_id();
+    if (j >= kWave) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            int r2 = r ^ (j / kWave);
+            if (r2 > r) {
+                bool asc = ((base + r * kWave) & k) == 0;   // k >= 128: the lane bits do not matter
+                bool sw = asc ? key[r] > key[r2] : key[r] < key[r2];
+                if (sw) {
+                    uint32_t tk = key[r];
+                    key[r] = key[r2];
+                    key[r2] = tk;
+                    int32_t tv = val[r];
+                    val[r] = val[r2];
+                    val[r2] = tv;
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            uint32_t ok = __shfl_xor(key[r], j, kWave);
+            int32_t ov = __shfl_xor(val[r], j, kWave);
+            bool asc = ((base + r * kWave + lane) & k) == 0;
+            bool lower = (lane & j) == 0;
+            bool take = (lower == asc) ? ok < key[r] : ok > key[r];
+            if (take) {
+                key[r] = ok;
+                val[r] = ov;
+            }
+        }
+    }
+}
+
+// Bitonic sort of 8*R*64 entries by a block of 8 waves (kSegBS = 512
+// threads): wave w holds entries [w*C, w*C + C), C = R*64, as entry
+// w*C + r*64 + lane in register r of its lanes.  Stages closer than C stay in
+// the wave (shuffles and register swaps); the log2(8) farther distances of
+// each merge go through LDS (sk/sw hold the block's entries), two barriers
+// each.  Every entry ends where a plain bitonic network over the whole range
+// would put it.
+template <int R>
+__device__ inline void block_bitonic(uint32_t (&key)[R], int32_t (&val)[R], uint32_t *sk, int32_t *sw)
+{
+    constexpr int C = R * kWave, P = (kSegBS / kWave) * C;
+    const int w = threadIdx.x / kWave, lane = lane_id(), base = w * C;
+#pragma unroll
+    for (int k = 2; k <= C; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) bitonic_stage<R>(key, val, base, k, j);
+    for (int k = 2 * C; k <= P; k <<= 1) {
+        for (int j = k >> 1; j >= C; j >>= 1) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                sk[base + r * kWave + lane] = key[r];
+                sw[base + r * kWave + lane] = val[r];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                int i = base + r * kWave + lane, pi = i ^ j;
+                uint32_t ok = sk[pi];
+                int32_t ov = sw[pi];
+                bool asc = (i & k) == 0, lower = (i & j) == 0;
+                bool take = (lower == asc) ? ok < key[r] : ok > key[r];
+                if (take) {
+                    key[r] = ok;
+                    val[r] = ov;
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = C >> 1; j > 0; j >>= 1) bitonic_stage<R>(key, val, base, k, j);
+    }
+}
+
 // Rows of 65..KR*64 entries: one wave per row, sorted in registers (padded
 // to 128, 256, 512 or 1024 entries); tails ranked by ballots, weights by wave
 // scans carried across the registers.  KR = 8 serves rows of <= 512 entries
@@ -847,40 +930,46 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
         int32_t v = rep[c], u = match[v];
         CatRow row(xadj, v, u);
         int len = (int)(row.dv + (u != v ? xadj[u + 1] - row.bu : 0));
-        int P = 64;
-        while (P < len) P <<= 1;
-        for (int i = tid; i < P; i += kSegBS) {
-            uint32_t key = kSent;
-            int32_t w = 0;
-            if (i < len) {
-                eid_t j = row.at(i);
-                int32_t cc = cmap[adjncy[j]];
-                if (cc != c) {
-                    key = (uint32_t)cc;
-                    w = adjwgt[j];
-                }
-            }
-            sk[i] = key;
-            sw[i] = w;
-        }
-        __syncthreads();
-        for (int k = 2; k <= P; k <<= 1) {
-            for (int j = k >> 1; j > 0; j >>= 1) {
-                for (int t = tid; t < P / 2; t += kSegBS) {
-                    int i = ((t & ~(j - 1)) << 1) | (t & (j - 1));
-                    int l = i | j;
-                    uint32_t a = sk[i], b = sk[l];
-                    bool asc = (i & k) == 0;
-                    if (asc ? a > b : a < b) {
-                        sk[i] = b;
-                        sk[l] = a;
-                        int32_t x = sw[i];
-                        sw[i] = sw[l];
-                        sw[l] = x;
+        const int P = len <= kSegLds / 2 ? kSegLds / 2 : kSegLds;   // rows here have more than kSegWave entries
+        {
+            // entries straight into registers (entry w*C + r*64 + lane), sorted by the block
+            const int R = P / kSegBS, w = tid / kWave, lane = lane_id();
+            uint32_t key[kSegE];
+            int32_t val[kSegE];
+#pragma unroll
+            for (int r = 0; r < kSegE; r++) {
+                int i = w * R * kWave + r * kWave + lane;
+                key[r] = kSent;
+                val[r] = 0;
+                if (r < R && i < len) {
+                    eid_t j = row.at(i);
+                    int32_t cc = cmap[adjncy[j]];
+                    if (cc != c) {
+                        key[r] = (uint32_t)cc;
+                        val[r] = adjwgt[j];
                     }
                 }
-                __syncthreads();
             }
+            if (R == kSegE) {
+                block_bitonic<kSegE>(key, val, sk, sw);
+            } else {
+                uint32_t k4[kSegE / 2];
+                int32_t v4[kSegE / 2];
+#pragma unroll
+                for (int r = 0; r < kSegE / 2; r++) { k4[r] = key[r]; v4[r] = val[r]; }
+                block_bitonic<kSegE / 2>(k4, v4, sk, sw);
+#pragma unroll
+                for (int r = 0; r < kSegE / 2; r++) { key[r] = k4[r]; val[r] = v4[r]; }
+            }
+            __syncthreads();   // the last cross-wave stage's reads are done
+#pragma unroll
+            for (int r = 0; r < kSegE; r++) {
+                if (r < R) {
+                    sk[w * R * kWave + r * kWave + lane] = key[r];
+                    sw[w * R * kWave + r * kWave + lane] = val[r];
+                }
+            }
+            __syncthreads();
         }
         // Thread tid owns entries [tid*E, tid*E + E), E = P / kSegBS (one each when P < kSegBS).
         const int E = P >= kSegBS ? P / kSegBS : 1;
