@@ -989,37 +989,63 @@ __global__ void k_rb_evict(int32_t n, int32_t k, const uint32_t *__restrict__ ke
 }
 
 // Cookie cutter (:2710-2745): slice the evicted list (in sorted order) into
-// consecutive runs, part by part, each filling maxwgt - pwgts.
-__global__ void k_rs_cookie(int32_t k, const int64_t *__restrict__ bs, const int64_t *__restrict__ ne_ptr,
-                            const int64_t *__restrict__ pwgts, const JetCtl *ctl, int32_t *__restrict__ es,
-                            int32_t *__restrict__ ee)
+// consecutive runs, part by part, each filling maxwgt - pwgts.  The parts
+// stay sequential (a slice starts where the last one ended); one wave finds
+// each slice end.  Vertex weights are >= 1, so bs is strictly increasing: a
+// slice of capacity 0 (or an exhausted list) ends where it starts, and the
+// first prefix reaching bs[start] + cap lies within cap entries of start.
+// Over (start, limit], limit = min(ne, start + cap), the serial binary search
+// ends on hi = the first index whose prefix reaches find (limit if none) and
+// lo = hi - 1; the wave finds that hi by 64-way narrowing (each round one
+// load per lane and a ballot), then keeps the nearer of lo and hi.
+__global__ void __launch_bounds__(kWave) k_rs_cookie(int32_t k, const int64_t *__restrict__ bs,
+                                                     const int64_t *__restrict__ ne_ptr,
+                                                     const int64_t *__restrict__ pwgts, const JetCtl *ctl,
+                                                     int32_t *__restrict__ es, int32_t *__restrict__ ee)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int32_t ne = (int32_t)*ne_ptr;
-    int64_t maxwgt = ctl->maxwgt;
+    if (blockIdx.x != 0) return;
+    const int lane = lane_id();
+    const int32_t ne = (int32_t)*ne_ptr;
+    const int64_t maxwgt = ctl->maxwgt;
     int32_t start = 0;
     for (int32_t p = 0; p < k; p++) {
-        es[p] = start;
-        int64_t cap = maxwgt > pwgts[p] ? maxwgt - pwgts[p] : 0;
+        if (lane == 0) es[p] = start;
+        const int64_t cap = maxwgt > pwgts[p] ? maxwgt - pwgts[p] : 0;
         int32_t end = start;
-        // Vertex weights are >= 1, so bs is strictly increasing: a slice of
-        // capacity 0 (or an exhausted list) ends where it starts, and the
-        // first prefix reaching bs[start] + cap lies within cap entries of
-        // start, which bounds the search (the binary search then ends on the
-        // same pair lo, hi as over [start, ne)).
         if (cap > 0 && start < ne) {
-            int64_t find = bs[start] + cap;
-            int32_t lo = start, hi = (int32_t)min((int64_t)ne, (int64_t)start + cap);
-            while (lo + 1 < hi) {
-                int32_t mid = (lo + hi) / 2;
-                if (bs[mid] >= find) hi = mid; else lo = mid;
+            const int64_t find = bs[start] + cap;
+            const int32_t limit = (int32_t)min((int64_t)ne, (int64_t)start + cap);
+            int32_t L = start + 1, H = limit;   // the answer lies in [L, H]; H stands for "none below H"
+            while (L < H) {
+                const int32_t span = H - L;
+                const int32_t step = span <= kWave ? 1 : (span + kWave - 1) / kWave;
+                const int32_t pos = L + lane * step;
+                const bool ge = pos < H && bs[pos] >= find;
+                const uint64_t m = __ballot(ge);
+                if (step == 1) {   // every index of [L, H) was tested
+                    if (m) H = L + (__ffsll((long long)m) - 1);
+                    break;
+                }
+                if (m == 0) {   // every sample below find: past the last one that lies below H
+                    const int32_t nsamp = (span + step - 1) / step;
+                    L = L + (nsamp - 1) * step + 1;
+                    continue;
+                }
+                const int f = __ffsll((long long)m) - 1;
+                if (f == 0) {
+                    H = L;
+                    break;
+                }
+                H = L + f * step;
+                L = L + (f - 1) * step + 1;
             }
+            const int32_t hi = H, lo = H - 1;
             int64_t dh = bs[hi] - find, dl = bs[lo] - find;
             dh = dh < 0 ? -dh : dh;
             dl = dl < 0 ? -dl : dl;
             end = dh < dl ? hi : lo;
         }
-        ee[p] = end;
+        if (lane == 0) ee[p] = end;
         start = end;
     }
 }
