@@ -1003,17 +1003,21 @@ __global__ void __launch_bounds__(kWave) k_rs_cookie(int32_t k, const int64_t *_
                                                      const int64_t *__restrict__ pwgts, const JetCtl *ctl,
                                                      int32_t *__restrict__ es, int32_t *__restrict__ ee)
 {
+    extern __shared__ int64_t room[];   // maxwgt - pwgts[p], clamped at 0
     if (blockIdx.x != 0) return;
     const int lane = lane_id();
     const int32_t ne = (int32_t)*ne_ptr;
     const int64_t maxwgt = ctl->maxwgt;
+    for (int32_t p = lane; p < k; p += kWave) room[p] = maxwgt > pwgts[p] ? maxwgt - pwgts[p] : 0;
+    __syncthreads();
     int32_t start = 0;
+    int64_t bstart = bs[0];   // bs[start], carried from slice to slice
     for (int32_t p = 0; p < k; p++) {
         if (lane == 0) es[p] = start;
-        const int64_t cap = maxwgt > pwgts[p] ? maxwgt - pwgts[p] : 0;
+        const int64_t cap = room[p];
         int32_t end = start;
         if (cap > 0 && start < ne) {
-            const int64_t find = bs[start] + cap;
+            const int64_t find = bstart + cap;
             const int32_t limit = (int32_t)min((int64_t)ne, (int64_t)start + cap);
             int32_t L = start + 1, H = limit;   // the answer lies in [L, H]; H stands for "none below H"
             while (L < H) {
@@ -1040,10 +1044,12 @@ __global__ void __launch_bounds__(kWave) k_rs_cookie(int32_t k, const int64_t *_
                 L = L + (f - 1) * step + 1;
             }
             const int32_t hi = H, lo = H - 1;
-            int64_t dh = bs[hi] - find, dl = bs[lo] - find;
+            const int64_t bh = bs[hi], bl = bs[lo];
+            int64_t dh = bh - find, dl = bl - find;
             dh = dh < 0 ? -dh : dh;
             dl = dl < 0 ? -dl : dl;
             end = dh < dl ? hi : lo;
+            bstart = dh < dl ? bh : bl;
         }
         if (lane == 0) ee[p] = end;
         start = end;
@@ -2086,7 +2092,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                 k_gather_w<<<gv, kBS, 0, c.stream>>>(d_ne, elist, val, g.vwgt, bs);
                 HGP_LAUNCH_CHECK();
                 scan_inclusive(c, bs + 1, bs + 1, n);
-                k_rs_cookie<<<1, 64, 0, c.stream>>>(k, bs, d_ne, pwgts, ctl, es, ee);
+                k_rs_cookie<<<1, kWave, sizeof(int64_t) * (size_t)k, c.stream>>>(k, bs, d_ne, pwgts, ctl, es, ee);
                 HGP_LAUNCH_CHECK();
                 k_rs_assign<<<gv, kBS, 0, c.stream>>>(k, d_ne, elist, val, es, ee, where, dest, pd16, pd32);
                 HGP_LAUNCH_CHECK();
