@@ -311,3 +311,15 @@ def test_files_end_to_end(hgp, orc, tmp_path):
         np.testing.assert_array_equal(p_g, p_o, err_msg=name)
         out = hgp.write_partition(str(tmp_path / os.path.basename(name)), p_g, k)
         assert np.loadtxt(out, dtype=np.int32, ndmin=1).tolist() == p_g.tolist()
+
+
+@pytest.mark.parametrize("n,k", [(400, 200), (400, 300), (20000, 300), (3000, 64)])
+def test_pipeline_level_paths(hgp, orc, n, k):
+    """Levels of <= 512 vertices run the whole Jet loop in one block when
+    k <= 256 and the multi-kernel loop otherwise; both equal the oracle."""
+    x, a, vw, aw = hgp.gen_rgg3d(n, hgp.rgg_radius(n, 10.0), seed=n + k, vw=(1, 3), ew=(1, 9))
+    p_g, st_g = hgp.partition(x, a, k, vw, aw, seed=2)
+    p_o, st_o = orc.dga_partition(x, a, k, vw, aw, seed=2)
+    np.testing.assert_array_equal(p_g, p_o, err_msg=f"n={n} k={k}")
+    assert st_g["cut"] == st_o["cut"] == orc.edge_cut(x, a, aw, p_g)
+    assert st_g["level_iters"] == st_o["level_iters"]
