@@ -1565,7 +1565,6 @@ struct TinyArgs {
 
 struct TinyShared {
     NextScratch nx;
-    int32_t ncand, nsort, ne;
     uint32_t cnt[256], base_d[256];
     uint32_t wcnt[kTinyWaves][256];
     uint32_t ssh[kTinyWaves + 1];
