@@ -197,7 +197,8 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
             pw1 -= wv;
             if (pw1 <= onemax) break;
             drain = 0;
-            eid_t rb = fx[s0 + v], re = fx[s0 + v + 1];
+            // once every vertex is queued a row has nothing fresh to append
+            eid_t rb = fx[s0 + v], re = nleft > 0 ? fx[s0 + v + 1] : rb;
             for (eid_t j0 = rb; j0 < re; j0 += kWave) {
                 eid_t j = j0 + lane;
                 int32_t x = j < re ? fadj[j] - s0 : -1;
