@@ -224,7 +224,20 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
         int32_t e = 0, in = 0;
         uint8_t me = M.where[v];
         eid_t rb = fx[s0 + v], re = fx[s0 + v + 1];
-        for (eid_t j = rb; j < re; j++) {
+        eid_t j = rb;
+        for (; j + 8 <= re; j += 8) {   // eight entries' loads in flight
+            int32_t x[8], w[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                x[t] = fadj[j + t];
+                w[t] = fw[j + t];
+            }
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                if (M.where[x[t] - s0] != me) e += w[t]; else in += w[t];
+            }
+        }
+        for (; j < re; j++) {
             if (M.where[fadj[j] - s0] != me) e += fw[j]; else in += fw[j];
         }
         M.ed[v] = e;
