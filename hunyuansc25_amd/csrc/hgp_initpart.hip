@@ -447,7 +447,16 @@ __global__ void k_ip_pos(int32_t n, const int32_t *__restrict__ node_of, const N
         int32_t rank1 = (v - nd.start) - rank0;
         p = sd == 0 ? nd.start + rank0 : nd.start + nd.cnt0 + rank1;
         nnode_of[p] = 2 * id + sd;
-        for (eid_t j = fx[v]; j < fx[v + 1]; j++) d += side[fadj[j]] == sd;
+        eid_t j = fx[v];
+        const eid_t je = fx[v + 1];
+        for (; j + 8 <= je; j += 8) {   // eight entries' loads in flight
+            int32_t x[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) x[t] = fadj[j + t];
+#pragma unroll
+            for (int t = 0; t < 8; t++) d += side[x[t]] == sd;
+        }
+        for (; j < je; j++) d += side[fadj[j]] == sd;
     }
     pos[v] = p;
     nlabel[p] = label[v];
@@ -464,7 +473,26 @@ __global__ void k_ip_edges(int32_t n, const int8_t *__restrict__ side, const int
     int8_t sd = side[v];
     if (sd < 0) return;
     eid_t o = nfx[pos[v]];
-    for (eid_t j = fx[v]; j < fx[v + 1]; j++) {
+    eid_t j = fx[v];
+    const eid_t je = fx[v + 1];
+    for (; j + 8 <= je; j += 8) {   // eight entries' loads in flight, written in row order
+        int32_t x[8], w[8], sx[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            x[t] = fadj[j + t];
+            w[t] = fw[j + t];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; t++) sx[t] = side[x[t]] == sd ? pos[x[t]] : -1;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            if (sx[t] < 0) continue;
+            nfadj[o] = sx[t];
+            nfw[o] = w[t];
+            o++;
+        }
+    }
+    for (; j < je; j++) {
         int32_t x = fadj[j];
         if (side[x] != sd) continue;
         nfadj[o] = pos[x];

@@ -1013,6 +1013,10 @@ __global__ void __launch_bounds__(kWave) k_rs_cookie(int32_t k, const int64_t *_
     int32_t start = 0;
     int64_t bstart = bs[0];   // bs[start], carried from slice to slice
     for (int32_t p = 0; p < k; p++) {
+        if (start >= ne) {   // the list is used up: every later slice is empty
+            for (int32_t q = p + lane; q < k; q += kWave) es[q] = ee[q] = ne;
+            break;
+        }
         if (lane == 0) es[p] = start;
         const int64_t cap = room[p];
         int32_t end = start;
