@@ -857,19 +857,23 @@ __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__
         int len = (int)(row.dv + (u != v ? xadj[u + 1] - row.bu : 0));
         uint32_t key[KR];
         int32_t val[KR];
+        int32_t nb[KR];
 #pragma unroll
-        for (int r = 0; r < KR; r++) {
+        for (int r = 0; r < KR; r++) {   // every index and weight load first, then the cmap gathers
             int i = r * kWave + lane;
-            key[r] = kSent;
+            nb[r] = -1;
             val[r] = 0;
             if (i < len) {
                 eid_t j = row.at(i);
-                int32_t cc = cmap[adjncy[j]];
-                if (cc != c) {
-                    key[r] = (uint32_t)cc;
-                    val[r] = adjwgt[j];
-                }
+                nb[r] = adjncy[j];
+                val[r] = adjwgt[j];
             }
+        }
+#pragma unroll
+        for (int r = 0; r < KR; r++) {
+            int32_t cc = nb[r] >= 0 ? cmap[nb[r]] : c;
+            key[r] = cc != c ? (uint32_t)cc : kSent;
+            val[r] = cc != c ? val[r] : 0;
         }
         int R = len <= 128 ? 2 : (len <= 256 ? 4 : (len <= 512 ? 8 : 16));
         if (R == 2) wave_bitonic<2>(key, val);
@@ -936,19 +940,23 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
             const int R = P / kSegBS, w = tid / kWave, lane = lane_id();
             uint32_t key[kSegE];
             int32_t val[kSegE];
+            int32_t nb[kSegE];
 #pragma unroll
-            for (int r = 0; r < kSegE; r++) {
+            for (int r = 0; r < kSegE; r++) {   // every index and weight load first, then the cmap gathers
                 int i = w * R * kWave + r * kWave + lane;
-                key[r] = kSent;
+                nb[r] = -1;
                 val[r] = 0;
                 if (r < R && i < len) {
                     eid_t j = row.at(i);
-                    int32_t cc = cmap[adjncy[j]];
-                    if (cc != c) {
-                        key[r] = (uint32_t)cc;
-                        val[r] = adjwgt[j];
-                    }
+                    nb[r] = adjncy[j];
+                    val[r] = adjwgt[j];
                 }
+            }
+#pragma unroll
+            for (int r = 0; r < kSegE; r++) {
+                int32_t cc = nb[r] >= 0 ? cmap[nb[r]] : c;
+                key[r] = cc != c ? (uint32_t)cc : kSent;
+                val[r] = cc != c ? val[r] : 0;
             }
             if (R == kSegE) {
                 block_bitonic<kSegE>(key, val, sk, sw);
