@@ -1546,7 +1546,7 @@ void project(Ctx &c, int32_t n, const int32_t *cmap, const int32_t *where_c, int
 // (selection, afterburner, LP filter, RW/RS eviction with its sort and cookie
 // cutter, move accounting, commit, bookkeeping) separated by block barriers
 // instead of launches, and no host round trip per iteration.  Rows are
-// walked one per wave in chunks of 512 entries; per-vertex phases stride the
+// walked one per wave in chunks of 256 entries; per-vertex phases stride the
 // block.  The block's 16 waves walk rows one after another, so the path
 // pays off only while a level's rows are few: measured on grids, 3-D RGGs
 // and banded graphs, 512 vertices beat both 256 and 1024.
@@ -1555,7 +1555,8 @@ constexpr int kTinyWaves = kTinyBS / kWave;
 constexpr int32_t kTinyN = 4096;   // capacity of the block-local scans
 constexpr int32_t kTinyK = 256;
 constexpr int32_t kTinyUse = 512;   // levels of at most this many vertices take the block path
-constexpr int kTinyChunk = (int)(kBinMid / kWave);   // entries per lane and chunk
+constexpr int kTinyChunk = 4;                         // entries per lane and chunk
+constexpr eid_t kTinyRow = (eid_t)kTinyChunk * kWave;  // row entries per chunk
 
 struct TinyArgs {
     SelArgs A;                   // selection inputs and outputs (dest, pregain, key, ab, pairs, under, ctl)
@@ -1698,8 +1699,8 @@ __global__ void __launch_bounds__(kTinyBS) k_jet_tiny(TinyArgs T)
             for (int t = 0; t < kTinyChunk; t++) q[t] = -1;
             if (mode == kModeRS) {
                 int32_t own = 0;
-                for (eid_t c0 = b; c0 < e; c0 += kBinMid) {
-                    load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + (eid_t)kBinMid, e), lane, u, w);
+                for (eid_t c0 = b; c0 < e; c0 += kTinyRow) {
+                    load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + kTinyRow, e), lane, u, w);
                     A.wq.gather(u, q);
 #pragma unroll
                     for (int t = 0; t < kTinyChunk; t++) own += q[t] == p ? w[t] : 0;
@@ -1708,9 +1709,9 @@ __global__ void __launch_bounds__(kTinyBS) k_jet_tiny(TinyArgs T)
                 if (lane == 0) sel_decide(A, C, mode, v, r, true, own, PartKey{0, -1});
                 continue;
             }
-            const bool one = e - b <= (eid_t)kBinMid;
-            for (eid_t c0 = b; c0 < e; c0 += kBinMid) {
-                load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + (eid_t)kBinMid, e), lane, u, w);
+            const bool one = e - b <= kTinyRow;
+            for (eid_t c0 = b; c0 < e; c0 += kTinyRow) {
+                load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + kTinyRow, e), lane, u, w);
                 A.wq.gather(u, q);
 #pragma unroll
                 for (int t = 0; t < kTinyChunk; t++)
@@ -1760,10 +1761,10 @@ __global__ void __launch_bounds__(kTinyBS) k_jet_tiny(TinyArgs T)
                 const int32_t p = T.where[v], bd = A.dest[v];
                 const eid_t b = A.xadj[v], e = A.xadj[v + 1];
                 int64_t change = 0;
-                for (eid_t c0 = b; c0 < e; c0 += kBinMid) {
+                for (eid_t c0 = b; c0 < e; c0 += kTinyRow) {
                     int32_t u[kTinyChunk], w[kTinyChunk];
                     uint64_t x[kTinyChunk];
-                    load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + (eid_t)kBinMid, e), lane, u, w);
+                    load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + kTinyRow, e), lane, u, w);
 #pragma unroll
                     for (int t = 0; t < kTinyChunk; t++) x[t] = u[t] >= 0 ? A.ab[u[t]] : 0ull;
 #pragma unroll
@@ -1896,9 +1897,9 @@ __global__ void __launch_bounds__(kTinyBS) k_jet_tiny(TinyArgs T)
                 atomicAdd(&pdelta[dv], (unsigned long long)(int64_t)wvv);
             }
             const eid_t b = A.xadj[v], e = A.xadj[v + 1];
-            for (eid_t c0 = b; c0 < e; c0 += kBinMid) {
+            for (eid_t c0 = b; c0 < e; c0 += kTinyRow) {
                 int32_t u[kTinyChunk], w[kTinyChunk], pu[kTinyChunk], du[kTinyChunk];
-                load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + (eid_t)kBinMid, e), lane, u, w);
+                load_entries<kTinyChunk, kWave>(A.adjncy, A.adjwgt, c0, min(c0 + kTinyRow, e), lane, u, w);
                 pq.gather(u, pu, du);
 #pragma unroll
                 for (int t = 0; t < kTinyChunk; t++)
