@@ -513,6 +513,75 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_g(const int32_t *__restri
     kb_flush(A.kb, kbytes);
 }
 
+// Rows of 9..32 entries (class m): 16-lane groups, four rows per wave at a
+// time, each group with its own k-int LDS row.  The groups of a wave run in
+// lockstep, so a wave-level fence orders each group's accumulation, reads
+// and clearing.  Used when k <= kQuarterK.
+constexpr int kQuarterG = 16;
+constexpr int32_t kQuarterK = 512;
+
+__global__ void __launch_bounds__(kRowBS) k_jet_select_q(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
+{
+    extern __shared__ int32_t acc[];   // k ints per group, then k part flags
+    constexpr int G = kQuarterG, NG = kRowBS / G, E = (int)(kBinHalf / G);
+    const JetCtl &C = *A.ctl;
+    if (C.done) return;
+    const int32_t mode = C.mode;
+    const int64_t maxwgt = C.maxwgt, opt = C.opt;
+    const int k = A.k, gid = threadIdx.x / G, gl = threadIdx.x % G;
+    int32_t *my = acc + (size_t)gid * k;
+    uint8_t *tg = (uint8_t *)(acc + (size_t)NG * k);
+    for (int i = gl; i < k; i += G) my[i] = 0;
+    load_targets(A, mode, maxwgt, tg);
+    int32_t ncand = 0;
+    uint64_t kbytes = 0;
+    for (int64_t row = (int64_t)blockIdx.x * NG + gid; row < cnt; row += (int64_t)gridDim.x * NG) {
+        const int32_t v = ids[row];
+        const SelRow r = sel_row(A, mode, v);
+        const int32_t p = r.p;
+        if (!sel_eligible(mode, maxwgt, opt, r)) {
+            if (gl == 0) sel_decide(A, C, mode, v, r, false, 0, PartKey{0, -1});
+            continue;
+        }
+        const eid_t b = A.xadj[v], e = A.xadj[v + 1];
+        int32_t u[E], w[E], q[E];
+        load_entries<E, G>(A.adjncy, A.adjwgt, b, e, gl, u, w);
+        A.wq.gather(u, q);
+        if (gl == 0) kbytes += 16 + 12 * (uint64_t)(e - b);
+        if (mode == kModeRS) {
+            int32_t own = 0;
+#pragma unroll
+            for (int t = 0; t < E; t++) own += q[t] == p ? w[t] : 0;
+            own = group_sum<G>(own);
+            if (gl == 0) sel_decide(A, C, mode, v, r, true, own, PartKey{0, -1});
+            continue;
+        }
+#pragma unroll
+        for (int t = 0; t < E; t++)
+            if (q[t] >= 0) atomicAdd(&my[q[t]], w[t]);
+        wave_lds_sync();
+        const int32_t pc = my[p];
+        PartKey best{0, -1};
+#pragma unroll
+        for (int t = 0; t < E; t++) {
+            if (q[t] >= 0 && q[t] != p && tg[q[t]]) {
+                PartKey cand{my[q[t]], q[t]};
+                if (BetterPart()(cand, best)) best = cand;
+            }
+        }
+        best = group_best<G>(best, BetterPart());
+        wave_lds_sync();
+#pragma unroll
+        for (int t = 0; t < E; t++)
+            if (q[t] >= 0) my[q[t]] = 0;
+        wave_lds_sync();
+        if (gl == 0) ncand += sel_decide(A, C, mode, v, r, true, pc, best);
+    }
+    sel_count(A, mode, ncand);
+    kb_flush(A.kb, kbytes);
+}
+
+
 // Rows of 9..512 entries (classes 1 and 2): a wave takes 64 rows of the list
 // at a time, decides the ineligible ones lane by lane and then walks the
 // eligible ones, one row at a time with its entries in registers (8 per
@@ -1473,14 +1542,25 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
         k_jet_select_g<8><<<blocks, kRowBS, tg_bytes, c.stream>>>(b.list(0), b.count(0), A);
         HGP_LAUNCH_CHECK();
     }
-    // classes 1 and 2 are consecutive in the id list: one wave-per-row launch
-    // over both (32-lane groups for class 1 measured slower here)
-    if (int64_t nw = b.few() ? 0 : b.count(1) + b.count(2)) {
+    // class m: 16-lane groups while k is small enough for their LDS rows;
+    // otherwise classes m and w (consecutive in the id list) share one
+    // wave-per-row launch
+    const bool quarter = !b.few() && k <= kQuarterK && b.count(1) > 0;
+    if (quarter) {
+        KScope ks(c, "k_jet_select_q", 24.0 * (double)b.count(1), true);
+        A.kb = ks.dev();
+        constexpr int NG = kRowBS / kQuarterG;
+        unsigned blocks = (unsigned)std::min<int64_t>((b.count(1) + NG - 1) / NG, 32768);
+        k_jet_select_q<<<blocks, kRowBS, sizeof(int32_t) * NG * (size_t)k + tg_bytes, c.stream>>>(b.list(1),
+                                                                                               b.count(1), A);
+        HGP_LAUNCH_CHECK();
+    }
+    if (int64_t nw = b.few() ? 0 : (quarter ? 0 : b.count(1)) + b.count(2)) {
         KScope ks(c, "k_jet_select_w", 24.0 * (double)nw, true);
         A.kb = ks.dev();
         int32_t rpw = rows_per_wave(nw);
         k_jet_select_w<<<wave_blocks(nw, rpw, 4096), kRowBS, sizeof(int32_t) * 4 * (size_t)k + tg_bytes, c.stream>>>(
-            b.list(1), nw, rpw, A);
+            quarter ? b.list(2) : b.list(1), nw, rpw, A);
         HGP_LAUNCH_CHECK();
     }
     if (g.segs.nseg) {
