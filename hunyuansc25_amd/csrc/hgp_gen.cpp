@@ -16,6 +16,23 @@
 #include <chrono>
 #include <cstdio>
 
+namespace hgp {
+
+// The arrays of a library-allocated host graph (n vertices, m directed
+// entries); false if any allocation failed (hgp_graph_free releases the rest).
+bool alloc_host_graph(hgp_graph *g, int32_t n, int64_t m)
+{
+    g->n = n;
+    g->m = m;
+    g->xadj = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    g->adjncy = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+    g->adjwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
+    g->vwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    return g->xadj && g->adjncy && g->adjwgt && g->vwgt;
+}
+
+}  // namespace hgp
+
 namespace {
 
 inline uint64_t mix64(uint64_t z)
@@ -63,17 +80,6 @@ void parallel_for(int64_t n, F f)
     for (auto &x : th) x.join();
 }
 
-bool alloc_graph(hgp_graph *g, int32_t n, int64_t m)
-{
-    g->n = n;
-    g->m = m;
-    g->xadj = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
-    g->adjncy = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
-    g->adjwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
-    g->vwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
-    return g->xadj && g->adjncy && g->adjwgt && g->vwgt;
-}
-
 // LSD radix sort of 64-bit keys (8 passes of 8 bits, skipping constant digits).
 void radix_sort_u64(std::vector<uint64_t> &a)
 {
@@ -94,7 +100,7 @@ void radix_sort_u64(std::vector<uint64_t> &a)
 // Builds a symmetric CSR from unique undirected pairs (lo < hi) packed as
 // lo << 32 | hi; rows come out sorted, so the result does not depend on the
 // order of the pairs or on thread scheduling.
-void csr_from_pairs(int32_t n, const std::vector<uint64_t> &pairs, hgp_graph *g)
+bool csr_from_pairs(int32_t n, const std::vector<uint64_t> &pairs, hgp_graph *g)
 {
     int64_t P = (int64_t)pairs.size();
     std::vector<std::atomic<int64_t>> deg(n);
@@ -103,7 +109,10 @@ void csr_from_pairs(int32_t n, const std::vector<uint64_t> &pairs, hgp_graph *g)
         deg[pairs[i] >> 32].fetch_add(1, std::memory_order_relaxed);
         deg[pairs[i] & 0xffffffffu].fetch_add(1, std::memory_order_relaxed);
     });
-    alloc_graph(g, n, 2 * P);
+    if (!hgp::alloc_host_graph(g, n, 2 * P)) {
+        hgp_graph_free(g);
+        return false;
+    }
     g->xadj[0] = 0;
     for (int32_t v = 0; v < n; v++) g->xadj[v + 1] = g->xadj[v] + deg[v].load(std::memory_order_relaxed);
     parallel_for(n, [&](int64_t v) { deg[v].store(g->xadj[v], std::memory_order_relaxed); });
@@ -117,6 +126,7 @@ void csr_from_pairs(int32_t n, const std::vector<uint64_t> &pairs, hgp_graph *g)
         g->vwgt[v] = 1;
     });
     parallel_for(2 * P, [&](int64_t j) { g->adjwgt[j] = 1; });
+    return true;
 }
 
 }  // namespace
@@ -138,7 +148,7 @@ int hgp_gen_grid2d(int32_t rows, int32_t cols, hgp_graph *g)
     if (!g || rows < 1 || cols < 1 || (int64_t)rows * cols > INT32_MAX) return HGP_ERR_INVALID;
     int32_t n = rows * cols;
     int64_t m = 2 * ((int64_t)(rows - 1) * cols + (int64_t)rows * (cols - 1));
-    if (!alloc_graph(g, n, m)) return HGP_ERR_OOM;
+    if (!hgp::alloc_host_graph(g, n, m)) return HGP_ERR_OOM;
     int64_t e = 0;
     g->xadj[0] = 0;
     for (int32_t r = 0; r < rows; r++) {
@@ -212,7 +222,7 @@ int hgp_gen_rgg3d(int32_t n, double radius, uint64_t seed, int32_t vw_lo, int32_
     });
     int64_t m = 0;
     for (int32_t i = 0; i < n; i++) m += deg[i];
-    if (!alloc_graph(g, n, m)) return HGP_ERR_OOM;
+    if (!hgp::alloc_host_graph(g, n, m)) return HGP_ERR_OOM;
     g->xadj[0] = 0;
     for (int32_t i = 0; i < n; i++) g->xadj[i + 1] = g->xadj[i] + deg[i];
     bool wv = vw_hi > 0 && vw_hi >= vw_lo && vw_lo > 0;
@@ -383,7 +393,7 @@ int hgp_gen_rmat(int32_t scale, int64_t nedges, double a, double b, double c, ui
         keys[i] = u < v ? (u << 32) | v : (v << 32) | u;
     });
     GEN_T("relabel");
-    csr_from_pairs((int32_t)n, keys, g);
+    if (!csr_from_pairs((int32_t)n, keys, g)) return HGP_ERR_OOM;
     GEN_T("csr");
     return HGP_OK;
 }
@@ -414,7 +424,7 @@ int hgp_gen_banded(int32_t n, int32_t bw, double frac, uint64_t seed, hgp_graph 
     }
     pairs.insert(pairs.end(), off.begin(), off.end());
     radix_sort_u64(pairs);
-    csr_from_pairs(n, pairs, g);
+    if (!csr_from_pairs(n, pairs, g)) return HGP_ERR_OOM;
     return HGP_OK;
 }
 

@@ -19,6 +19,7 @@
 
 namespace hgp {
 void set_error(int code, const std::string &msg);
+bool alloc_host_graph(hgp_graph *g, int32_t n, int64_t m);   // hgp_gen.cpp
 }
 
 namespace {
@@ -95,18 +96,6 @@ bool parse_int(const char *&p, const char *e, int64_t &out)
     return true;
 }
 
-bool alloc_graph(hgp_graph *g, int32_t n, int64_t m)
-{
-    memset(g, 0, sizeof(*g));
-    g->n = n;
-    g->m = m;
-    g->xadj = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
-    g->adjncy = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
-    g->adjwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(m > 0 ? m : 1));
-    g->vwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
-    return g->xadj && g->adjncy && g->adjwgt && g->vwgt;
-}
-
 std::string at(const Lines &L) { return " (line " + std::to_string(L.lineno) + ")"; }
 
 void read_metis(const char *path, hgp_graph *g)
@@ -130,7 +119,7 @@ void read_metis(const char *path, hgp_graph *g)
     // and edge weights; a digit other than 1 leaves its field out
     const bool has_vsize = fmt / 100 == 1, has_vw = (fmt / 10) % 10 == 1, has_ew = fmt % 10 == 1;
     const int64_t m = 2 * ne;
-    if (!alloc_graph(g, (int32_t)nv, m)) fail(HGP_ERR_OOM, "host out of memory");
+    if (!hgp::alloc_host_graph(g, (int32_t)nv, m)) fail(HGP_ERR_OOM, "host out of memory");
     int64_t k = 0;
     g->xadj[0] = 0;
     for (int32_t v = 0; v < (int32_t)nv; v++) {
@@ -219,7 +208,7 @@ void read_mtx(const char *path, hgp_graph *g)
     std::sort(key.begin(), key.end());
     key.erase(std::unique(key.begin(), key.end()), key.end());
     const int64_t m = (int64_t)key.size();
-    if (!alloc_graph(g, n, m)) fail(HGP_ERR_OOM, "host out of memory");
+    if (!hgp::alloc_host_graph(g, n, m)) fail(HGP_ERR_OOM, "host out of memory");
     std::fill(g->xadj, g->xadj + n + 1, 0);
     for (int64_t t = 0; t < m; t++) {
         g->xadj[(key[t] >> 32) + 1]++;
