@@ -18,6 +18,8 @@
 
 namespace hgp {
 
+bool alloc_host_graph(hgp_graph *g, int32_t n, int64_t m);   // hgp_gen.cpp
+
 namespace {
 
 std::mutex g_api_mu;   // one call at a time per process on the default contexts
@@ -544,14 +546,9 @@ int hgp_coarsen(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int
             HGP_CHECK_HIP(hipEventRecord(c.ev[1], c.stream));
             const DevGraph &cg = lv.back();
             hgp_graph h{};
-            h.n = cg.n;
-            h.m = cg.m;
-            h.xadj = (int64_t *)malloc(sizeof(int64_t) * ((size_t)cg.n + 1));
-            h.adjncy = (int32_t *)malloc(sizeof(int32_t) * (size_t)(cg.m > 0 ? cg.m : 1));
-            h.adjwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)(cg.m > 0 ? cg.m : 1));
-            h.vwgt = (int32_t *)malloc(sizeof(int32_t) * (size_t)cg.n);
+            const bool got = alloc_host_graph(&h, cg.n, cg.m);
             try {
-                if (!h.xadj || !h.adjncy || !h.adjwgt || !h.vwgt) throw Error{-3, "host out of memory"};
+                if (!got) throw Error{-3, "host out of memory"};
                 copy_coarsest(c, cg, h, lv, tvwgt, probe, stats);
             } catch (...) {
                 hgp_graph_free(&h);
