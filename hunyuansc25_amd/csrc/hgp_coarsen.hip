@@ -97,7 +97,9 @@ __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ 
             EdgeKey best{-1, 0, 0, 0u};
             eid_t beg = xadj[v], end = xadj[v + 1];
             const uint64_t tv = mix64(s1 ^ (uint64_t)(uint32_t)v);
-            constexpr int T = G == kWave ? (int)(kBinMid / kWave) : 1;   // entries per lane and phase
+            // entries per lane and phase: a whole class-w row for a wave, a whole
+            // class-m row for a 16-lane group, one entry otherwise
+            constexpr int T = G == kWave ? (int)(kBinMid / kWave) : (G == 16 ? (int)(kBinHalf / 16) : 1);
             for (eid_t c0 = beg; c0 < end; c0 += (eid_t)T * G) {
                 int32_t u[T], w[T], wu[T];
                 load_entries<T, G>(adjncy, adjwgt, c0, end, lane, u, w);
@@ -1429,7 +1431,7 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
                                                                                avail, maxvwgt, s1, prop, prev,
                                                                                r == 0, ks.dev());
                 else if (cl == 1)
-                    k_propose<32><<<grid_groups(cnt, 32), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
+                    k_propose<16><<<grid_groups(cnt, 16), kRowBS, 0, c.stream>>>(ids, cnt, g.xadj, g.adjncy, g.adjwgt,
                                                                                  avail, maxvwgt, s1, prop, prev,
                                                                                  r == 0, ks.dev());
                 else
