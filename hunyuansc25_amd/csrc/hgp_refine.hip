@@ -676,7 +676,8 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
 __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A, int32_t *__restrict__ conn)
 {
     extern __shared__ int32_t acc[];   // k ints, then k part flags
-    constexpr int T = (int)(kSegLen / kRowBS);
+    constexpr int T = (int)(kSegLen / kRowBS) / 2;   // a segment in two phases of T entries per thread
+    constexpr eid_t kHalf = (eid_t)T * kRowBS;
     const JetCtl &C = *A.ctl;
     if (C.done) return;
     const int32_t mode = C.mode;
@@ -695,14 +696,22 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         bool eligible = sel_eligible(mode, maxwgt, opt, rr);
         eid_t b = S.j0[s], e = min(b + (eid_t)kSegLen, A.xadj[v + 1]);
         if (threadIdx.x == 0 && eligible) kbytes += 8 + 12 * (uint64_t)(e - b);
+        const eid_t ee = eligible ? e : b;
+        const bool two = ee - b > kHalf;
         int32_t u[T], w[T], q[T];
-        load_entries<T, kRowBS>(A.adjncy, A.adjwgt, b, eligible ? e : b, threadIdx.x, u, w);
+        load_entries<T, kRowBS>(A.adjncy, A.adjwgt, b, min(ee, b + kHalf), threadIdx.x, u, w);
         A.wq.gather(u, q);
         if (mode == kModeRS) {
             // only the connectivity to the own part: a block sum, no k-int row
             int32_t own = 0;
 #pragma unroll
             for (int t = 0; t < T; t++) own += q[t] == p ? w[t] : 0;
+            if (two) {
+                load_entries<T, kRowBS>(A.adjncy, A.adjwgt, b + kHalf, ee, threadIdx.x, u, w);
+                A.wq.gather(u, q);
+#pragma unroll
+                for (int t = 0; t < T; t++) own += q[t] == p ? w[t] : 0;
+            }
             own = group_sum<kRowBS>(own);
             if (threadIdx.x == 0) {
                 if (mi < 0) sel_decide(A, C, mode, v, rr, eligible, own, PartKey{0, -1});
@@ -710,10 +719,18 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
             }
             continue;
         }
-        bool walk_parts = e - b > (eid_t)k;
+        // a two-phase segment keeps only its second half in registers: walk the parts
+        bool walk_parts = two || e - b > (eid_t)k;
 #pragma unroll
         for (int t = 0; t < T; t++)
             if (q[t] >= 0) atomicAdd(&acc[q[t]], w[t]);
+        if (two) {
+            load_entries<T, kRowBS>(A.adjncy, A.adjwgt, b + kHalf, ee, threadIdx.x, u, w);
+            A.wq.gather(u, q);
+#pragma unroll
+            for (int t = 0; t < T; t++)
+                if (q[t] >= 0) atomicAdd(&acc[q[t]], w[t]);
+        }
         __syncthreads();
         if (mi >= 0) {
             // partial sums of a multi row: move them to conn and clear
