@@ -397,7 +397,7 @@ extern "C" {
 
 const char *hgp_last_error(void) { return hgp::last_error_text(); }
 
-const char *hgp_version(void) { return "hgp 0.1.0 (gfx950)"; }
+const char *hgp_version(void) { return "hgp 0.2.0 (gfx950)"; }
 
 int hgp_device_count(void)
 {
