@@ -248,7 +248,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1",
+        print(json.dumps({"error": "the partitioner runs on one GPU (a graph is not sharded across GPUs); use --gpus 1",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
 
