@@ -590,7 +590,7 @@ __global__ void k_rep(int32_t n, const eid_t *__restrict__ xadj, const wgt_t *__
 // and equal ids are merged by summing weights; the merged row is written at
 // toff[c] (the concatenation's offset) of tadj/twgt and its length to ccnt[c].
 // Rows of <= 64 entries sort in registers (bitonic over xor-shuffles), rows of
-// <= kSegLds entries in LDS (bitonic, one block per row), longer rows through
+// <= kSegLds entries in LDS (radix, one block per row), longer rows through
 // the device radix sort.
 
 constexpr uint32_t kSent = 0xFFFFFFFFu;   // dropped entry; sorts last
@@ -751,86 +751,81 @@ __device__ inline void wave_bitonic(uint32_t (&key)[KR], int32_t (&val)[KR])
     }
 }
 
-// One compare-exchange stage at distance j (j < 64: xor-shuffles, else
-// register swaps) of a bitonic network whose entry e = base + r*64 + lane
-// sorts ascending where (e & k) == 0.
+// Stable LSD radix sort of a block's P = 8*R*64 entries by the low 8*passes
+// bits of the key (8-bit digits), entry w*C + r*64 + lane (C = R*64) held in
+// register r of lane `lane` of wave w.  Each pass ranks a wave's entries by
+// digit (entries of equal digit keep their order: register-major, then lane)
+// from 8 ballots per register and a per-wave count of each digit (wc, 256
+// ints per wave), turns the counts into scatter bases with a scan over
+// (digit, wave) and scatters into sk/sw; the entries are read back into
+// registers for the next pass.  The last pass leaves the sorted entries in
+// sk/sw, P of them.  Keys beyond the low bits (the sentinel) must sort last
+// on those bits alone.
 template <int R>
-__device__ inline void bitonic_stage(uint32_t (&key)[R], int32_t (&val)[R], int base, int k, int j)
+__device__ inline void block_radix(uint32_t (&key)[R], int32_t (&val)[R], uint32_t *sk, int32_t *sw, int32_t *wc,
+                                   int32_t *dsum, int passes)
 {
-    const int lane = lane_id();
-    if (j >= kWave) {
+    constexpr int C = R * kWave, NW = kSegBS / kWave;
+    const int w = threadIdx.x / kWave, lane = lane_id();
+    const uint64_t lt = lanemask_lt();
+    int32_t *my = wc + w * 256;
+    for (int pass = 0; pass < passes; pass++) {
+        const int shift = 8 * pass;
+        for (int i = lane; i < 256; i += kWave) my[i] = 0;
+        wave_lds_sync();
+        int32_t rk[R];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            int r2 = r ^ (j / kWave);
-            if (r2 > r) {
-                bool asc = ((base + r * kWave) & k) == 0;   // k >= 128: the lane bits do not matter
-                bool sw = asc ? key[r] > key[r2] : key[r] < key[r2];
-                if (sw) {
-                    uint32_t tk = key[r];
-                    key[r] = key[r2];
-                    key[r2] = tk;
-                    int32_t tv = val[r];
-                    val[r] = val[r2];
-                    val[r2] = tv;
-                }
+            const uint32_t d = (key[r] >> shift) & 255u;
+            uint64_t peers = ~0ull;   // lanes whose register r holds the same digit
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const bool bit = (d >> b) & 1u;
+                const uint64_t bb = __ballot(bit);
+                peers &= bit ? bb : ~bb;
+            }
+            const int32_t base = my[d];
+            rk[r] = base + __popcll(peers & lt);
+            wave_lds_sync();   // every lane has its base
+            if ((peers & lt) == 0) my[d] = base + __popcll(peers);
+            wave_lds_sync();
+        }
+        __syncthreads();
+        // base of (digit d, wave x): entries of smaller digits, then of digit d in waves < x
+        const int d = threadIdx.x & 255;
+        int32_t run = 0;
+        if (threadIdx.x < 256) {
+#pragma unroll
+            for (int x = 0; x < NW; x++) {
+                const int32_t c = wc[x * 256 + d];
+                wc[x * 256 + d] = run;
+                run += c;
             }
         }
-    } else {
+        const int32_t incl = wave_inclusive_scan(run);
+        if (threadIdx.x < 256 && lane == kWave - 1) dsum[w] = incl;
+        __syncthreads();
+        if (threadIdx.x < 256) {
+            int32_t pre = incl - run;
+            for (int x = 0; x < w; x++) pre += dsum[x];
+#pragma unroll
+            for (int x = 0; x < NW; x++) wc[x * 256 + d] += pre;
+        }
+        __syncthreads();
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            uint32_t ok = __shfl_xor(key[r], j, kWave);
-            int32_t ov = __shfl_xor(val[r], j, kWave);
-            bool asc = ((base + r * kWave + lane) & k) == 0;
-            bool lower = (lane & j) == 0;
-            bool take = (lower == asc) ? ok < key[r] : ok > key[r];
-            if (take) {
-                key[r] = ok;
-                val[r] = ov;
-            }
+            const int32_t at = wc[w * 256 + ((key[r] >> shift) & 255u)] + rk[r];
+            sk[at] = key[r];
+            sw[at] = val[r];
         }
-    }
-}
-
-// Bitonic sort of 8*R*64 entries by a block of 8 waves (kSegBS = 512
-// threads): wave w holds entries [w*C, w*C + C), C = R*64, as entry
-// w*C + r*64 + lane in register r of its lanes.  Stages closer than C stay in
-// the wave (shuffles and register swaps); the log2(8) farther distances of
-// each merge go through LDS (sk/sw hold the block's entries), two barriers
-// each.  Every entry ends where a plain bitonic network over the whole range
-// would put it.
-template <int R>
-__device__ inline void block_bitonic(uint32_t (&key)[R], int32_t (&val)[R], uint32_t *sk, int32_t *sw)
-{
-    constexpr int C = R * kWave, P = (kSegBS / kWave) * C;
-    const int w = threadIdx.x / kWave, lane = lane_id(), base = w * C;
-#pragma unroll
-    for (int k = 2; k <= C; k <<= 1)
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) bitonic_stage<R>(key, val, base, k, j);
-    for (int k = 2 * C; k <= P; k <<= 1) {
-        for (int j = k >> 1; j >= C; j >>= 1) {
+        __syncthreads();
+        if (pass + 1 < passes) {
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                sk[base + r * kWave + lane] = key[r];
-                sw[base + r * kWave + lane] = val[r];
+                key[r] = sk[w * C + r * kWave + lane];
+                val[r] = sw[w * C + r * kWave + lane];
             }
-            __syncthreads();
-#pragma unroll
-            for (int r = 0; r < R; r++) {
-                int i = base + r * kWave + lane, pi = i ^ j;
-                uint32_t ok = sk[pi];
-                int32_t ov = sw[pi];
-                bool asc = (i & k) == 0, lower = (i & j) == 0;
-                bool take = (lower == asc) ? ok < key[r] : ok > key[r];
-                if (take) {
-                    key[r] = ok;
-                    val[r] = ov;
-                }
-            }
-            __syncthreads();
         }
-#pragma unroll
-        for (int j = C >> 1; j > 0; j >>= 1) bitonic_stage<R>(key, val, base, k, j);
     }
 }
 
@@ -914,9 +909,10 @@ __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__
     kb_flush(kb, kbytes);
 }
 
-// Rows of kSegWave+1..kSegLds entries: one block per row, bitonic sort in LDS padded
-// to a power of two, then tails (last entry of each run of equal ids) are
-// ranked by a block scan and their inclusive weight prefixes differenced.
+// Rows of kSegWave+1..kSegLds entries: one block per row, radix sort through
+// LDS (padded to 2048 or 4096 entries; `passes` 8-bit digits cover every
+// coarse id), then tails (last entry of each run of equal ids) are ranked by
+// a block scan and their inclusive weight prefixes differenced.
 __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ ids, int64_t cnt,
                                                     const int32_t *__restrict__ rep,
                                                     const int32_t *__restrict__ match,
@@ -924,11 +920,13 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
                                                     const wgt_t *__restrict__ adjwgt,
                                                     const int32_t *__restrict__ cmap, const int64_t *__restrict__ toff,
                                                     int32_t *__restrict__ tadj, int32_t *__restrict__ twgt,
-                                                    int32_t *__restrict__ ccnt, unsigned long long *kb)
+                                                    int32_t *__restrict__ ccnt, int passes, unsigned long long *kb)
 {
     __shared__ uint32_t sk[kSegLds];
     uint64_t kbytes = 0;
     __shared__ int32_t sw[kSegLds];
+    __shared__ int32_t wc[kSegBS / kWave * 256];
+    __shared__ int32_t dsum[4];
     __shared__ int64_t sh[kSegBS / kWave + 1];
     const int tid = threadIdx.x;
     for (int64_t gi = blockIdx.x; gi < cnt; gi += gridDim.x) {
@@ -961,25 +959,14 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
                 val[r] = cc != c ? val[r] : 0;
             }
             if (R == kSegE) {
-                block_bitonic<kSegE>(key, val, sk, sw);
+                block_radix<kSegE>(key, val, sk, sw, wc, dsum, passes);
             } else {
                 uint32_t k4[kSegE / 2];
                 int32_t v4[kSegE / 2];
 #pragma unroll
                 for (int r = 0; r < kSegE / 2; r++) { k4[r] = key[r]; v4[r] = val[r]; }
-                block_bitonic<kSegE / 2>(k4, v4, sk, sw);
-#pragma unroll
-                for (int r = 0; r < kSegE / 2; r++) { key[r] = k4[r]; val[r] = v4[r]; }
+                block_radix<kSegE / 2>(k4, v4, sk, sw, wc, dsum, passes);
             }
-            __syncthreads();   // the last cross-wave stage's reads are done
-#pragma unroll
-            for (int r = 0; r < kSegE; r++) {
-                if (r < R) {
-                    sk[w * R * kWave + r * kWave + lane] = key[r];
-                    sw[w * R * kWave + r * kWave + lane] = val[r];
-                }
-            }
-            __syncthreads();
         }
         // Thread tid owns entries [tid*E, tid*E + E), E = P / kSegBS (one each when P < kSegBS).
         const int E = P >= kSegBS ? P / kSegBS : 1;
@@ -1670,8 +1657,9 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
         if (cb.count(4)) {
             KScope ks(c, "k_seg_lds", bytes(4), true);
             unsigned blocks = (unsigned)std::min<int64_t>(cb.count(4), 8192);
+            const int passes = std::max(1, (bits_for((uint64_t)cn) + 7) / 8);   // coarse ids < 2^(8*passes) - 1
             k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(4), cb.count(4), rep, match, g.xadj, g.adjncy,
-                                                       g.adjwgt, cmap, toff, tadj, twgt, ccnt, ks.dev());
+                                                       g.adjwgt, cmap, toff, tadj, twgt, ccnt, passes, ks.dev());
             HGP_LAUNCH_CHECK();
         }
     }

@@ -202,6 +202,13 @@ __device__ inline uint64_t lanemask_lt()
     return l == 0 ? 0ull : (~0ull >> (64 - l));
 }
 
+// Orders a wave's LDS accesses around this point (the wave's own lanes only).
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <class T>
 __device__ inline T wave_sum(T v)
 {

@@ -406,12 +406,6 @@ __device__ inline void sel_count(const SelArgs &A, int32_t mode, int32_t ncand)
 
 // Orders this wave's LDS accesses before and after the point (the wave runs
 // in lockstep, so completing its own LDS operations is enough).
-__device__ inline void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-}
-
 // Part flags of the iteration for the block: tg[q] = 1 if q may receive a
 // vertex (sel_target), read once per block instead of once per entry.
 __device__ inline void load_targets(const SelArgs &A, int32_t mode, int64_t maxwgt, uint8_t *tg)

@@ -243,6 +243,21 @@ def test_contract_hub_rows(hgp, orc):
         np.testing.assert_array_equal(g_arr, e_arr)
 
 
+def test_contract_wide_coarse_ids(hgp, orc):
+    """More than 2^16 coarse vertices: the block-sorted rows (1025..4096
+    entries) take three 8-bit digit passes."""
+    x, a, vw, aw = hgp.gen_rmat(18, 1 << 22, seed=21)
+    deg = np.diff(x)
+    assert ((deg > 1024) & (deg <= 4096)).sum() > 20
+    tv = int(vw.sum())
+    mt, _ = orc.dga_match(x, a, vw, aw, orc.maxvwgt(tv, orc.coarsen_threshold(64)), 5, 0, 8)
+    got = hgp.contract(x, a, aw, vw, mt)
+    exp = orc.contract(x, a, aw, vw, mt)
+    assert len(exp[2]) - 1 > (1 << 16)
+    for g_arr, e_arr, what in zip(got, exp, ["match", "cmap", "cxadj", "cadjncy", "cadjwgt", "cvwgt"]):
+        np.testing.assert_array_equal(g_arr, e_arr, err_msg=what)
+
+
 def test_pipeline_hub_graph(hgp, orc):
     x, a, vw, aw = _hub_graph(hgp)
     p_g, st_g = hgp.partition(x, a, 64, vw, aw, seed=2)
