@@ -594,8 +594,7 @@ __global__ void k_rep(int32_t n, const eid_t *__restrict__ xadj, const wgt_t *__
 // the device radix sort.
 
 constexpr uint32_t kSent = 0xFFFFFFFFu;   // dropped entry; sorts last
-constexpr int kSegWave = 1024;  // rows of 65..kSegWave entries: one wave, up to 16 entries per lane
-constexpr int kSegWaveHalf = 512;   // rows of 65..512 entries take the 8-register variant
+constexpr int kSegWaveMax = 512;   // rows of 65..512 entries: one wave, up to 8 entries per lane
 constexpr int kClsSegLong = 5;      // contraction class of the rows longer than kSegLds
 constexpr int kSegLds = 4096;
 constexpr int kSegBS = 512;
@@ -829,11 +828,9 @@ __device__ inline void block_radix(uint32_t (&key)[R], int32_t (&val)[R], uint32
     }
 }
 
-// Rows of 65..KR*64 entries: one wave per row, sorted in registers (padded
-// to 128, 256, 512 or 1024 entries); tails ranked by ballots, weights by wave
-// scans carried across the registers.  KR = 8 serves rows of <= 512 entries
-// at twice the occupancy of KR = 16.
-template <int KR>
+// Rows of 65..kSegWaveMax entries: one wave per row, sorted in registers
+// (padded to 128, 256 or 512 entries); tails ranked by ballots, weights by
+// wave scans carried across the registers.
 __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__ ids, int64_t cnt,
                                                      const int32_t *__restrict__ rep,
                                                      const int32_t *__restrict__ match,
@@ -844,6 +841,7 @@ __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__
                                                      int32_t *__restrict__ twgt, int32_t *__restrict__ ccnt,
                                                      unsigned long long *kb)
 {
+    constexpr int KR = kSegWaveMax / kWave;
     const int lane = lane_id();
     const int64_t gi = ((int64_t)blockIdx.x * kRowBS + threadIdx.x) / kWave;
     uint64_t kbytes = 0;
@@ -872,11 +870,10 @@ __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__
             key[r] = cc != c ? (uint32_t)cc : kSent;
             val[r] = cc != c ? val[r] : 0;
         }
-        int R = len <= 128 ? 2 : (len <= 256 ? 4 : (len <= 512 ? 8 : 16));
+        int R = len <= 128 ? 2 : (len <= 256 ? 4 : 8);
         if (R == 2) wave_bitonic<2>(key, val);
         else if (R == 4) wave_bitonic<4>(key, val);
-        else if (KR == 8 || R == 8) wave_bitonic<8>(key, val);
-        else if constexpr (KR >= 16) wave_bitonic<16>(key, val);
+        else wave_bitonic<8>(key, val);
         const int64_t o = toff[c];
         int32_t carry = 0, lastcum = 0, rank = 0;
 #pragma unroll
@@ -909,9 +906,9 @@ __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__
     kb_flush(kb, kbytes);
 }
 
-// Rows of kSegWave+1..kSegLds entries: one block per row, radix sort through
-// LDS (padded to 2048 or 4096 entries; `passes` 8-bit digits cover every
-// coarse id), then tails (last entry of each run of equal ids) are ranked by
+// Rows of kSegWaveMax+1..kSegLds entries: one block per row, radix sort
+// through LDS (padded to 1024, 2048 or 4096 entries; `passes` 8-bit digits
+// cover every coarse id), then tails (last entry of each run of equal ids) are ranked by
 // a block scan and their inclusive weight prefixes differenced.
 __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ ids, int64_t cnt,
                                                     const int32_t *__restrict__ rep,
@@ -934,7 +931,7 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
         int32_t v = rep[c], u = match[v];
         CatRow row(xadj, v, u);
         int len = (int)(row.dv + (u != v ? xadj[u + 1] - row.bu : 0));
-        const int P = len <= kSegLds / 2 ? kSegLds / 2 : kSegLds;   // rows here have more than kSegWave entries
+        const int P = len <= kSegLds / 4 ? kSegLds / 4 : (len <= kSegLds / 2 ? kSegLds / 2 : kSegLds);
         {
             // entries straight into registers (entry w*C + r*64 + lane), sorted by the block
             const int R = P / kSegBS, w = tid / kWave, lane = lane_id();
@@ -960,12 +957,18 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
             }
             if (R == kSegE) {
                 block_radix<kSegE>(key, val, sk, sw, wc, dsum, passes);
-            } else {
+            } else if (R == kSegE / 2) {
                 uint32_t k4[kSegE / 2];
                 int32_t v4[kSegE / 2];
 #pragma unroll
                 for (int r = 0; r < kSegE / 2; r++) { k4[r] = key[r]; v4[r] = val[r]; }
                 block_radix<kSegE / 2>(k4, v4, sk, sw, wc, dsum, passes);
+            } else {
+                uint32_t k2[kSegE / 4];
+                int32_t v2[kSegE / 4];
+#pragma unroll
+                for (int r = 0; r < kSegE / 4; r++) { k2[r] = key[r]; v2[r] = val[r]; }
+                block_radix<kSegE / 4>(k2, v2, sk, sw, wc, dsum, passes);
             }
         }
         // Thread tid owns entries [tid*E, tid*E + E), E = P / kSegBS (one each when P < kSegBS).
@@ -1623,7 +1626,7 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     int32_t *tadj = c.arena.ralloc<int32_t>((size_t)m);
     int32_t *twgt = c.arena.ralloc<int32_t>((size_t)m);
     int32_t *ccnt = c.arena.ralloc<int32_t>((size_t)cn);
-    RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegWaveHalf, kSegWave, kSegLds);
+    RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegWaveMax, kSegLds, kSegLds);   // class 4 empty
     {
         // per coarse row: ids, rep, match, 2 xadj pairs, toff, count (56 B); per fine entry: adjncy, adjwgt,
         // cmap gather (12 B); the kernels count the merged entries they write (8 B each)
@@ -1641,24 +1644,17 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
             HGP_LAUNCH_CHECK();
         }
         if (cb.count(2)) {
-            KScope ks(c, "k_seg_wave8", bytes(2), true);
-            k_seg_wave<8><<<grid_groups(cb.count(2), kWave), kRowBS, 0, c.stream>>>(
+            KScope ks(c, "k_seg_wave", bytes(2), true);
+            k_seg_wave<<<grid_groups(cb.count(2), kWave), kRowBS, 0, c.stream>>>(
                 cb.list(2), cb.count(2), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt,
                 ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (cb.count(3)) {
-            KScope ks(c, "k_seg_wave16", bytes(3), true);
-            k_seg_wave<16><<<grid_groups(cb.count(3), kWave), kRowBS, 0, c.stream>>>(
-                cb.list(3), cb.count(3), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt,
-                ks.dev());
-            HGP_LAUNCH_CHECK();
-        }
-        if (cb.count(4)) {
-            KScope ks(c, "k_seg_lds", bytes(4), true);
-            unsigned blocks = (unsigned)std::min<int64_t>(cb.count(4), 8192);
+            KScope ks(c, "k_seg_lds", bytes(3), true);
+            unsigned blocks = (unsigned)std::min<int64_t>(cb.count(3), 8192);
             const int passes = std::max(1, (bits_for((uint64_t)cn) + 7) / 8);   // coarse ids < 2^(8*passes) - 1
-            k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(4), cb.count(4), rep, match, g.xadj, g.adjncy,
+            k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(3), cb.count(3), rep, match, g.xadj, g.adjncy,
                                                        g.adjwgt, cmap, toff, tadj, twgt, ccnt, passes, ks.dev());
             HGP_LAUNCH_CHECK();
         }

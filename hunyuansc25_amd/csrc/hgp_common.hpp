@@ -79,7 +79,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     Arena arena;
     int64_t *pinned = nullptr;     // pinned host buffer: scalar read-backs + upload staging
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[8] = {};         // 0-3: phase times of a partition; 7: read_scalars_start
 };
 
 constexpr size_t kPinnedBytes = 1 << 20;
@@ -136,6 +136,12 @@ class KScope {
 
 // Copies `bytes` from device to the pinned buffer and waits.
 void read_scalars(Ctx &c, const void *dsrc, size_t bytes);
+
+// The same copy in two halves: start queues it (and an event behind it),
+// wait blocks until it has landed; work queued in between keeps the device
+// busy meanwhile.  The pinned buffer must not be reused in between.
+void read_scalars_start(Ctx &c, const void *dsrc, size_t bytes);
+void read_scalars_wait(Ctx &c);
 
 // Host-to-device copy through the pinned staging area; waits for the copy, so
 // the host source may die right after the call.

@@ -1480,7 +1480,7 @@ void for_small_bins(const RowBins &b, F f)
     if (b.count(2)) f(std::integral_constant<int, 64>{}, b.list(2), b.count(2));
 }
 
-inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, 16384); }
+inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, 65536); }
 
 __global__ void k_half(int64_t *d) { *d /= 2; }
 
@@ -2118,10 +2118,13 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         HGP_LAUNCH_CHECK();
     }
     for (; !(n <= kTinyUse && k <= kTinyK);) {
-        read_scalars(c, ctl, sizeof(JetCtl));
+        // the selection does not depend on the mode on the host: it runs while
+        // the control block travels (and does nothing once the loop is done)
+        read_scalars_start(c, ctl, sizeof(JetCtl));
+        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32);
+        read_scalars_wait(c);
         JetCtl h = *(JetCtl *)c.pinned;
         if (h.done) break;
-        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32);
         if (h.mode == kModeLP) {
             {
                 // fixed: ids, pregain, sel (9 B per row); candidates add xadj, where, dest and 16 B per entry

@@ -217,4 +217,12 @@ void read_scalars(Ctx &c, const void *dsrc, size_t bytes)
     HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
 }
 
+void read_scalars_start(Ctx &c, const void *dsrc, size_t bytes)
+{
+    HGP_CHECK_HIP(hipMemcpyAsync(c.pinned, dsrc, bytes, hipMemcpyDeviceToHost, c.stream));
+    HGP_CHECK_HIP(hipEventRecord(c.ev[7], c.stream));
+}
+
+void read_scalars_wait(Ctx &c) { HGP_CHECK_HIP(hipEventSynchronize(c.ev[7])); }
+
 }  // namespace hgp
