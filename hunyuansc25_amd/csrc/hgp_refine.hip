@@ -404,8 +404,6 @@ __device__ inline void sel_count(const SelArgs &A, int32_t mode, int32_t ncand)
     if (__syncthreads_or(ncand > 0) && threadIdx.x == 0) A.ctl->ncand = 1;
 }
 
-// Orders this wave's LDS accesses before and after the point (the wave runs
-// in lockstep, so completing its own LDS operations is enough).
 // Part flags of the iteration for the block: tg[q] = 1 if q may receive a
 // vertex (sel_target), read once per block instead of once per entry.
 __device__ inline void load_targets(const SelArgs &A, int32_t mode, int64_t maxwgt, uint8_t *tg)
@@ -578,23 +576,30 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_q(const int32_t *__restri
 
 // Rows of 9..512 entries (classes 1 and 2): a wave takes 64 rows of the list
 // at a time, decides the ineligible ones lane by lane and then walks the
-// eligible ones, one row at a time with its entries in registers (8 per
-// lane, loaded in phases) and a k-int LDS accumulator per wave.  Waves loop
-// on their own, without block barriers.
+// eligible ones.  With Q (k <= kQuarterK) the rows of <= kQuarterRow entries
+// go four at a time, one per 16-lane quarter with 8 entries per lane and its
+// own k-int LDS row, so that one chain of loads and gathers serves four rows;
+// the longer rows (and all rows without Q) go one at a time with their
+// entries in registers (8 per lane, loaded in phases) and a k-int LDS row per
+// wave.  Waves loop on their own, without block barriers.
+constexpr int64_t kQuarterRow = kQuarterG * 8;
+
+template <bool Q>
 __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restrict__ ids, int64_t cnt, int32_t rpw,
                                                          SelArgs A)
 {
-    extern __shared__ int32_t acc[];   // k ints per wave, then k part flags
-    constexpr int S = (int)(kBinMid / kWave);
+    extern __shared__ int32_t acc[];   // k ints per wave (4k with Q), then k part flags
+    constexpr int S = (int)(kBinMid / kWave), NQ = Q ? kWave / kQuarterG : 1;
     const JetCtl &C = *A.ctl;
     if (C.done) return;
     const int32_t mode = C.mode;
     const int64_t maxwgt = C.maxwgt, opt = C.opt;
     const int k = A.k, lane = lane_id();
     const int wv = threadIdx.x / kWave, nw = kRowBS / kWave;
-    int32_t *my = acc + (size_t)wv * k;
-    uint8_t *tg = (uint8_t *)(acc + (size_t)nw * k);
-    for (int i = lane; i < k; i += kWave) my[i] = 0;
+    const int quarter = lane / kQuarterG, gl = lane % kQuarterG;
+    int32_t *my = acc + (size_t)wv * NQ * k;
+    uint8_t *tg = (uint8_t *)(acc + (size_t)nw * NQ * k);
+    for (int i = lane; i < NQ * k; i += kWave) my[i] = 0;
     load_targets(A, mode, maxwgt, tg);
     int32_t ncand = 0;
     uint64_t kbytes = 0;
@@ -611,6 +616,59 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
             me = A.xadj[mv + 1];
         }
         uint64_t todo = __ballot(mel);
+        if constexpr (Q) {
+            uint64_t four = __ballot(mel && me - mb <= kQuarterRow);
+            todo &= ~four;
+            while (four) {
+                int ql = -1;   // the lane (row) of this lane's quarter, -1 when the rows ran out
+#pragma unroll
+                for (int g = 0; g < NQ; g++) {
+                    const int l = four ? __ffsll((long long)four) - 1 : -1;
+                    if (l >= 0) four &= four - 1;
+                    if (g == quarter) ql = l;
+                }
+                const int src = ql >= 0 ? ql : 0;
+                const int32_t v = __shfl(mv, src, kWave);
+                const SelRow rr = shfl_row(mr, src);
+                const int32_t p = rr.p;
+                const eid_t b = __shfl(mb, src, kWave), e1 = __shfl(me, src, kWave);   // every lane shuffles
+                const eid_t e = ql >= 0 ? e1 : b;
+                int32_t u[8], w[8], q[8];
+                load_entries<8, kQuarterG>(A.adjncy, A.adjwgt, b, e, gl, u, w);
+                A.wq.gather(u, q);
+                const bool lead = gl == 0 && ql >= 0;
+                if (lead) kbytes += 16 + 12 * (uint64_t)(e - b);
+                if (mode == kModeRS) {
+                    int32_t own = 0;
+#pragma unroll
+                    for (int t = 0; t < 8; t++) own += q[t] == p ? w[t] : 0;
+                    own = group_sum<kQuarterG>(own);
+                    if (lead) sel_decide(A, C, mode, v, rr, true, own, PartKey{0, -1});
+                    continue;
+                }
+                int32_t *mq = my + (size_t)quarter * k;
+#pragma unroll
+                for (int t = 0; t < 8; t++)
+                    if (q[t] >= 0) atomicAdd(&mq[q[t]], w[t]);
+                wave_lds_sync();
+                const int32_t pc = ql >= 0 ? mq[p] : 0;
+                PartKey best{0, -1};
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    if (q[t] >= 0 && q[t] != p && tg[q[t]]) {
+                        PartKey cand{mq[q[t]], q[t]};
+                        if (BetterPart()(cand, best)) best = cand;
+                    }
+                }
+                best = group_best<kQuarterG>(best, BetterPart());
+                wave_lds_sync();
+#pragma unroll
+                for (int t = 0; t < 8; t++)
+                    if (q[t] >= 0) mq[q[t]] = 0;
+                wave_lds_sync();
+                if (lead) ncand += sel_decide(A, C, mode, v, rr, true, pc, best);
+            }
+        }
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
             todo &= todo - 1;
@@ -869,8 +927,9 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
 }
 
 // Rows of 33..512 entries: 64 rows per wave at a time; non-candidates are
-// settled lane by lane, candidates walked one row at a time by the wave with
-// the row's entries loaded in phases (8 per lane).
+// settled lane by lane, candidates of <= kQuarterRow entries four at a time
+// (a 16-lane quarter and 8 entries per lane each), longer ones one row at a
+// time by the wave with the row's entries loaded in phases (8 per lane).
 __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restrict__ ids, int64_t cnt,
                                                           const eid_t *__restrict__ xadj,
                                                           const vid_t *__restrict__ adjncy,
@@ -902,6 +961,37 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
             md = dest[mv];
         }
         uint64_t todo = __ballot(live);
+        // rows of <= kQuarterRow entries four at a time, a 16-lane quarter each
+        uint64_t four = __ballot(live && me - mb <= kQuarterRow);
+        todo &= ~four;
+        while (four) {
+            int ql = -1;
+#pragma unroll
+            for (int g = 0; g < kWave / kQuarterG; g++) {
+                const int l = four ? __ffsll((long long)four) - 1 : -1;
+                if (l >= 0) four &= four - 1;
+                if (g == lane / kQuarterG) ql = l;
+            }
+            const int src = ql >= 0 ? ql : 0;
+            const int32_t v = __shfl(mv, src, kWave), ig = __shfl(mg, src, kWave);
+            const int32_t p = __shfl(mp, src, kWave), best = __shfl(md, src, kWave);
+            const eid_t b = __shfl(mb, src, kWave), e1 = __shfl(me, src, kWave);   // every lane shuffles
+            const eid_t e = ql >= 0 ? e1 : b;
+            int32_t u[8], w[8];
+            uint64_t x[8];
+            load_entries<8, kQuarterG>(adjncy, adjwgt, b, e, lane % kQuarterG, u, w);
+#pragma unroll
+            for (int t = 0; t < 8; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
+            int64_t change = 0;
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                if (u[t] >= 0) change += ab_change(v, ig, p, best, u[t], w[t], x[t]);
+            change = group_sum<kQuarterG>(change);
+            if (lane % kQuarterG == 0 && ql >= 0) {
+                sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
+                kbytes += 24 + 16 * (uint64_t)(e - b);
+            }
+        }
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
             todo &= todo - 1;
@@ -1269,7 +1359,9 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_g(const int32_t *__restri
 }
 
 // Rows of 33..512 entries: 64 rows per wave at a time, only the moved ones
-// walked, each with its entries loaded in phases (8 per lane).
+// walked: rows of <= kQuarterRow entries four at a time (a 16-lane quarter
+// and 8 entries per lane each), longer ones one at a time with their entries
+// loaded in phases (8 per lane).
 __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restrict__ ids, int64_t cnt, int32_t k,
                                                          const eid_t *__restrict__ xadj,
                                                          const vid_t *__restrict__ adjncy,
@@ -1307,6 +1399,28 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
             kbytes += 20 + 16 * (uint64_t)(me - mb);
         }
         uint64_t todo = __ballot(moved);
+        // rows of <= kQuarterRow entries four at a time, a 16-lane quarter each
+        uint64_t four = __ballot(moved && me - mb <= kQuarterRow);
+        todo &= ~four;
+        while (four) {
+            int ql = -1;
+#pragma unroll
+            for (int g = 0; g < kWave / kQuarterG; g++) {
+                const int l = four ? __ffsll((long long)four) - 1 : -1;
+                if (l >= 0) four &= four - 1;
+                if (g == lane / kQuarterG) ql = l;
+            }
+            const int src = ql >= 0 ? ql : 0;
+            const int32_t pv = __shfl(mpv, src, kWave), dv = __shfl(mdv, src, kWave);
+            const eid_t b = __shfl(mb, src, kWave), e1 = __shfl(me, src, kWave);   // every lane shuffles
+            const eid_t e = ql >= 0 ? e1 : b;
+            int32_t u[8], w[8], pu[8], du[8];
+            load_entries<8, kQuarterG>(adjncy, adjwgt, b, e, lane % kQuarterG, u, w);
+            pq.gather(u, pu, du);
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
+        }
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
             todo &= todo - 1;
@@ -1480,7 +1594,7 @@ void for_small_bins(const RowBins &b, F f)
     if (b.count(2)) f(std::integral_constant<int, 64>{}, b.list(2), b.count(2));
 }
 
-inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, 65536); }
+inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, 16384); }
 
 __global__ void k_half(int64_t *d) { *d /= 2; }
 
@@ -1526,6 +1640,12 @@ inline unsigned wave_blocks(int64_t rows, int32_t rpw, int64_t cap)
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, cap));
 }
 
+// LDS of k_jet_select_w: a k-int row per wave, or per quarter wave with Q.
+inline size_t sel_w_lds(int32_t k, bool q)
+{
+    return sizeof(int32_t) * (kRowBS / kWave) * (q ? kWave / kQuarterG : 1) * (size_t)k;
+}
+
 void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, const int64_t *pwgts,
                    const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest, int32_t *pregain,
                    uint32_t *key, int32_t *conn, NarrowParts wq, uint64_t *ab, uint16_t *pd16, uint32_t *pd32)
@@ -1541,8 +1661,12 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
             KScope ks(c, "k_jet_select_few", 24.0 * (double)nw, true);
             A.kb = ks.dev();
             int32_t rpw = rows_per_wave(nw);
-            k_jet_select_w<<<wave_blocks(nw, rpw, 4096), kRowBS, sizeof(int32_t) * 4 * (size_t)k + tg_bytes,
-                             c.stream>>>(b.list(0), nw, rpw, A);
+            if (k <= kQuarterK)
+                k_jet_select_w<true><<<wave_blocks(nw, rpw, 4096), kRowBS, sel_w_lds(k, true) + tg_bytes, c.stream>>>(
+                    b.list(0), nw, rpw, A);
+            else
+                k_jet_select_w<false><<<wave_blocks(nw, rpw, 4096), kRowBS, sel_w_lds(k, false) + tg_bytes,
+                                        c.stream>>>(b.list(0), nw, rpw, A);
             HGP_LAUNCH_CHECK();
         }
     }
@@ -1570,8 +1694,13 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
         KScope ks(c, "k_jet_select_w", 24.0 * (double)nw, true);
         A.kb = ks.dev();
         int32_t rpw = rows_per_wave(nw);
-        k_jet_select_w<<<wave_blocks(nw, rpw, 4096), kRowBS, sizeof(int32_t) * 4 * (size_t)k + tg_bytes, c.stream>>>(
-            quarter ? b.list(2) : b.list(1), nw, rpw, A);
+        const int32_t *list = quarter ? b.list(2) : b.list(1);
+        if (k <= kQuarterK)
+            k_jet_select_w<true><<<wave_blocks(nw, rpw, 4096), kRowBS, sel_w_lds(k, true) + tg_bytes, c.stream>>>(
+                list, nw, rpw, A);
+        else
+            k_jet_select_w<false><<<wave_blocks(nw, rpw, 4096), kRowBS, sel_w_lds(k, false) + tg_bytes, c.stream>>>(
+                list, nw, rpw, A);
         HGP_LAUNCH_CHECK();
     }
     if (g.segs.nseg) {

@@ -258,6 +258,21 @@ def test_contract_wide_coarse_ids(hgp, orc):
         np.testing.assert_array_equal(g_arr, e_arr, err_msg=what)
 
 
+def test_pipeline_quarter_rows(hgp, orc):
+    """About 330k rows of 33..128 entries: the wave-per-row kernels take 41
+    rows per wave chunk and walk them four at a time, one per 16-lane quarter,
+    so the last round of a chunk often has fewer rows than quarters and its
+    rows sit in lanes of the idle quarters."""
+    n = 400000
+    x, a, vw, aw = hgp.gen_rgg3d(n, hgp.rgg_radius(n, 40), seed=33, vw=(1, 3), ew=(1, 7))
+    deg = np.diff(x)
+    assert ((deg > 32) & (deg <= 128)).sum() > 16 * 8192
+    p_g, st_g = hgp.partition(x, a, 32, vw, aw, seed=2)
+    p_o, st_o = orc.dga_partition(x, a, 32, vw, aw, seed=2)
+    np.testing.assert_array_equal(p_g, p_o)
+    assert st_g["cut"] == st_o["cut"] == orc.edge_cut(x, a, aw, p_g)
+
+
 def test_pipeline_hub_graph(hgp, orc):
     x, a, vw, aw = _hub_graph(hgp)
     p_g, st_g = hgp.partition(x, a, 64, vw, aw, seed=2)
