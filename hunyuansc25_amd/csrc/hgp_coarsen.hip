@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(kRowBS) k_propose(const int32_t *__restrict__ 
 // Class-2 rows by segments (a hub row is shared by several blocks): the
 // segment of a single-segment row writes the proposal; the segments of a
 // multi row leave their best edge in sbest for k_propose_fin.
-__global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *__restrict__ xadj,
+__global__ void __launch_bounds__(kSegBlk) k_propose_seg(RowSegs S, const eid_t *__restrict__ xadj,
                                                         const vid_t *__restrict__ adjncy,
                                                         const wgt_t *__restrict__ adjwgt,
                                                         const int32_t *__restrict__ avail, int32_t maxvwgt,
@@ -155,9 +155,9 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
         const uint64_t tv = mix64(s1 ^ (uint64_t)(uint32_t)v);
         constexpr int T = 8;   // entries per thread and phase
-        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kSegBlk) {
             int32_t u[T], w[T], wu[T];
-            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+            load_entries<T, kSegBlk>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
             gather_slots<T>(avail, u, wu, -1);
 #pragma unroll
             for (int t = 0; t < T; t++) {
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(kRowBS) k_propose_seg(RowSegs S, const eid_t *
                 if (BetterEdge()(e, best)) best = e;
             }
         }
-        best = group_best<kRowBS>(best, BetterEdge());
+        best = group_best<kSegBlk>(best, BetterEdge());
         if (threadIdx.x == 0) {
             if (mi < 0) prop[v] = best.w < 0 ? -1 : (best.lo == v ? best.hi : best.lo);
             else sbest[s] = best;
@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(kRowBS) k_rsum_g(const int32_t *__restrict__ i
     if (lane == 0) rsum[v] = sum;
 }
 
-__global__ void __launch_bounds__(kRowBS) k_rsum_seg(RowSegs S, const eid_t *__restrict__ xadj,
+__global__ void __launch_bounds__(kSegBlk) k_rsum_seg(RowSegs S, const eid_t *__restrict__ xadj,
                                                      const vid_t *__restrict__ adjncy,
                                                      const wgt_t *__restrict__ adjwgt,
                                                      const int32_t *__restrict__ match,
@@ -366,13 +366,13 @@ __global__ void __launch_bounds__(kRowBS) k_rsum_seg(RowSegs S, const eid_t *__r
         if (match[v] != -1) continue;   // block-uniform
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
         int64_t sum = 0;
-        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kSegBlk) {
             int32_t u[T], w[T];
-            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+            load_entries<T, kSegBlk>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
 #pragma unroll
             for (int t = 0; t < T; t++) sum += u[t] >= 0 ? u[t] : 0;
         }
-        sum = group_sum<kRowBS>(sum);
+        sum = group_sum<kSegBlk>(sum);
         if (threadIdx.x == 0 && sum) atomicAdd(&rsum[v], (unsigned long long)sum);
     }
 }
@@ -490,7 +490,7 @@ __global__ void __launch_bounds__(kRowBS) k_rel_dist(const int32_t *__restrict__
 // Long rows by segments: a single-segment row writes dist, a multi row leaves
 // the best of each segment in dbest for k_rel_dist_fin.  (Degrees are
 // differences of xadj entries, exact in 32 bits for rows of < 2^31 entries.)
-__global__ void __launch_bounds__(kRowBS) k_rel_dist_seg(RowSegs S, const eid_t *__restrict__ xadj,
+__global__ void __launch_bounds__(kSegBlk) k_rel_dist_seg(RowSegs S, const eid_t *__restrict__ xadj,
                                                          const vid_t *__restrict__ adjncy,
                                                          const wgt_t *__restrict__ adjwgt,
                                                          const int32_t *__restrict__ match,
@@ -507,10 +507,10 @@ __global__ void __launch_bounds__(kRowBS) k_rel_dist_seg(RowSegs S, const eid_t 
         const eid_t b = xadj[v];
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
         DistKey best{-1, 0, 0, -1};
-        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kSegBlk) {
             int32_t u[T], w[T];
             int64_t d0[T], d1[T];
-            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+            load_entries<T, kSegBlk>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
 #pragma unroll
             for (int t = 0; t < T; t++) {
                 d0[t] = u[t] >= 0 ? xadj[u[t]] : 0;
@@ -519,11 +519,11 @@ __global__ void __launch_bounds__(kRowBS) k_rel_dist_seg(RowSegs S, const eid_t 
 #pragma unroll
             for (int t = 0; t < T; t++) {
                 if (u[t] < 0) continue;
-                DistKey k{(int32_t)(d1[t] - d0[t]), w[t], (int32_t)(c0 + threadIdx.x + (eid_t)t * kRowBS - b), u[t]};
+                DistKey k{(int32_t)(d1[t] - d0[t]), w[t], (int32_t)(c0 + threadIdx.x + (eid_t)t * kSegBlk - b), u[t]};
                 if (BetterDist()(k, best)) best = k;
             }
         }
-        best = group_best<kRowBS>(best, BetterDist());
+        best = group_best<kSegBlk>(best, BetterDist());
         if (threadIdx.x == 0) {
             if (mi < 0) dist[v] = best.u;
             else dbest[s] = best;
@@ -1433,7 +1433,7 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
             if (g.segs.nseg) {
                 {
                     KScope ks(c, "k_propose_seg", 24.0 * (double)g.segs.nseg, true);
-                    k_propose_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
+                    k_propose_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kSegBlk, 0, c.stream>>>(
                         g.segs, g.xadj, g.adjncy, g.adjwgt, avail, maxvwgt, s1, prop, sbest, prev, r == 0, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
@@ -1507,7 +1507,7 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
                 HGP_LAUNCH_CHECK();
             }
             if (g.segs.nseg) {
-                k_rsum_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
+                k_rsum_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kSegBlk, 0, c.stream>>>(
                     g.segs, g.xadj, g.adjncy, g.adjwgt, match, (unsigned long long *)rsum);
                 HGP_LAUNCH_CHECK();
             }
@@ -1550,7 +1550,7 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
         }
         if (g.segs.nseg) {
             DistKey *dbest = c.arena.ralloc<DistKey>((size_t)g.segs.nseg);
-            k_rel_dist_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kRowBS, 0, c.stream>>>(
+            k_rel_dist_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kSegBlk, 0, c.stream>>>(
                 g.segs, g.xadj, g.adjncy, g.adjwgt, match, dist, dbest);
             HGP_LAUNCH_CHECK();
             if (g.segs.nmulti) {

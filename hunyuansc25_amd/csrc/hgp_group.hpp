@@ -2,9 +2,9 @@
 //
 // Row-parallel kernels take their vertices from degree classes (RowBins):
 // rows of <= 8 entries get 8-lane groups, rows of <= 512 entries one wave,
-// longer rows a 256-thread block.  Group reductions are xor-shuffles inside a
-// wave for G <= 64 and go through LDS across the block's four waves for
-// G = 256.  Every reduction here uses a strict total order (or an integer
+// longer rows one block per segment.  Group reductions are xor-shuffles
+// inside a wave for G <= 64 and go through LDS across the block's waves for
+// a whole block (G > 64).  Every reduction here uses a strict total order (or an integer
 // sum), so results do not depend on which lane saw which entry.
 #pragma once
 
@@ -15,7 +15,7 @@ namespace hgp {
 constexpr int kRowBS = 256;             // block size of row-parallel kernels
 // Degree classes of the rows of a level: 0 = <= 8 entries (8 lanes per row),
 // 1 = <= 32 (32 lanes), 2 = <= 512 (one wave), 3 = longer (segments of
-// kSegLen entries, a block each).
+// kSegLen entries, a block of kSegBlk threads each).
 constexpr int64_t kBinSmall = 8;
 constexpr int64_t kBinHalf = 32;
 constexpr int64_t kBinMid = 512;
@@ -46,8 +46,12 @@ RowBins row_bins(Ctx &c, int32_t n, const eid_t *xadj, const int64_t *len, bool 
 // Rows of class kClsLong cut into segments of at most kSegLen entries, so that a hub
 // row is shared by several blocks.  A segment of a single-segment row is
 // finished by the segment kernel itself; the segments of a "multi" row leave
-// partial results that a second kernel combines per row.
-constexpr int64_t kSegLen = 4096;
+// partial results that a second kernel combines per row.  A segment kernel's
+// block has kSegBlk threads (8 entries per thread and load phase): small
+// blocks keep many segments in flight per CU, which is what bounds these
+// kernels (each segment is a short chain of dependent loads).
+constexpr int64_t kSegLen = 2048;
+constexpr int kSegBlk = 128;
 
 struct RowSegs {
     int64_t nseg = 0;

@@ -721,22 +721,22 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_w(const int32_t *__restri
 }
 
 // Rows of > 512 entries by segments (RowSegs), a k-int LDS accumulator per
-// block.  The segment's entries stay in registers (kSegLen / kRowBS per
+// block.  The segment's entries stay in registers (kSegLen / kSegBlk per
 // thread, loaded in phases) for the accumulation, the arg-max and the
 // clearing.  A single-segment row is decided here; a multi row's segments add
 // their sums to its k-int row of conn, which k_jet_select_fin reads.
-__global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A, int32_t *__restrict__ conn)
+__global__ void __launch_bounds__(kSegBlk) k_jet_select_seg(RowSegs S, SelArgs A, int32_t *__restrict__ conn)
 {
     extern __shared__ int32_t acc[];   // k ints, then k part flags
-    constexpr int T = (int)(kSegLen / kRowBS) / 2;   // a segment in two phases of T entries per thread
-    constexpr eid_t kHalf = (eid_t)T * kRowBS;
+    constexpr int T = (int)(kSegLen / kSegBlk) / 2;   // a segment in two phases of T entries per thread
+    constexpr eid_t kHalf = (eid_t)T * kSegBlk;
     const JetCtl &C = *A.ctl;
     if (C.done) return;
     const int32_t mode = C.mode;
     const int64_t maxwgt = C.maxwgt, opt = C.opt;
     const int k = A.k;
     uint8_t *tg = (uint8_t *)(acc + k);
-    for (int i = threadIdx.x; i < k; i += kRowBS) acc[i] = 0;
+    for (int i = threadIdx.x; i < k; i += kSegBlk) acc[i] = 0;
     load_targets(A, mode, maxwgt, tg);
     int32_t ncand = 0;
     uint64_t kbytes = 0;
@@ -751,7 +751,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         const eid_t ee = eligible ? e : b;
         const bool two = ee - b > kHalf;
         int32_t u[T], w[T], q[T];
-        load_entries<T, kRowBS>(A.adjncy, A.adjwgt, b, min(ee, b + kHalf), threadIdx.x, u, w);
+        load_entries<T, kSegBlk>(A.adjncy, A.adjwgt, b, min(ee, b + kHalf), threadIdx.x, u, w);
         A.wq.gather(u, q);
         if (mode == kModeRS) {
             // only the connectivity to the own part: a block sum, no k-int row
@@ -759,12 +759,12 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
 #pragma unroll
             for (int t = 0; t < T; t++) own += q[t] == p ? w[t] : 0;
             if (two) {
-                load_entries<T, kRowBS>(A.adjncy, A.adjwgt, b + kHalf, ee, threadIdx.x, u, w);
+                load_entries<T, kSegBlk>(A.adjncy, A.adjwgt, b + kHalf, ee, threadIdx.x, u, w);
                 A.wq.gather(u, q);
 #pragma unroll
                 for (int t = 0; t < T; t++) own += q[t] == p ? w[t] : 0;
             }
-            own = group_sum<kRowBS>(own);
+            own = group_sum<kSegBlk>(own);
             if (threadIdx.x == 0) {
                 if (mi < 0) sel_decide(A, C, mode, v, rr, eligible, own, PartKey{0, -1});
                 else if (eligible && own) atomicAdd(&conn[(size_t)mi * k + p], own);
@@ -777,7 +777,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         for (int t = 0; t < T; t++)
             if (q[t] >= 0) atomicAdd(&acc[q[t]], w[t]);
         if (two) {
-            load_entries<T, kRowBS>(A.adjncy, A.adjwgt, b + kHalf, ee, threadIdx.x, u, w);
+            load_entries<T, kSegBlk>(A.adjncy, A.adjwgt, b + kHalf, ee, threadIdx.x, u, w);
             A.wq.gather(u, q);
 #pragma unroll
             for (int t = 0; t < T; t++)
@@ -789,7 +789,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
             if (eligible) {
                 int32_t *row = conn + (size_t)mi * k;
                 if (walk_parts) {
-                    for (int r = threadIdx.x; r < k; r += kRowBS) {
+                    for (int r = threadIdx.x; r < k; r += kSegBlk) {
                         int32_t x = acc[r];
                         if (x) {
                             atomicAdd(&row[r], x);
@@ -812,7 +812,7 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
         PartKey best{0, -1};
         if (eligible) {
             if (walk_parts) {
-                for (int r = threadIdx.x; r < k; r += kRowBS) {
+                for (int r = threadIdx.x; r < k; r += kSegBlk) {
                     if (r == p || !tg[r]) continue;
                     PartKey cand{acc[r], r};
                     if (BetterPart()(cand, best)) best = cand;
@@ -826,10 +826,10 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_seg(RowSegs S, SelArgs A,
                 }
             }
         }
-        best = group_best<kRowBS>(best, BetterPart());
+        best = group_best<kSegBlk>(best, BetterPart());
         if (eligible) {
             if (walk_parts) {
-                for (int r = threadIdx.x; r < k; r += kRowBS) acc[r] = 0;
+                for (int r = threadIdx.x; r < k; r += kSegBlk) acc[r] = 0;
             } else {
 #pragma unroll
                 for (int t = 0; t < T; t++)
@@ -1020,7 +1020,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
 // Long rows by segments: a single-segment row decides here, a multi row adds
 // its partial change to macc for k_afterburner_fin.  Entries in phases of 8
 // per thread.
-__global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid_t *__restrict__ xadj,
+__global__ void __launch_bounds__(kSegBlk) k_afterburner_seg(RowSegs S, const eid_t *__restrict__ xadj,
                                                             const vid_t *__restrict__ adjncy,
                                                             const wgt_t *__restrict__ adjwgt,
                                                             const int32_t *__restrict__ where,
@@ -1045,17 +1045,17 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_seg(RowSegs S, const eid
         int32_t p = where[v], best = dest[v];
         int64_t change = 0;
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
-        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kSegBlk) {
             int32_t u[T], w[T];
             uint64_t x[T];
-            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+            load_entries<T, kSegBlk>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
 #pragma unroll
             for (int t = 0; t < T; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
 #pragma unroll
             for (int t = 0; t < T; t++)
                 if (u[t] >= 0) change += ab_change(v, ig, p, best, u[t], w[t], x[t]);
         }
-        change = group_sum<kRowBS>(change);
+        change = group_sum<kSegBlk>(change);
         if (threadIdx.x == 0) {
             if (mi < 0) sel[v] = ((int64_t)ig + change >= 0) ? 1 : 0;
             else if (change) atomicAdd(&macc[mi], (unsigned long long)change);
@@ -1445,7 +1445,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
 
 // Long rows by segments; the part weights move with the row's first segment.
 // Entries in phases of 8 per thread.
-__global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k, const eid_t *__restrict__ xadj,
+__global__ void __launch_bounds__(kSegBlk) k_move_delta_seg(RowSegs S, int32_t k, const eid_t *__restrict__ xadj,
                                                            const vid_t *__restrict__ adjncy,
                                                            const wgt_t *__restrict__ adjwgt,
                                                            const wgt_t *__restrict__ vwgt,
@@ -1457,7 +1457,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
     __shared__ unsigned long long bdelta;
     if (ctl->done) return;
     constexpr int T = 8;
-    for (int i = threadIdx.x; i < k; i += kRowBS) pdelta[i] = 0;
+    for (int i = threadIdx.x; i < k; i += kSegBlk) pdelta[i] = 0;
     if (threadIdx.x == 0) bdelta = 0;
     __syncthreads();
     uint64_t kbytes = 0;
@@ -1467,9 +1467,9 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
         int32_t pv = where[v], dv = dest[v];
         if (pv == dv) continue;
         eid_t j0 = S.j0[s], j1 = min(j0 + (eid_t)kSegLen, xadj[v + 1]);
-        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kRowBS) {
+        for (eid_t c0 = j0; c0 < j1; c0 += (eid_t)T * kSegBlk) {
             int32_t u[T], w[T], pu[T], du[T];
-            load_entries<T, kRowBS>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
+            load_entries<T, kSegBlk>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
             pq.gather(u, pu, du);
 #pragma unroll
             for (int t = 0; t < T; t++)
@@ -1483,11 +1483,11 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_seg(RowSegs S, int32_t k,
             }
         }
     }
-    d2 = group_sum<kRowBS>(d2);
+    d2 = group_sum<kSegBlk>(d2);
     if (threadIdx.x == 0 && d2) atomicAdd(&bdelta, (unsigned long long)d2);
     kb_flush(kb, kbytes);
     __syncthreads();
-    for (int i = threadIdx.x; i < k; i += kRowBS)
+    for (int i = threadIdx.x; i < k; i += kSegBlk)
         if (pdelta[i]) atomicAdd((unsigned long long *)&pwgts[i], pdelta[i]);
     if (threadIdx.x == 0 && bdelta) atomicAdd(&ctl->delta2, bdelta);
 }
@@ -1569,7 +1569,7 @@ __global__ void __launch_bounds__(kRowBS) k_cut(const int32_t *__restrict__ ids,
     if (lane == 0 && s) atomicAdd(out, (unsigned long long)s);
 }
 
-__global__ void __launch_bounds__(kRowBS) k_cut_seg(RowSegs S, const eid_t *__restrict__ xadj,
+__global__ void __launch_bounds__(kSegBlk) k_cut_seg(RowSegs S, const eid_t *__restrict__ xadj,
                                                     const vid_t *__restrict__ adjncy,
                                                     const wgt_t *__restrict__ adjwgt,
                                                     const int32_t *__restrict__ where, unsigned long long *out)
@@ -1579,9 +1579,9 @@ __global__ void __launch_bounds__(kRowBS) k_cut_seg(RowSegs S, const eid_t *__re
         int32_t v = S.row[s];
         int32_t p = where[v];
         eid_t j1 = min(S.j0[s] + (eid_t)kSegLen, xadj[v + 1]);
-        for (eid_t j = S.j0[s] + threadIdx.x; j < j1; j += kRowBS) tot += where[adjncy[j]] != p ? adjwgt[j] : 0;
+        for (eid_t j = S.j0[s] + threadIdx.x; j < j1; j += kSegBlk) tot += where[adjncy[j]] != p ? adjwgt[j] : 0;
     }
-    tot = group_sum<kRowBS>(tot);
+    tot = group_sum<kSegBlk>(tot);
     if (threadIdx.x == 0 && tot) atomicAdd(out, (unsigned long long)tot);
 }
 
@@ -1707,7 +1707,7 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
         {
             KScope ks(c, "k_jet_select_seg", 24.0 * (double)g.segs.nseg, true);
             A.kb = ks.dev();
-            k_jet_select_seg<<<seg_grid(g.segs), kRowBS, sizeof(int32_t) * (size_t)k + tg_bytes, c.stream>>>(
+            k_jet_select_seg<<<seg_grid(g.segs), kSegBlk, sizeof(int32_t) * (size_t)k + tg_bytes, c.stream>>>(
                 g.segs, A, conn);
             HGP_LAUNCH_CHECK();
         }
@@ -1734,7 +1734,7 @@ void edge_cut(Ctx &c, const DevGraph &g, const int32_t *where, int64_t *d_cut)
         HGP_LAUNCH_CHECK();
     });
     if (g.segs.nseg) {
-        k_cut_seg<<<seg_grid(g.segs), kRowBS, 0, c.stream>>>(g.segs, g.xadj, g.adjncy, g.adjwgt, where,
+        k_cut_seg<<<seg_grid(g.segs), kSegBlk, 0, c.stream>>>(g.segs, g.xadj, g.adjncy, g.adjwgt, where,
                                                             (unsigned long long *)d_cut);
         HGP_LAUNCH_CHECK();
     }
@@ -2288,7 +2288,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                 if (g.segs.nseg) {
                     {
                         KScope ks(c, "k_afterburner_seg", 24.0 * (double)g.segs.nseg, true);
-                        k_afterburner_seg<<<seg_grid(g.segs), kRowBS, 0, c.stream>>>(
+                        k_afterburner_seg<<<seg_grid(g.segs), kSegBlk, 0, c.stream>>>(
                             g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel,
                             (unsigned long long *)macc, ks.dev());
                         HGP_LAUNCH_CHECK();
@@ -2358,8 +2358,8 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         }
         if (g.segs.nseg) {
             KScope ks(c, "k_move_delta_seg", 24.0 * (double)g.segs.nseg, true);
-            unsigned blocks = (unsigned)std::min<int64_t>(g.segs.nseg, 2048);
-            k_move_delta_seg<<<blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
+            unsigned blocks = (unsigned)std::min<int64_t>(g.segs.nseg, 4096);
+            k_move_delta_seg<<<blocks, kSegBlk, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
                 g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
