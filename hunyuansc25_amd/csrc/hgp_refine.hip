@@ -843,16 +843,18 @@ __global__ void __launch_bounds__(kSegBlk) k_jet_select_seg(RowSegs S, SelArgs A
     kb_flush(A.kb, kbytes);
 }
 
-// Multi rows: one block per row reads the summed k-int row, decides, clears it.
+// Multi rows: one wave per row reads the summed k-int row (k / 64 parts per
+// lane), decides, clears it.
 __global__ void __launch_bounds__(kRowBS) k_jet_select_fin(RowSegs S, SelArgs A, int32_t *__restrict__ conn)
 {
     const JetCtl &C = *A.ctl;
     if (C.done) return;
     const int32_t mode = C.mode;
     const int64_t maxwgt = C.maxwgt, opt = C.opt;
-    const int k = A.k;
+    const int k = A.k, lane = lane_id();
+    const int64_t nwv = (int64_t)gridDim.x * (kRowBS / kWave);
     int32_t ncand = 0;
-    for (int32_t t = blockIdx.x; t < S.nmulti; t += gridDim.x) {
+    for (int64_t t = ((int64_t)blockIdx.x * kRowBS + threadIdx.x) / kWave; t < S.nmulti; t += nwv) {
         int32_t v = S.mrow[t];
         const SelRow rr = sel_row(A, mode, v);
         const int32_t p = rr.p;
@@ -861,15 +863,15 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_fin(RowSegs S, SelArgs A,
         int32_t pc = eligible ? row[p] : 0;
         PartKey best{0, -1};
         if (eligible)
-            for (int q = threadIdx.x; q < k; q += kRowBS) {
+            for (int q = lane; q < k; q += kWave) {
                 if (q == p || !sel_target(A, mode, maxwgt, q)) continue;
                 PartKey cand{row[q], q};
                 if (BetterPart()(cand, best)) best = cand;
             }
-        best = group_best<kRowBS>(best, BetterPart());
+        best = group_best<kWave>(best, BetterPart());
         if (eligible)
-            for (int q = threadIdx.x; q < k; q += kRowBS) row[q] = 0;
-        if (threadIdx.x == 0) ncand += sel_decide(A, C, mode, v, rr, eligible, pc, best);
+            for (int q = lane; q < k; q += kWave) row[q] = 0;
+        if (lane == 0) ncand += sel_decide(A, C, mode, v, rr, eligible, pc, best);
     }
     sel_count(A, mode, ncand);
 }
@@ -1714,8 +1716,9 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
         A.kb = nullptr;
         if (g.segs.nmulti) {
             KScope ks(c, "k_jet_select_fin", (8.0 * k + 24.0) * (double)g.segs.nmulti);
-            k_jet_select_fin<<<(unsigned)std::min<int32_t>(g.segs.nmulti, 4096), kRowBS, 0, c.stream>>>(g.segs, A,
-                                                                                                      conn);
+            const int32_t rows_per_block = kRowBS / kWave;
+            k_jet_select_fin<<<(unsigned)std::min<int32_t>((g.segs.nmulti + rows_per_block - 1) / rows_per_block, 4096),
+                               kRowBS, 0, c.stream>>>(g.segs, A, conn);
             HGP_LAUNCH_CHECK();
         }
     }
