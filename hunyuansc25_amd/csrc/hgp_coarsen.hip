@@ -590,15 +590,17 @@ __global__ void k_rep(int32_t n, const eid_t *__restrict__ xadj, const wgt_t *__
 // and equal ids are merged by summing weights; the merged row is written at
 // toff[c] (the concatenation's offset) of tadj/twgt and its length to ccnt[c].
 // Rows of <= 64 entries sort in registers (bitonic over xor-shuffles), rows of
-// <= kSegLds entries in LDS (radix, one block per row), longer rows through
+// <= kSegBig entries in LDS (radix, one block per row), longer rows through
 // the device radix sort.
 
 constexpr uint32_t kSent = 0xFFFFFFFFu;   // dropped entry; sorts last
 constexpr int kSegWaveMax = 512;   // rows of 65..512 entries: one wave, up to 8 entries per lane
-constexpr int kClsSegLong = 5;      // contraction class of the rows longer than kSegLds
-constexpr int kSegLds = 4096;
+constexpr int kClsSegLong = 5;      // contraction class of the rows longer than kSegBig
+constexpr int kSegLds = 4096;       // rows of <= kSegLds entries: a 512-thread block, <= 8 entries per thread
 constexpr int kSegBS = 512;
-constexpr int kSegE = kSegLds / kSegBS;   // entries per thread in the LDS merge
+constexpr int kSegE = kSegLds / kSegBS;
+constexpr int kSegBig = 16384;      // rows of <= kSegBig entries: a 1024-thread block, <= 16 per thread (144 KB LDS)
+constexpr int kSegBigBS = 1024;
 
 // Entry i of coarse row c's concatenation (fine rows of v, then of u).
 struct CatRow {
@@ -750,7 +752,7 @@ __device__ inline void wave_bitonic(uint32_t (&key)[KR], int32_t (&val)[KR])
     }
 }
 
-// Stable LSD radix sort of a block's P = 8*R*64 entries by the low 8*passes
+// Stable LSD radix sort of a block's P = (BS/64)*R*64 entries by the low 8*passes
 // bits of the key (8-bit digits), entry w*C + r*64 + lane (C = R*64) held in
 // register r of lane `lane` of wave w.  Each pass ranks a wave's entries by
 // digit (entries of equal digit keep their order: register-major, then lane)
@@ -760,11 +762,11 @@ __device__ inline void wave_bitonic(uint32_t (&key)[KR], int32_t (&val)[KR])
 // registers for the next pass.  The last pass leaves the sorted entries in
 // sk/sw, P of them.  Keys beyond the low bits (the sentinel) must sort last
 // on those bits alone.
-template <int R>
+template <int R, int BS>
 __device__ inline void block_radix(uint32_t (&key)[R], int32_t (&val)[R], uint32_t *sk, int32_t *sw, int32_t *wc,
                                    int32_t *dsum, int passes)
 {
-    constexpr int C = R * kWave, NW = kSegBS / kWave;
+    constexpr int C = R * kWave, NW = BS / kWave;
     const int w = threadIdx.x / kWave, lane = lane_id();
     const uint64_t lt = lanemask_lt();
     int32_t *my = wc + w * 256;
@@ -906,11 +908,12 @@ __global__ void __launch_bounds__(kRowBS) k_seg_wave(const int32_t *__restrict__
     kb_flush(kb, kbytes);
 }
 
-// Rows of kSegWaveMax+1..kSegLds entries: one block per row, radix sort
-// through LDS (padded to 1024, 2048 or 4096 entries; `passes` 8-bit digits
-// cover every coarse id), then tails (last entry of each run of equal ids) are ranked by
+// Rows of up to BS*RMAX entries: one block of BS threads per row, radix sort
+// through LDS (padded to BS*RMAX, half or a quarter of it; `passes` 8-bit
+// digits cover every coarse id), then tails (last entry of each run of equal ids) are ranked by
 // a block scan and their inclusive weight prefixes differenced.
-__global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ ids, int64_t cnt,
+template <int BS, int RMAX>
+__global__ void __launch_bounds__(BS) k_seg_lds(const int32_t *__restrict__ ids, int64_t cnt,
                                                     const int32_t *__restrict__ rep,
                                                     const int32_t *__restrict__ match,
                                                     const eid_t *__restrict__ xadj, const vid_t *__restrict__ adjncy,
@@ -919,27 +922,28 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
                                                     int32_t *__restrict__ tadj, int32_t *__restrict__ twgt,
                                                     int32_t *__restrict__ ccnt, int passes, unsigned long long *kb)
 {
-    __shared__ uint32_t sk[kSegLds];
+    constexpr int PMAX = BS * RMAX;
+    __shared__ uint32_t sk[PMAX];
     uint64_t kbytes = 0;
-    __shared__ int32_t sw[kSegLds];
-    __shared__ int32_t wc[kSegBS / kWave * 256];
+    __shared__ int32_t sw[PMAX];
+    __shared__ int32_t wc[BS / kWave * 256];
     __shared__ int32_t dsum[4];
-    __shared__ int64_t sh[kSegBS / kWave + 1];
+    __shared__ int64_t sh[BS / kWave + 1];
     const int tid = threadIdx.x;
     for (int64_t gi = blockIdx.x; gi < cnt; gi += gridDim.x) {
         int32_t c = ids[gi];
         int32_t v = rep[c], u = match[v];
         CatRow row(xadj, v, u);
         int len = (int)(row.dv + (u != v ? xadj[u + 1] - row.bu : 0));
-        const int P = len <= kSegLds / 4 ? kSegLds / 4 : (len <= kSegLds / 2 ? kSegLds / 2 : kSegLds);
+        const int P = len <= PMAX / 4 ? PMAX / 4 : (len <= PMAX / 2 ? PMAX / 2 : PMAX);
         {
             // entries straight into registers (entry w*C + r*64 + lane), sorted by the block
-            const int R = P / kSegBS, w = tid / kWave, lane = lane_id();
-            uint32_t key[kSegE];
-            int32_t val[kSegE];
-            int32_t nb[kSegE];
+            const int R = P / BS, w = tid / kWave, lane = lane_id();
+            uint32_t key[RMAX];
+            int32_t val[RMAX];
+            int32_t nb[RMAX];
 #pragma unroll
-            for (int r = 0; r < kSegE; r++) {   // every index and weight load first, then the cmap gathers
+            for (int r = 0; r < RMAX; r++) {   // every index and weight load first, then the cmap gathers
                 int i = w * R * kWave + r * kWave + lane;
                 nb[r] = -1;
                 val[r] = 0;
@@ -950,35 +954,35 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
                 }
             }
 #pragma unroll
-            for (int r = 0; r < kSegE; r++) {
+            for (int r = 0; r < RMAX; r++) {
                 int32_t cc = nb[r] >= 0 ? cmap[nb[r]] : c;
                 key[r] = cc != c ? (uint32_t)cc : kSent;
                 val[r] = cc != c ? val[r] : 0;
             }
-            if (R == kSegE) {
-                block_radix<kSegE>(key, val, sk, sw, wc, dsum, passes);
-            } else if (R == kSegE / 2) {
-                uint32_t k4[kSegE / 2];
-                int32_t v4[kSegE / 2];
+            if (R == RMAX) {
+                block_radix<RMAX, BS>(key, val, sk, sw, wc, dsum, passes);
+            } else if (R == RMAX / 2) {
+                uint32_t k4[RMAX / 2];
+                int32_t v4[RMAX / 2];
 #pragma unroll
-                for (int r = 0; r < kSegE / 2; r++) { k4[r] = key[r]; v4[r] = val[r]; }
-                block_radix<kSegE / 2>(k4, v4, sk, sw, wc, dsum, passes);
+                for (int r = 0; r < RMAX / 2; r++) { k4[r] = key[r]; v4[r] = val[r]; }
+                block_radix<RMAX / 2, BS>(k4, v4, sk, sw, wc, dsum, passes);
             } else {
-                uint32_t k2[kSegE / 4];
-                int32_t v2[kSegE / 4];
+                uint32_t k2[RMAX / 4];
+                int32_t v2[RMAX / 4];
 #pragma unroll
-                for (int r = 0; r < kSegE / 4; r++) { k2[r] = key[r]; v2[r] = val[r]; }
-                block_radix<kSegE / 4>(k2, v2, sk, sw, wc, dsum, passes);
+                for (int r = 0; r < RMAX / 4; r++) { k2[r] = key[r]; v2[r] = val[r]; }
+                block_radix<RMAX / 4, BS>(k2, v2, sk, sw, wc, dsum, passes);
             }
         }
-        // Thread tid owns entries [tid*E, tid*E + E), E = P / kSegBS (one each when P < kSegBS).
-        const int E = P >= kSegBS ? P / kSegBS : 1;
+        // Thread tid owns entries [tid*E, tid*E + E), E = P / BS (one each when P < BS).
+        const int E = P >= BS ? P / BS : 1;
         const int first = tid * E;
-        uint32_t kk[kSegE];
-        int32_t ww[kSegE];
+        uint32_t kk[RMAX];
+        int32_t ww[RMAX];
         uint32_t after = first + E < P ? sk[first + E] : kSent;
 #pragma unroll
-        for (int e = 0; e < kSegE; e++) {
+        for (int e = 0; e < RMAX; e++) {
             int i = first + e;
             bool own = e < E && i < P;
             kk[e] = own ? sk[i] : kSent;
@@ -986,8 +990,8 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
         }
         int32_t nt = 0, wsum = 0;
 #pragma unroll
-        for (int e = 0; e < kSegE; e++) {
-            uint32_t nx = e + 1 < E ? kk[e + 1 < kSegE ? e + 1 : e] : after;
+        for (int e = 0; e < RMAX; e++) {
+            uint32_t nx = e + 1 < E ? kk[e + 1 < RMAX ? e + 1 : e] : after;
             if (e < E && kk[e] != kSent && nx != kk[e]) nt++;
             wsum += ww[e];
         }
@@ -997,8 +1001,8 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
         const int32_t ntot = (int32_t)(tot >> 32);
         __syncthreads();   // every entry has been read: sk/sw now hold the merged row
 #pragma unroll
-        for (int e = 0; e < kSegE; e++) {
-            uint32_t nx = e + 1 < E ? kk[e + 1 < kSegE ? e + 1 : e] : after;
+        for (int e = 0; e < RMAX; e++) {
+            uint32_t nx = e + 1 < E ? kk[e + 1 < RMAX ? e + 1 : e] : after;
             run += ww[e];
             if (e < E && kk[e] != kSent && nx != kk[e]) {
                 sk[rank] = kk[e];
@@ -1008,7 +1012,7 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
         }
         __syncthreads();
         const int64_t o = toff[c];
-        for (int r = tid; r < ntot; r += kSegBS) {
+        for (int r = tid; r < ntot; r += BS) {
             tadj[o + r] = (int32_t)sk[r];
             twgt[o + r] = sw[r] - (r > 0 ? sw[r - 1] : 0);
         }
@@ -1021,7 +1025,7 @@ __global__ void __launch_bounds__(kSegBS) k_seg_lds(const int32_t *__restrict__ 
     kb_flush(kb, kbytes);
 }
 
-// Rows longer than kSegLds, by coarse-id ranges: the entries of each long row
+// Rows longer than kSegBig, by coarse-id ranges: the entries of each long row
 // are counted and scattered into bins (row, id >> kBktBits); each bin is then
 // merged in a dense LDS array of kBkt weights with a presence bitmap and
 // emitted in id order.  Bins of one row are consecutive, so a row's merged
@@ -1223,7 +1227,7 @@ __global__ void k_lb_rowcnt(int64_t nl, int32_t nbk, const int32_t *__restrict__
     if (li < nl) ccnt[lids[li]] = (int32_t)(uoff[(li + 1) * nbk] - uoff[li * nbk]);
 }
 
-// Rows longer than kSegLds: keys (list index << 32 | coarse id) of all such
+// Rows longer than kSegBig: keys (list index << 32 | coarse id) of all such
 // rows, laid out at loff, go through the device radix sort.
 __global__ void __launch_bounds__(kRowBS) k_long_gather(const int32_t *__restrict__ ids, int64_t cnt, int32_t cn,
                                                         const int32_t *__restrict__ rep,
@@ -1626,7 +1630,7 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     int32_t *tadj = c.arena.ralloc<int32_t>((size_t)m);
     int32_t *twgt = c.arena.ralloc<int32_t>((size_t)m);
     int32_t *ccnt = c.arena.ralloc<int32_t>((size_t)cn);
-    RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegWaveMax, kSegLds, kSegLds);   // class 4 empty
+    RowBins cb = row_bins(c, cn, nullptr, len, false, 8, 64, kSegWaveMax, kSegLds, kSegBig);
     {
         // per coarse row: ids, rep, match, 2 xadj pairs, toff, count (56 B); per fine entry: adjncy, adjwgt,
         // cmap gather (12 B); the kernels count the merged entries they write (8 B each)
@@ -1654,8 +1658,18 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
             KScope ks(c, "k_seg_lds", bytes(3), true);
             unsigned blocks = (unsigned)std::min<int64_t>(cb.count(3), 8192);
             const int passes = std::max(1, (bits_for((uint64_t)cn) + 7) / 8);   // coarse ids < 2^(8*passes) - 1
-            k_seg_lds<<<blocks, kSegBS, 0, c.stream>>>(cb.list(3), cb.count(3), rep, match, g.xadj, g.adjncy,
-                                                       g.adjwgt, cmap, toff, tadj, twgt, ccnt, passes, ks.dev());
+            k_seg_lds<kSegBS, kSegE><<<blocks, kSegBS, 0, c.stream>>>(cb.list(3), cb.count(3), rep, match, g.xadj,
+                                                                      g.adjncy, g.adjwgt, cmap, toff, tadj, twgt,
+                                                                      ccnt, passes, ks.dev());
+            HGP_LAUNCH_CHECK();
+        }
+        if (cb.count(4)) {
+            KScope ks(c, "k_seg_lds_big", bytes(4), true);
+            unsigned blocks = (unsigned)std::min<int64_t>(cb.count(4), 4096);
+            const int passes = std::max(1, (bits_for((uint64_t)cn) + 7) / 8);
+            k_seg_lds<kSegBigBS, kSegBig / kSegBigBS><<<blocks, kSegBigBS, 0, c.stream>>>(
+                cb.list(4), cb.count(4), rep, match, g.xadj, g.adjncy, g.adjwgt, cmap, toff, tadj, twgt, ccnt, passes,
+                ks.dev());
             HGP_LAUNCH_CHECK();
         }
     }

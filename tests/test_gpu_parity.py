@@ -210,9 +210,9 @@ def test_invalid_inputs_raise(hgp):
 
 
 def _hub_graph(hgp):
-    """R-MAT with hubs of > 8k entries: its coarse rows reach every contraction
-    class (registers, wave registers, LDS, long-row bins) and its hub rows are
-    cut into several segments."""
+    """R-MAT with hubs of > 8k entries: its coarse rows reach the register,
+    wave-register and both LDS contraction classes (<= 4096 and <= 16384
+    entries), and its hub rows are cut into several segments."""
     x, a, vw, aw = hgp.gen_rmat(16, 1 << 20, seed=12)
     deg = np.diff(x)
     assert deg.max() > 2 * 4096
@@ -244,11 +244,13 @@ def test_contract_hub_rows(hgp, orc):
 
 
 def test_contract_wide_coarse_ids(hgp, orc):
-    """More than 2^16 coarse vertices: the block-sorted rows (1025..4096
-    entries) take three 8-bit digit passes."""
+    """More than 2^16 coarse vertices: the block-sorted rows (513..16384
+    entries) take three 8-bit digit passes; a row of > 16384 entries takes
+    the coarse-id bins."""
     x, a, vw, aw = hgp.gen_rmat(18, 1 << 22, seed=21)
     deg = np.diff(x)
     assert ((deg > 1024) & (deg <= 4096)).sum() > 20
+    assert ((deg > 4096) & (deg <= 16384)).sum() > 20 and (deg > 16384).sum() >= 1
     tv = int(vw.sum())
     mt, _ = orc.dga_match(x, a, vw, aw, orc.maxvwgt(tv, orc.coarsen_threshold(64)), 5, 0, 8)
     got = hgp.contract(x, a, aw, vw, mt)
