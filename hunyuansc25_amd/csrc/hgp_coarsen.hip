@@ -752,9 +752,10 @@ __device__ inline void wave_bitonic(uint32_t (&key)[KR], int32_t (&val)[KR])
     }
 }
 
-// Stable LSD radix sort of a block's P = (BS/64)*R*64 entries by the low 8*passes
-// bits of the key (8-bit digits), entry w*C + r*64 + lane (C = R*64) held in
-// register r of lane `lane` of wave w.  Each pass ranks a wave's entries by
+// Stable LSD radix sort of a block's P = (BS/64)*R*64 entries (registers
+// 0..R-1 of RMAX in use) by the low 8*passes bits of the key (8-bit digits),
+// entry w*C + r*64 + lane (C = R*64) held in register r of lane `lane` of
+// wave w.  Each pass ranks a wave's entries by
 // digit (entries of equal digit keep their order: register-major, then lane)
 // from 8 ballots per register and a per-wave count of each digit (wc, 256
 // ints per wave), turns the counts into scatter bases with a scan over
@@ -762,11 +763,12 @@ __device__ inline void wave_bitonic(uint32_t (&key)[KR], int32_t (&val)[KR])
 // registers for the next pass.  The last pass leaves the sorted entries in
 // sk/sw, P of them.  Keys beyond the low bits (the sentinel) must sort last
 // on those bits alone.
-template <int R, int BS>
-__device__ inline void block_radix(uint32_t (&key)[R], int32_t (&val)[R], uint32_t *sk, int32_t *sw, int32_t *wc,
-                                   int32_t *dsum, int passes)
+template <int RMAX, int BS>
+__device__ inline void block_radix(uint32_t (&key)[RMAX], int32_t (&val)[RMAX], int R, uint32_t *sk, int32_t *sw,
+                                   int32_t *wc, int32_t *dsum, int passes)
 {
-    constexpr int C = R * kWave, NW = BS / kWave;
+    constexpr int NW = BS / kWave;
+    const int C = R * kWave;
     const int w = threadIdx.x / kWave, lane = lane_id();
     const uint64_t lt = lanemask_lt();
     int32_t *my = wc + w * 256;
@@ -774,9 +776,10 @@ __device__ inline void block_radix(uint32_t (&key)[R], int32_t (&val)[R], uint32
         const int shift = 8 * pass;
         for (int i = lane; i < 256; i += kWave) my[i] = 0;
         wave_lds_sync();
-        int32_t rk[R];
+        int32_t rk[RMAX];
 #pragma unroll
-        for (int r = 0; r < R; r++) {
+        for (int r = 0; r < RMAX; r++) {
+            if (r >= R) break;   // R is the same for the whole block
             const uint32_t d = (key[r] >> shift) & 255u;
             uint64_t peers = ~0ull;   // lanes whose register r holds the same digit
 #pragma unroll
@@ -814,7 +817,8 @@ __device__ inline void block_radix(uint32_t (&key)[R], int32_t (&val)[R], uint32
         }
         __syncthreads();
 #pragma unroll
-        for (int r = 0; r < R; r++) {
+        for (int r = 0; r < RMAX; r++) {
+            if (r >= R) break;
             const int32_t at = wc[w * 256 + ((key[r] >> shift) & 255u)] + rk[r];
             sk[at] = key[r];
             sw[at] = val[r];
@@ -822,7 +826,8 @@ __device__ inline void block_radix(uint32_t (&key)[R], int32_t (&val)[R], uint32
         __syncthreads();
         if (pass + 1 < passes) {
 #pragma unroll
-            for (int r = 0; r < R; r++) {
+            for (int r = 0; r < RMAX; r++) {
+                if (r >= R) break;
                 key[r] = sk[w * C + r * kWave + lane];
                 val[r] = sw[w * C + r * kWave + lane];
             }
@@ -959,21 +964,7 @@ __global__ void __launch_bounds__(BS) k_seg_lds(const int32_t *__restrict__ ids,
                 key[r] = cc != c ? (uint32_t)cc : kSent;
                 val[r] = cc != c ? val[r] : 0;
             }
-            if (R == RMAX) {
-                block_radix<RMAX, BS>(key, val, sk, sw, wc, dsum, passes);
-            } else if (R == RMAX / 2) {
-                uint32_t k4[RMAX / 2];
-                int32_t v4[RMAX / 2];
-#pragma unroll
-                for (int r = 0; r < RMAX / 2; r++) { k4[r] = key[r]; v4[r] = val[r]; }
-                block_radix<RMAX / 2, BS>(k4, v4, sk, sw, wc, dsum, passes);
-            } else {
-                uint32_t k2[RMAX / 4];
-                int32_t v2[RMAX / 4];
-#pragma unroll
-                for (int r = 0; r < RMAX / 4; r++) { k2[r] = key[r]; v2[r] = val[r]; }
-                block_radix<RMAX / 4, BS>(k2, v2, sk, sw, wc, dsum, passes);
-            }
+            block_radix<RMAX, BS>(key, val, R, sk, sw, wc, dsum, passes);
         }
         // Thread tid owns entries [tid*E, tid*E + E), E = P / BS (one each when P < BS).
         const int E = P >= BS ? P / BS : 1;
