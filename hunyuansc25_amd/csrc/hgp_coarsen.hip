@@ -755,11 +755,12 @@ __device__ inline void wave_bitonic(uint32_t (&key)[KR], int32_t (&val)[KR])
 // Stable LSD radix sort of a block's P = (BS/64)*R*64 entries (registers
 // 0..R-1 of RMAX in use) by the low 8*passes bits of the key (8-bit digits),
 // entry w*C + r*64 + lane (C = R*64) held in register r of lane `lane` of
-// wave w.  Each pass ranks a wave's entries by
-// digit (entries of equal digit keep their order: register-major, then lane)
-// from 8 ballots per register and a per-wave count of each digit (wc, 256
-// ints per wave), turns the counts into scatter bases with a scan over
-// (digit, wave) and scatters into sk/sw; the entries are read back into
+// wave w.  Each pass after the first ranks a wave's entries by digit
+// (entries of equal digit keep their order: register-major, then lane) from
+// 8 ballots per register and a per-wave count of each digit (wc, 256 ints
+// per wave); the first pass, whose input order is arbitrary anyway, takes
+// its ranks from LDS atomics on the same counts.  The counts become scatter
+// bases by a scan over (digit, wave), and the entries scatter into sk/sw; the entries are read back into
 // registers for the next pass.  The last pass leaves the sorted entries in
 // sk/sw, P of them.  Keys beyond the low bits (the sentinel) must sort last
 // on those bits alone.
@@ -777,22 +778,33 @@ __device__ inline void block_radix(uint32_t (&key)[RMAX], int32_t (&val)[RMAX], 
         for (int i = lane; i < 256; i += kWave) my[i] = 0;
         wave_lds_sync();
         int32_t rk[RMAX];
+        if (pass == 0) {
+            // the first pass may order equal digits any way (the input order
+            // is arbitrary anyway): ranks from LDS atomics
 #pragma unroll
-        for (int r = 0; r < RMAX; r++) {
-            if (r >= R) break;   // R is the same for the whole block
-            const uint32_t d = (key[r] >> shift) & 255u;
-            uint64_t peers = ~0ull;   // lanes whose register r holds the same digit
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const bool bit = (d >> b) & 1u;
-                const uint64_t bb = __ballot(bit);
-                peers &= bit ? bb : ~bb;
+            for (int r = 0; r < RMAX; r++) {
+                if (r >= R) break;
+                rk[r] = atomicAdd(&my[(key[r] >> shift) & 255u], 1);
             }
-            const int32_t base = my[d];
-            rk[r] = base + __popcll(peers & lt);
-            wave_lds_sync();   // every lane has its base
-            if ((peers & lt) == 0) my[d] = base + __popcll(peers);
             wave_lds_sync();
+        } else {
+#pragma unroll
+            for (int r = 0; r < RMAX; r++) {
+                if (r >= R) break;   // R is the same for the whole block
+                const uint32_t d = (key[r] >> shift) & 255u;
+                uint64_t peers = ~0ull;   // lanes whose register r holds the same digit
+#pragma unroll
+                for (int b = 0; b < 8; b++) {
+                    const bool bit = (d >> b) & 1u;
+                    const uint64_t bb = __ballot(bit);
+                    peers &= bit ? bb : ~bb;
+                }
+                const int32_t base = my[d];
+                rk[r] = base + __popcll(peers & lt);
+                wave_lds_sync();   // every lane has its base
+                if ((peers & lt) == 0) my[d] = base + __popcll(peers);
+                wave_lds_sync();
+            }
         }
         __syncthreads();
         // base of (digit d, wave x): entries of smaller digits, then of digit d in waves < x
