@@ -10,6 +10,7 @@ namespace {
 
 constexpr int kScanBS = 256;    // threads per block
 constexpr int kScanR = 8;       // rounds per tile: tile = 2048 items
+constexpr int64_t kRawTiles = 2048;   // multi-tile scans of up to this many tiles take two launches
 
 // Per-tile sums of a striped tile (item r*256+t of the tile in round r).
 template <class T>
@@ -36,11 +37,23 @@ __global__ void __launch_bounds__(kScanBS) k_tile_reduce(const T *__restrict__ i
 // Inclusive scan of each tile plus the inclusive prefix of the previous
 // tiles (incl_sums[tile-1]).
 template <class T>
-__global__ void __launch_bounds__(kScanBS) k_tile_scan(const T *in, T *out, int64_t n, const T *__restrict__ incl_sums)
+__global__ void __launch_bounds__(kScanBS) k_tile_scan(const T *in, T *out, int64_t n, const T *__restrict__ sums,
+                                                      int raw)
 {
     __shared__ T sh[kScanBS / kWave + 1];
     int64_t base = (int64_t)blockIdx.x * (kScanBS * kScanR);
-    T carry = (incl_sums && blockIdx.x > 0) ? incl_sums[blockIdx.x - 1] : (T)0;
+    T carry = 0;
+    if (sums && blockIdx.x > 0) {
+        if (raw) {
+            // sums[t] = total of tile t (not scanned): the block adds those of
+            // the tiles before its own (few tiles, so this costs less than a launch)
+            T part = 0;
+            for (int i = threadIdx.x; i < (int)blockIdx.x; i += kScanBS) part += sums[i];
+            block_exclusive_scan(part, sh, &carry);
+        } else {
+            carry = sums[blockIdx.x - 1];   // inclusive scan of the tile totals
+        }
+    }
     for (int r = 0; r < kScanR; r++) {
         int64_t i = base + r * kScanBS + threadIdx.x;
         T v = i < n ? in[i] : (T)0;
@@ -59,7 +72,7 @@ void scan_impl(Ctx &c, const T *in, T *out, int64_t n)
     int64_t tile = kScanBS * kScanR;
     int64_t tiles = (n + tile - 1) / tile;
     if (tiles == 1) {
-        k_tile_scan<T><<<1, kScanBS, 0, c.stream>>>(in, out, n, nullptr);
+        k_tile_scan<T><<<1, kScanBS, 0, c.stream>>>(in, out, n, nullptr, 0);
         HGP_LAUNCH_CHECK();
         return;
     }
@@ -67,10 +80,14 @@ void scan_impl(Ctx &c, const T *in, T *out, int64_t n)
     T *sums = c.arena.ralloc<T>((size_t)tiles);
     k_tile_reduce<T><<<(unsigned)tiles, kScanBS, 0, c.stream>>>(in, n, sums);
     HGP_LAUNCH_CHECK();
-    scan_impl<T>(c, sums, sums, tiles);
+    // up to kRawTiles tiles each block sums the tile totals before it itself;
+    // beyond, the totals are scanned first (the per-block sums would grow
+    // with the square of the tile count)
+    const bool raw = tiles <= kRawTiles;
+    if (!raw) scan_impl<T>(c, sums, sums, tiles);
     {
         KScope ks(c, sizeof(T) == 8 ? "k_tile_scan<i64>" : "k_tile_scan<i32>", 2.0 * sizeof(T) * (double)n);
-        k_tile_scan<T><<<(unsigned)tiles, kScanBS, 0, c.stream>>>(in, out, n, sums);
+        k_tile_scan<T><<<(unsigned)tiles, kScanBS, 0, c.stream>>>(in, out, n, sums, raw ? 1 : 0);
         HGP_LAUNCH_CHECK();
     }
     c.arena.right_release(mark);
