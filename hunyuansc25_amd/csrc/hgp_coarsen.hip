@@ -1315,7 +1315,8 @@ __global__ void k_long_count(int64_t cnt, const int32_t *__restrict__ ids, const
     ccnt[ids[li]] = (e > b ? hsc[e - 1] : before) - before;
 }
 
-// Moves each merged row from toff[c] to cxadj[c].
+// Moves each merged row from toff[c] to cxadj[c], four loads per lane in
+// flight before the stores.
 template <int G>
 __global__ void __launch_bounds__(kRowBS) k_seg_compact(const int32_t *__restrict__ ids, int64_t cnt,
                                                         const int64_t *__restrict__ toff,
@@ -1331,7 +1332,21 @@ __global__ void __launch_bounds__(kRowBS) k_seg_compact(const int32_t *__restric
         int32_t c = ids[gi];
         int64_t src = toff[c];
         eid_t dst = cxadj[c], l = cxadj[c + 1] - dst;
-        for (eid_t j = lane; j < l; j += G) {
+        eid_t j = lane;
+        for (; j + 3 * G < l; j += 4 * G) {
+            int32_t a[4], w[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                a[t] = tadj[src + j + t * G];
+                w[t] = twgt[src + j + t * G];
+            }
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                cadjncy[dst + j + t * G] = a[t];
+                cadjwgt[dst + j + t * G] = w[t];
+            }
+        }
+        for (; j < l; j += G) {
             cadjncy[dst + j] = tadj[src + j];
             cadjwgt[dst + j] = twgt[src + j];
         }
@@ -1775,8 +1790,9 @@ DevGraph contract(Ctx &c, const DevGraph &g, const int32_t *match, const int32_t
     cg.adjwgt = c.arena.lalloc<wgt_t>((size_t)(cm > 0 ? cm : 1));
     {
         // classes 2..5 are consecutive in the id list: one launch of 256-lane groups
-        const int64_t cnt[3] = {cb.count(0), cb.count(1), cb.off[kClsSegLong + 1] - cb.off[2]};
-        const int32_t *lst[3] = {cb.list(0), cb.list(1), cb.list(2)};
+        // rows of <= 8 / <= 512 / longer pre-merge entries: 8 lanes, a wave, a block
+        const int64_t cnt[3] = {cb.count(0), cb.count(1) + cb.count(2), cb.off[kClsSegLong + 1] - cb.off[3]};
+        const int32_t *lst[3] = {cb.list(0), cb.list(1), cb.list(3)};
         static const char *names[3] = {"k_seg_compact8", "k_seg_compact64", "k_seg_compact256"};
         for (int cl = 0; cl < 3; cl++) {
             if (!cnt[cl]) continue;
