@@ -321,6 +321,47 @@ __global__ void k_compact(const uint8_t *__restrict__ f, const int32_t *__restri
     if (i == n - 1) *count = incl[i];
 }
 
+// Flag compaction in two launches (up to kRawTiles tiles): per-tile flag
+// counts, then each block adds the counts of the tiles before its own, ranks
+// its tile's flags with block scans and writes their indices in order.
+__global__ void __launch_bounds__(kScanBS) k_flag_tile_count(const uint8_t *__restrict__ f, int64_t n,
+                                                             int32_t *__restrict__ tsum)
+{
+    __shared__ int32_t sh[kScanBS / kWave + 1];
+    const int64_t base = (int64_t)blockIdx.x * (kScanBS * kScanR);
+    int32_t cnt = 0;
+#pragma unroll
+    for (int r = 0; r < kScanR; r++) {
+        const int64_t i = base + r * kScanBS + threadIdx.x;
+        cnt += (i < n && f[i]) ? 1 : 0;
+    }
+    int32_t tot;
+    block_exclusive_scan(cnt, sh, &tot);
+    if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(kScanBS) k_flag_compact(const uint8_t *__restrict__ f, int64_t n,
+                                                          const int32_t *__restrict__ tsum,
+                                                          int32_t *__restrict__ idx, int64_t *count)
+{
+    __shared__ int32_t sh[kScanBS / kWave + 1];
+    const int64_t base = (int64_t)blockIdx.x * (kScanBS * kScanR);
+    int32_t part = 0;
+    for (int i = threadIdx.x; i < (int)blockIdx.x; i += kScanBS) part += tsum[i];
+    int32_t carry;
+    block_exclusive_scan(part, sh, &carry);
+    for (int r = 0; r < kScanR; r++) {
+        const int64_t i = base + r * kScanBS + threadIdx.x;
+        const int32_t v = (i < n && f[i]) ? 1 : 0;
+        int32_t tot;
+        const int32_t ex = block_exclusive_scan(v, sh, &tot);
+        if (v) idx[carry + ex] = (int32_t)i;
+        carry += tot;
+        if (base + (r + 1) * kScanBS >= n) break;   // uniform across the block
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *count = carry;
+}
+
 // ------------------------------------------------------------ degree classes
 
 constexpr int kNumCls = 6;
@@ -586,6 +627,16 @@ void compact_indices(Ctx &c, const uint8_t *flag, int64_t n, int32_t *idx, int64
         return;
     }
     size_t mark = c.arena.right_mark();
+    const int64_t tiles = (n + kScanBS * kScanR - 1) / (kScanBS * kScanR);
+    if (tiles <= kRawTiles) {
+        int32_t *tsum = c.arena.ralloc<int32_t>((size_t)tiles);
+        k_flag_tile_count<<<(unsigned)tiles, kScanBS, 0, c.stream>>>(flag, n, tsum);
+        HGP_LAUNCH_CHECK();
+        k_flag_compact<<<(unsigned)tiles, kScanBS, 0, c.stream>>>(flag, n, tsum, idx, count);
+        HGP_LAUNCH_CHECK();
+        c.arena.right_release(mark);
+        return;
+    }
     int32_t *incl = c.arena.ralloc<int32_t>((size_t)n);
     k_flag_to_i32<<<grid_for(n, 256), 256, 0, c.stream>>>(flag, n, incl);
     HGP_LAUNCH_CHECK();
