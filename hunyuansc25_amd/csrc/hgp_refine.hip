@@ -1109,25 +1109,21 @@ __global__ void k_iota_u32(int32_t n, uint32_t *__restrict__ val)
     if (v < n) val[v] = (uint32_t)v;
 }
 
-// After the sort by (key, id): exclusive prefix weight of each candidate
-// inside its part, via an inclusive scan of weights and the part's first
-// position.
+// After the sort by (key, id): each candidate's weight (scanned next into
+// the prefix weights of its part's candidates) and the first position of
+// each part in the sorted list.
 __global__ void k_rb_weights(int32_t n, int32_t k, const uint32_t *__restrict__ key, const uint32_t *__restrict__ val,
-                             const wgt_t *__restrict__ vwgt, int64_t *__restrict__ w)
+                             const wgt_t *__restrict__ vwgt, int64_t *__restrict__ w, int32_t *__restrict__ pstart)
 {
     int32_t i = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
-    w[i] = key[i] < (uint32_t)k * kMaxBuckets ? vwgt[val[i]] : 0;
-}
-
-__global__ void k_rb_pstart(int32_t n, int32_t k, const uint32_t *__restrict__ key, int32_t *__restrict__ pstart)
-{
-    int32_t i = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (i >= n) return;
-    uint32_t kk = key[i];
-    if (kk >= (uint32_t)k * kMaxBuckets) return;
-    uint32_t p = kk / kMaxBuckets;
-    if (i == 0 || key[i - 1] / kMaxBuckets != p) pstart[p] = i;
+    const uint32_t kk = key[i];
+    const bool cand = kk < (uint32_t)k * kMaxBuckets;
+    w[i] = cand ? vwgt[val[i]] : 0;
+    if (cand) {
+        uint32_t p = kk / kMaxBuckets;
+        if (i == 0 || key[i - 1] / kMaxBuckets != p) pstart[p] = i;
+    }
 }
 
 // Evict a candidate iff the candidates ahead of it in its part weigh less
@@ -2309,11 +2305,9 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             k_iota_u32<<<gv, kBS, 0, c.stream>>>(n, val);
             HGP_LAUNCH_CHECK();
             radix_sort(c, key, val, n, 0, bits_for((uint64_t)k * kMaxBuckets));
-            k_rb_weights<<<gv, kBS, 0, c.stream>>>(n, k, key, val, g.vwgt, wsc);
+            k_rb_weights<<<gv, kBS, 0, c.stream>>>(n, k, key, val, g.vwgt, wsc, pstart);
             HGP_LAUNCH_CHECK();
             scan_inclusive(c, wsc, wsc, n);
-            k_rb_pstart<<<gv, kBS, 0, c.stream>>>(n, k, key, pstart);
-            HGP_LAUNCH_CHECK();
             k_rb_evict<<<gv, kBS, 0, c.stream>>>(n, k, key, val, wsc, g.vwgt, pstart, pwgts, ctl, where, dest, sel,
                                                  pd16, pd32);
             HGP_LAUNCH_CHECK();
