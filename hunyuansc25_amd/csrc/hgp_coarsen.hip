@@ -1455,7 +1455,7 @@ int64_t match_level(Ctx &c, const DevGraph &g, int32_t maxvwgt, uint64_t seed, i
             if (g.segs.nseg) {
                 {
                     KScope ks(c, "k_propose_seg", 24.0 * (double)g.segs.nseg, true);
-                    k_propose_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 16384), kSegBlk, 0, c.stream>>>(
+                    k_propose_seg<<<(unsigned)std::min<int64_t>(g.segs.nseg, 32768), kSegBlk, 0, c.stream>>>(
                         g.segs, g.xadj, g.adjncy, g.adjwgt, avail, maxvwgt, s1, prop, sbest, prev, r == 0, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }

@@ -1592,7 +1592,7 @@ void for_small_bins(const RowBins &b, F f)
     if (b.count(2)) f(std::integral_constant<int, 64>{}, b.list(2), b.count(2));
 }
 
-inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, 16384); }
+inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, 32768); }
 
 __global__ void k_half(int64_t *d) { *d /= 2; }
 
