@@ -118,6 +118,8 @@ SIGNATURES = {
     "hgp_default_options": (None, [_P]),
     "hgp_partition": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "hgp_partition_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
+    "hgp_check_graph_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I32]),
+    "hgp_evaluate_device": (C.c_int, [_I32, _P, _P, _P, _P, _P, _I32, _I32, _P]),
     "hgp_partition_batch": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "hgp_server_create": (C.c_int, [_P, _I32, _I64, _P]),
     "hgp_server_submit": (C.c_int64, [_P, _I32, _P, _P, _P, _P, _I32, _U64, _P]),
@@ -344,6 +346,22 @@ def partition_device(n, d_xadj, d_adjncy, d_vwgt, d_adjwgt, k, d_part, **opts) -
                                       C.c_void_p(d_adjwgt), int(k), C.byref(o), C.c_void_p(d_part),
                                       C.byref(st)))
     return st.to_dict()
+
+
+def check_graph_device(n, d_xadj, d_adjncy, d_vwgt=0, d_adjwgt=0, symmetric=False, device=0) -> None:
+    """Checks a device CSR (device pointers as ints; 0 = unit weights); raises
+    HgpError naming the first fault."""
+    _check(lib().hgp_check_graph_device(int(n), C.c_void_p(d_xadj), C.c_void_p(d_adjncy),
+                                        C.c_void_p(d_vwgt or None), C.c_void_p(d_adjwgt or None),
+                                        1 if symmetric else 0, int(device)))
+
+
+def evaluate_device(n, d_xadj, d_adjncy, d_adjwgt, d_vwgt, d_part, k, d_out, device=0) -> None:
+    """Writes {cut, heaviest part, bad part ids, bad neighbour ids} (4 int64)
+    to the device array d_out."""
+    _check(lib().hgp_evaluate_device(int(n), C.c_void_p(d_xadj), C.c_void_p(d_adjncy),
+                                     C.c_void_p(d_adjwgt or None), C.c_void_p(d_vwgt or None),
+                                     C.c_void_p(d_part), int(k), int(device), C.c_void_p(d_out)))
 
 
 # ------------------------------------------------------------------ kernel timer

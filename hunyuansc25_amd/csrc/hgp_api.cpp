@@ -2,6 +2,7 @@
 // uncoarsen/refine) and the C ABI.
 #include "hunyuangraph.h"
 
+#include "hgp_check.hpp"
 #include "hgp_graph.hpp"
 #include "hgp_prims.hpp"
 #include "hgp_host.hpp"
@@ -504,8 +505,12 @@ int hgp_partition_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjn
         HGP_CHECK_HIP(hipSetDevice(c.device));
         int64_t m = 0;
         HGP_CHECK_HIP(hipMemcpy(&m, d_xadj + n, sizeof(int64_t), hipMemcpyDeviceToHost));
-        if (m < 0 || m > INT32_MAX) throw Error{-4, "directed entry count out of range"};
+        if (m < 0) throw Error{-1, "xadj[n] < 0"};
+        if (m > INT32_MAX) throw Error{-4, "more than 2^31-1 directed entries"};
         with_arena(c, estimate_arena(n, m, k, false), [&] {
+            // the CSR invariants of check_graph (hunyuangraph_GPU_match.h:1718-1800),
+            // checked on the device before any kernel gathers through it
+            check_graph_device(c, n, d_xadj, d_adjncy, d_vwgt, d_adjwgt, o.validate != 0);
             DevGraph g;
             g.n = n;
             g.m = m;
@@ -514,6 +519,39 @@ int hgp_partition_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjn
             g.adjwgt = const_cast<wgt_t *>(d_adjwgt);
             g.vwgt = const_cast<wgt_t *>(d_vwgt);
             run_pipeline(c, g, k, p, d_part, stats);
+        });
+    });
+}
+
+int hgp_check_graph_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy, const int32_t *d_vwgt,
+                           const int32_t *d_adjwgt, int32_t symmetric, int32_t device)
+{
+    return guarded([&] {
+        if (n < 0) throw Error{-1, "nvtxs < 0"};
+        if (n == 0) return;
+        if (!d_xadj) throw Error{-1, "xadj is NULL"};
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(device);
+        HGP_CHECK_HIP(hipSetDevice(c.device));
+        with_arena(c, 1 << 20, [&] { check_graph_device(c, n, d_xadj, d_adjncy, d_vwgt, d_adjwgt, symmetric != 0); });
+    });
+}
+
+int hgp_evaluate_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy, const int32_t *d_adjwgt,
+                        const int32_t *d_vwgt, const int32_t *d_part, int32_t k, int32_t device, int64_t *d_out)
+{
+    return guarded([&] {
+        if (n < 0) throw Error{-1, "nvtxs < 0"};
+        if (k < 1) throw Error{-1, "nparts must be >= 1"};
+        if (k > kEvalMaxParts) throw Error{-4, "nparts > 4096 is not supported"};
+        if (!d_out) throw Error{-1, "out is NULL"};
+        if (n > 0 && (!d_xadj || !d_adjncy || !d_part)) throw Error{-1, "NULL device pointer"};
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(device);
+        HGP_CHECK_HIP(hipSetDevice(c.device));
+        with_arena(c, sizeof(int64_t) * ((size_t)kEvalHead + (size_t)k) + 4096, [&] {
+            int64_t *scr = c.arena.ralloc<int64_t>((size_t)kEvalHead + (size_t)k);
+            evaluate_launch(c.stream, n, d_xadj, d_adjncy, d_adjwgt, d_vwgt, d_part, k, scr, d_out);
         });
     });
 }

@@ -1,0 +1,50 @@
+"""Synthetic workloads of the batched and serving modes (host-side generators
+over the library's own graph generators; seeded, so every run and test sees
+the same graphs)."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def batch_graphs(hgp, count, seed, n_lo=1000, n_hi=65536):
+    """The batched workload: vertex counts log-uniform in [n_lo, n_hi]
+    (default 1k..64k), cycling 2-D grids (about sqrt(n) x sqrt(n)), 3-D random
+    geometric graphs (mean degree 12) and banded FEM-like patterns (bandwidth
+    32, 10% random off-band entries); unit weights; k drawn from
+    {2, 4, 8, 16, 32, 64}.  Returns (graphs, ks); a graph is (xadj, adjncy,
+    vwgt, adjwgt)."""
+    rng = np.random.default_rng(seed)
+    gs, ks = [], []
+    for i in range(count):
+        n = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        kind = i % 3
+        if kind == 0:
+            side = max(2, int(round(np.sqrt(n))))
+            g = hgp.gen_grid2d(side, max(2, n // side))
+        elif kind == 1:
+            g = hgp.gen_rgg3d(n, hgp.rgg_radius(n, 12.0), seed=seed + i)
+        else:
+            g = hgp.gen_banded(n, 32, 0.1, seed=seed + i)
+        gs.append(g)
+        ks.append(int(rng.choice([2, 4, 8, 16, 32, 64])))
+    return gs, ks
+
+
+def serve_requests(hgp, count, seed, n_lo=1e4, n_hi=4e6):
+    """Serving requests: vertex counts log-uniform in [n_lo, n_hi] (default
+    10k..4M), alternating R-MAT (edge factor 16, power-of-two vertex counts)
+    and 3-D random geometric graphs (mean degree 12); k log-uniform over
+    {8, 16, ..., 512}.  Returns [((xadj, adjncy, vwgt, adjwgt), k), ...]."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        n = float(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        k = int(2 ** rng.integers(3, 10))
+        if i % 2 == 0:
+            scale = int(round(np.log2(n)))
+            g = hgp.gen_rmat(scale, 16 << scale, seed=seed + i)
+        else:
+            n = int(n)
+            g = hgp.gen_rgg3d(n, hgp.rgg_radius(n, 12.0), seed=seed + i)
+        reqs.append((g, k))
+    return reqs

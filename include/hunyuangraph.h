@@ -59,7 +59,7 @@ typedef struct hgp_options {
     int32_t trials;         /* initial bisection trials per node (default 1024)  */
     int32_t fm_passes;      /* FM passes per trial (default 10)                  */
     int32_t patience;       /* Jet non-improving rounds (default 6)              */
-    int32_t validate;       /* 1: check CSR symmetry on the host before running  */
+    int32_t validate;       /* 1: also check CSR symmetry before running         */
     int32_t device;         /* HIP device ordinal (default 0)                    */
     int32_t level_stats;    /* 1: fill level_adjwgt and level_ms of hgp_stats    */
 } hgp_options;
@@ -106,6 +106,33 @@ int hgp_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy, const i
 int hgp_partition_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy,
                          const int32_t *d_vwgt, const int32_t *d_adjwgt, int32_t k,
                          const hgp_options *opt, int32_t *d_part, hgp_stats *stats);
+
+/* Checks a device-resident CSR (device pointers on `device`) with the
+ * invariants of check_graph (hunyuangraph_GPU_match.h:1718-1800): xadj[0] = 0,
+ * monotone offsets, vertex weights >= 1, neighbour ids in [0, n), no
+ * self-loops, edge weights >= 1, and a total edge weight below 2^31.  With
+ * symmetric != 0, also that every edge appears in both rows with the same
+ * weight (compared through two 64-bit hash sums of the edge multiset, so an
+ * asymmetric graph passes with probability about 2^-128).  vwgt/adjwgt may be
+ * NULL (unit weights).  Returns HGP_OK or HGP_ERR_INVALID, with the first
+ * fault (the smallest vertex showing it) in hgp_last_error().  The offsets
+ * are proved monotone before any row is read, so a malformed xadj never
+ * sends a kernel outside adjncy.  hgp_partition_device runs this check on
+ * every call (symmetric when opt->validate is set). */
+int hgp_check_graph_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy, const int32_t *d_vwgt,
+                           const int32_t *d_adjwgt, int32_t symmetric, int32_t device);
+
+/* Quality of a device-resident partition (the device forms of
+ * compute_edgecut_gpu, hunyuangraph_graph.h:184-268, and of the part-weight
+ * sums of calculateSum, hunyuangraph_GPU_uncoarsen.h:19-45).  d_out (device,
+ * 4 int64) receives {edge cut, heaviest part weight, number of vertices whose
+ * part id is outside [0, k), number of entries whose neighbour id is outside
+ * [0, n)}; such vertices and entries are counted, never dereferenced.  The
+ * offsets must be valid (as hgp_check_graph_device accepts them); vwgt/adjwgt
+ * may be NULL (unit weights); k <= 4096.  Runs on the library's stream of
+ * `device` and waits for it. */
+int hgp_evaluate_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy, const int32_t *d_adjwgt,
+                        const int32_t *d_vwgt, const int32_t *d_part, int32_t k, int32_t device, int64_t *d_out);
 
 /* ------------------------------------------------------------ batches
  * No counterpart in the reference, which partitions one graph per call.
