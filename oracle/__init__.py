@@ -77,6 +77,11 @@ _SIG = {
     "oracle_dga_coarsen": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "oracle_coarsen_threshold": (_I32, [_I32]),
     "oracle_maxvwgt": (_I32, [_I64, _I32]),
+    "oracle_fb_apply": (None, [_I32, _I32, _P, _P, _P, _P, _I32, _P]),
+    "oracle_jet_rw_step": (_I64, [_I32, _P, _P, _P, _P, _I32, _P, _P, _I64, _I64, _P]),
+    "oracle_jet_rs_step": (_I64, [_I32, _P, _P, _P, _P, _I32, _P, _P, _I64, _I64, _P]),
+    "oracle_cookie_cutter": (None, [_I32, _I32, _P, _I64, _P, _P, _P]),
+    "oracle_jet_accept": (C.c_int, [_I64, _I64, _I64, _P, _P, _P]),
     "oracle_dga_match": (_I64, [_I32, _P, _P, _P, _P, _I32, _U64, _I32, _I32, _P]),
     "oracle_edge_hash": (_U32, [_U64, _I32, _I32, _I32]),
     "oracle_dga_initpart": (None, [_I32, _P, _P, _P, _P, _I32, _P, _P]),
@@ -289,3 +294,51 @@ def cpu_rb_partition(xadj, adjncy, k, vwgt=None, adjwgt=None, seed=1):
     part = np.zeros(n, dtype=np.int32)
     cut = lib().oracle_cpu_rb_partition(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(k), int(seed), _p(part))
     return part, int(cut)
+
+
+def _rebalance(fn, xadj, adjncy, adjwgt, vwgt, k, where, pwgts, maxwgt, opt):
+    xadj, adjncy, adjwgt, vwgt, where = _i64(xadj), _i32(adjncy), _i32(adjwgt), _i32(vwgt), _i32(where)
+    pwgts = _i64(pwgts)
+    dest = np.zeros(len(where), np.int32)
+    m = fn(len(where), _p(xadj), _p(adjncy), _p(adjwgt), _p(vwgt), int(k), _p(where), _p(pwgts), int(maxwgt),
+           int(opt), _p(dest))
+    return dest, int(m)
+
+
+def jet_rw_step(xadj, adjncy, adjwgt, vwgt, k, where, pwgts, maxwgt, opt):
+    """One weak-rebalance step: (dest, moves)."""
+    return _rebalance(lib().oracle_jet_rw_step, xadj, adjncy, adjwgt, vwgt, k, where, pwgts, maxwgt, opt)
+
+
+def jet_rs_step(xadj, adjncy, adjwgt, vwgt, k, where, pwgts, maxwgt, opt):
+    """One strong-rebalance step: (dest, moves)."""
+    return _rebalance(lib().oracle_jet_rs_step, xadj, adjncy, adjwgt, vwgt, k, where, pwgts, maxwgt, opt)
+
+
+def cookie_cutter(bs, maxwgt, pwgts):
+    """Slices (es, ee) of the evicted list with weight prefix bs[0..ne]."""
+    bs, pwgts = _i64(bs), _i64(pwgts)
+    k = len(pwgts)
+    es = np.zeros(k, np.int32)
+    ee = np.zeros(k, np.int32)
+    lib().oracle_cookie_cutter(int(k), int(len(bs) - 1), _p(bs), int(maxwgt), _p(pwgts), _p(es), _p(ee))
+    return es, ee
+
+
+def jet_accept(cur_cut, curmax, maxwgt, best_cut, best_max, count):
+    """Acceptance rule: (accepted, best_cut, best_max, count) after one iteration."""
+    bc, bm, cn = C.c_int64(best_cut), C.c_int64(best_max), C.c_int32(count)
+    r = lib().oracle_jet_accept(int(cur_cut), int(curmax), int(maxwgt), C.byref(bc), C.byref(bm), C.byref(cn))
+    return bool(r), int(bc.value), int(bm.value), int(cn.value)
+
+
+FALLBACKS = {"leaf": 0, "isolate": 1, "twin": 2, "relative": 3, "forced": 4}
+
+
+def fb_apply(kind, xadj, adjncy, adjwgt, vwgt, maxvwgt, match):
+    """One matching fallback pass on a copy of match; returns the new match."""
+    xadj, adjncy, adjwgt, vwgt = _i64(xadj), _i32(adjncy), _i32(adjwgt), _i32(vwgt)
+    m = np.array(match, dtype=np.int32)
+    lib().oracle_fb_apply(FALLBACKS[kind], len(xadj) - 1, _p(xadj), _p(adjncy), _p(adjwgt), _p(vwgt),
+                          int(maxvwgt), _p(m))
+    return m

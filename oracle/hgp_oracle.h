@@ -135,6 +135,10 @@ int64_t oracle_dga_match(int32_t n, const int64_t *xadj, const int32_t *adjncy,
                          uint64_t seed, int32_t level, int32_t rounds, int32_t *match);
 
 /* Edge priority hash of the handshake matching. */
+/* One matching fallback pass (0 leaf, 1 isolate, 2 twin, 3 relative,
+ * 4 forced) on its own, match updated in place. */
+void oracle_fb_apply(int32_t kind, int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                     const int32_t *adjwgt, const int32_t *vwgt, int32_t maxvwgt, int32_t *match);
 uint32_t oracle_edge_hash(uint64_t seed, int32_t level, int32_t lo, int32_t hi);
 
 /* Recursive-bisection initial partition of a (coarsest) graph into k parts. */
@@ -167,6 +171,21 @@ int64_t oracle_jet_lp_step(int32_t n, const int64_t *xadj, const int32_t *adjncy
 
 /* Integer restatement of gain_bucket (hunyuangraph_GPU_krefine.h:2477-2513). */
 int32_t oracle_gain_bucket(int64_t gain, int64_t vwgt);
+
+/* Jet rebalancing steps on their own (weak: jetrw, strong: jetrs), the
+ * cookie cutter of jetrs and the acceptance rule of k_refine, for tests. */
+int64_t oracle_jet_rw_step(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                           const int32_t *adjwgt, const int32_t *vwgt, int32_t k,
+                           const int32_t *where, const int64_t *pwgts, int64_t maxwgt,
+                           int64_t opt, int32_t *dest);
+int64_t oracle_jet_rs_step(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                           const int32_t *adjwgt, const int32_t *vwgt, int32_t k,
+                           const int32_t *where, const int64_t *pwgts, int64_t maxwgt,
+                           int64_t opt, int32_t *dest);
+void oracle_cookie_cutter(int32_t k, int32_t ne, const int64_t *bs, int64_t maxwgt,
+                          const int64_t *pwgts, int32_t *es, int32_t *ee);
+int oracle_jet_accept(int64_t cur_cut, int64_t curmax, int64_t maxwgt, int64_t *best_cut,
+                      int64_t *best_max, int32_t *count);
 
 /* ---------------------------------------------------------------- (3) */
 

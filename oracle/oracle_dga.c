@@ -224,6 +224,21 @@ static void fb_forced(int32_t n, const int32_t *vwgt, int32_t maxvwgt, int32_t *
     }
 }
 
+/* One fallback pass on its own, for the tests: kind 0 leaf, 1 isolate,
+ * 2 twin, 3 relative, 4 forced; match (n) is updated in place (-1 = free). */
+void oracle_fb_apply(int32_t kind, int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                     const int32_t *adjwgt, const int32_t *vwgt, int32_t maxvwgt, int32_t *match)
+{
+    switch (kind) {
+    case 0: fb_leaf(n, xadj, adjncy, vwgt, maxvwgt, match); break;
+    case 1: fb_isolate(n, xadj, vwgt, maxvwgt, match); break;
+    case 2: fb_twin(n, xadj, adjncy, vwgt, maxvwgt, match); break;
+    case 3: fb_relative(n, xadj, adjncy, adjwgt, vwgt, maxvwgt, match); break;
+    case 4: fb_forced(n, vwgt, maxvwgt, match); break;
+    default: break;
+    }
+}
+
 /* Handshake heavy-edge matching.  Replaces the level-0 CAS random matching
  * (hunyuangraph_GPU_match.h:454-510) and the level>=1 weight-sorted top-k
  * handshake (:800-861, loop :2373-2507): in each synchronous round every free
@@ -770,6 +785,31 @@ static int64_t jet_rw_step(connctx *c, const int32_t *where, const int64_t *pwgt
     return moves;
 }
 
+/* cookie_cutter (hunyuangraph_GPU_krefine.h:2710-2745): the evicted list,
+ * whose vertex weights prefix to bs[0..ne], is cut into consecutive slices
+ * [es[p], ee[p]), one per part in part order; each slice starts where the
+ * previous ended and ends at the prefix index closest to
+ * bs[start] + max(maxwgt - pwgts[p], 0), found by the reference's binary
+ * search (ties to the lower index). */
+void oracle_cookie_cutter(int32_t k, int32_t ne, const int64_t *bs, int64_t maxwgt,
+                          const int64_t *pwgts, int32_t *es, int32_t *ee)
+{
+    int32_t start = 0;
+    for (int32_t p = 0; p < k; p++) {
+        es[p] = start;
+        int64_t cap = maxwgt > pwgts[p] ? maxwgt - pwgts[p] : 0;
+        int64_t find = bs[start] + cap;
+        int32_t lo = start, hi = ne;
+        while (lo + 1 < hi) {
+            int32_t mid = (lo + hi) / 2;
+            if (bs[mid] >= find) hi = mid; else lo = mid;
+        }
+        int32_t end = (iabs64(bs[hi] - find) < iabs64(bs[lo] - find)) ? hi : lo;
+        ee[p] = end;
+        start = end;
+    }
+}
+
 /* jetrs (hunyuangraph_GPU_krefine.h:2774-3134, kernels :2515-2766).  The
  * reference's loss term averages connectivity over parts gated by
  * poverload[p] of the vertex's own (overloaded) part, so it reduces to
@@ -816,20 +856,7 @@ static int64_t jet_rs_step(connctx *c, const int32_t *where, const int64_t *pwgt
     for (int32_t i = 0; i < ne; i++) bs[i + 1] = bs[i] + c->vwgt[ev[i]];
     int32_t *es = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
     int32_t *ee = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
-    int32_t start = 0;
-    for (int32_t p = 0; p < k; p++) {
-        es[p] = start;
-        int64_t cap = maxwgt > pwgts[p] ? maxwgt - pwgts[p] : 0;
-        int64_t find = bs[start] + cap;
-        int32_t lo = start, hi = ne;
-        while (lo + 1 < hi) {
-            int32_t mid = (lo + hi) / 2;
-            if (bs[mid] >= find) hi = mid; else lo = mid;
-        }
-        int32_t end = (iabs64(bs[hi] - find) < iabs64(bs[lo] - find)) ? hi : lo;
-        ee[p] = end;
-        start = end;
-    }
+    oracle_cookie_cutter(k, ne, bs, maxwgt, pwgts, es, ee);
     int64_t moves = 0;
     int32_t p = 0;
     for (int32_t i = 0; i < ne; i++) {
@@ -839,6 +866,35 @@ static int64_t jet_rs_step(connctx *c, const int32_t *where, const int64_t *pwgt
     }
     free(cand); free(ev); free(bs); free(es); free(ee);
     return moves;
+}
+
+/* One weak (jetrw) or strong (jetrs) rebalancing step on its own, for the
+ * tests: pwgts, maxwgt and opt as the refinement loop holds them; dest[v]
+ * receives the chosen part (where[v] if v stays).  Returns the moves. */
+int64_t oracle_jet_rw_step(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                           const int32_t *adjwgt, const int32_t *vwgt, int32_t k,
+                           const int32_t *where, const int64_t *pwgts, int64_t maxwgt,
+                           int64_t opt, int32_t *dest)
+{
+    connctx c = {n, k, xadj, adjncy, adjwgt, vwgt, NULL, NULL};
+    c.conn = (int64_t *)calloc((size_t)k, sizeof(int64_t));
+    c.touched = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
+    int64_t m = jet_rw_step(&c, where, pwgts, maxwgt, opt, dest);
+    free(c.conn); free(c.touched);
+    return m;
+}
+
+int64_t oracle_jet_rs_step(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                           const int32_t *adjwgt, const int32_t *vwgt, int32_t k,
+                           const int32_t *where, const int64_t *pwgts, int64_t maxwgt,
+                           int64_t opt, int32_t *dest)
+{
+    connctx c = {n, k, xadj, adjncy, adjwgt, vwgt, NULL, NULL};
+    c.conn = (int64_t *)calloc((size_t)k, sizeof(int64_t));
+    c.touched = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
+    int64_t m = jet_rs_step(&c, where, pwgts, maxwgt, opt, dest);
+    free(c.conn); free(c.touched);
+    return m;
 }
 
 /* perform_moves (hunyuangraph_GPU_krefine.h:4034-4155): apply dest[], update
@@ -874,6 +930,33 @@ static int64_t max_of(const int64_t *a, int32_t k)
     int64_t m = a[0];
     for (int32_t i = 1; i < k; i++) if (a[i] > m) m = a[i];
     return m;
+}
+
+/* The acceptance rule of k_refine (hunyuangraph_GPU_krefine.h:4805-4839)
+ * after an iteration that left cut cur_cut and heaviest part curmax: an
+ * unbalanced best (best_max > maxwgt) is replaced by any state with a lighter
+ * heaviest part, and the patience counter restarts; otherwise a smaller cut
+ * is kept if the state is balanced or no less balanced than the best, and
+ * the counter restarts only if the cut fell by more than 0.1 % (the
+ * reference's tol 0.999, as 1000 cur < 999 best).  The reference compares
+ * max(pwgts)/opt; the heaviest part weight orders identically.  Returns 1
+ * when the state becomes the best. */
+int oracle_jet_accept(int64_t cur_cut, int64_t curmax, int64_t maxwgt, int64_t *best_cut,
+                      int64_t *best_max, int32_t *count)
+{
+    if (*best_max > maxwgt && curmax < *best_max) {
+        *best_max = curmax;
+        *best_cut = cur_cut;
+        *count = 0;
+        return 1;
+    }
+    if (cur_cut < *best_cut && (curmax <= maxwgt || curmax <= *best_max)) {
+        if (1000 * cur_cut < 999 * *best_cut) *count = 0;
+        *best_max = curmax;
+        *best_cut = cur_cut;
+        return 1;
+    }
+    return 0;
 }
 
 /* k_refine (hunyuangraph_GPU_krefine.h:4434-4882) with the driver limits of
@@ -913,17 +996,8 @@ int32_t oracle_dga_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
         cur_cut += apply_moves(n, xadj, adjncy, adjwgt, vwgt, where, dest, pwgts);
         iters++;
         curmax = max_of(pwgts, k);
-        if (best_max > maxwgt && curmax < best_max) {
-            best_max = curmax;
-            best_cut = cur_cut;
+        if (oracle_jet_accept(cur_cut, curmax, maxwgt, &best_cut, &best_max, &count))
             memcpy(best_where, where, sizeof(int32_t) * (size_t)n);
-            count = 0;
-        } else if (cur_cut < best_cut && (curmax <= maxwgt || curmax <= best_max)) {
-            if (1000 * cur_cut < 999 * best_cut) count = 0;
-            best_max = curmax;
-            best_cut = cur_cut;
-            memcpy(best_where, where, sizeof(int32_t) * (size_t)n);
-        }
     }
     memcpy(where, best_where, sizeof(int32_t) * (size_t)n);
     *cut = best_cut;
