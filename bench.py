@@ -10,8 +10,13 @@ Prints one JSON line: throughput in M edges/s (undirected |E| / time per
 step), the HBM roofline of the dominant kernel (per-launch algorithmic bytes /
 average launch duration from HIP events on the library's stream), and the
 host-CPU baseline: the C restatement of the reference CPU path
-(hunyuangraph_rbbisection) timed on one core on a bounded sample of the same
-workload (R-MAT scale 19, same parameters and edge factor, k = 256).
+(hunyuangraph_rbbisection) timed on one pinned core, in a child process
+started before any GPU call, on the 256x256 grid (k = 8) and the 3-D RGG 2^20
+(k = 64) at full size and on a bounded sample of the R-MAT workload (scale 19,
+the same generator and edge factor, k = 256); the GPU partitions the same
+three graphs for a same-graph speedup and cut ratio.  Also the batched mode
+(4096 graphs of 1k..64k vertices) and the serving mode (saturating and
+Poisson arrivals).
 """
 import argparse
 import json
@@ -32,35 +37,104 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def host_cut_and_maxpart(xadj, adjncy, part, k):
+def host_cut_and_maxpart(xadj, adjncy, part, k, vwgt=None, adjwgt=None):
+    """Exact cut and heaviest part weight, recounted on the host."""
     n = len(xadj) - 1
     deg = np.diff(xadj)
     rows = np.repeat(np.arange(n, dtype=np.int32), deg)
-    cut = int(np.count_nonzero(part[rows] != part[adjncy])) // 2
-    pw = np.bincount(part, minlength=k)
+    diff = part[rows] != part[adjncy]
+    cut = (int(np.count_nonzero(diff)) if adjwgt is None else int(adjwgt[diff].astype(np.int64).sum())) // 2
+    pw = np.bincount(part, weights=None if vwgt is None else vwgt.astype(np.float64), minlength=k)
     return cut, int(pw.max())
 
 
-def cpu_baseline(scale, k, seed, hgp):
-    """Reference CPU path restated in C (oracle), single thread, bounded sample.
-    The same sample is partitioned on the GPU too, for the cut comparison."""
+def cpu_graphs(hgp, scale, seed):
+    """The CPU baseline's graphs: (name, graph, k)."""
+    n2 = 1 << 20
+    return [
+        ("grid2d_256x256_k8", lambda: hgp.gen_grid2d(256, 256), 8),
+        ("rgg3d_2^20_w_k64", lambda: hgp.gen_rgg3d(n2, hgp.rgg_radius(n2, 12.0), seed=seed, vw=(1, 4), ew=(1, 16)), 64),
+        (f"rmat_s{scale}_e{scale + 4}_k256", lambda: hgp.gen_rmat(scale, 16 << scale, seed=seed), 256),
+    ]
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_leg_child(out_path, core, scale, seed):
+    """Child process: pins itself to one core and times the C restatement of
+    the reference CPU path (oracle, one thread) on each graph; generation is
+    outside the clock.  Touches no GPU."""
+    os.sched_setaffinity(0, {core})
+    import hunyuansc25_amd as hgp    # host-side generators only
     import oracle as O
-    x, a, vw, aw = hgp.gen_rmat(scale, 16 << scale, seed=seed)
-    E = len(a) // 2
-    t0 = time.perf_counter()
-    part, cut = O.cpu_rb_partition(x, a, k, vw, aw, seed=seed)
-    dt = time.perf_counter() - t0
-    gpart, gst = hgp.partition(x, a, k, vw, aw, seed=seed)
+    res = {}
+    for name, make, k in cpu_graphs(hgp, scale, seed):
+        x, a, vw, aw = make()
+        t0 = time.perf_counter()
+        part, cut = O.cpu_rb_partition(x, a, k, vw, aw, seed=seed)
+        dt = time.perf_counter() - t0
+        E = len(a) // 2
+        tv = int(vw.astype(np.int64).sum())
+        pw = np.bincount(part, weights=vw.astype(np.float64), minlength=k)
+        res[name] = {"edges": E, "k": k, "seconds": round(dt, 3), "M_edges_per_s": round(E / dt / 1e6, 4),
+                     "cut": int(cut), "imbalance": round(float(pw.max()) * k / tv, 5)}
+    with open(out_path, "w") as f:
+        json.dump(res, f)
+
+
+def start_cpu_leg(args):
+    """Starts the CPU baseline in a child process before this process makes
+    any GPU call; the child takes the last allowed core and this process the
+    others."""
+    import subprocess
+    import tempfile
+    allowed = sorted(os.sched_getaffinity(0))
+    core = allowed[-1]
+    out = os.path.join(tempfile.mkdtemp(prefix="hgp_cpu_"), "cpu.json")
+    proc = subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-leg-out", out, "--cpu-core", str(core),
+                             "--cpu-scale", str(args.cpu_scale), "--seed", str(args.seed)])
+    if len(allowed) > 1:
+        os.sched_setaffinity(0, set(allowed[:-1]))
+    return proc, out, core, len(allowed)
+
+
+def finish_cpu_leg(leg, gpu_rates):
+    """Waits for the child and joins its times with the GPU's on the same graphs."""
+    proc, out, core, nallowed = leg
+    rc = proc.wait(timeout=900)
+    if rc != 0:
+        raise SystemExit(f"cpu baseline child failed with status {rc}")
+    res = json.load(open(out))
+    for name, r in res.items():
+        g = gpu_rates.get(name)
+        if g:
+            r["gpu_M_edges_per_s"] = g["M_edges_per_s"]
+            r["gpu_cut"] = g["cut"]
+            r["gpu_cut_ratio"] = round(g["cut"] / max(r["cut"], 1), 4)
+            r["speedup"] = round(g["M_edges_per_s"] / r["M_edges_per_s"], 1)
+    sample = [k for k in res if k.startswith("rmat_")][0]
+    s = res[sample]
     return {
-        "value": round(E / dt / 1e6, 4),
+        "value": s["M_edges_per_s"],
         "unit": "M edges/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"rmat scale {scale} ({E} edges), k={k}, one partition in {dt:.2f} s",
-        "cut": int(cut),
-        "cut_fraction": round(cut / E, 4),
-        "gpu_cut_same_sample": int(gst["cut"]),
-        "gpu_cut_ratio": round(gst["cut"] / max(cut, 1), 4),
+        "sample": f"{sample}: R-MAT scale {sample.split('_')[1][1:]} ({s['edges']} edges, edge factor 16 as the "
+                  f"2^22/2^26 workload), k=256, one partition in {s['seconds']} s on one pinned core; the 256x256 "
+                  f"grid (k=8) and the 3-D RGG 2^20 (k=64) at full size beside it",
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "cores_allowed": nallowed,
+        "pinned_core": core,
+        "configs": res,
     }
 
 
@@ -98,39 +172,20 @@ def bench_other(hgp, torch, name, graph, k, seed, steps=3):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     part = d_part.cpu().numpy()
-    cut, maxp = host_cut_and_maxpart(x, a, part, k) if int(aw.min(initial=1)) == 1 == int(aw.max(initial=1)) \
-        else (st["cut"], st["max_pwgt"])
-    tv = int(vw.sum())
+    cut, maxp = host_cut_and_maxpart(x, a, part, k, vw, aw)
+    if cut != st["cut"]:
+        raise SystemExit(f"{name}: library cut {st['cut']} != host recount {cut}")
+    tv = int(vw.astype(np.int64).sum())
     return {"workload": name, "n": n, "edges": E, "k": k, "M_edges_per_s": round(E / dt / 1e6, 3),
-            "ms": round(dt * 1e3, 3), "cut": int(st["cut"]), "balanced_1.03": bool(100 * k * st["max_pwgt"] <= 103 * tv)}
-
-
-def batch_graphs(hgp, count, seed):
-    """The batched workload: sizes log-uniform in [1k, 64k] vertices, cycling
-    2-D grids, 3-D random geometric graphs (mean degree 12) and banded
-    FEM-like patterns (bandwidth 32, 10% off-band entries); unit weights;
-    k drawn from {2, 4, 8, 16, 32, 64}."""
-    rng = np.random.default_rng(seed)
-    gs, ks = [], []
-    for i in range(count):
-        n = int(np.exp(rng.uniform(np.log(1000), np.log(65536))))
-        kind = i % 3
-        if kind == 0:
-            side = max(2, int(round(np.sqrt(n))))
-            g = hgp.gen_grid2d(side, max(2, n // side))
-        elif kind == 1:
-            g = hgp.gen_rgg3d(n, hgp.rgg_radius(n, 12.0), seed=seed + i)
-        else:
-            g = hgp.gen_banded(n, 32, 0.1, seed=seed + i)
-        gs.append(g)
-        ks.append(int(rng.choice([2, 4, 8, 16, 32, 64])))
-    return gs, ks
+            "ms": round(dt * 1e3, 3), "cut": cut, "imbalance": round(maxp * k / tv, 5),
+            "balanced_1.03": bool(100 * k * maxp <= 103 * tv)}
 
 
 def bench_batch(hgp, count, lanes, seed):
     """Graphs/s of the plain per-graph path (one hgp_partition call after
     another) and of hgp_partition_batch on the same graphs; both from host
     arrays.  The partitions must be identical."""
+    from hunyuansc25_amd.workloads import batch_graphs
     gs, ks = batch_graphs(hgp, count, seed)
     E = sum(len(g[1]) // 2 for g in gs)
     hgp.partition_batch(gs[: 2 * lanes], ks[: 2 * lanes], lanes=lanes, seed=seed)   # lanes' arenas warm
@@ -154,74 +209,78 @@ def bench_batch(hgp, count, lanes, seed):
     }
 
 
-def serve_requests(hgp, count, seed):
-    """Serving requests: sizes log-uniform in [10k, 4M] vertices, alternating
-    R-MAT (edge factor 16, power-of-two vertex counts) and 3-D random
-    geometric graphs (mean degree 12); k log-uniform over {8, ..., 512}."""
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for i in range(count):
-        n = float(np.exp(rng.uniform(np.log(1e4), np.log(4e6))))
-        k = int(2 ** rng.integers(3, 10))
-        if i % 2 == 0:
-            scale = int(round(np.log2(n)))
-            g = hgp.gen_rmat(scale, 16 << scale, seed=seed + i)
-        else:
-            n = int(n)
-            g = hgp.gen_rgg3d(n, hgp.rgg_radius(n, 12.0), seed=seed + i)
-        reqs.append((g, k))
-    return reqs
-
-
-def bench_serve(hgp, count, slots, budget, seed):
-    """Poisson arrivals at a fixed rate 25% above what the plain path sustains
-    (measured on the same requests); latency = submit to result."""
+def bench_serve(hgp, pool_size, count, slots, budget, seed):
+    """Serving mode on a pool of pool_size requests (sizes log-uniform in
+    [10k, 4M] vertices, R-MAT and 3-D RGG, k in {8..512}); request j is pool
+    entry j mod pool_size with that entry's seed, so every result can be
+    compared with the entry partitioned alone.
+    * plain: the pool one request after another (hgp_partition);
+    * saturating: the whole pool submitted at once (arrivals above any
+      capacity): served graphs/s against plain graphs/s on the same requests;
+    * Poisson: `count` requests at a fixed rate of 60 % of the saturated
+      capacity: p50/p99 latency (submit to result)."""
     import threading
-    reqs = serve_requests(hgp, count, seed)
+    from hunyuansc25_amd.workloads import serve_requests
+    pool = serve_requests(hgp, pool_size, seed)
+    seeds = [seed + i for i in range(pool_size)]
+    hgp.partition(pool[0][0][0], pool[0][0][1], pool[0][1], seed=seeds[0])   # warm
     t0 = time.perf_counter()
-    alone = [hgp.partition(g[0], g[1], k, g[2], g[3], seed=seed + i)[0] for i, (g, k) in enumerate(reqs)]
+    alone = [hgp.partition(g[0], g[1], k, g[2], g[3], seed=s)[0] for (g, k), s in zip(pool, seeds)]
     t_plain = time.perf_counter() - t0
-    rate = 1.25 * count / t_plain
-    rng = np.random.default_rng(seed + 7)
-    arrive = np.cumsum(rng.exponential(1.0 / rate, size=count))
-    lat = [0.0] * count
-    same = [False] * count
+    same = True
     with hgp.Server(slots=slots, budget_bytes=budget) as srv:
+        # saturating arrivals
+        start = time.perf_counter()
+        tickets = [srv.submit(g[0], g[1], k, g[2], g[3], seed=s) for (g, k), s in zip(pool, seeds)]
+        for i, t in enumerate(tickets):
+            part, _, _ = srv.wait(t)
+            same &= bool(np.array_equal(part, alone[i]))
+        t_sat = time.perf_counter() - start
+        cap = pool_size / t_sat
+        # Poisson arrivals at a fixed rate
+        rate = 0.6 * cap
+        rng = np.random.default_rng(seed + 7)
+        arrive = np.cumsum(rng.exponential(1.0 / rate, size=count))
+        lat = [0.0] * count
         tickets = [None] * count
         ready = threading.Semaphore(0)
+        ok = [True]
 
         def waiter():
-            for i in range(count):
+            for j in range(count):
                 ready.acquire()
-                part, st, ms = srv.wait(tickets[i])
-                lat[i] = ms
-                same[i] = bool(np.array_equal(part, alone[i]))
+                part, st, ms = srv.wait(tickets[j])
+                lat[j] = ms
+                ok[0] &= bool(np.array_equal(part, alone[j % pool_size]))
 
         th = threading.Thread(target=waiter)
         th.start()
         start = time.perf_counter()
-        for i, (g, k) in enumerate(reqs):
-            delay = start + arrive[i] - time.perf_counter()
+        for j in range(count):
+            delay = start + arrive[j] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            tickets[i] = srv.submit(g[0], g[1], k, g[2], g[3], seed=seed + i)
+            (g, k), s = pool[j % pool_size], seeds[j % pool_size]
+            tickets[j] = srv.submit(g[0], g[1], k, g[2], g[3], seed=s)
             ready.release()
         th.join()
         span = time.perf_counter() - start
         info = srv.info()
     lat_s = np.sort(np.array(lat))
     return {
-        "workload": f"serve_{count}_requests_10k-4M",
-        "requests": count,
+        "workload": f"serve_pool{pool_size}_10k-4M",
         "slots": slots,
         "budget_GB": round(budget / 2**30, 1),
-        "arrival_rate_per_s": round(rate, 3),
-        "graphs_per_s": round(count / span, 3),
-        "plain_graphs_per_s": round(count / t_plain, 3),
+        "plain_graphs_per_s": round(pool_size / t_plain, 3),
+        "saturated_graphs_per_s": round(cap, 3),
+        "saturated_speedup": round(cap / (pool_size / t_plain), 3),
+        "poisson_requests": count,
+        "poisson_rate_per_s": round(rate, 3),
+        "poisson_graphs_per_s": round(count / span, 3),
         "p50_ms": round(float(np.percentile(lat_s, 50)), 2),
         "p99_ms": round(float(np.percentile(lat_s, 99)), 2),
         "peak_reserved_GB": round(info["peak_reserved"] / 2**30, 2),
-        "identical_to_alone": bool(all(same)),
+        "identical_to_alone": bool(same and ok[0]),
     }
 
 
@@ -240,17 +299,29 @@ def main():
     ap.add_argument("--no-others", action="store_true", help="skip the grid and RGG configurations")
     ap.add_argument("--calib-copy", action="store_true",
                     help="one 1 GiB device-to-device copy before the steps (counter calibration)")
-    ap.add_argument("--batch-graphs", type=int, default=256, help="batched-mode sample (0: skip)")
+    ap.add_argument("--batch-graphs", type=int, default=4096, help="batched-mode graphs (0: skip)")
     ap.add_argument("--lanes", type=int, default=8)
-    ap.add_argument("--serve-requests", type=int, default=24, help="serving-mode sample (0: skip)")
+    ap.add_argument("--serve-pool", type=int, default=24, help="serving-mode distinct requests (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=200, help="Poisson-arrival requests")
+    ap.add_argument("--cpu-leg-out", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-core", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=200.0)
     args = ap.parse_args()
+
+    if args.cpu_leg_out:
+        cpu_leg_child(args.cpu_leg_out, args.cpu_core, args.cpu_scale, args.seed)
+        return
 
     if args.gpus != 1:
         print(json.dumps({"error": "the partitioner runs on one GPU (a graph is not sharded across GPUs); use --gpus 1",
                           "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
+
+    leg = None
+    if not args.no_cpu:
+        leg = start_cpu_leg(args)
+        log(f"cpu baseline: child process on core {leg[2]} (this process on the other {leg[3] - 1})")
 
     log("bench: loading library and torch")
     import hunyuansc25_amd as hgp
@@ -339,12 +410,13 @@ def main():
     pipeline_gbs = st["alg_bytes"] / (st["ms_total"] * 1e-3) / 1e9
 
     others = []
+    gpu_rates = {}
     if not args.no_others:
-        log("other configurations: grid 256x256 k=8, 3-D RGG 2^20 weighted k=64 ...")
-        others.append(bench_other(hgp, torch, "grid2d_256x256_k8", hgp.gen_grid2d(256, 256), 8, args.seed))
-        n2 = 1 << 20
-        rgg = hgp.gen_rgg3d(n2, hgp.rgg_radius(n2, 12.0), seed=args.seed, vw=(1, 4), ew=(1, 16))
-        others.append(bench_other(hgp, torch, "rgg3d_2^20_w_k64", rgg, 64, args.seed))
+        log("other configurations: grid 256x256 k=8, 3-D RGG 2^20 weighted k=64, R-MAT sample ...")
+        for name, make, k in cpu_graphs(hgp, args.cpu_scale, args.seed):
+            r = bench_other(hgp, torch, name, make(), k, args.seed)
+            others.append(r)
+            gpu_rates[name] = r
         log(f"other configurations: {others}")
 
     batched = None
@@ -353,15 +425,16 @@ def main():
         batched = bench_batch(hgp, args.batch_graphs, args.lanes, args.seed)
         log(f"batched mode: {batched}")
     serving = None
-    if args.serve_requests > 0:
-        log(f"serving mode: {args.serve_requests} requests, {args.slots} slots ...")
-        serving = bench_serve(hgp, args.serve_requests, args.slots, int(args.budget_gb * 2**30), args.seed)
+    if args.serve_pool > 0:
+        log(f"serving mode: pool of {args.serve_pool}, {args.serve_requests} Poisson requests, {args.slots} slots ...")
+        serving = bench_serve(hgp, args.serve_pool, args.serve_requests, args.slots, int(args.budget_gb * 2**30),
+                              args.seed)
         log(f"serving mode: {serving}")
 
     cpu = None
-    if not args.no_cpu:
-        log(f"cpu baseline: rmat scale {args.cpu_scale} k={args.k} ...")
-        cpu = cpu_baseline(args.cpu_scale, args.k, args.seed, hgp)
+    if leg:
+        log("cpu baseline: waiting for the child ...")
+        cpu = finish_cpu_leg(leg, gpu_rates)
         log(f"cpu baseline: {cpu['value']} M edges/s ({cpu['sample']})")
 
     out = {
@@ -406,7 +479,10 @@ def main():
         "serving": serving,
     }
     if cpu:
-        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+        # the same graph on both sides: the R-MAT sample
+        same = cpu["configs"][[k for k in cpu["configs"] if k.startswith("rmat_")][0]]
+        if "speedup" in same:
+            out["speedup_vs_cpu"] = same["speedup"]
     print(json.dumps(out), flush=True)
 
 

@@ -299,6 +299,9 @@ class Server:
         o = options(**opts)
         _check(lib().hgp_server_create(C.byref(o), int(slots), int(budget_bytes), C.byref(self._h)))
         self._live = {}
+        import threading
+        self._mu = threading.Lock()
+        self._waiting = set()
 
     def submit(self, xadj, adjncy, k, vwgt=None, adjwgt=None, seed=1) -> int:
         x, a = _i64(xadj), _i32(adjncy)
@@ -313,12 +316,22 @@ class Server:
         return t
 
     def wait(self, ticket):
-        st = HgpStats()
-        lat = C.c_double()
-        rc = lib().hgp_server_wait(self._h, int(ticket), C.byref(st), C.byref(lat))
-        arrs = self._live.pop(ticket, None)
-        _check(rc)
-        return arrs[4], st.to_dict(), lat.value
+        """A ticket is waited for once: a second wait raises HgpError (-1)
+        and leaves the first waiter's job alone."""
+        with self._mu:
+            if ticket in self._waiting:
+                raise HgpError(-1, "ticket is already being waited for")
+            self._waiting.add(ticket)
+        try:
+            st = HgpStats()
+            lat = C.c_double()
+            rc = lib().hgp_server_wait(self._h, int(ticket), C.byref(st), C.byref(lat))
+            arrs = self._live.pop(ticket, None) if rc == HGP_OK or ticket in self._live else None
+            _check(rc)
+            return arrs[4], st.to_dict(), lat.value
+        finally:
+            with self._mu:
+                self._waiting.discard(ticket)
 
     def info(self) -> dict:
         r, p, c = C.c_int64(), C.c_int64(), C.c_int64()

@@ -13,6 +13,7 @@
 // finds the row of the first by a binary search over xadj and then steps
 // through the rows, so hub rows cost no more than short ones.
 #include "hgp_check.hpp"
+#include "hgp_host.hpp"
 
 #include <algorithm>
 #include <climits>

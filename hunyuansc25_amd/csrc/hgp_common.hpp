@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "hgp_hostcheck.hpp"
+
 #include <cstdint>
 #include <cstdio>
 #include <memory>
@@ -23,13 +25,6 @@ constexpr int kMidBucket = 25;
 
 // ------------------------------------------------------------------ errors
 
-struct Error {
-    int code;
-    std::string msg;
-    bool overflow = false;   // an arena request did not fit (a larger arena may succeed)
-};
-
-void set_error(int code, const std::string &msg);
 void clear_error();
 
 #define HGP_CHECK_HIP(expr)                                                        \

@@ -6,6 +6,7 @@
 
 #include "hgp_common.hpp"
 
+#include <algorithm>
 #include <cstdint>
 #include <new>
 #include <stdexcept>
@@ -53,8 +54,11 @@ int guarded(F f)
 }
 
 // Runs body with an arena of at least `bytes`; on arena overflow the arena
-// doubles (up to max_bytes, three times at most) and the body is re-run (the
-// body must be restartable).
+// grows to twice its size or to 5/4 of what the failed request needed,
+// whichever is larger (up to max_bytes, three times at most), and the body
+// is re-run (the body must be restartable).  Growing by the need lets a run
+// whose coarsest graph stays large (its bisection trials' scratch grows with
+// the square of that size) succeed on the first retry.
 template <class F>
 void with_arena(Ctx &c, size_t bytes, F body, size_t max_bytes = SIZE_MAX)
 {
@@ -69,8 +73,13 @@ void with_arena(Ctx &c, size_t bytes, F body, size_t max_bytes = SIZE_MAX)
         } catch (const Error &e) {
             (void)hipStreamSynchronize(c.stream);
             c.arena.reset();
-            if (!e.overflow || attempt >= 3 || bytes > max_bytes / 2) throw;
-            bytes *= 2;
+            if (!e.overflow || attempt >= 3 || bytes >= max_bytes) throw;
+            size_t grow = std::max(bytes <= SIZE_MAX / 2 ? 2 * bytes : SIZE_MAX, e.need / 4 * 5 + (64 << 20));
+            if (grow > max_bytes) {
+                if (e.need > max_bytes) throw;
+                grow = max_bytes;
+            }
+            bytes = grow;
         }
     }
 }

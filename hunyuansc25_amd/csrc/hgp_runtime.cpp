@@ -63,7 +63,8 @@ void *Arena::left(size_t bytes)
 {
     bytes = round_up(bytes);
     if (lo_ + bytes > hi_)
-        throw Error{-3, "arena: left allocation of " + std::to_string(bytes) + " bytes does not fit", true};
+        throw Error{-3, "arena: left allocation of " + std::to_string(bytes) + " bytes does not fit", true,
+                    lo_ + (cap_ - hi_) + bytes};
     void *p = base_ + lo_;
     lo_ += bytes;
     if (lo_ + (cap_ - hi_) > peak_) peak_ = lo_ + (cap_ - hi_);
@@ -74,7 +75,8 @@ void *Arena::right(size_t bytes)
 {
     bytes = round_up(bytes);
     if (lo_ + bytes > hi_)
-        throw Error{-3, "arena: right allocation of " + std::to_string(bytes) + " bytes does not fit", true};
+        throw Error{-3, "arena: right allocation of " + std::to_string(bytes) + " bytes does not fit", true,
+                    lo_ + (cap_ - hi_) + bytes};
     hi_ -= bytes;
     if (lo_ + (cap_ - hi_) > peak_) peak_ = lo_ + (cap_ - hi_);
     return base_ + hi_;

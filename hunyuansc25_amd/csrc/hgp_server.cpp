@@ -1,6 +1,7 @@
 // hgp_server.cpp -- the serving loop: a FIFO of partition jobs admitted by
 // free slot and device-memory budget, run concurrently on per-slot streams.
 #include "hgp_host.hpp"
+#include "hgp_tickets.hpp"
 
 #include <chrono>
 #include <condition_variable>
@@ -17,7 +18,6 @@ struct hgp_server {
         hgp_stats stats;
         size_t bytes = 0;
         int status = 0;
-        bool done = false;
         std::string msg;
         Clock::time_point t_submit, t_done;
     };
@@ -28,11 +28,11 @@ struct hgp_server {
     std::vector<std::unique_ptr<hgp::Ctx>> ctx;
     std::vector<char> busy;
     std::deque<int64_t> queue;
-    std::map<int64_t, std::unique_ptr<Job>> jobs;
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    hgp::TicketBook<Job> jobs{mu};   // jobs by ticket, each waited for once
+    std::condition_variable cv_work;
     bool stop = false;
-    int64_t next_ticket = 0, completed = 0;
+    int64_t completed = 0;
     size_t peak = 0;
     std::vector<std::thread> workers;
 
@@ -74,26 +74,29 @@ struct hgp_server {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
             cv_work.wait(lk, [&] {
-                return (stop && queue.empty()) || (!queue.empty() && admit(w, jobs[queue.front()]->bytes));
+                return (stop && queue.empty()) || (!queue.empty() && admit(w, jobs.get(queue.front())->bytes));
             });
             if (queue.empty()) break;   // stopping and drained
             int64_t t = queue.front();
             queue.pop_front();
-            Job *job = jobs[t].get();
+            Job *job = jobs.get(t);
             size_t need = job->bytes;
             busy[w] = 1;
             claim[w] = std::max(ctx[w]->arena.capacity(), need);
             peak = std::max(peak, reserved());
             lk.unlock();
             // The arena is capped at the claim; a run that needs more goes back
-            // to the head of the queue with twice the claim (up to the budget).
+            // to the head of the queue with twice the claim, or 5/4 of what the
+            // failed request needed if that is more (up to the budget).
             bool again = false;
+            size_t regrow = 0;
             int rc = hgp::guarded([&] {
                 try {
                     hgp::exec_job(*ctx[w], job->o, job->j, need, claim[w]);
                 } catch (const hgp::Error &e) {
-                    if (e.overflow && need < budget) {
+                    if (e.overflow && need < budget && e.need <= budget) {
                         again = true;
+                        regrow = std::max(2 * need, e.need / 4 * 5 + ((size_t)64 << 20));
                         return;
                     }
                     throw;
@@ -103,15 +106,14 @@ struct hgp_server {
             lk.lock();
             busy[w] = 0;
             if (again) {
-                job->bytes = std::min(2 * need, budget);
+                job->bytes = std::min(regrow, budget);
                 queue.push_front(t);
             } else {
                 job->status = rc;
                 job->msg = msg;
                 job->t_done = Clock::now();
-                job->done = true;
                 completed++;
-                cv_done.notify_all();
+                jobs.finish(t);
             }
             cv_work.notify_all();
         }
@@ -158,8 +160,7 @@ int64_t hgp_server_submit(hgp_server *s, int32_t n, const int64_t *xadj, const i
         job->t_submit = Clock::now();
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop) throw hgp::Error{-1, "server is stopping"};
-        ticket = s->next_ticket++;
-        s->jobs[ticket] = std::move(job);
+        ticket = s->jobs.add(std::move(job));
         s->queue.push_back(ticket);
         s->cv_work.notify_all();
     });
@@ -172,16 +173,15 @@ int hgp_server_wait(hgp_server *s, int64_t ticket, hgp_stats *stats, double *lat
     std::string msg;
     int rc = hgp::guarded([&] {
         if (!s) throw hgp::Error{-1, "server is NULL"};
-        std::unique_lock<std::mutex> lk(s->mu);
-        auto it = s->jobs.find(ticket);
-        if (it == s->jobs.end()) throw hgp::Error{-1, "unknown ticket"};
-        hgp_server::Job *job = it->second.get();
-        s->cv_done.wait(lk, [&] { return job->done; });
-        if (stats) *stats = job->stats;
-        if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(job->t_done - job->t_submit).count();
-        status = job->status;
-        msg = job->msg;
-        s->jobs.erase(it);
+        int w = s->jobs.wait(ticket, [&](hgp_server::Job &job) {
+            if (stats) *stats = job.stats;
+            if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(job.t_done - job.t_submit).count();
+            status = job.status;
+            msg = job.msg;
+        });
+        if (w == hgp::TicketBook<hgp_server::Job>::kUnknown) throw hgp::Error{-1, "unknown ticket"};
+        if (w == hgp::TicketBook<hgp_server::Job>::kClaimed)
+            throw hgp::Error{-1, "ticket is already being waited for"};
     });
     if (rc != HGP_OK) return rc;
     if (status != HGP_OK) hgp::set_error(status, msg);
@@ -209,6 +209,9 @@ int hgp_server_destroy(hgp_server *s)
             s->cv_work.notify_all();
         }
         for (auto &t : s->workers) t.join();
+        // every job has finished: the waits in flight return; the server
+        // outlives them
+        s->jobs.drain();
         for (auto &c : s->ctx) hgp::destroy_context(std::move(c));
         delete s;
     });

@@ -171,12 +171,15 @@ int64_t hgp_server_submit(hgp_server *s, int32_t n, const int64_t *xadj, const i
                           int32_t *part);
 /* Blocks until the job is done and returns its status; stats and latency_ms
  * (submit to result, host clock) are optional outputs.  A ticket is waited
- * for once. */
+ * for once: a second wait on it, concurrent or later, returns
+ * HGP_ERR_INVALID and leaves the job to the first. */
 int hgp_server_wait(hgp_server *s, int64_t ticket, hgp_stats *stats, double *latency_ms);
 /* Device bytes held by the slots' arenas now and at most so far, and the
  * number of finished jobs (any output may be NULL). */
 int hgp_server_info(hgp_server *s, int64_t *reserved, int64_t *peak_reserved, int64_t *completed);
-/* Finishes queued jobs, stops the workers and frees the device memory. */
+/* Finishes queued jobs, stops the workers, waits until every
+ * hgp_server_wait in flight has returned, and frees the device memory.  No
+ * call on the server may start once destroy has begun. */
 int hgp_server_destroy(hgp_server *s);
 
 /* ------------------------------------------------------------ kernel timer

@@ -72,3 +72,29 @@ def test_serve_rejects(hgp):
         t = srv.submit(g[0], g[1], 4)
         part, st, _ = srv.wait(t)
         assert st["cut"] > 0 and part.max() == 3
+
+
+def test_serve_ticket_waited_once(hgp):
+    """Two threads waiting on one ticket: one gets the result, the other an
+    error; a wait after the result was taken is refused too."""
+    g = hgp.gen_rmat(12, 1 << 14, seed=4)
+    res, errs = [], []
+    with hgp.Server(slots=2, budget_bytes=1 << 30) as srv:
+        t = srv.submit(g[0], g[1], 8, seed=2)
+
+        def waiter():
+            try:
+                res.append(srv.wait(t))
+            except hgp.HgpError as e:
+                errs.append(e.code)
+
+        th = [threading.Thread(target=waiter) for _ in range(2)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        with pytest.raises(hgp.HgpError):
+            srv.wait(t)
+    assert len(res) == 1 and errs == [-1]
+    p, _ = hgp.partition(g[0], g[1], 8, seed=2)
+    np.testing.assert_array_equal(res[0][0], p)
