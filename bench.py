@@ -297,6 +297,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ktimer", action="store_true")
     ap.add_argument("--no-others", action="store_true", help="skip the grid and RGG configurations")
+    ap.add_argument("--no-fm-kinds", action="store_true",
+                    help="skip the bisection-FM comparison (scan, queue, reference queue FM) on the headline graph")
     ap.add_argument("--calib-copy", action="store_true",
                     help="one 1 GiB device-to-device copy before the steps (counter calibration)")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="batched-mode graphs (0: skip)")
@@ -324,8 +326,8 @@ def main():
         log(f"cpu baseline: child process on core {leg[2]} (this process on the other {leg[3] - 1})")
 
     log("bench: loading library and torch")
+    import torch                      # first: libhgp.so then binds to torch's HIP runtime
     import hunyuansc25_amd as hgp
-    import torch
     log("bench: torch ready")
 
     if hgp.device_count() < 1:
@@ -409,6 +411,21 @@ def main():
         }
     pipeline_gbs = st["alg_bytes"] / (st["ms_total"] * 1e-3) / 1e9
 
+    fm_kinds = None
+    if not args.no_fm_kinds:
+        # the initial partition of the headline graph's coarsest level with
+        # each FM of the bisection trials (device time of that phase)
+        fm_kinds = {}
+        for kind, label in ((0, "wave_argmax_scan"), (1, "priority_queue_same_moves"),
+                            (2, "reference_priority_queue_fm")):
+            d_tmp = torch.zeros(n, dtype=torch.int32, device=dev)
+            s_k = hgp.partition_device(n, d_x.data_ptr(), d_a.data_ptr(), d_vw.data_ptr(), d_aw.data_ptr(), args.k,
+                                       d_tmp.data_ptr(), seed=args.seed, fm_kind=kind)
+            fm_kinds[label] = {"fm_kind": kind, "coarsest_n": s_k["coarsest_n"], "init_ms": round(s_k["ms_init"], 3),
+                               "init_cut": s_k["init_cut"], "cut": s_k["cut"], "ms_total": round(s_k["ms_total"], 3)}
+            del d_tmp
+        log(f"bisection FM kinds: {fm_kinds}")
+
     others = []
     gpu_rates = {}
     if not args.no_others:
@@ -473,6 +490,7 @@ def main():
         "pipeline_alg_GBs": round(pipeline_gbs, 2),
         "roofline": roofline,
         "kernels": {k: [v[0], round(v[1], 3), v[2]] for k, v in kt.items()},
+        "bisection_fm": fm_kinds,
         "cpu_baseline": cpu,
         "other_configs": others,
         "batched": batched,

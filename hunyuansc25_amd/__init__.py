@@ -43,6 +43,7 @@ class HgpOptions(C.Structure):
         ("validate", C.c_int32),
         ("device", C.c_int32),
         ("level_stats", C.c_int32),
+        ("fm_kind", C.c_int32),
     ]
 
 
@@ -120,6 +121,8 @@ SIGNATURES = {
     "hgp_partition_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "hgp_check_graph_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I32]),
     "hgp_evaluate_device": (C.c_int, [_I32, _P, _P, _P, _P, _P, _I32, _I32, _P]),
+    "hgp_jet_trace_enable": (C.c_int, [C.c_int]),
+    "hgp_jet_trace": (C.c_int64, [_P, C.c_int64]),
     "hgp_partition_batch": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "hgp_server_create": (C.c_int, [_P, _I32, _I64, _P]),
     "hgp_server_submit": (C.c_int64, [_P, _I32, _P, _P, _P, _P, _I32, _U64, _P]),
@@ -375,6 +378,22 @@ def evaluate_device(n, d_xadj, d_adjncy, d_adjwgt, d_vwgt, d_part, k, d_out, dev
     _check(lib().hgp_evaluate_device(int(n), C.c_void_p(d_xadj), C.c_void_p(d_adjncy),
                                      C.c_void_p(d_adjwgt or None), C.c_void_p(d_vwgt or None),
                                      C.c_void_p(d_part), int(k), int(device), C.c_void_p(d_out)))
+
+
+
+# ------------------------------------------------------------------ Jet trace
+
+def jet_trace_enable(on: bool = True) -> None:
+    _check(lib().hgp_jet_trace_enable(1 if on else 0))
+
+
+def jet_trace() -> np.ndarray:
+    """Records of the Jet loops since enabling: rows of (level vertices,
+    iterations done, mode of the coming iteration, cut so far, loop ended)."""
+    total = int(lib().hgp_jet_trace(None, 0))
+    buf = np.zeros(max(total, 1), dtype=np.int64)
+    lib().hgp_jet_trace(buf.ctypes.data_as(C.c_void_p), total)
+    return buf[:total].reshape(-1, 5)
 
 
 # ------------------------------------------------------------------ kernel timer

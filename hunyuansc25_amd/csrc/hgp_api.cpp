@@ -269,6 +269,7 @@ Params params_from(const hgp_options *o)
     p.fm_passes = o->fm_passes;
     p.patience = o->patience;
     p.level_stats = o->level_stats != 0;
+    p.fm_kind = o->fm_kind;
     return p;
 }
 
@@ -277,7 +278,8 @@ void check_params(const Params &p, int32_t k)
     if (k < 1) throw Error{-1, "nparts must be >= 1"};
     if (k > 4096) throw Error{-4, "nparts > 4096 is not supported"};
     if (p.ub_num <= 0 || p.ub_den <= 0 || p.ub_num < p.ub_den) throw Error{-1, "balance bound must be >= 1"};
-    if (p.match_rounds < 0 || p.trials < 1 || p.fm_passes < 0 || p.patience < 1)
+    if (p.match_rounds < 0 || p.trials < 1 || p.fm_passes < 0 || p.patience < 1 || p.fm_kind < -1 ||
+        p.fm_kind > 2)
         throw Error{-1, "invalid option values"};
 }
 
@@ -378,6 +380,7 @@ void hgp_default_options(hgp_options *o)
     o->trials = 1024;
     o->fm_passes = 10;
     o->patience = 6;
+    o->fm_kind = -1;
     o->validate = 0;
     o->device = 0;
     o->level_stats = 0;
@@ -573,6 +576,25 @@ void hunyuangraph_PartitionGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, 
         int rc = hgp_partition(n, x64.data(), adjncy, vwgt, adjwgt, *nparts, &o, part, nullptr);
         if (rc != HGP_OK) throw Error{rc, hgp_last_error()};
     });
+}
+
+// ---------------------------------------------------------------- Jet trace
+
+int hgp_jet_trace_enable(int on)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(jet_trace().mu);
+        jet_trace().on = on != 0;
+        jet_trace().recs.clear();
+    });
+}
+
+int64_t hgp_jet_trace(int64_t *buf, int64_t cap)
+{
+    std::lock_guard<std::mutex> lk(jet_trace().mu);
+    const std::vector<int64_t> &r = jet_trace().recs;
+    if (buf && cap > 0) std::copy(r.begin(), r.begin() + std::min<int64_t>(cap, (int64_t)r.size()), buf);
+    return (int64_t)r.size();
 }
 
 // ---------------------------------------------------------------- kernel timer

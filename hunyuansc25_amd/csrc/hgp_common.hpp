@@ -112,6 +112,19 @@ struct KTimer {
 
 KTimer &ktimer();
 
+// Optional trace of the Jet refinement loops: one record per read of the
+// control block (before each iteration, and the final read), holding the
+// level's vertex count, the iterations done, the mode of the coming
+// iteration, the cut so far and whether the loop has ended.  Off unless
+// enabled.
+struct JetTrace {
+    bool on = false;
+    std::mutex mu;
+    std::vector<int64_t> recs;   // kJetTraceFields per record
+};
+constexpr int kJetTraceFields = 5;
+JetTrace &jet_trace();
+
 class KScope {
   public:
     KScope(Ctx &c, const char *name, double bytes, bool counted = false);

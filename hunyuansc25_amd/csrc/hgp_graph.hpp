@@ -30,6 +30,7 @@ struct Params {
     int32_t trials = 1024;
     int32_t fm_passes = 10;
     int32_t patience = 6;
+    int32_t fm_kind = -1;       // bisection-trial FM: 0 scan, 1 heap (same moves), 2 reference heap FM, -1 auto
     bool level_stats = false;   // per-level edge-weight sums and build times
 };
 

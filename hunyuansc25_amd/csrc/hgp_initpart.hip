@@ -19,7 +19,20 @@ namespace hgp {
 namespace {
 
 constexpr int kTrialBytesPerVertex = 14;   // where, moved (u8) + ed, id, queue (i32)
+// heap FM: + position index and two heaps of (key, vertex), n + 1 slots each (i32)
+constexpr int kHeapBytesPerVertex = 20;
 constexpr int kLdsBudget = 60 * 1024;
+// fm_kind -1 keeps the scan: on the dense coarsest graphs this pipeline
+// produces (R-MAT 2^22: 995 vertices, average degree ~300) the queues took
+// 188 ms against 7 ms, a move's row being ~300 serial queue updates
+constexpr int32_t kAutoHeapMinVertices = INT32_MAX;
+
+__host__ __device__ inline int64_t trial_bytes(int64_t n, bool heap)
+{
+    int64_t b = ((2 * n + 3) & ~3ll) + 12 * n;
+    if (heap) b += 4 * n + 16 * (n + 1);
+    return b;
+}
 
 struct Node {
     int32_t start, end, k, fpart;
@@ -72,7 +85,7 @@ __global__ void __launch_bounds__(256) k_ip_setup(const int32_t *__restrict__ ac
 
 // Exclusive offsets of trials and scratch over the active nodes (one block).
 __global__ void k_ip_offsets(const int32_t *__restrict__ act, int32_t nact, Node *nodes, int32_t lds_cap,
-                             int32_t *total)
+                             bool heap, int32_t *total)
 {
     if (threadIdx.x != 0) return;
     int32_t t = 0;
@@ -85,7 +98,7 @@ __global__ void k_ip_offsets(const int32_t *__restrict__ act, int32_t nact, Node
         nd.soff = s;
         t += nd.T;
         w += (int64_t)nd.T * nn;
-        if (nn > lds_cap) s += (int64_t)nd.T * ((nn * kTrialBytesPerVertex + 255) & ~255ll);
+        if (nn > lds_cap) s += (int64_t)nd.T * ((trial_bytes(nn, heap) + 255) & ~255ll);
     }
     *total = t;
 }
@@ -93,6 +106,86 @@ __global__ void k_ip_offsets(const int32_t *__restrict__ act, int32_t nact, Node
 struct TrialMem {
     uint8_t *where, *moved;
     int32_t *ed, *id, *queue;
+    int32_t *loc, *hkey[2], *hval[2];   // heap FM only
+};
+
+// ---- device priority queue (hunyuangraph_GPU_priorityqueue.h:27-225), run by
+// one lane: a max-heap of vertices with a position index.  An element moves
+// above another when its key is strictly greater, or, with `ties`, equal with
+// a smaller vertex id (the top is then the unique (gain desc, id asc)
+// maximum); a removal refills the hole from the last slot.
+struct Heap {
+    int32_t *key, *val, *loc;
+    int32_t cnt;
+    bool ties;
+    __device__ bool above(int32_t ka, int32_t va, int32_t kb, int32_t vb) const
+    {
+        return ka > kb || (ties && ka == kb && va < vb);
+    }
+    __device__ void place(int32_t i, int32_t k, int32_t v)
+    {
+        key[i] = k;
+        val[i] = v;
+        loc[v] = i;
+    }
+    __device__ void up(int32_t i, int32_t k, int32_t v)
+    {
+        while (i > 0) {
+            int32_t j = (i - 1) >> 1;
+            int32_t kj = key[j], vj = val[j];
+            if (!above(k, v, kj, vj)) break;
+            place(i, kj, vj);
+            i = j;
+        }
+        place(i, k, v);
+    }
+    __device__ void down(int32_t i, int32_t k, int32_t v)
+    {
+        int32_t j;
+        while ((j = 2 * i + 1) < cnt) {
+            int32_t kj = key[j], vj = val[j];
+            if (j + 1 < cnt) {
+                int32_t k1 = key[j + 1], v1 = val[j + 1];
+                if (above(kj, vj, k, v)) {
+                    if (above(k1, v1, kj, vj)) { j++; kj = k1; vj = v1; }
+                } else if (above(k1, v1, k, v)) {
+                    j++; kj = k1; vj = v1;
+                } else {
+                    break;
+                }
+            } else if (!above(kj, vj, k, v)) {
+                break;
+            }
+            place(i, kj, vj);
+            i = j;
+        }
+        place(i, k, v);
+    }
+    __device__ void insert(int32_t v, int32_t k) { up(cnt++, k, v); }
+    __device__ int32_t top()
+    {
+        if (cnt == 0) return -1;
+        int32_t t = val[0];
+        loc[t] = -1;
+        if (--cnt > 0) down(0, key[cnt], val[cnt]);
+        return t;
+    }
+    __device__ void remove(int32_t v)
+    {
+        int32_t i = loc[v];
+        loc[v] = -1;
+        if (--cnt > 0 && val[cnt] != v) {
+            int32_t nk = key[cnt], nv = val[cnt];
+            if (above(nk, nv, key[i], v)) up(i, nk, nv);
+            else down(i, nk, nv);
+        }
+    }
+    __device__ void update(int32_t v, int32_t k)
+    {
+        int32_t i = loc[v];
+        if (k > key[i]) up(i, k, v);
+        else down(i, k, v);
+    }
 };
 
 __device__ inline void fence_wave()
@@ -106,11 +199,13 @@ __device__ inline void fence_wave()
 // (gain desc, id asc) and neighbour updates are spread over the 64 lanes.
 // The boundary flag is derived: bnd(v) = deg(v) == 0 || ed[v] > 0, which is
 // the invariant the reference's incremental updates maintain.
+template <bool HEAP>
 __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__ act, int32_t nact,
                                                      const Node *__restrict__ nodes, const int32_t *total,
                                                      const eid_t *__restrict__ fx, const vid_t *__restrict__ fadj,
                                                      const wgt_t *__restrict__ fw, const wgt_t *__restrict__ fvw,
-                                                     uint64_t seed, int32_t fm_passes, int32_t lds_cap,
+                                                     uint64_t seed, int32_t fm_passes, int32_t fm_kind,
+                                                     int32_t lds_cap,
                                                      uint8_t *__restrict__ trial_where, char *__restrict__ gscratch,
                                                      int64_t *__restrict__ tcut)
 {
@@ -129,13 +224,20 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
     const int32_t n = nd.end - nd.start, s0 = nd.start;
     char *base;
     if (n <= lds_cap) base = lds;
-    else base = gscratch + nd.soff + (int64_t)t * (((int64_t)n * kTrialBytesPerVertex + 255) & ~255ll);
+    else base = gscratch + nd.soff + (int64_t)t * ((trial_bytes(n, HEAP) + 255) & ~255ll);
     TrialMem M;
     M.where = (uint8_t *)base;
     M.moved = M.where + n;
     M.ed = (int32_t *)(base + (((int64_t)2 * n + 3) & ~3ll));
     M.id = M.ed + n;
     M.queue = M.id + n;
+    if (HEAP) {
+        M.loc = M.queue + n;
+        M.hkey[0] = M.loc + n;
+        M.hval[0] = M.hkey[0] + (n + 1);
+        M.hkey[1] = M.hval[0] + (n + 1);
+        M.hval[1] = M.hkey[1] + (n + 1);
+    }
     const int64_t tv = nd.tv, onemin = nd.onemin, onemax = nd.onemax, ideal0 = nd.ideal0;
 
     for (int32_t i = lane; i < n; i += kWave) { M.where[i] = 1; M.moved[i] = 0; }
@@ -255,7 +357,149 @@ __global__ void __launch_bounds__(kWave) k_ip_trials(const int32_t *__restrict__
     if (limit > 100) limit = 100;
     int32_t *swaps = M.queue;
 
-    for (int32_t pass = 0; pass < fm_passes; pass++) {
+    if (HEAP) {
+        // FM driven by two priority queues (FM_2way_cut_refinement,
+        // hunyuangraph_GPU_initialpartition.h:160-300).  Lane 0 runs every
+        // queue operation in the serial order; the lanes share the gain
+        // updates of a move's row.  fm_kind 1: (gain desc, id asc) keys and
+        // the part bounds of the scan FM below, so the moves are the same;
+        // fm_kind 2: the reference's queue order and balance rule
+        // (|ideal0 - pw0| <= origdiff + avgvwgt, weights applied before the
+        // test as hunyuangraph_CPU_2wayrefine.h:133-145 does).
+        Heap q[2];
+        for (int s = 0; s < 2; s++) {
+            q[s].key = M.hkey[s];
+            q[s].val = M.hval[s];
+            q[s].loc = M.loc;
+            q[s].cnt = 0;
+            q[s].ties = fm_kind == 1;
+        }
+        const bool live_rule = fm_kind == 1;
+        const int64_t a20 = (pw[0] + pw[1]) / 20, a2n = 2 * (pw[0] + pw[1]) / n;
+        const int64_t avgvwgt = a20 < a2n ? a20 : a2n;
+        const int64_t origdiff = ideal[0] - pw[0] < 0 ? pw[0] - ideal[0] : ideal[0] - pw[0];
+        for (int32_t pass = 0; pass < fm_passes; pass++) {
+            int64_t mincutorder = -1;
+            int64_t newcut = cut, mincut = cut, initcut = cut;
+            int64_t d0 = ideal[0] - pw[0];
+            int64_t mindiff = d0 < 0 ? -d0 : d0;
+            if (lane == 0) {
+                q[0].cnt = q[1].cnt = 0;
+                for (int32_t v = 0; v < n; v++) {
+                    int32_t e = M.ed[v];
+                    if (e > 0 || (M.moved[v] & 2)) q[M.where[v]].insert(v, e - M.id[v]);
+                }
+            }
+            int32_t nswaps;
+            for (nswaps = 0; nswaps < n; nswaps++) {
+                int from = (ideal[0] - pw[0] < ideal[1] - pw[1]) ? 0 : 1;
+                int to = from ^ 1;
+                int32_t bv = -1, bg = 0;
+                if (lane == 0) {
+                    bv = q[from].top();
+                    if (bv >= 0) bg = M.ed[bv] - M.id[bv];
+                }
+                bv = __shfl(bv, 0, kWave);
+                bg = __shfl(bg, 0, kWave);
+                if (bv == -1) break;
+                int32_t wv = fvw[s0 + bv];
+                newcut -= bg;
+                bool accept;
+                if (live_rule) {
+                    int64_t dd = ideal[0] - pw[0];
+                    dd = dd < 0 ? -dd : dd;
+                    accept = (newcut < mincut && pw[to] + wv <= onemax && pw[from] - wv <= onemax) ||
+                             (newcut == mincut && dd < mindiff);
+                    if (accept) mindiff = dd;
+                    pw[to] += wv;
+                    pw[from] -= wv;
+                } else {
+                    pw[to] += wv;
+                    pw[from] -= wv;
+                    int64_t dd = ideal[0] - pw[0];
+                    dd = dd < 0 ? -dd : dd;
+                    accept = (newcut < mincut && dd <= origdiff + avgvwgt) || (newcut == mincut && dd < mindiff);
+                    if (accept) mindiff = dd;
+                }
+                if (accept) {
+                    mincut = newcut;
+                    mincutorder = nswaps;
+                } else if (nswaps - mincutorder > limit) {
+                    newcut += bg;
+                    pw[from] += wv;
+                    pw[to] -= wv;
+                    break;
+                }
+                if (lane == 0) {
+                    M.where[bv] = (uint8_t)to;
+                    M.moved[bv] |= 1;
+                    swaps[nswaps] = bv;
+                    int32_t tmp = M.ed[bv];
+                    M.ed[bv] = M.id[bv];
+                    M.id[bv] = tmp;
+                }
+                fence_wave();
+                const eid_t rb = fx[s0 + bv], re = fx[s0 + bv + 1];
+                for (eid_t j = rb + lane; j < re; j += kWave) {
+                    int32_t x = fadj[j] - s0;
+                    int32_t kw = (M.where[x] == from) ? -fw[j] : fw[j];
+                    M.id[x] += kw;     // rows hold no duplicates: one lane per neighbour
+                    M.ed[x] -= kw;
+                }
+                fence_wave();
+                if (lane == 0) {
+                    // queue upkeep in row order: a neighbour that left the
+                    // boundary is deleted, one that stays is re-keyed, one
+                    // that joined is inserted (moved vertices stay out)
+                    for (eid_t j = rb; j < re; j++) {
+                        int32_t x = fadj[j] - s0;
+                        uint8_t st = M.moved[x];
+                        if (st & 1) continue;
+                        int32_t e = M.ed[x];
+                        int32_t kw = (M.where[x] == from) ? -fw[j] : fw[j];
+                        bool was = e + kw > 0;
+                        int sx = M.where[x];
+                        if (was) {
+                            if (e == 0) q[sx].remove(x);
+                            else q[sx].update(x, e - M.id[x]);
+                        } else if (e > 0) {
+                            q[sx].insert(x, e - M.id[x]);
+                        }
+                    }
+                }
+                fence_wave();
+            }
+            for (int32_t i = lane; i < nswaps; i += kWave) M.moved[swaps[i]] &= 2;
+            fence_wave();
+            for (nswaps--; nswaps > mincutorder; nswaps--) {
+                int32_t v = swaps[nswaps];
+                int from = M.where[v];
+                int to = from ^ 1;
+                int32_t wv = fvw[s0 + v];
+                fence_wave();
+                if (lane == 0) {
+                    M.where[v] = (uint8_t)to;
+                    int32_t tmp = M.ed[v];
+                    M.ed[v] = M.id[v];
+                    M.id[v] = tmp;
+                }
+                pw[to] += wv;
+                pw[from] -= wv;
+                fence_wave();
+                for (eid_t j = fx[s0 + v] + lane; j < fx[s0 + v + 1]; j += kWave) {
+                    int32_t x = fadj[j] - s0;
+                    int32_t kw = (M.where[x] == from) ? -fw[j] : fw[j];
+                    M.id[x] += kw;
+                    M.ed[x] -= kw;
+                }
+                fence_wave();
+            }
+            cut = mincut;
+            if (mincutorder <= 0 || mincut == initcut) break;
+        }
+    }
+
+    for (int32_t pass = 0; pass < (HEAP ? 0 : fm_passes); pass++) {
         int64_t mincutorder = -1;
         int64_t newcut = cut, mincut = cut, initcut = cut;
         int64_t d0 = ideal[0] - pw[0];
@@ -503,6 +747,15 @@ __global__ void k_ip_edges(int32_t n, const int8_t *__restrict__ side, const int
 
 struct TreeNode { int32_t id, k, fpart; };
 
+// fm_kind -1 picks the queue-driven FM (the same moves as the scan) once the
+// coarsest graph is large enough for O(log n) queue steps to beat an O(n/64)
+// scan per move.
+bool fm_uses_heap(int32_t fm_kind, int32_t n)
+{
+    if (fm_kind >= 0) return fm_kind != 0;
+    return n >= kAutoHeapMinVertices;
+}
+
 }  // namespace
 
 void initial_partition(Ctx &c, const DevGraph &g, int32_t k, const Params &p, int32_t *where)
@@ -549,9 +802,11 @@ void initial_partition(Ctx &c, const DevGraph &g, int32_t k, const Params &p, in
     }
 
     double ub = std::pow((double)p.ub_num / (double)p.ub_den, 1.0 / std::log((double)k));
-    int32_t lds_cap = kLdsBudget / kTrialBytesPerVertex;
+    const bool heap = fm_uses_heap(p.fm_kind, n);
+    const int32_t fm_kind = p.fm_kind < 0 ? (heap ? 1 : 0) : p.fm_kind;
+    int32_t lds_cap = (kLdsBudget - 64) / (kTrialBytesPerVertex + (heap ? kHeapBytesPerVertex : 0));
     int32_t lds_n = std::min(n, lds_cap);
-    size_t lds_bytes = (size_t)(((int64_t)2 * lds_n + 3) & ~3ll) + (size_t)lds_n * 12 + 16;
+    size_t lds_bytes = (size_t)trial_bytes(lds_n, heap) + 16;
 
     size_t mark = c.arena.right_mark();
     Node *nodes = c.arena.ralloc<Node>((size_t)ntab);
@@ -570,8 +825,10 @@ void initial_partition(Ctx &c, const DevGraph &g, int32_t k, const Params &p, in
     uint8_t *trial_where = c.arena.ralloc<uint8_t>((size_t)(maxtrials * n));
     char *gscratch = nullptr;
     if (n > lds_cap) {
-        // sum over nodes of T * round256(14 nn) <= maxtrials * (14 n + 256 * (n / lds_cap + 1))
-        size_t gs = (size_t)maxtrials * ((size_t)n * kTrialBytesPerVertex + 256 * ((size_t)(n / lds_cap) + 1));
+        // sum over nodes of T * round256(trial_bytes(nn)): trial_bytes is affine
+        // in nn with a constant below 20, and at most n / lds_cap + 1 nodes
+        // use global scratch
+        size_t gs = (size_t)maxtrials * ((size_t)trial_bytes(n, heap) + 512 * ((size_t)(n / lds_cap) + 1));
         gscratch = c.arena.ralloc<char>(gs);
     }
     // trials of one depth: sum of min(trials, n_node) <= n
@@ -610,13 +867,14 @@ void initial_partition(Ctx &c, const DevGraph &g, int32_t k, const Params &p, in
         const int32_t *act = act_all + act_off[d];
         k_ip_setup<<<nact, 256, 0, c.stream>>>(act, nodes, fvw[cb], ub, p.trials);
         HGP_LAUNCH_CHECK();
-        k_ip_offsets<<<1, 64, 0, c.stream>>>(act, nact, nodes, lds_cap, total);
+        k_ip_offsets<<<1, 64, 0, c.stream>>>(act, nact, nodes, lds_cap, heap, total);
         HGP_LAUNCH_CHECK();
         int64_t waves = std::min<int64_t>((int64_t)nact * p.trials, n);
         KScope ks(c, "k_ip_trials", 0.0);
-        k_ip_trials<<<(unsigned)std::max<int64_t>(waves, 1), kWave, lds_bytes, c.stream>>>(
-            act, nact, nodes, total, fx[cb], fadj[cb], fw[cb], fvw[cb], p.seed, p.fm_passes, lds_cap, trial_where,
-            gscratch, tcut);
+        auto trials_kernel = heap ? k_ip_trials<true> : k_ip_trials<false>;
+        trials_kernel<<<(unsigned)std::max<int64_t>(waves, 1), kWave, lds_bytes, c.stream>>>(
+            act, nact, nodes, total, fx[cb], fadj[cb], fw[cb], fvw[cb], p.seed, p.fm_passes, fm_kind, lds_cap,
+            trial_where, gscratch, tcut);
         HGP_LAUNCH_CHECK();
         k_ip_select<<<nact, 256, 0, c.stream>>>(act, nodes, tcut);
         HGP_LAUNCH_CHECK();

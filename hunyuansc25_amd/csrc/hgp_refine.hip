@@ -2252,6 +2252,11 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32);
         read_scalars_wait(c);
         JetCtl h = *(JetCtl *)c.pinned;
+        if (jet_trace().on) {
+            std::lock_guard<std::mutex> lk(jet_trace().mu);
+            jet_trace().recs.insert(jet_trace().recs.end(),
+                                    {(int64_t)n, (int64_t)h.iters, (int64_t)h.mode, h.cut, (int64_t)h.done});
+        }
         if (h.done) break;
         if (h.mode == kModeLP) {
             {

@@ -176,6 +176,12 @@ int64_t KTimer::counter_slot()
     return s;
 }
 
+JetTrace &jet_trace()
+{
+    static JetTrace t;
+    return t;
+}
+
 KScope::KScope(Ctx &c, const char *name, double bytes, bool counted)
     : c_(c), on_(ktimer().on), name_(name), bytes_(bytes)
 {

@@ -62,6 +62,11 @@ typedef struct hgp_options {
     int32_t validate;       /* 1: also check CSR symmetry before running         */
     int32_t device;         /* HIP device ordinal (default 0)                    */
     int32_t level_stats;    /* 1: fill level_adjwgt and level_ms of hgp_stats    */
+    int32_t fm_kind;        /* FM of the bisection trials: 0 wave arg-max scan,  */
+                            /* 1 device priority queues with the same moves,     */
+                            /* 2 the reference's priority-queue FM               */
+                            /* (FM_2way_cut_refinement); -1 (default) picks 0 or */
+                            /* 1 by the coarsest graph's size (same result)      */
 } hgp_options;
 
 #define HGP_MAX_LEVELS 64
@@ -133,6 +138,19 @@ int hgp_check_graph_device(int32_t n, const int64_t *d_xadj, const int32_t *d_ad
  * `device` and waits for it. */
 int hgp_evaluate_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy, const int32_t *d_adjwgt,
                         const int32_t *d_vwgt, const int32_t *d_part, int32_t k, int32_t device, int64_t *d_out);
+
+
+/* Trace of the Jet refinement loops (the reference prints per-iteration
+ * state only under debug macros, hunyuangraph_GPU_krefine.h:4434-4882).
+ * hgp_jet_trace_enable(1) clears and starts recording, 0 stops; while on,
+ * every read of a level's control block appends 5 int64: the level's vertex
+ * count, the iterations done, the mode of the coming iteration (0 label
+ * propagation, 1 weak rebalance, 2 strong rebalance), the cut so far, and 1
+ * on the read that ends the loop (else 0).  Levels of <= 512 vertices run
+ * their loop in one kernel and record nothing.  hgp_jet_trace copies up to
+ * cap values into buf and returns how many are recorded. */
+int hgp_jet_trace_enable(int on);
+int64_t hgp_jet_trace(int64_t *buf, int64_t cap);
 
 /* ------------------------------------------------------------ batches
  * No counterpart in the reference, which partitions one graph per call.

@@ -27,6 +27,7 @@ class OracleOpts(C.Structure):
         ("trials", C.c_int32),
         ("fm_passes", C.c_int32),
         ("patience", C.c_int32),
+        ("fm_kind", C.c_int32),
     ]
 
 
@@ -85,7 +86,10 @@ _SIG = {
     "oracle_dga_match": (_I64, [_I32, _P, _P, _P, _P, _I32, _U64, _I32, _I32, _P]),
     "oracle_edge_hash": (_U32, [_U64, _I32, _I32, _I32]),
     "oracle_dga_initpart": (None, [_I32, _P, _P, _P, _P, _I32, _P, _P]),
-    "oracle_dga_trial": (_I64, [_I32, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I32, _I32, _U64, _U32, _I32, _P]),
+    "oracle_jet_trace": (None, [_P, _I64]),
+    "oracle_jet_trace_len": (_I64, []),
+    "oracle_heap_run": (_I32, [_I32, _I32, _P, _P, _P, _I32, _P, _P, _P, _P]),
+    "oracle_dga_trial": (_I64, [_I32, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I32, _I32, _U64, _U32, _I32, _I32, _P]),
     "oracle_dga_refine": (_I32, [_I32, _P, _P, _P, _P, _I32, _I64, _I32, _P, _P, _P]),
     "oracle_jet_lp_step": (_I64, [_I32, _P, _P, _P, _I32, _P, _I32, _I32, _P, _P]),
     "oracle_gain_bucket": (_I32, [_I64, _I64]),
@@ -252,13 +256,55 @@ def dga_initpart(xadj, adjncy, vwgt, adjwgt, k, **kw):
     return where
 
 
-def dga_trial(xadj, adjncy, vwgt, adjwgt, tvwgt, onemin, onemax, ideal0, ntrials, trial, seed, node_id, fm_passes):
+def dga_trial(xadj, adjncy, vwgt, adjwgt, tvwgt, onemin, onemax, ideal0, ntrials, trial, seed, node_id, fm_passes,
+              fm_kind=0):
     xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
     n = len(xadj) - 1
     w = np.zeros(n, dtype=np.uint8)
     cut = lib().oracle_dga_trial(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(tvwgt), int(onemin), int(onemax),
-                                 int(ideal0), int(ntrials), int(trial), int(seed), int(node_id), int(fm_passes), _p(w))
+                                 int(ideal0), int(ntrials), int(trial), int(seed), int(node_id), int(fm_passes),
+                                 int(fm_kind), _p(w))
     return w, int(cut)
+
+
+_trace_buf = None
+
+
+def jet_trace_start(cap=1 << 22):
+    """Starts recording the Jet loops of the following oracle calls."""
+    global _trace_buf
+    _trace_buf = np.zeros(cap, dtype=np.int64)
+    lib().oracle_jet_trace(_p(_trace_buf), cap)
+
+
+def jet_trace_stop():
+    """Stops recording; returns rows of (level vertices, iterations done,
+    mode of the coming iteration, cut so far, loop ended)."""
+    global _trace_buf
+    n = int(lib().oracle_jet_trace_len())
+    out = _trace_buf[:min(n, len(_trace_buf))].reshape(-1, 5).copy()
+    lib().oracle_jet_trace(None, 0)
+    _trace_buf = None
+    return out
+
+
+def heap_run(nv, ops, ties=False):
+    """Applies ops = [(op, v, key), ...] (op 0 insert, 1 top, 2 delete,
+    3 update) to one device priority queue; returns (tops, vals, keys, loc):
+    the vertex each top returned (None for other ops), the final slots and
+    the final position index."""
+    m = len(ops)
+    op = np.array([o[0] for o in ops], dtype=np.int32)
+    vv = np.array([o[1] for o in ops], dtype=np.int32)
+    kk = np.array([o[2] for o in ops], dtype=np.int64)
+    tops = np.full(max(m, 1), -2, dtype=np.int32)
+    fv = np.zeros(max(nv, 1), dtype=np.int32)
+    fk = np.zeros(max(nv, 1), dtype=np.int64)
+    fl = np.zeros(max(nv, 1), dtype=np.int32)
+    cnt = lib().oracle_heap_run(int(nv), int(m), _p(op), _p(vv), _p(kk), 1 if ties else 0, _p(tops), _p(fv), _p(fk),
+                                _p(fl))
+    return ([int(tops[i]) if op[i] == 1 else None for i in range(m)], fv[:cnt].tolist(), fk[:cnt].tolist(),
+            fl[:nv].tolist())
 
 
 def dga_refine(xadj, adjncy, vwgt, adjwgt, k, tvwgt, is_finest, where, cut, **kw):

@@ -89,6 +89,8 @@ typedef struct oracle_opts {
     int32_t trials;             /* initial-bisection trials, default 1024        */
     int32_t fm_passes;          /* FM passes per trial, default 10               */
     int32_t patience;           /* Jet non-improving rounds, default 6           */
+    int32_t fm_kind;            /* 0 wave arg-max FM, 1 heap FM with the same    */
+                                /* moves, 2 the reference's heap FM (default 0)  */
 } oracle_opts;
 
 typedef struct oracle_stats {
@@ -152,7 +154,24 @@ int64_t oracle_dga_trial(int32_t n, const int64_t *xadj, const int32_t *adjncy,
                          const int32_t *vwgt, const int32_t *adjwgt,
                          int64_t tvwgt, int64_t onemin, int64_t onemax, int64_t ideal0,
                          int32_t ntrials, int32_t trial, uint64_t seed, uint32_t node_id,
-                         int32_t fm_passes, uint8_t *where);
+                         int32_t fm_passes, int32_t fm_kind, uint8_t *where);
+
+/* The device priority queue alone (hunyuangraph_GPU_priorityqueue.h:27-225):
+ * applies nops operations (op 0 insert (v, key), 1 top, 2 delete v,
+ * 3 update (v, key)) to one queue over vertices [0, nv); tops[i] receives
+ * what a top returned; final_val/final_key (nv slots) and final_loc (nv) the
+ * final layout.  ties != 0 orders equal keys by the smaller vertex.  Returns
+ * the final count. */
+int32_t oracle_heap_run(int32_t nv, int32_t nops, const int32_t *op, const int32_t *v, const int64_t *key,
+                        int32_t ties, int32_t *tops, int32_t *final_val, int64_t *final_key, int32_t *final_loc);
+
+/* Trace of the Jet loops of the following calls into buf (cap int64; NULL
+ * stops): records of 5 int64 (level vertices, iterations done, mode of the
+ * coming iteration, cut so far, 1 on the loop's last record), as the GPU
+ * library's hgp_jet_trace; levels of <= 512 vertices record nothing.
+ * oracle_jet_trace_len returns the values recorded (also past cap). */
+void oracle_jet_trace(int64_t *buf, int64_t cap);
+int64_t oracle_jet_trace_len(void);
 
 /* Jet refinement of one level in place.  Returns the number of iterations.
  * cut is in/out. */

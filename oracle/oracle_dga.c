@@ -49,6 +49,7 @@ void oracle_default_opts(oracle_opts *o)
     o->match_rounds = 8;
     o->trials = 1024;
     o->fm_passes = 10;
+    o->fm_kind = 0;
     o->patience = 6;
 }
 
@@ -336,6 +337,241 @@ static int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
 static int64_t imax64(int64_t a, int64_t b) { return a > b ? a : b; }
 static int64_t iabs64(int64_t a) { return a < 0 ? -a : a; }
 
+/* ------------------------------------------------ device priority queue
+ * Binary max-heap of vertices with a position index, restating
+ * priority_queue_Insert/_Delete/_Update/_GetTop
+ * (hunyuangraph_GPU_priorityqueue.h:27-225): a key moves above another when
+ * it is strictly greater; a removal refills the hole from the last slot and
+ * sifts that element up or down.  With `ties` set, equal keys are ordered by
+ * the smaller vertex id, so the top is the unique (gain desc, id asc)
+ * maximum; without it equal keys keep the reference's heap order. */
+typedef struct {
+    int32_t cnt;
+    int64_t *key;
+    int32_t *val;
+} oheap;
+
+static int heap_above(int64_t ka, int32_t va, int64_t kb, int32_t vb, int ties)
+{
+    return ka > kb || (ties && ka == kb && va < vb);
+}
+
+static void heap_place(oheap *h, int32_t *loc, int32_t i, int64_t key, int32_t v)
+{
+    h->key[i] = key;
+    h->val[i] = v;
+    loc[v] = i;
+}
+
+static void heap_up(oheap *h, int32_t *loc, int32_t i, int64_t key, int32_t v, int ties)
+{
+    while (i > 0) {
+        int32_t j = (i - 1) >> 1;
+        if (!heap_above(key, v, h->key[j], h->val[j], ties)) break;
+        heap_place(h, loc, i, h->key[j], h->val[j]);
+        i = j;
+    }
+    heap_place(h, loc, i, key, v);
+}
+
+static void heap_down(oheap *h, int32_t *loc, int32_t i, int64_t key, int32_t v, int ties)
+{
+    int32_t j;
+    while ((j = 2 * i + 1) < h->cnt) {
+        if (heap_above(h->key[j], h->val[j], key, v, ties)) {
+            if (j + 1 < h->cnt && heap_above(h->key[j + 1], h->val[j + 1], h->key[j], h->val[j], ties)) j++;
+        } else if (j + 1 < h->cnt && heap_above(h->key[j + 1], h->val[j + 1], key, v, ties)) {
+            j++;
+        } else {
+            break;
+        }
+        heap_place(h, loc, i, h->key[j], h->val[j]);
+        i = j;
+    }
+    heap_place(h, loc, i, key, v);
+}
+
+static void heap_insert(oheap *h, int32_t *loc, int32_t v, int64_t key, int ties)
+{
+    heap_up(h, loc, h->cnt++, key, v, ties);
+}
+
+static int32_t heap_top(oheap *h, int32_t *loc, int ties)
+{
+    if (h->cnt == 0) return -1;
+    int32_t top = h->val[0];
+    loc[top] = -1;
+    if (--h->cnt > 0) heap_down(h, loc, 0, h->key[h->cnt], h->val[h->cnt], ties);
+    return top;
+}
+
+static void heap_delete(oheap *h, int32_t *loc, int32_t v, int ties)
+{
+    int32_t i = loc[v];
+    loc[v] = -1;
+    if (--h->cnt > 0 && h->val[h->cnt] != v) {
+        int64_t nk = h->key[h->cnt], ok = h->key[i];
+        int32_t nv = h->val[h->cnt];
+        if (heap_above(nk, nv, ok, v, ties)) heap_up(h, loc, i, nk, nv, ties);
+        else heap_down(h, loc, i, nk, nv, ties);
+    }
+}
+
+static void heap_update(oheap *h, int32_t *loc, int32_t v, int64_t key, int ties)
+{
+    int32_t i = loc[v];
+    if (key > h->key[i]) heap_up(h, loc, i, key, v, ties);
+    else heap_down(h, loc, i, key, v, ties);
+}
+
+/* Runs a sequence of operations on one queue over vertices [0, nv):
+ * op 0 insert (v, key), 1 top, 2 delete v, 3 update (v, key).  tops[i]
+ * receives the vertex a top returned (-1 when empty; unchanged for other
+ * ops).  The final slots (vertex, key) and the position index are written
+ * out; returns the final count.  Exposed for tests of the queue alone. */
+int32_t oracle_heap_run(int32_t nv, int32_t nops, const int32_t *op, const int32_t *v, const int64_t *key,
+                        int32_t ties, int32_t *tops, int32_t *final_val, int64_t *final_key, int32_t *final_loc)
+{
+    oheap h;
+    h.cnt = 0;
+    h.key = final_key;
+    h.val = final_val;
+    for (int32_t i = 0; i < nv; i++) final_loc[i] = -1;
+    for (int32_t i = 0; i < nops; i++) {
+        switch (op[i]) {
+        case 0: heap_insert(&h, final_loc, v[i], key[i], ties); break;
+        case 1: tops[i] = heap_top(&h, final_loc, ties); break;
+        case 2: heap_delete(&h, final_loc, v[i], ties); break;
+        default: heap_update(&h, final_loc, v[i], key[i], ties); break;
+        }
+    }
+    return h.cnt;
+}
+
+/* FM passes driven by two device priority queues, one per side
+ * (FM_2way_cut_refinement, hunyuangraph_GPU_initialpartition.h:160-300):
+ * every pass inserts the boundary vertices in id order with key ed - id,
+ * takes the top of the heavier side's queue, and keeps the queues current
+ * as the neighbours' gains change (delete when a neighbour leaves the
+ * boundary, update, insert when one joins it; moved vertices are never
+ * queued again in the pass), then rolls back to the best prefix.
+ *   fm_kind 1: keys ordered (gain desc, id asc) and the acceptance rule of
+ *     the live wave FM (:3244-3505, the part bounds): the move sequence is
+ *     exactly that of the arg-max scan, found in O(log n) per step.
+ *   fm_kind 2: the reference's queue semantics (equal gains in heap order)
+ *     and its balance rule (|tpwgts0 - pwgts0| <= origdiff + avgvwgt,
+ *     avgvwgt = min(tvwgt/20, 2 tvwgt/n)), with the move's weights applied
+ *     before the test as in hunyuangraph_CPU_2wayrefine.h:133-145 (the
+ *     device version tests the weights before the move, :196-203).
+ * ed/id/bnd/where/pw are the trial's state after growing; returns the cut. */
+static int64_t fm_heap_passes(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                              const int32_t *adjwgt, int64_t tvwgt, int64_t onemax, int64_t ideal0,
+                              int32_t fm_passes, int fm_kind, int64_t cut, uint8_t *where, int64_t *ed,
+                              int64_t *id, uint8_t *bnd, uint8_t *moved, int32_t *swaps, int64_t *pw)
+{
+    const int ties = fm_kind == 1;
+    int64_t *keys = (int64_t *)malloc(sizeof(int64_t) * (size_t)(2 * n + 2));
+    int32_t *vals = (int32_t *)malloc(sizeof(int32_t) * (size_t)(2 * n + 2));
+    int32_t *loc = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    oheap q[2];
+    q[0].key = keys; q[0].val = vals;
+    q[1].key = keys + n + 1; q[1].val = vals + n + 1;
+    for (int32_t v = 0; v < n; v++) loc[v] = -1;
+    int64_t ideal[2] = {ideal0, tvwgt - ideal0};
+    int64_t limit = imin64(imax64(n / 100, 15), 100);
+    int64_t avgvwgt = imin64((pw[0] + pw[1]) / 20, 2 * (pw[0] + pw[1]) / n);
+    int64_t origdiff = iabs64(ideal[0] - pw[0]);
+    for (int32_t pass = 0; pass < fm_passes; pass++) {
+        int64_t mincutorder = -1;
+        int64_t newcut = cut, mincut = cut, initcut = cut;
+        int64_t mindiff = iabs64(ideal[0] - pw[0]);
+        q[0].cnt = q[1].cnt = 0;
+        for (int32_t v = 0; v < n; v++)
+            if (bnd[v]) heap_insert(&q[where[v]], loc, v, ed[v] - id[v], ties);
+        int32_t nswaps;
+        for (nswaps = 0; nswaps < n; nswaps++) {
+            int from = (ideal[0] - pw[0] < ideal[1] - pw[1]) ? 0 : 1;
+            int to = from ^ 1;
+            int32_t v = heap_top(&q[from], loc, ties);
+            if (v == -1) break;
+            int64_t g = ed[v] - id[v];
+            newcut -= g;
+            int accept;
+            if (fm_kind == 1) {
+                int64_t dd = iabs64(ideal[0] - pw[0]);
+                accept = (newcut < mincut && pw[to] + vwgt[v] <= onemax && pw[from] - vwgt[v] <= onemax) ||
+                         (newcut == mincut && dd < mindiff);
+                if (accept) mindiff = dd;
+                pw[to] += vwgt[v];
+                pw[from] -= vwgt[v];
+            } else {
+                pw[to] += vwgt[v];
+                pw[from] -= vwgt[v];
+                int64_t dd = iabs64(ideal[0] - pw[0]);
+                accept = (newcut < mincut && dd <= origdiff + avgvwgt) || (newcut == mincut && dd < mindiff);
+                if (accept) mindiff = dd;
+            }
+            if (accept) {
+                mincut = newcut;
+                mincutorder = nswaps;
+            } else if (nswaps - mincutorder > limit) {
+                newcut += g;
+                pw[from] += vwgt[v];
+                pw[to] -= vwgt[v];
+                break;
+            }
+            where[v] = (uint8_t)to;
+            moved[v] = 1;
+            swaps[nswaps] = v;
+            { int64_t t = ed[v]; ed[v] = id[v]; id[v] = t; }
+            if (ed[v] == 0 && xadj[v] < xadj[v + 1]) bnd[v] = 0;
+            for (int64_t j = xadj[v]; j < xadj[v + 1]; j++) {
+                int32_t x = adjncy[j];
+                int64_t kw = (where[x] == to) ? (int64_t)adjwgt[j] : -(int64_t)adjwgt[j];
+                id[x] += kw;
+                ed[x] -= kw;
+                if (bnd[x]) {
+                    if (ed[x] == 0) {
+                        bnd[x] = 0;
+                        if (!moved[x]) heap_delete(&q[where[x]], loc, x, ties);
+                    } else if (!moved[x]) {
+                        heap_update(&q[where[x]], loc, x, ed[x] - id[x], ties);
+                    }
+                } else if (ed[x] > 0) {
+                    bnd[x] = 1;
+                    if (!moved[x]) heap_insert(&q[where[x]], loc, x, ed[x] - id[x], ties);
+                }
+            }
+        }
+        for (int32_t i = 0; i < nswaps; i++) moved[swaps[i]] = 0;
+        for (int32_t i = 0; i < q[0].cnt; i++) loc[q[0].val[i]] = -1;
+        for (int32_t i = 0; i < q[1].cnt; i++) loc[q[1].val[i]] = -1;
+        for (nswaps--; nswaps > mincutorder; nswaps--) {
+            int32_t v = swaps[nswaps];
+            int from = where[v];
+            int to = from ^ 1;
+            where[v] = (uint8_t)to;
+            { int64_t t = ed[v]; ed[v] = id[v]; id[v] = t; }
+            if (ed[v] == 0 && bnd[v] && xadj[v] < xadj[v + 1]) bnd[v] = 0;
+            else if (ed[v] > 0 && !bnd[v]) bnd[v] = 1;
+            pw[to] += vwgt[v];
+            pw[from] -= vwgt[v];
+            for (int64_t j = xadj[v]; j < xadj[v + 1]; j++) {
+                int32_t x = adjncy[j];
+                int64_t kw = (where[x] == from) ? -(int64_t)adjwgt[j] : (int64_t)adjwgt[j];
+                id[x] += kw;
+                ed[x] -= kw;
+                if (bnd[x] && ed[x] == 0) bnd[x] = 0;
+                else if (!bnd[x] && ed[x] > 0) bnd[x] = 1;
+            }
+        }
+        cut = mincut;
+        if (mincutorder <= 0 || mincut == initcut) break;
+    }
+    free(keys); free(vals); free(loc);
+    return cut;
+}
+
 /* One GGGP + FM trial, one wave per trial on the GPU
  * (hunyuangraph_gpu_BFS_warp_2wayrefine_noedit[_sampling],
  * hunyuangraph_GPU_initialpartition.h:2426-3659).  Start vertices come from a
@@ -345,7 +581,7 @@ int64_t oracle_dga_trial(int32_t n, const int64_t *xadj, const int32_t *adjncy,
                          const int32_t *vwgt, const int32_t *adjwgt,
                          int64_t tvwgt, int64_t onemin, int64_t onemax, int64_t ideal0,
                          int32_t ntrials, int32_t trial, uint64_t seed, uint32_t node_id,
-                         int32_t fm_passes, uint8_t *where)
+                         int32_t fm_passes, int32_t fm_kind, uint8_t *where)
 {
     int32_t *queue = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
     int64_t *ed = (int64_t *)malloc(sizeof(int64_t) * (size_t)n);
@@ -417,6 +653,14 @@ int64_t oracle_dga_trial(int32_t n, const int64_t *xadj, const int32_t *adjncy,
     }
     cut /= 2;
     memset(moved, 0, (size_t)n);
+
+    if (fm_kind != 0) {
+        int64_t pwh[2] = {pw[0], pw[1]};
+        cut = fm_heap_passes(n, xadj, adjncy, vwgt, adjwgt, tvwgt, onemax, ideal0, fm_passes, fm_kind, cut,
+                             where, ed, id, bnd, moved, queue, pwh);
+        free(queue); free(ed); free(id); free(moved); free(bnd);
+        return cut;
+    }
 
     /* FM passes (:3244-3505) */
     int32_t *swaps = queue;
@@ -523,7 +767,7 @@ static void rb_node(rbnode *nd, const oracle_opts *o, double ub, int32_t *answer
     int64_t bcut = -1;
     for (int32_t t = 0; t < T; t++) {
         int64_t c = oracle_dga_trial(n, nd->xadj, nd->adjncy, nd->vwgt, nd->adjwgt, tv, onemin,
-                                     onemax, ideal0, T, t, o->seed, nd->id, o->fm_passes, w);
+                                     onemax, ideal0, T, t, o->seed, nd->id, o->fm_passes, o->fm_kind, w);
         if (bcut < 0 || c < bcut) { bcut = c; memcpy(bw, w, (size_t)n); }
     }
     for (int32_t v = 0; v < n; v++) answer[nd->label[v]] = nd->fpart + bw[v];
@@ -964,6 +1208,30 @@ int oracle_jet_accept(int64_t cur_cut, int64_t curmax, int64_t maxwgt, int64_t *
  * ub bound floor(ub_num*tvwgt/(ub_den*k)) (the reference uses (int)(opt*1.04)).
  * Imbalance comparisons of the acceptance rule (:4805-4839) are made on the
  * maximum part weight, which orders identically to max(pwgts)/opt. */
+/* Trace of the refinement loops, in the records the GPU library's trace
+ * uses (level vertices, iterations done, mode of the coming iteration, cut
+ * so far, loop ended); levels of <= 512 vertices record nothing, as there. */
+static int64_t *g_trace_buf, g_trace_cap, g_trace_len;
+
+void oracle_jet_trace(int64_t *buf, int64_t cap)
+{
+    g_trace_buf = buf;
+    g_trace_cap = buf ? cap : 0;
+    g_trace_len = 0;
+}
+
+int64_t oracle_jet_trace_len(void) { return g_trace_len; }
+
+static void trace_rec(int32_t n, int32_t iters, int32_t mode, int64_t cut, int32_t done)
+{
+    if (!g_trace_buf || n <= 512) return;
+    int64_t r[5] = {n, iters, mode, cut, done};
+    for (int i = 0; i < 5; i++) {
+        if (g_trace_len < g_trace_cap) g_trace_buf[g_trace_len] = r[i];
+        g_trace_len++;
+    }
+}
+
 int32_t oracle_dga_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
                           const int32_t *vwgt, const int32_t *adjwgt, int32_t k,
                           int64_t tvwgt, int32_t is_finest, const oracle_opts *opts,
@@ -985,6 +1253,7 @@ int32_t oracle_dga_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
     int32_t count = 0, balance_counter = 0, iters = 0;
     while (count++ < opts->patience) {
         int64_t curmax = max_of(pwgts, k);
+        trace_rec(n, iters, curmax <= maxwgt ? 0 : (balance_counter < 2 ? 1 : 2), cur_cut, 0);
         if (curmax <= maxwgt) {
             oracle_jet_lp_step(n, xadj, adjncy, adjwgt, k, where, fnum, fden, lock, dest);
             balance_counter = 0;
@@ -999,6 +1268,7 @@ int32_t oracle_dga_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
         if (oracle_jet_accept(cur_cut, curmax, maxwgt, &best_cut, &best_max, &count))
             memcpy(best_where, where, sizeof(int32_t) * (size_t)n);
     }
+    trace_rec(n, iters, 0, cur_cut, 1);
     memcpy(where, best_where, sizeof(int32_t) * (size_t)n);
     *cut = best_cut;
     free(pwgts); free(best_where); free(dest); free(lock); free(c.conn); free(c.touched);

@@ -9,6 +9,17 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# PyTorch (the device buffers of some GPU tests) ships its own HIP runtime and
+# loads it as torch/lib/libamdhip64.so, a name the dynamic loader does not
+# match against the libamdhip64.so.7 that libhgp.so links.  If libhgp.so is
+# loaded first, torch then loads a second HIP runtime, and whichever of the
+# two initialises the GPU second finds no device.  Importing torch before the
+# library makes libhgp.so bind to torch's runtime: one runtime per process.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
