@@ -47,6 +47,17 @@ class HgpOptions(C.Structure):
     ]
 
 
+class HgpTuning(C.Structure):
+    _fields_ = [
+        ("tiny_max_vertices", C.c_int32),
+        ("few_rows", C.c_int32),
+        ("fill_waves", C.c_int32),
+        ("seg_grid", C.c_int32),
+        ("batch_lanes", C.c_int32),
+        ("hub_track", C.c_int32),
+    ]
+
+
 class HgpStats(C.Structure):
     _fields_ = [
         ("cut", C.c_int64),
@@ -121,13 +132,22 @@ SIGNATURES = {
     "hgp_partition_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "hgp_check_graph_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I32]),
     "hgp_evaluate_device": (C.c_int, [_I32, _P, _P, _P, _P, _P, _I32, _I32, _P]),
+    "hgp_partition_cpu": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _U64, _P, _P]),
+    "hgp_get_tuning": (C.c_int, [_P]),
+    "hgp_set_tuning": (C.c_int, [_P]),
     "hgp_jet_trace_enable": (C.c_int, [C.c_int]),
     "hgp_jet_trace": (C.c_int64, [_P, C.c_int64]),
+    "hgp_jet_trace_dump": (C.c_int, [C.c_int64, C.c_int32]),
+    "hgp_jet_trace_dumped": (C.c_int64, [_P, _P, C.c_int64]),
     "hgp_partition_batch": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "hgp_server_create": (C.c_int, [_P, _I32, _I64, _P]),
     "hgp_server_submit": (C.c_int64, [_P, _I32, _P, _P, _P, _P, _I32, _U64, _P]),
     "hgp_server_wait": (C.c_int, [_P, _I64, _P, _P]),
     "hgp_server_info": (C.c_int, [_P, _P, _P, _P]),
+    "hgp_server_register": (C.c_int64, [_P, _I32, _P, _P, _P, _P]),
+    "hgp_server_submit_graph": (C.c_int64, [_P, C.c_int64, _I32, _U64, _P]),
+    "hgp_server_unregister": (C.c_int, [_P, C.c_int64]),
+    "hgp_server_kept": (C.c_int, [_P, _P, _P, _P, _P]),
     "hgp_server_destroy": (C.c_int, [_P]),
     "hgp_ktimer_enable": (C.c_int, [C.c_int]),
     "hgp_ktimer_report": (C.c_int64, [_P, _I64]),
@@ -166,7 +186,51 @@ def lib() -> C.CDLL:
             fn.restype = res
             fn.argtypes = args
         _lib_handle = h
+        _apply_tuning_file()
     return _lib_handle
+
+
+def partition_cpu(xadj, adjncy, k, vwgt=None, adjwgt=None, seed=1):
+    """The reference's CPU recursive bisection on the host; returns (part, cut)."""
+    xadj, adjncy = _i64(xadj), _i32(adjncy)
+    n = len(xadj) - 1
+    vw = None if vwgt is None else _i32(vwgt)
+    aw = None if adjwgt is None else _i32(adjwgt)
+    part = np.zeros(max(n, 1), np.int32)
+    cut = np.zeros(1, np.int64)
+    _check(lib().hgp_partition_cpu(n, _ptr(xadj), _ptr(adjncy), _ptr(vw), _ptr(aw), int(k), int(seed), _ptr(part),
+                                   _ptr(cut)))
+    return part[:n], int(cut[0])
+
+
+TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+
+
+def get_tuning() -> dict:
+    t = HgpTuning()
+    _check(lib().hgp_get_tuning(C.byref(t)))
+    return {name: getattr(t, name) for name, _ in HgpTuning._fields_}
+
+
+def set_tuning(**kw) -> None:
+    """Sets scheduling knobs (see hgp_tuning); results never depend on them."""
+    t = HgpTuning()
+    _check(lib().hgp_get_tuning(C.byref(t)))
+    for name, v in kw.items():
+        if name not in dict(HgpTuning._fields_):
+            raise KeyError(name)
+        setattr(t, name, int(v))
+    _check(lib().hgp_set_tuning(C.byref(t)))
+
+
+def _apply_tuning_file() -> None:
+    # values measured by autotune.py on this kind of device, if present
+    if os.environ.get("HGP_NO_TUNING_FILE") or not os.path.exists(TUNING_PATH):
+        return
+    import json
+    with open(TUNING_PATH) as f:
+        vals = json.load(f).get("tuning", {})
+    set_tuning(**{k: v for k, v in vals.items() if k in dict(HgpTuning._fields_)})
 
 
 class HgpError(RuntimeError):
@@ -302,6 +366,7 @@ class Server:
         o = options(**opts)
         _check(lib().hgp_server_create(C.byref(o), int(slots), int(budget_bytes), C.byref(self._h)))
         self._live = {}
+        self._graph_n = {}
         import threading
         self._mu = threading.Lock()
         self._waiting = set()
@@ -318,6 +383,35 @@ class Server:
         self._live[t] = (x, a, vw, aw, part)
         return t
 
+    def register(self, xadj, adjncy, vwgt=None, adjwgt=None) -> int:
+        """Uploads a graph once; returns its id for submit_graph()."""
+        x, a = _i64(xadj), _i32(adjncy)
+        vw = None if vwgt is None else _i32(vwgt)
+        aw = None if adjwgt is None else _i32(adjwgt)
+        g = lib().hgp_server_register(self._h, len(x) - 1, _ptr(x), _ptr(a), _ptr(vw), _ptr(aw))
+        if g < 0:
+            _check(int(g))
+        self._graph_n[g] = len(x) - 1
+        return g
+
+    def submit_graph(self, graph, k, seed=1) -> int:
+        """Queues a partition of a registered graph; returns a ticket."""
+        part = np.zeros(max(self._graph_n.get(graph, 1), 1), np.int32)
+        t = lib().hgp_server_submit_graph(self._h, int(graph), int(k), int(seed), _ptr(part))
+        if t < 0:
+            _check(int(t))
+        self._live[t] = (part,)
+        return t
+
+    def unregister(self, graph) -> None:
+        _check(lib().hgp_server_unregister(self._h, int(graph)))
+        self._graph_n.pop(graph, None)
+
+    def kept(self) -> dict:
+        h, m, kn, kb = (C.c_int64() for _ in range(4))
+        _check(lib().hgp_server_kept(self._h, C.byref(h), C.byref(m), C.byref(kn), C.byref(kb)))
+        return {"hits": h.value, "misses": m.value, "kept": kn.value, "kept_bytes": kb.value}
+
     def wait(self, ticket):
         """A ticket is waited for once: a second wait raises HgpError (-1)
         and leaves the first waiter's job alone."""
@@ -331,7 +425,7 @@ class Server:
             rc = lib().hgp_server_wait(self._h, int(ticket), C.byref(st), C.byref(lat))
             arrs = self._live.pop(ticket, None) if rc == HGP_OK or ticket in self._live else None
             _check(rc)
-            return arrs[4], st.to_dict(), lat.value
+            return arrs[-1], st.to_dict(), lat.value
         finally:
             with self._mu:
                 self._waiting.discard(ticket)
@@ -380,7 +474,6 @@ def evaluate_device(n, d_xadj, d_adjncy, d_adjwgt, d_vwgt, d_part, k, d_out, dev
                                      C.c_void_p(d_part), int(k), int(device), C.c_void_p(d_out)))
 
 
-
 # ------------------------------------------------------------------ Jet trace
 
 def jet_trace_enable(on: bool = True) -> None:
@@ -394,6 +487,22 @@ def jet_trace() -> np.ndarray:
     buf = np.zeros(max(total, 1), dtype=np.int64)
     lib().hgp_jet_trace(buf.ctypes.data_as(C.c_void_p), total)
     return buf[:total].reshape(-1, 5)
+
+
+def jet_trace_dump(n, it) -> None:
+    """While tracing, keep where/dest of iteration `it` of the level with n vertices."""
+    _check(lib().hgp_jet_trace_dump(int(n), int(it)))
+
+
+def jet_trace_dumped():
+    """(where before the moves, dest) of the kept iteration, or None."""
+    n = int(lib().hgp_jet_trace_dumped(None, None, 0))
+    if n == 0:
+        return None
+    w = np.zeros(n, dtype=np.int32)
+    d = np.zeros(n, dtype=np.int32)
+    lib().hgp_jet_trace_dumped(w.ctypes.data_as(C.c_void_p), d.ctypes.data_as(C.c_void_p), n)
+    return w, d
 
 
 # ------------------------------------------------------------------ kernel timer

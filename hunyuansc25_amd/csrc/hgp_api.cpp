@@ -168,14 +168,66 @@ void copy_coarsest(Ctx &c, const DevGraph &cg, hgp_graph &h, const std::vector<D
     probe.fill(c, stats, lv.size());
 }
 
+// Copies the left-end region [m0, m1) of c's arena, which holds the levels
+// built by coarsen_levels, into a device allocation of its own and points a
+// copy of the level records at it.  Returns false (nothing kept) if the
+// allocation fails.
+bool snapshot_hierarchy(Ctx &c, const std::vector<DevGraph> &lv, const std::vector<int32_t *> &cmaps, size_t m0,
+                        size_t m1, int64_t tvwgt, CachedHierarchy &h)
+{
+    const size_t bytes = m1 - m0;
+    void *buf = nullptr;
+    if (hipMalloc(&buf, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    char *lo = c.arena.base() + m0, *hi = c.arena.base() + m1;
+    HGP_CHECK_HIP(hipMemcpyAsync(buf, lo, bytes, hipMemcpyDeviceToDevice, c.stream));
+    // the copy completes before another stream can read the hierarchy
+    HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+    auto rb = [&](auto *ptr) {
+        char *q = (char *)ptr;
+        return (q >= lo && q < hi) ? (decltype(ptr))((char *)buf + (q - lo)) : ptr;
+    };
+    h.buf = buf;
+    h.bytes = bytes;
+    h.tvwgt = tvwgt;
+    h.lv = lv;
+    for (DevGraph &g : h.lv) {
+        g.xadj = rb(g.xadj);
+        g.adjncy = rb(g.adjncy);
+        g.adjwgt = rb(g.adjwgt);
+        g.vwgt = rb(g.vwgt);
+        g.bins.ids = rb(g.bins.ids);
+        g.segs.row = rb(g.segs.row);
+        g.segs.j0 = rb(g.segs.j0);
+        g.segs.mi = rb(g.segs.mi);
+        g.segs.mrow = rb(g.segs.mrow);
+        g.segs.mseg = rb(g.segs.mseg);
+    }
+    h.cmaps.clear();
+    for (int32_t *cm : cmaps) h.cmaps.push_back(rb(cm));
+    return true;
+}
+
 // The multilevel pipeline on a device-resident level-0 graph
-// (hunyuangraph_kway_partition, hunyuangraph_partitiongraph.h:13-149).
-void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_t *d_part, hgp_stats *st)
+// (hunyuangraph_kway_partition, hunyuangraph_partitiongraph.h:13-149).  With
+// a store, the coarsening of `key` is taken from it when kept there, else
+// built and offered to it.
+void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_t *d_part, hgp_stats *st,
+                  HierarchyStore *store = nullptr, const HierKey *key = nullptr)
 {
     int32_t n = g0.n;
     int64_t *dsc = c.arena.lalloc<int64_t>(8);
     HGP_CHECK_HIP(hipEventRecord(c.ev[0], c.stream));
-    int64_t tvwgt = total_vwgt(c, g0, dsc);
+    const CachedHierarchy *held = nullptr;
+    if (store && k > 1 && !p.level_stats) held = store->acquire(*key);
+    struct Release {
+        HierarchyStore *s;
+        const CachedHierarchy *&h;
+        ~Release() { if (h) s->release(h); }
+    } release_guard{store, held};
+    int64_t tvwgt = held ? held->tvwgt : total_vwgt(c, g0, dsc);
     if (k == 1) {
         fill<int32_t>(c, d_part, n, 0);
         HGP_CHECK_HIP(hipEventRecord(c.ev[1], c.stream));
@@ -188,7 +240,19 @@ void run_pipeline(Ctx &c, const DevGraph &g0, int32_t k, const Params &p, int32_
     std::vector<DevGraph> lv{g0};
     std::vector<int32_t *> cmaps;
     LevelProbe probe(p.level_stats && st);
-    coarsen_levels(c, k, p, tvwgt, lv, cmaps, probe);
+    if (held) {
+        lv = held->lv;
+        cmaps = held->cmaps;
+    } else {
+        const size_t m0 = c.arena.left_mark();
+        coarsen_levels(c, k, p, tvwgt, lv, cmaps, probe);
+        const size_t m1 = c.arena.left_mark();
+        if (store && !p.level_stats && store->reserve(m1 - m0)) {
+            CachedHierarchy h;
+            if (snapshot_hierarchy(c, lv, cmaps, m0, m1, tvwgt, h)) held = store->insert(*key, std::move(h));
+            else store->unreserve(m1 - m0);
+        }
+    }
     HGP_CHECK_HIP(hipEventRecord(c.ev[1], c.stream));
     int32_t L = (int32_t)lv.size() - 1;
     const DevGraph &cg = lv[L];
@@ -337,6 +401,35 @@ void check_job(const hgp_options &o, const HostJob &j)
     if (j.n > 0 && !j.part) throw Error{-1, "part is NULL"};
 }
 
+void free_hierarchy(CachedHierarchy &h)
+{
+    if (h.buf) (void)hipFree(h.buf);
+    h.buf = nullptr;
+    h.bytes = 0;
+    h.lv.clear();
+    h.cmaps.clear();
+}
+
+size_t resident_job_bytes(int32_t n, int64_t m, int32_t k) { return estimate_arena(n, m, k, false); }
+
+void check_options(const hgp_options &o, int32_t k) { check_params(params_from(&o), k); }
+
+void exec_resident(Ctx &c, const hgp_options &o, const DevGraph &g0, int32_t k, int32_t *part, hgp_stats *stats,
+                   HierarchyStore *store, int64_t graph, size_t bytes, size_t max_bytes)
+{
+    if (g0.n == 0) return;
+    Params p = params_from(&o);
+    check_params(p, k);
+    if (stats) memset(stats, 0, sizeof(*stats));
+    HierKey key{graph, p.seed, coarsen_threshold(k), p.match_rounds};
+    HGP_CHECK_HIP(hipSetDevice(c.device));
+    with_arena(c, bytes, [&] {
+        int32_t *d_part = c.arena.lalloc<int32_t>((size_t)g0.n);
+        run_pipeline(c, g0, k, p, d_part, stats, store, &key);
+        to_host(c, part, d_part, (size_t)g0.n);
+    }, max_bytes);
+}
+
 void exec_job(Ctx &c, const hgp_options &o, const HostJob &j, size_t bytes, size_t max_bytes)
 {
     if (j.n == 0) return;
@@ -410,7 +503,7 @@ int hgp_partition_batch(int32_t count, const int32_t *n, const int64_t *const *x
         if (!n || !xadj || !adjncy || !k || !part) throw Error{-1, "NULL array argument"};
         hgp_options o;
         if (opt) o = *opt; else hgp_default_options(&o);
-        int nl = lanes > 0 ? lanes : 8;
+        int nl = lanes > 0 ? lanes : tuning().batch_lanes;
         nl = std::min(std::min(nl, 64), count);
         std::lock_guard<std::mutex> lk(g_api_mu);
         std::vector<Ctx *> ctx = lane_contexts(o.device, nl);
@@ -578,6 +671,40 @@ void hunyuangraph_PartitionGraph(int *nvtxs, int *xadj, int *adjncy, int *vwgt, 
     });
 }
 
+// ---------------------------------------------------------------- tuning
+
+int hgp_get_tuning(hgp_tuning *t)
+{
+    return guarded([&] {
+        if (!t) throw Error{-1, "tuning is NULL"};
+        const Tuning &x = tuning();
+        t->tiny_max_vertices = x.tiny_max;
+        t->few_rows = (int32_t)x.few_rows;
+        t->fill_waves = (int32_t)x.fill_waves;
+        t->seg_grid = x.seg_grid;
+        t->batch_lanes = x.batch_lanes;
+        t->hub_track = x.hub_track;
+    });
+}
+
+int hgp_set_tuning(const hgp_tuning *t)
+{
+    return guarded([&] {
+        if (!t) throw Error{-1, "tuning is NULL"};
+        if (t->tiny_max_vertices < 0 || t->tiny_max_vertices > 4096 || t->few_rows < 0 || t->fill_waves < 1 ||
+            t->seg_grid < 1 || t->batch_lanes < 1 || t->batch_lanes > 64 || t->hub_track < 0 || t->hub_track > 1)
+            throw Error{-1, "tuning value out of range"};
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Tuning &x = tuning();
+        x.tiny_max = t->tiny_max_vertices;
+        x.few_rows = t->few_rows;
+        x.fill_waves = t->fill_waves;
+        x.seg_grid = t->seg_grid;
+        x.batch_lanes = t->batch_lanes;
+        x.hub_track = t->hub_track;
+    });
+}
+
 // ---------------------------------------------------------------- Jet trace
 
 int hgp_jet_trace_enable(int on)
@@ -587,6 +714,27 @@ int hgp_jet_trace_enable(int on)
         jet_trace().on = on != 0;
         jet_trace().recs.clear();
     });
+}
+
+int hgp_jet_trace_dump(int64_t n, int32_t iter)
+{
+    return guarded([&] {
+        std::lock_guard<std::mutex> lk(jet_trace().mu);
+        jet_trace().dump_n = n;
+        jet_trace().dump_iter = iter;
+        jet_trace().where.clear();
+        jet_trace().dest.clear();
+    });
+}
+
+int64_t hgp_jet_trace_dumped(int32_t *where, int32_t *dest, int64_t cap)
+{
+    std::lock_guard<std::mutex> lk(jet_trace().mu);
+    const JetTrace &t = jet_trace();
+    int64_t c = std::min<int64_t>(cap, (int64_t)t.where.size());
+    if (where && c > 0) std::copy(t.where.begin(), t.where.begin() + c, where);
+    if (dest && c > 0) std::copy(t.dest.begin(), t.dest.begin() + c, dest);
+    return (int64_t)t.where.size();
 }
 
 int64_t hgp_jet_trace(int64_t *buf, int64_t cap)

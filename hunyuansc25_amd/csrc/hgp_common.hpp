@@ -58,6 +58,7 @@ class Arena {
     void free_block();                   // return the block to HIP (nothing may be live)
     size_t capacity() const { return cap_; }
     size_t high_water() const { return peak_; }
+    char *base() const { return base_; }  // left allocations start here (left_mark() bytes in use)
 
     template <class T> T *lalloc(size_t n) { return (T *)left(sizeof(T) * (n ? n : 1)); }
     template <class T> T *ralloc(size_t n) { return (T *)right(sizeof(T) * (n ? n : 1)); }
@@ -83,6 +84,23 @@ constexpr size_t kStageOffset = 64 << 10;   // staging area starts here
 Ctx &context(int device);                        // per-device default context (lazily created)
 std::unique_ptr<Ctx> create_context(int device);   // a private context (own stream, arena, staging)
 void destroy_context(std::unique_ptr<Ctx> c);      // releases a private context's resources
+
+// ------------------------------------------------------------------ tuning
+// Scheduling knobs: launch shapes and path thresholds that change how fast a
+// partition is computed, never what it is (every path computes the same
+// exact result).  Defaults are the values measured best by hand; the
+// autotuner (hunyuansc25_amd/autotune.py) measures them on the device in use
+// and sets them through hgp_set_tuning.
+struct Tuning {
+    int32_t tiny_max = 512;      // levels of <= this many vertices (k <= 256) run the Jet loop in one block
+    int64_t few_rows = 8192;     // levels with <= this many rows of <= 512 entries run them in one launch
+    int64_t fill_waves = 8192;   // waves the wave-per-row kernels spread a level's rows over
+    int32_t seg_grid = 32768;    // grid cap of the segment kernels
+    int32_t batch_lanes = 8;     // lanes of hgp_partition_batch when the caller passes <= 0
+    int32_t hub_track = 0;       // 1: long rows keep their connectivity rows current through a level's
+                                 // iterations (k <= 512) instead of rebuilding them in each selection
+};
+Tuning &tuning();
 
 // ------------------------------------------------------------------ kernel timer
 // Optional per-kernel timing with HIP events on the context's stream: a
@@ -121,6 +139,11 @@ struct JetTrace {
     bool on = false;
     std::mutex mu;
     std::vector<int64_t> recs;   // kJetTraceFields per record
+    // one iteration's state: where (before its moves) and dest (the moves)
+    // of the level with dump_n vertices at iteration dump_iter
+    int64_t dump_n = -1;
+    int32_t dump_iter = -1;
+    std::vector<int32_t> where, dest;
 };
 constexpr int kJetTraceFields = 5;
 JetTrace &jet_trace();

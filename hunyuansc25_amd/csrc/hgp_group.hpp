@@ -21,11 +21,10 @@ constexpr int64_t kBinHalf = 32;
 constexpr int64_t kBinMid = 512;
 constexpr int kClsLong = 3;
 
-// A level with at most kFewRows rows in classes 0-2 runs their row kernels
-// as one wave-per-row launch over the three classes (consecutive in the id
-// list): on such levels a row gets a wave of its own anyway, and one launch
-// instead of three saves launch latency, which dominates there.
-constexpr int64_t kFewRows = 8192;
+// A level with at most Tuning::few_rows rows in classes 0-2 runs their row
+// kernels as one wave-per-row launch over the three classes (consecutive in
+// the id list): on such levels a row gets a wave of its own anyway, and one
+// launch instead of three saves launch latency, which dominates there.
 
 struct RowBins {
     int32_t *ids = nullptr;   // row ids grouped by class (order inside a class is arbitrary)
@@ -34,7 +33,7 @@ struct RowBins {
     int64_t count(int c) const { return off[c + 1] - off[c]; }
     const int32_t *list(int c) const { return ids + off[c]; }
     int64_t short_rows() const { return off[3] - off[0]; }   // classes 0-2
-    bool few() const { return short_rows() <= kFewRows; }
+    bool few() const { return short_rows() <= tuning().few_rows; }
 };
 
 // Classes of the rows of xadj (or of the lengths len, if xadj is null): class

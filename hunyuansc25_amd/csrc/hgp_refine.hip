@@ -100,6 +100,45 @@ struct PairParts {
     }
 };
 
+// Connectivity rows of the long rows (class kClsLong, > 512 entries), kept
+// current through the Jet iterations of a level, as the reference keeps its
+// connectivity lists (set_gain_val_warp / update_large,
+// hunyuangraph_GPU_krefine.h:4332-4432, 3728-3871), here only for the rows
+// whose k-int row is far shorter than the row itself: tconn[h * k + q] =
+// total edge weight from long row h to part q.  Built once per level from
+// where; the move accounting then moves each moved vertex's weight between
+// the rows of its long neighbours.  Integer sums, so the order of the
+// updates does not matter.
+struct HubConn {
+    const int32_t *hidx = nullptr;   // long-row index of each vertex, -1 for the others; null: not tracked
+    int32_t *tconn = nullptr;
+    int32_t k = 0;
+    // long-row index of u[t] (-1 for dead slots and other rows)
+    template <int T>
+    __device__ void index(const int32_t (&u)[T], int32_t (&h)[T]) const
+    {
+#pragma unroll
+        for (int t = 0; t < T; t++) h[t] = (hidx && u[t] >= 0) ? hidx[u[t]] : -1;
+    }
+    // moves weight w of a neighbour from part pv to dv in long row h
+    __device__ void move(int32_t h, int32_t w, int32_t pv, int32_t dv) const
+    {
+        if (h < 0) return;
+        int32_t *row = tconn + (size_t)h * k;
+        atomicAdd(&row[pv], -w);
+        atomicAdd(&row[dv], w);
+    }
+    template <int T>
+    __device__ void moved(const int32_t (&u)[T], const int32_t (&w)[T], int32_t pv, int32_t dv) const
+    {
+        if (!hidx) return;
+        int32_t h[T];
+        index<T>(u, h);
+#pragma unroll
+        for (int t = 0; t < T; t++) move(h[t], w[t], pv, dv);
+    }
+};
+
 // Writes the (where, dest) word of v.
 __device__ inline void store_pair(uint16_t *p16, uint32_t *p32, int32_t v, int32_t w, int32_t d)
 {
@@ -276,6 +315,7 @@ struct SelArgs {
     uint64_t *ab;             // LP: the afterburner's (pregain, where, dest) records
     uint16_t *pd16;           // RW/RS: (where, dest) words of the move accounting
     uint32_t *pd32;
+    uint32_t *cbits;          // LP: one bit per LP candidate (may be null)
 };
 
 // The per-vertex inputs of a decision besides its connectivity: part, vertex
@@ -345,7 +385,12 @@ __device__ inline SelOut sel_compute(const SelArgs &A, const JetCtl &C, int32_t 
             if (bq >= 0 && bc > 0) { d = bq; gain = (int64_t)bc - pc; mv = true; }
             else if (C.nunder > 0) { d = A.under[v % C.nunder]; gain = -(int64_t)pc; mv = true; }
             if (mv && d != p) {
-                kk = (uint32_t)p * kMaxBuckets + (uint32_t)gain_bucket(gain, r.vw);
+                // gain_bucket may return kMaxBuckets itself; the reference's
+                // bucket id (:3243-3266) then falls into the next part's
+                // first bucket and counts against that part's cut-off.  The
+                // vertex stays in its own part's last bucket here.
+                int32_t bkt = gain_bucket(gain, r.vw);
+                kk = (uint32_t)p * kMaxBuckets + (uint32_t)(bkt < kMaxBuckets ? bkt : kMaxBuckets - 1);
                 o.d = d;
             }
         }
@@ -371,6 +416,7 @@ __device__ inline void sel_store(const SelArgs &A, int32_t mode, int32_t v, int3
     if (mode == kModeLP) {
         A.pregain[v] = o.val;
         A.ab[v] = ab_pack(o.val, p, o.d);
+        if (o.cand && A.cbits) atomicOr(&A.cbits[v >> 5], 1u << (v & 31));
     } else {
         A.key[v] = (uint32_t)o.val;
         store_pair(A.pd16, A.pd32, v, p, o.d);
@@ -511,6 +557,10 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_g(const int32_t *__restri
 // and clearing.  Used when k <= kQuarterK.
 constexpr int kQuarterG = 16;
 constexpr int32_t kQuarterK = 512;
+// The long rows (> 512 entries) keep connectivity rows while k is at most
+// this: a k-int row is then at most the size of the shortest long row's
+// entries, and the selection reads it instead of the row.
+constexpr int32_t kTrackMaxK = 512;
 
 __global__ void __launch_bounds__(kRowBS) k_jet_select_q(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A)
 {
@@ -876,6 +926,92 @@ __global__ void __launch_bounds__(kRowBS) k_jet_select_fin(RowSegs S, SelArgs A,
     sel_count(A, mode, ncand);
 }
 
+// Long-row index of every vertex: position in the long rows' id list.
+__global__ void k_hub_index(const int32_t *__restrict__ ids, int64_t cnt, int32_t *__restrict__ hidx)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cnt) hidx[ids[i]] = (int32_t)i;
+}
+
+// Connectivity rows of the long rows from the current partition, one block
+// per segment: the segment's entries are summed per part in a k-int LDS row
+// (two load phases of 8 entries per thread), then added to the long row's
+// connectivity row (tconn zeroed beforehand).
+__global__ void __launch_bounds__(kSegBlk) k_hub_build(RowSegs S, const eid_t *__restrict__ xadj,
+                                                       const vid_t *__restrict__ adjncy,
+                                                       const wgt_t *__restrict__ adjwgt, NarrowParts wq, HubConn hc)
+{
+    extern __shared__ int32_t acc[];   // k ints
+    constexpr int T = (int)(kSegLen / kSegBlk) / 2;
+    constexpr eid_t kHalf = (eid_t)T * kSegBlk;
+    const int k = hc.k;
+    for (int i = threadIdx.x; i < k; i += kSegBlk) acc[i] = 0;
+    __syncthreads();
+    for (int64_t s = blockIdx.x; s < S.nseg; s += gridDim.x) {
+        const int32_t v = S.row[s];
+        const eid_t b = S.j0[s], e = min(b + (eid_t)kSegLen, xadj[v + 1]);
+        for (eid_t c0 = b; c0 < e; c0 += kHalf) {
+            int32_t u[T], w[T], q[T];
+            load_entries<T, kSegBlk>(adjncy, adjwgt, c0, min(e, c0 + kHalf), threadIdx.x, u, w);
+            wq.gather(u, q);
+#pragma unroll
+            for (int t = 0; t < T; t++)
+                if (q[t] >= 0) atomicAdd(&acc[q[t]], w[t]);
+        }
+        __syncthreads();
+        int32_t *row = hc.tconn + (size_t)hc.hidx[v] * k;
+        for (int r = threadIdx.x; r < k; r += kSegBlk) {
+            int32_t x = acc[r];
+            if (x) {
+                atomicAdd(&row[r], x);
+                acc[r] = 0;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Decisions of the long rows from their connectivity rows, one wave per row
+// (row i of the long rows' list is connectivity row i): the own part's
+// connectivity and the best target part (k / 64 parts per lane), then the
+// decision of the mode as for any row.
+__global__ void __launch_bounds__(kRowBS) k_jet_select_hub(const int32_t *__restrict__ ids, int64_t cnt, SelArgs A,
+                                                           HubConn hc)
+{
+    extern __shared__ uint8_t tg[];   // k part flags
+    const JetCtl &C = *A.ctl;
+    if (C.done) return;
+    const int32_t mode = C.mode;
+    const int64_t maxwgt = C.maxwgt, opt = C.opt;
+    const int k = A.k, lane = lane_id();
+    load_targets(A, mode, maxwgt, tg);
+    int32_t ncand = 0;
+    uint64_t kbytes = 0;
+    const int64_t nwv = (int64_t)gridDim.x * (kRowBS / kWave);
+    for (int64_t i = ((int64_t)blockIdx.x * kRowBS + threadIdx.x) / kWave; i < cnt; i += nwv) {
+        const int32_t v = ids[i];
+        const SelRow rr = sel_row(A, mode, v);
+        const int32_t p = rr.p;
+        const bool eligible = sel_eligible(mode, maxwgt, opt, rr);
+        const int32_t *row = hc.tconn + (size_t)i * k;
+        int32_t pc = eligible ? row[p] : 0;
+        PartKey best{0, -1};
+        if (eligible && mode != kModeRS)
+            for (int q = lane; q < k; q += kWave) {
+                if (q == p || !tg[q]) continue;
+                PartKey cand{row[q], q};
+                if (BetterPart()(cand, best)) best = cand;
+            }
+        best = group_best<kWave>(best, BetterPart());
+        if (lane == 0) {
+            ncand += sel_decide(A, C, mode, v, rr, eligible, pc, best);
+            if (eligible) kbytes += 4 * (uint64_t)k;
+        }
+    }
+    sel_count(A, mode, ncand);
+    kb_flush(A.kb, kbytes);
+}
+
 // Afterburner (afterburner_heuristic, :2137-2178): recompute a candidate's
 // gain assuming every higher-priority neighbour (greater gain, ties to the
 // smaller id) has already moved.  A neighbour's pregain, where and dest come
@@ -892,6 +1028,24 @@ __device__ inline int64_t ab_change(int32_t v, int32_t ig, int32_t p, int32_t be
     return ch;
 }
 
+// A neighbour that is no LP candidate reads as this record: pregain kNoGain,
+// which ab_change never counts (a candidate's own pregain is above it).
+constexpr uint64_t kAbNone = (uint64_t)(uint32_t)kNoGain;
+
+// Gathers the records of T neighbours (u < 0: none): the candidate bits
+// first, from a bitmap of n/8 bytes that stays in L2, then the 8-byte
+// records of the candidates alone.
+template <int T>
+__device__ inline void ab_gather(const uint32_t *__restrict__ cbits, const uint64_t *__restrict__ ab,
+                                 const int32_t (&u)[T], uint64_t (&x)[T])
+{
+    uint32_t bw[T];
+#pragma unroll
+    for (int t = 0; t < T; t++) bw[t] = u[t] >= 0 ? cbits[u[t] >> 5] : 0u;
+#pragma unroll
+    for (int t = 0; t < T; t++) x[t] = (u[t] >= 0 && ((bw[t] >> (u[t] & 31)) & 1u)) ? ab[u[t]] : kAbNone;
+}
+
 template <int G>
 __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restrict__ ids, int64_t cnt,
                                                         const eid_t *__restrict__ xadj,
@@ -899,6 +1053,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
                                                         const wgt_t *__restrict__ adjwgt,
                                                         const int32_t *__restrict__ where,
                                                         const int32_t *__restrict__ dest, const uint64_t *__restrict__ ab,
+                                                        const uint32_t *__restrict__ cbits,
                                                         const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                         uint8_t *__restrict__ sel, unsigned long long *kb)
 {
@@ -915,8 +1070,10 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
             int32_t p = where[v], best = dest[v];
             int64_t change = 0;
             for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) {
-                int32_t u = adjncy[j], w = adjwgt[j];
-                change += ab_change(v, ig, p, best, u, w, ab[u]);
+                int32_t u[1] = {adjncy[j]}, w = adjwgt[j];
+                uint64_t x[1];
+                ab_gather<1>(cbits, ab, u, x);
+                change += ab_change(v, ig, p, best, u[0], w, x[0]);
             }
             change = group_sum<G>(change);
             if (lane == 0) {
@@ -938,6 +1095,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
                                                           const wgt_t *__restrict__ adjwgt,
                                                           const int32_t *__restrict__ where,
                                                           const int32_t *__restrict__ dest, const uint64_t *__restrict__ ab,
+                                                          const uint32_t *__restrict__ cbits,
                                                           const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                           uint8_t *__restrict__ sel, int32_t rpw,
                                                           unsigned long long *kb)
@@ -982,8 +1140,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
             int32_t u[8], w[8];
             uint64_t x[8];
             load_entries<8, kQuarterG>(adjncy, adjwgt, b, e, lane % kQuarterG, u, w);
-#pragma unroll
-            for (int t = 0; t < 8; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
+            ab_gather<8>(cbits, ab, u, x);
             int64_t change = 0;
 #pragma unroll
             for (int t = 0; t < 8; t++)
@@ -1003,8 +1160,7 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
             int32_t u[S], w[S];
             uint64_t x[S];
             load_entries<S, kWave>(adjncy, adjwgt, b, e, lane, u, w);
-#pragma unroll
-            for (int t = 0; t < S; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
+            ab_gather<S>(cbits, ab, u, x);
             int64_t change = 0;
 #pragma unroll
             for (int t = 0; t < S; t++)
@@ -1028,6 +1184,7 @@ __global__ void __launch_bounds__(kSegBlk) k_afterburner_seg(RowSegs S, const ei
                                                             const int32_t *__restrict__ where,
                                                             const int32_t *__restrict__ dest,
                                                             const uint64_t *__restrict__ ab,
+                                                            const uint32_t *__restrict__ cbits,
                                                             const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                             uint8_t *__restrict__ sel,
                                                             unsigned long long *__restrict__ macc,
@@ -1051,8 +1208,7 @@ __global__ void __launch_bounds__(kSegBlk) k_afterburner_seg(RowSegs S, const ei
             int32_t u[T], w[T];
             uint64_t x[T];
             load_entries<T, kSegBlk>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
-#pragma unroll
-            for (int t = 0; t < T; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
+            ab_gather<T>(cbits, ab, u, x);
 #pragma unroll
             for (int t = 0; t < T; t++)
                 if (u[t] >= 0) change += ab_change(v, ig, p, best, u[t], w[t], x[t]);
@@ -1083,11 +1239,13 @@ __global__ void k_afterburner_fin(RowSegs S, const int32_t *__restrict__ pregain
 // locks become exactly the selected set; without any, nothing changes.
 __global__ void k_lp_finish(int32_t n, const JetCtl *ctl, const uint8_t *__restrict__ sel,
                             const int32_t *__restrict__ where, int32_t *__restrict__ dest,
-                            uint8_t *__restrict__ lock, uint16_t *__restrict__ pd16, uint32_t *__restrict__ pd32)
+                            uint8_t *__restrict__ lock, uint16_t *__restrict__ pd16, uint32_t *__restrict__ pd32,
+                            uint32_t *__restrict__ cbits)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (v >= n) return;
+    if ((v & 31) == 0) cbits[v >> 5] = 0u;   // the next LP selection starts from no candidates
     const int32_t w = where[v];
     int32_t d = w;
     if (ctl->ncand != 0) {
@@ -1281,8 +1439,9 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_g(const int32_t *__restri
                                                          const wgt_t *__restrict__ adjwgt,
                                                          const wgt_t *__restrict__ vwgt,
                                                          const int32_t *__restrict__ where,
-                                                         const int32_t *__restrict__ dest, PairParts pq, JetCtl *ctl,
-                                                         int64_t *__restrict__ pwgts, unsigned long long *kb)
+                                                         const int32_t *__restrict__ dest, PairParts pq, HubConn hc,
+                                                         JetCtl *ctl, int64_t *__restrict__ pwgts,
+                                                         unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
     __shared__ unsigned long long bdelta;
@@ -1337,11 +1496,16 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_g(const int32_t *__restri
                 }
             }
             pq.gather(u, pu, du);
+            int32_t hh[PB];
+            hc.index<PB>(u, hh);
 #pragma unroll
             for (int t = 0; t < PB; t++) {
                 if (t0 + t < np) {
                     int32_t pv = __shfl(mpv, src[t], kWave), dv = __shfl(mdv, src[t], kWave);
-                    if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
+                    if (u[t] >= 0) {
+                        d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
+                        hc.move(hh[t], w[t], pv, dv);
+                    }
                 }
             }
         }
@@ -1366,8 +1530,8 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
                                                          const wgt_t *__restrict__ adjwgt,
                                                          const wgt_t *__restrict__ vwgt,
                                                          const int32_t *__restrict__ where,
-                                                         const int32_t *__restrict__ dest, PairParts pq, JetCtl *ctl,
-                                                         int64_t *__restrict__ pwgts, int32_t rpw,
+                                                         const int32_t *__restrict__ dest, PairParts pq, HubConn hc,
+                                                         JetCtl *ctl, int64_t *__restrict__ pwgts, int32_t rpw,
                                                          unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
@@ -1418,6 +1582,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
 #pragma unroll
             for (int t = 0; t < 8; t++)
                 if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
+            hc.moved<8>(u, w, pv, dv);
         }
         while (todo) {
             int l = __ffsll((long long)todo) - 1;
@@ -1430,6 +1595,7 @@ __global__ void __launch_bounds__(kRowBS) k_move_delta_w(const int32_t *__restri
 #pragma unroll
             for (int t = 0; t < S; t++)
                 if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
+            hc.moved<S>(u, w, pv, dv);
         }
     }
     d2 = wave_sum(d2);
@@ -1448,8 +1614,9 @@ __global__ void __launch_bounds__(kSegBlk) k_move_delta_seg(RowSegs S, int32_t k
                                                            const wgt_t *__restrict__ adjwgt,
                                                            const wgt_t *__restrict__ vwgt,
                                                            const int32_t *__restrict__ where,
-                                                           const int32_t *__restrict__ dest, PairParts pq, JetCtl *ctl,
-                                                           int64_t *__restrict__ pwgts, unsigned long long *kb)
+                                                           const int32_t *__restrict__ dest, PairParts pq, HubConn hc,
+                                                           JetCtl *ctl, int64_t *__restrict__ pwgts,
+                                                           unsigned long long *kb)
 {
     extern __shared__ unsigned long long pdelta[];
     __shared__ unsigned long long bdelta;
@@ -1472,6 +1639,7 @@ __global__ void __launch_bounds__(kSegBlk) k_move_delta_seg(RowSegs S, int32_t k
 #pragma unroll
             for (int t = 0; t < T; t++)
                 if (u[t] >= 0) d2 += move_d2(pv, dv, pu[t], du[t], w[t]);
+            hc.moved<T>(u, w, pv, dv);
         }
         if (threadIdx.x == 0) {
             kbytes += 16 + 16 * (uint64_t)(j1 - j0);
@@ -1592,7 +1760,7 @@ void for_small_bins(const RowBins &b, F f)
     if (b.count(2)) f(std::integral_constant<int, 64>{}, b.list(2), b.count(2));
 }
 
-inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, 32768); }
+inline unsigned seg_grid(const RowSegs &S) { return (unsigned)std::min<int64_t>(S.nseg, tuning().seg_grid); }
 
 __global__ void k_half(int64_t *d) { *d /= 2; }
 
@@ -1621,12 +1789,11 @@ __global__ void k_project(int32_t n, const int32_t *__restrict__ cmap, const int
 // Rows per wave chunk of the wave-per-row kernels.  A wave screens a chunk
 // of rows and walks the live ones one after another, so on small levels a
 // 64-row chunk is a long serial chain on few waves; the chunk shrinks until
-// about kFillWaves waves share the rows (64 rows once the level is large).
-constexpr int64_t kFillWaves = 8192;
-
+// about Tuning::fill_waves waves share the rows (64 rows once the level is large).
 inline int32_t rows_per_wave(int64_t rows)
 {
-    int64_t r = (rows + kFillWaves - 1) / kFillWaves;
+    const int64_t fill = tuning().fill_waves;
+    int64_t r = (rows + fill - 1) / fill;
     return (int32_t)std::max<int64_t>(1, std::min<int64_t>(r, kWave));
 }
 
@@ -1646,11 +1813,12 @@ inline size_t sel_w_lds(int32_t k, bool q)
 
 void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, const int64_t *pwgts,
                    const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest, int32_t *pregain,
-                   uint32_t *key, int32_t *conn, NarrowParts wq, uint64_t *ab, uint16_t *pd16, uint32_t *pd32)
+                   uint32_t *key, int32_t *conn, NarrowParts wq, uint64_t *ab, uint16_t *pd16, uint32_t *pd32,
+                   uint32_t *cbits, const HubConn &hc)
 {
     const RowBins &b = g.bins;
     SelArgs A{k,   g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl,
-              dest, pregain, key, nullptr, wq, ab, pd16, pd32};
+              dest, pregain, key, nullptr, wq, ab, pd16, pd32, cbits};
     const size_t tg_bytes = ((size_t)k + 3) & ~(size_t)3;   // part flags of the selection kernels
     // fixed bytes: ids, where, pwgts[p], vwgt, the decision (24 per row); eligible rows add xadj and
     // 12 B per entry (adjncy, adjwgt, where gather) in the kernel
@@ -1701,7 +1869,16 @@ void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, c
                 list, nw, rpw, A);
         HGP_LAUNCH_CHECK();
     }
-    if (g.segs.nseg) {
+    if (hc.hidx) {
+        // the long rows decide from their connectivity rows
+        const int64_t nh = b.count(kClsLong);
+        KScope ks(c, "k_jet_select_hub", 24.0 * (double)nh, true);
+        A.kb = ks.dev();
+        const int32_t rows_per_block = kRowBS / kWave;
+        k_jet_select_hub<<<(unsigned)std::min<int64_t>((nh + rows_per_block - 1) / rows_per_block, 8192), kRowBS,
+                           tg_bytes, c.stream>>>(b.list(kClsLong), nh, A, hc);
+        HGP_LAUNCH_CHECK();
+    } else if (g.segs.nseg) {
         {
             KScope ks(c, "k_jet_select_seg", 24.0 * (double)g.segs.nseg, true);
             A.kb = ks.dev();
@@ -1760,7 +1937,7 @@ void project(Ctx &c, int32_t n, const int32_t *cmap, const int32_t *where_c, int
 
 // ---------------------------------------------------------------- tiny levels
 
-// A level of at most kTinyUse vertices (k <= kTinyK) runs its whole Jet loop
+// A level of at most Tuning::tiny_max vertices (k <= kTinyK) runs its whole Jet loop
 // in one block of kTinyBS threads: the same phases as the kernels above
 // (selection, afterburner, LP filter, RW/RS eviction with its sort and cookie
 // cutter, move accounting, commit, bookkeeping) separated by block barriers
@@ -1773,7 +1950,6 @@ constexpr int kTinyBS = 1024;
 constexpr int kTinyWaves = kTinyBS / kWave;
 constexpr int32_t kTinyN = 4096;   // capacity of the block-local scans
 constexpr int32_t kTinyK = 256;
-constexpr int32_t kTinyUse = 512;   // levels of at most this many vertices take the block path
 constexpr int kTinyChunk = 4;                         // entries per lane and chunk
 constexpr eid_t kTinyRow = (eid_t)kTinyChunk * kWave;  // row entries per chunk
 
@@ -2191,6 +2367,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     uint16_t *pd16 = byte_parts ? c.arena.ralloc<uint16_t>((size_t)n) : nullptr;
     uint32_t *pd32 = byte_parts ? nullptr : c.arena.ralloc<uint32_t>((size_t)n);
     uint64_t *ab = c.arena.ralloc<uint64_t>((size_t)n);
+    uint32_t *cbits = c.arena.ralloc<uint32_t>(((size_t)n + 31) / 32);
     const NarrowParts wq{w8, w16};
     const PairParts pq{pd16, pd32};
     int32_t *conn = c.arena.ralloc<int32_t>((size_t)g.segs.nmulti * (size_t)k);
@@ -2201,13 +2378,38 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     }
     int64_t *d_ne = c.arena.ralloc<int64_t>(1);
     fill<uint8_t>(c, lock, n, 0);
+    fill<uint32_t>(c, cbits, (n + 31) / 32, 0);
     HGP_CHECK_HIP(hipMemcpyAsync(best, where, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
     part_weights(c, n, g.vwgt, where, k, pwgts);
     k_jet_init<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, d_cut, maxwgt, opt, p.patience, finest ? 1 : 3);
     k_narrow<<<grid_for(n, kBS), kBS, 0, c.stream>>>(n, where, w8, w16);   // refreshed by each commit
     HGP_LAUNCH_CHECK();
+    // connectivity rows of the long rows, when asked for, while k-int rows
+    // are short beside them (k <= kTrackMaxK) and the level runs the
+    // multi-launch loop.  Off by default: on R-MAT 2^22 / 2^26 at k = 256 the
+    // moved vertices' atomic updates of the rows of the heaviest hubs (the
+    // same few addresses from every wave) cost 211 ms per partition in the
+    // move accounting against 19 ms, and the selection saves only 17 ms.
+    HubConn hc;
+    hc.k = k;
+    const int64_t nhub = g.bins.count(kClsLong);
+    const bool tiny = n <= tuning().tiny_max && k <= kTinyK;
+    if (nhub > 0 && k <= kTrackMaxK && !tiny && tuning().hub_track) {
+        int32_t *hidx = c.arena.ralloc<int32_t>((size_t)n);
+        hc.tconn = c.arena.ralloc<int32_t>((size_t)nhub * (size_t)k);
+        fill<int32_t>(c, hidx, n, -1);
+        fill<int32_t>(c, hc.tconn, nhub * (int64_t)k, 0);
+        k_hub_index<<<grid_for(nhub, kBS), kBS, 0, c.stream>>>(g.bins.list(kClsLong), nhub, hidx);
+        HGP_LAUNCH_CHECK();
+        hc.hidx = hidx;
+        KScope ks(c, "k_hub_build", 24.0 * (double)g.segs.nseg + 9.0 * (double)g.bins.ent[kClsLong] +
+                                        4.0 * (double)nhub * k);
+        k_hub_build<<<seg_grid(g.segs), kSegBlk, sizeof(int32_t) * (size_t)k, c.stream>>>(g.segs, g.xadj, g.adjncy,
+                                                                                         g.adjwgt, wq, hc);
+        HGP_LAUNCH_CHECK();
+    }
     unsigned gv = grid_for(n, kBS);
-    if (n <= kTinyUse && k <= kTinyK) {
+    if (tiny) {
         // the whole loop in one block
         uint32_t *k2 = c.arena.ralloc<uint32_t>((size_t)n);
         uint32_t *v2 = c.arena.ralloc<uint32_t>((size_t)n);
@@ -2241,15 +2443,15 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         }
         read_scalars(c, ctl, sizeof(JetCtl));
     }
-    if (!(n <= kTinyUse && k <= kTinyK)) {
+    if (!tiny) {
         k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 1);
         HGP_LAUNCH_CHECK();
     }
-    for (; !(n <= kTinyUse && k <= kTinyK);) {
+    for (; !tiny;) {
         // the selection does not depend on the mode on the host: it runs while
         // the control block travels (and does nothing once the loop is done)
         read_scalars_start(c, ctl, sizeof(JetCtl));
-        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32);
+        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32, cbits, hc);
         read_scalars_wait(c);
         JetCtl h = *(JetCtl *)c.pinned;
         if (jet_trace().on) {
@@ -2265,27 +2467,27 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                     KScope ks(c, "k_afterburner_few", 9.0 * (double)nb, true);
                     int32_t rpw = rows_per_wave(nb);
                     k_afterburner_w<<<wave_blocks(nb, rpw, 4096), kRowBS, 0, c.stream>>>(
-                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, rpw,
+                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel, rpw,
                         ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.few() ? 0 : g.bins.count(0)) {
                     KScope ks(c, "k_afterburner_s", 9.0 * (double)nb, true);
                     k_afterburner<8><<<grid_groups(nb, 8), kRowBS, 0, c.stream>>>(
-                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, ks.dev());
+                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.few() ? 0 : g.bins.count(1)) {
                     KScope ks(c, "k_afterburner_m", 9.0 * (double)nb, true);
                     k_afterburner<32><<<grid_groups(nb, 32), kRowBS, 0, c.stream>>>(
-                        g.bins.list(1), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, ks.dev());
+                        g.bins.list(1), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.few() ? 0 : g.bins.count(2)) {
                     KScope ks(c, "k_afterburner_w", 9.0 * (double)nb, true);
                     int32_t rpw = rows_per_wave(nb);
                     k_afterburner_w<<<wave_blocks(nb, rpw, 4096), kRowBS, 0, c.stream>>>(
-                        g.bins.list(2), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, rpw,
+                        g.bins.list(2), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel, rpw,
                         ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
@@ -2293,7 +2495,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                     {
                         KScope ks(c, "k_afterburner_seg", 24.0 * (double)g.segs.nseg, true);
                         k_afterburner_seg<<<seg_grid(g.segs), kSegBlk, 0, c.stream>>>(
-                            g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel,
+                            g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel,
                             (unsigned long long *)macc, ks.dev());
                         HGP_LAUNCH_CHECK();
                     }
@@ -2304,7 +2506,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                     }
                 }
             }
-            k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock, pd16, pd32);
+            k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock, pd16, pd32, cbits);
             HGP_LAUNCH_CHECK();
         } else {
             k_iota_u32<<<gv, kBS, 0, c.stream>>>(n, val);
@@ -2332,7 +2534,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             KScope ks(c, "k_move_delta_few", 12.0 * (double)nb, true);
             int32_t rpw = rows_per_wave(nb);
             k_move_delta_w<<<wave_blocks(nb, rpw, 2048), kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, rpw,
+                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, hc, ctl, pwgts, rpw,
                 ks.dev());
             HGP_LAUNCH_CHECK();
         }
@@ -2340,21 +2542,21 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             KScope ks(c, "k_move_delta_s", 12.0 * (double)nb, true);
             int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_g<8><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
+                g.bins.list(0), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, hc, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (int64_t nb = g.bins.few() ? 0 : g.bins.count(1)) {
             KScope ks(c, "k_move_delta_m", 12.0 * (double)nb, true);
             int64_t blocks = std::min<int64_t>((nb + kRowBS - 1) / kRowBS, 2048);
             k_move_delta_g<32><<<(unsigned)blocks, kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
+                g.bins.list(1), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, hc, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
         }
         if (int64_t nb = g.bins.few() ? 0 : g.bins.count(2)) {
             KScope ks(c, "k_move_delta_w", 12.0 * (double)nb, true);
             int32_t rpw = rows_per_wave(nb);
             k_move_delta_w<<<wave_blocks(nb, rpw, 2048), kRowBS, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.bins.list(2), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, rpw,
+                g.bins.list(2), nb, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, hc, ctl, pwgts, rpw,
                 ks.dev());
             HGP_LAUNCH_CHECK();
         }
@@ -2362,8 +2564,21 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             KScope ks(c, "k_move_delta_seg", 24.0 * (double)g.segs.nseg, true);
             unsigned blocks = (unsigned)std::min<int64_t>(g.segs.nseg, 4096);
             k_move_delta_seg<<<blocks, kSegBlk, sizeof(unsigned long long) * (size_t)k, c.stream>>>(
-                g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, ctl, pwgts, ks.dev());
+                g.segs, k, g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, dest, pq, hc, ctl, pwgts, ks.dev());
             HGP_LAUNCH_CHECK();
+        }
+        {
+            JetTrace &jt = jet_trace();
+            std::lock_guard<std::mutex> lk(jt.mu);
+            if (jt.on && jt.dump_n == n && jt.dump_iter == h.iters) {
+                jt.where.resize((size_t)n);
+                jt.dest.resize((size_t)n);
+                HGP_CHECK_HIP(hipMemcpyAsync(jt.where.data(), where, sizeof(int32_t) * (size_t)n,
+                                             hipMemcpyDeviceToHost, c.stream));
+                HGP_CHECK_HIP(hipMemcpyAsync(jt.dest.data(), dest, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost,
+                                             c.stream));
+                HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+            }
         }
         k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where, w8, w16);
         HGP_LAUNCH_CHECK();

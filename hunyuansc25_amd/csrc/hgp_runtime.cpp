@@ -176,6 +176,12 @@ int64_t KTimer::counter_slot()
     return s;
 }
 
+Tuning &tuning()
+{
+    static Tuning t;
+    return t;
+}
+
 JetTrace &jet_trace()
 {
     static JetTrace t;

@@ -140,6 +140,43 @@ int hgp_evaluate_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjnc
                         const int32_t *d_vwgt, const int32_t *d_part, int32_t k, int32_t device, int64_t *d_out);
 
 
+/* ------------------------------------------------------------ CPU path
+ * The reference's CPU partitioner, hunyuangraph_rbbisection
+ * (hunyuangraph_CPU_initialpartition.h:361-374): serial multilevel recursive
+ * bisection on the host (random or heavy-edge matching with two-hop
+ * matching, greedy graph growing, boundary balancing and FM with priority
+ * queues, k-1 bisections), with the standalone admin's parameters
+ * (hunyuangraph_admin.h:7-31: coarsening threshold 200, one cut, ten FM
+ * passes, ubfactor 1.03 per bisection).  A path the caller chooses; the GPU
+ * entry points never fall back to it.  Its random choices come from `seed`
+ * (the reference calls rand() unseeded).  vwgt/adjwgt may be NULL (unit
+ * weights); m and the total vertex weight must fit int32, as in the
+ * reference.  *cut (optional) receives the partition's edge cut. */
+int hgp_partition_cpu(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                      const int32_t *adjwgt, int32_t k, uint64_t seed, int32_t *part, int64_t *cut);
+
+/* ------------------------------------------------------------ tuning
+ * No counterpart in the reference (its launch shapes are compile-time).
+ * Scheduling knobs that change how fast a partition is computed, never its
+ * result: every path they choose between computes the same exact partition.
+ * hunyuansc25_amd/autotune.py measures them on the device in use and writes
+ * them to hunyuansc25_amd/tuning.json, which the Python binding applies on
+ * load.  Set them while no partition runs. */
+typedef struct hgp_tuning {
+    int32_t tiny_max_vertices;  /* levels of <= this many vertices (k <= 256) run the
+                                   whole refinement loop in one block (0..4096, default 512) */
+    int32_t few_rows;           /* levels with <= this many rows of <= 512 entries run
+                                   those rows in one launch (default 8192) */
+    int32_t fill_waves;         /* waves the wave-per-row kernels spread a level over (default 8192) */
+    int32_t seg_grid;           /* grid cap of the long-row segment kernels (default 32768) */
+    int32_t batch_lanes;        /* lanes of hgp_partition_batch when it is passed <= 0 (1..64, default 8) */
+    int32_t hub_track;          /* 1: rows of > 512 entries keep their part-connectivity rows
+                                   current through a level's iterations, updated by every move
+                                   (k <= 512); 0: rebuilt by each selection (default 0) */
+} hgp_tuning;
+int hgp_get_tuning(hgp_tuning *t);
+int hgp_set_tuning(const hgp_tuning *t);
+
 /* Trace of the Jet refinement loops (the reference prints per-iteration
  * state only under debug macros, hunyuangraph_GPU_krefine.h:4434-4882).
  * hgp_jet_trace_enable(1) clears and starts recording, 0 stops; while on,
@@ -151,6 +188,13 @@ int hgp_evaluate_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjnc
  * cap values into buf and returns how many are recorded. */
 int hgp_jet_trace_enable(int on);
 int64_t hgp_jet_trace(int64_t *buf, int64_t cap);
+/* While tracing, also keep one iteration's state: at iteration `iter` of the
+ * level with n vertices, where[] before the iteration's moves and dest[] (the
+ * part each vertex moves to, its own part if it stays).  hgp_jet_trace_dumped
+ * copies up to cap entries of each and returns the level's n (0 if the
+ * iteration was not reached). */
+int hgp_jet_trace_dump(int64_t n, int32_t iter);
+int64_t hgp_jet_trace_dumped(int32_t *where, int32_t *dest, int64_t cap);
 
 /* ------------------------------------------------------------ batches
  * No counterpart in the reference, which partitions one graph per call.
@@ -158,7 +202,7 @@ int64_t hgp_jet_trace(int64_t *buf, int64_t cap);
  * (int64, n[i]+1 entries), adjncy[i], vwgt[i], adjwgt[i] with k[i] parts, and
  * part[i] (n[i] entries) receives its partition.  vwgt/adjwgt may be NULL, or
  * hold NULL entries, for unit weights.  The graphs run concurrently on
- * `lanes` streams of opt->device (<= 0: 8), each lane with its own arena and
+ * `lanes` streams of opt->device (<= 0: the tuning's batch_lanes), each lane with its own arena and
  * staging buffers; part[i] equals what hgp_partition returns for graph i
  * alone with the same options.  status[i] (optional) receives each graph's
  * status and stats[i] (optional) its statistics.  Returns HGP_OK when every
@@ -192,8 +236,29 @@ int64_t hgp_server_submit(hgp_server *s, int32_t n, const int64_t *xadj, const i
  * for once: a second wait on it, concurrent or later, returns
  * HGP_ERR_INVALID and leaves the job to the first. */
 int hgp_server_wait(hgp_server *s, int64_t ticket, hgp_stats *stats, double *latency_ms);
-/* Device bytes held by the slots' arenas now and at most so far, and the
- * number of finished jobs (any output may be NULL). */
+/* Resident graphs.  hgp_server_register uploads a graph once (validated,
+ * NULL weights = unit) and returns a graph id >= 0, or a negative status
+ * (HGP_ERR_OOM when it does not fit the budget); the host arrays may be
+ * freed afterwards.  hgp_server_submit_graph queues a partition of a
+ * resident graph into k parts with `seed` (part[] is written before the
+ * wait returns; returns a ticket or a negative status).  Partitions of a
+ * resident graph keep their coarsening hierarchy on the device: a later
+ * request with the same seed and coarsening threshold (hunyuangraph_partitiongraph.h:298-302:
+ * every k in 128..512 shares one) skips the coarsening.  Kept hierarchies
+ * count against the budget and are dropped least recently used first when
+ * a job, graph or hierarchy needs the room; results equal hgp_partition of
+ * the same graph in every case.  hgp_server_unregister waits for the
+ * graph's queued jobs, then frees it and its hierarchies. */
+int64_t hgp_server_register(hgp_server *s, int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                            const int32_t *vwgt, const int32_t *adjwgt);
+int64_t hgp_server_submit_graph(hgp_server *s, int64_t graph, int32_t k, uint64_t seed, int32_t *part);
+int hgp_server_unregister(hgp_server *s, int64_t graph);
+/* Hierarchy reuse so far: requests that found their hierarchy kept (hits)
+ * or built it (misses), and the hierarchies and bytes kept now. */
+int hgp_server_kept(hgp_server *s, int64_t *hits, int64_t *misses, int64_t *kept, int64_t *kept_bytes);
+/* Device bytes held by the slots' arenas, the resident graphs and the kept
+ * hierarchies, now and at most so far, and the number of finished jobs (any
+ * output may be NULL). */
 int hgp_server_info(hgp_server *s, int64_t *reserved, int64_t *peak_reserved, int64_t *completed);
 /* Finishes queued jobs, stops the workers, waits until every
  * hgp_server_wait in flight has returned, and frees the device memory.  No

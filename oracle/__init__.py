@@ -88,6 +88,8 @@ _SIG = {
     "oracle_dga_initpart": (None, [_I32, _P, _P, _P, _P, _I32, _P, _P]),
     "oracle_jet_trace": (None, [_P, _I64]),
     "oracle_jet_trace_len": (_I64, []),
+    "oracle_jet_dump": (None, [_I64, _I32, _P, _P]),
+    "oracle_jet_dumped": (_I32, []),
     "oracle_heap_run": (_I32, [_I32, _I32, _P, _P, _P, _I32, _P, _P, _P, _P]),
     "oracle_dga_trial": (_I64, [_I32, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I32, _I32, _U64, _U32, _I32, _I32, _P]),
     "oracle_dga_refine": (_I32, [_I32, _P, _P, _P, _P, _I32, _I64, _I32, _P, _P, _P]),

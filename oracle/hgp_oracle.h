@@ -172,6 +172,11 @@ int32_t oracle_heap_run(int32_t nv, int32_t nops, const int32_t *op, const int32
  * oracle_jet_trace_len returns the values recorded (also past cap). */
 void oracle_jet_trace(int64_t *buf, int64_t cap);
 int64_t oracle_jet_trace_len(void);
+/* Keeps where (before the moves) and dest of iteration `iter` of the level
+ * with n vertices in the caller's arrays (NULL stops); oracle_jet_dumped
+ * returns 1 once that iteration ran. */
+void oracle_jet_dump(int64_t n, int32_t iter, int32_t *where, int32_t *dest);
+int32_t oracle_jet_dumped(void);
 
 /* Jet refinement of one level in place.  Returns the number of iterations.
  * cut is in/out. */

@@ -1009,7 +1009,13 @@ static int64_t jet_rw_step(connctx *c, const int32_t *where, const int64_t *pwgt
         else continue;
         if (best == p) continue;
         dest[v] = best;
-        cand[nc].key = (int64_t)p * kMaxBuckets + oracle_gain_bucket(gain, c->vwgt[v]);
+        /* gain_bucket may return kMaxBuckets itself: the reference's bucket id
+         * (:3243-3266) then lies in the next part's first bucket and counts
+         * against that part's cut-off; here the vertex stays in its own
+         * part's last bucket (jetrs drops such vertices, :2549). */
+        int32_t b = oracle_gain_bucket(gain, c->vwgt[v]);
+        if (b >= kMaxBuckets) b = kMaxBuckets - 1;
+        cand[nc].key = (int64_t)p * kMaxBuckets + b;
         cand[nc].v = v;
         nc++;
     }
@@ -1222,6 +1228,20 @@ void oracle_jet_trace(int64_t *buf, int64_t cap)
 
 int64_t oracle_jet_trace_len(void) { return g_trace_len; }
 
+static int64_t g_dump_n = -1;
+static int32_t g_dump_iter = -1, *g_dump_where, *g_dump_dest, g_dumped;
+
+void oracle_jet_dump(int64_t n, int32_t iter, int32_t *where, int32_t *dest)
+{
+    g_dump_n = n;
+    g_dump_iter = iter;
+    g_dump_where = where;
+    g_dump_dest = dest;
+    g_dumped = 0;
+}
+
+int32_t oracle_jet_dumped(void) { return g_dumped; }
+
 static void trace_rec(int32_t n, int32_t iters, int32_t mode, int64_t cut, int32_t done)
 {
     if (!g_trace_buf || n <= 512) return;
@@ -1261,6 +1281,11 @@ int32_t oracle_dga_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
             if (balance_counter < 2) jet_rw_step(&c, where, pwgts, maxwgt, opt, dest);
             else jet_rs_step(&c, where, pwgts, maxwgt, opt, dest);
             balance_counter++;
+        }
+        if (g_dump_where && n == g_dump_n && iters == g_dump_iter) {
+            memcpy(g_dump_where, where, sizeof(int32_t) * (size_t)n);
+            memcpy(g_dump_dest, dest, sizeof(int32_t) * (size_t)n);
+            g_dumped = 1;
         }
         cur_cut += apply_moves(n, xadj, adjncy, adjwgt, vwgt, where, dest, pwgts);
         iters++;

@@ -139,6 +139,48 @@ def test_refine_bit_exact(hgp, orc, k, finest):
         assert c_g == orc.edge_cut(x, a, aw, w_g)
 
 
+def _bucket_fifty_case(n, npairs, seed):
+    """Random graph plus isolated pairs joined by an edge of weight 20000,
+    placed with ids spread over the graph, in three overloaded parts: under
+    weak rebalancing the pairs' vertices (no target part, gain -20000) fall
+    in gain bucket 50, beside the next part's first-bucket candidates."""
+    rng = np.random.default_rng(seed)
+    pairs = rng.choice(n, 2 * npairs, replace=False)
+    normal = np.setdiff1d(np.arange(n), pairs)
+    edges = {}
+    for _ in range(int(2.5 * n)):
+        u, v = rng.choice(normal, 2, replace=False)
+        edges.setdefault((int(min(u, v)), int(max(u, v))), int(rng.integers(1, 4)))
+    for i in range(0, 2 * npairs, 2):
+        edges[(int(min(pairs[i], pairs[i + 1])), int(max(pairs[i], pairs[i + 1])))] = 20000
+    x, a, aw = csr_from_edges(n, list(edges), list(edges.values()))
+    k = 8
+    where = rng.integers(0, k, size=n).astype(np.int32)
+    over = rng.random(n) < 0.45
+    where[over] = rng.integers(0, 3, size=int(over.sum()))
+    for i in range(0, 2 * npairs, 2):
+        where[pairs[i]] = where[pairs[i + 1]] = rng.integers(0, 3)
+    return x, a, np.ones(n, np.int32), aw, k, where
+
+
+@pytest.mark.parametrize("n,npairs", [(1600, 60), (400, 20)])
+def test_refine_weak_rebalance_bucket_fifty(hgp, orc, n, npairs):
+    """Jet refinement from a partition whose first iterations are weak
+    rebalancing with gain-bucket-50 candidates, on the row-class kernels
+    (1600 vertices) and the single-block loop (400), against the oracle."""
+    assert orc.gain_bucket(-20000, 1) == 50
+    x, a, vw, aw, k, where = _bucket_fifty_case(n, npairs, 5)
+    tv = int(vw.sum())
+    cut = orc.edge_cut(x, a, aw, where)
+    for finest in (0, 1):
+        print(f"bucket-50 case n={n} finest={finest}", flush=True)
+        w_g, c_g, i_g = hgp.refine(x, a, vw, aw, k, tv, finest, where, cut)
+        w_o, c_o, i_o = orc.dga_refine(x, a, vw, aw, k, tv, finest, where, cut)
+        np.testing.assert_array_equal(w_g, w_o)
+        assert (c_g, i_g) == (c_o, i_o)
+        assert c_g == orc.edge_cut(x, a, aw, w_g)
+
+
 @pytest.mark.parametrize("k", [2, 8, 32])
 def test_pipeline_bit_exact(hgp, orc, k):
     for name, (x, a, vw, aw) in graphs(hgp).items():
@@ -275,9 +317,18 @@ def test_pipeline_quarter_rows(hgp, orc):
     assert st_g["cut"] == st_o["cut"] == orc.edge_cut(x, a, aw, p_g)
 
 
-def test_pipeline_hub_graph(hgp, orc):
+@pytest.mark.parametrize("hub_track", [0, 1])
+def test_pipeline_hub_graph(hgp, orc, hub_track):
+    """Long rows rebuild their connectivity in every selection (hub_track 0)
+    or keep it current through the move accounting (1): the same partition
+    as the oracle either way."""
     x, a, vw, aw = _hub_graph(hgp)
-    p_g, st_g = hgp.partition(x, a, 64, vw, aw, seed=2)
+    t0 = hgp.get_tuning()
+    try:
+        hgp.set_tuning(hub_track=hub_track)
+        p_g, st_g = hgp.partition(x, a, 64, vw, aw, seed=2)
+    finally:
+        hgp.set_tuning(**t0)
     p_o, st_o = orc.dga_partition(x, a, 64, vw, aw, seed=2)
     np.testing.assert_array_equal(p_g, p_o)
     assert st_g["cut"] == st_o["cut"] == orc.edge_cut(x, a, aw, p_g)

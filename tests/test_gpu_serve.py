@@ -98,3 +98,69 @@ def test_serve_ticket_waited_once(hgp):
     assert len(res) == 1 and errs == [-1]
     p, _ = hgp.partition(g[0], g[1], 8, seed=2)
     np.testing.assert_array_equal(res[0][0], p)
+
+
+def _resident_cases(hgp):
+    g0 = hgp.gen_rmat(13, 1 << 17, seed=41)
+    n1 = 20000
+    g1 = hgp.gen_rgg3d(n1, hgp.rgg_radius(n1, 12), seed=42, vw=(1, 3), ew=(1, 7))
+    # (graph, k, seed): k 128, 256 and 512 share the coarsening threshold
+    # 1024, so the second and third requests of graph 0 reuse the first's
+    # hierarchy; another seed or a k with another threshold builds its own
+    reqs = [(0, 128, 1), (0, 256, 1), (0, 512, 1), (0, 64, 1), (0, 256, 2), (1, 8, 3), (1, 8, 3), (1, 16, 3),
+            (0, 300, 1)]
+    return [g0, g1], reqs
+
+
+def test_resident_graphs_reuse_hierarchies(hgp, orc):
+    graphs, reqs = _resident_cases(hgp)
+    with hgp.Server(slots=3, budget_bytes=8 << 30) as srv:
+        ids = [srv.register(*g) for g in graphs]
+        res = []
+        for gi, k, s in reqs:           # one at a time: the hits are deterministic
+            print(f"resident case graph {gi} k={k} seed={s}", flush=True)
+            res.append(srv.wait(srv.submit_graph(ids[gi], k, seed=s)))
+        kept = srv.kept()
+        before = srv.info()["reserved"]
+        srv.unregister(ids[0])
+        after = srv.info()["reserved"]
+        # the same requests again, all at once, on the graph left
+        tickets = [srv.submit_graph(ids[1], k, seed=s) for gi, k, s in reqs if gi == 1]
+        again = [srv.wait(t)[0] for t in tickets]
+    assert kept["hits"] == 4 and kept["misses"] == 5, kept   # 256/1, 512/1, 8/3, 300/1 reuse
+    assert kept["kept"] == 5 and kept["kept_bytes"] > 0     # graph 0: 1024/1, 512/1, 1024/2; graph 1: 64/3, 128/3
+    assert after < before
+    for (gi, k, s), (part, st, _) in zip(reqs, res):
+        x, a, vw, aw = graphs[gi]
+        p, st1 = hgp.partition(x, a, k, vw, aw, seed=s)
+        np.testing.assert_array_equal(part, p, err_msg=f"graph {gi} k={k} seed={s}")
+        assert st["cut"] == st1["cut"] == orc.edge_cut(x, a, aw, part)
+    mine = [r[0] for (gi, _, _), r in zip(reqs, res) if gi == 1]
+    for p0, p1 in zip(mine, again):
+        np.testing.assert_array_equal(p0, p1)
+
+
+def test_resident_hierarchy_not_kept_without_room(hgp):
+    graphs, reqs = _resident_cases(hgp)
+    g = graphs[0]
+    # room for the graph and one job's arena, not for a kept hierarchy as well
+    with hgp.Server(slots=1, budget_bytes=120 << 20) as srv:
+        gid = srv.register(*g)
+        parts = [srv.wait(srv.submit_graph(gid, k, seed=1))[0] for k in (128, 256)]
+        kept = srv.kept()
+        assert srv.info()["peak_reserved"] <= 120 << 20
+    for k, part in zip((128, 256), parts):
+        p, _ = hgp.partition(g[0], g[1], k, g[2], g[3], seed=1)
+        np.testing.assert_array_equal(part, p)
+    assert kept["kept"] <= 1
+
+
+def test_resident_rejects(hgp):
+    with hgp.Server(slots=1, budget_bytes=1 << 20) as srv:
+        g = hgp.gen_rmat(12, 1 << 16, seed=3)
+        with pytest.raises(hgp.HgpError) as e:
+            srv.register(*g)                    # 1 MB cannot hold the graph
+        assert e.value.code == -3
+        with pytest.raises(hgp.HgpError) as e:
+            srv.submit_graph(7, 8)              # unknown graph id
+        assert e.value.code == -1

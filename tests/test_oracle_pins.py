@@ -10,7 +10,11 @@ comments give the derivation), plus brute-force and semantic checks:
   cookie cutter (:2710-2745) against a brute-force closest-prefix search;
 * the Jet LP afterburner (:2137-2178): each kept move has a non-negative
   gain once the higher-priority moves are actually applied;
-* the acceptance rule of k_refine (:4805-4839) on a constructed sequence.
+* the acceptance rule of k_refine (:4805-4839) on a constructed sequence;
+* the CPU recursive bisection (hunyuangraph_CPU_initialpartition.h:361-374)
+  on graphs whose optimal partition is known in closed form: a wrong gain
+  sign, a dropped FM pass or a bad split leaves cut edges that the optimum
+  does not have.
 """
 import numpy as np
 import pytest
@@ -164,6 +168,25 @@ def test_rw_skips_heavy_vertices(orc):
     assert dest.tolist() == [0, 1, 1] and moves == 1
 
 
+def test_rw_bucket_fifty_stays_in_its_part(orc):
+    # gain_bucket returns 50 for -gain/vwgt in (1.5^24, 1.5^25]: vertices 1
+    # and 2 of part 0 (joined by an edge of weight 20000, no edge to the only
+    # target part 2, dest = undersized part 2, gain -20000).  They sort into
+    # their own part's last bucket, not into part 1's first bucket beside
+    # vertex 0 (part 1, gain +1 into part 2, bucket 0).
+    #   part 0 (pwgts 302, excess 102): 3 (bucket 1, weight 300), then 1, 2
+    #     -> only 3 is evicted;
+    #   part 1 (pwgts 301, excess 101): 0 (bucket 0), then 4 (bucket 1)
+    #     -> both evicted (weight ahead of 4 is 1).
+    # Counting 1 and 2 in part 1's run would evict them as well.
+    assert orc.gain_bucket(-20000, 1) == 50
+    x, a, w = _graph(6, [(1, 2), (0, 5)], [20000, 1])
+    vw = np.array([1, 1, 1, 300, 300, 1], np.int32)
+    where = np.array([1, 0, 0, 0, 1, 2], np.int32)
+    dest, moves = orc.jet_rw_step(x, a, w, vw, 3, where, [302, 301, 1], 200, 100)
+    assert dest.tolist() == [2, 0, 0, 2, 2, 2] and moves == 3
+
+
 # ------------------------------------------------------------------ strong rebalancing
 
 def test_rs_eviction_and_cookie_cutter(orc):
@@ -307,3 +330,63 @@ def test_acceptance_sequence(orc):
         got = orc.jet_accept(cut, mx, maxwgt, best_cut, best_max, count)
         assert got == exp, ((cut, mx, count), got, exp)
         _, best_cut, best_max, _ = got
+
+
+# ------------------------------------------------------------------ CPU recursive bisection
+
+def _clique(off, s):
+    return [(off + i, off + j) for i in range(s) for j in range(i + 1, s)]
+
+
+def _unit(n, edges, weights=None):
+    x, a, w = csr_from_edges(n, edges, weights)
+    return x, a, np.ones(n, np.int32), w
+
+
+@pytest.mark.parametrize("s", [8, 20, 50])
+def test_cpu_rb_barbell_cuts_the_bridge(orc, s):
+    # two K_s joined by one edge: the only balanced bisection of cut 1 puts
+    # each clique in its own part; any other split cuts >= s - 1 clique edges
+    x, a, vw, aw = _unit(2 * s, _clique(0, s) + _clique(s, s) + [(s - 1, s)])
+    for seed in (1, 2, 3):
+        p, cut = orc.cpu_rb_partition(x, a, 2, vw, aw, seed=seed)
+        assert cut == 1
+        assert len(set(p[:s])) == 1 and len(set(p[s:])) == 1 and p[0] != p[s]
+
+
+@pytest.mark.parametrize("n,k", [(64, 2), (64, 4), (200, 8)])
+def test_cpu_rb_path_is_cut_into_runs(orc, n, k):
+    # a path into k balanced parts: the optimum is k - 1 cut edges, each part
+    # one run of consecutive vertices
+    x, a, vw, aw = _unit(n, [(i, i + 1) for i in range(n - 1)])
+    p, cut = orc.cpu_rb_partition(x, a, k, vw, aw, seed=1)
+    assert cut == k - 1
+    assert int(np.count_nonzero(np.diff(p))) == k - 1
+    assert np.bincount(p, minlength=k).max() <= 1.03 ** 3 * n / k
+
+
+def test_cpu_rb_separates_components(orc):
+    # four disjoint K_8 into four parts: cut 0, one clique per part
+    edges = [e for c in range(4) for e in _clique(8 * c, 8)]
+    x, a, vw, aw = _unit(32, edges)
+    for seed in (1, 5):
+        p, cut = orc.cpu_rb_partition(x, a, 4, vw, aw, seed=seed)
+        assert cut == 0
+        assert sorted(p.reshape(4, 8)[:, 0].tolist()) == [0, 1, 2, 3]
+        assert all(len(set(r)) == 1 for r in p.reshape(4, 8))
+
+
+def test_cpu_rb_ladder_and_weighted_bridge(orc):
+    # a 2 x 32 ladder: the optimal bisection cuts the two rails once each
+    n = 32
+    e = [(i, i + 1) for i in range(n - 1)] + [(n + i, n + i + 1) for i in range(n - 1)] + [(i, n + i) for i in range(n)]
+    x, a, vw, aw = _unit(2 * n, e)
+    assert orc.cpu_rb_partition(x, a, 2, vw, aw, seed=1)[1] == 2
+    # two K_12 joined by three parallel bridges of weights 1, 1, 5: the
+    # bisection cuts the bridges (weight 7), not the unit clique edges
+    s = 12
+    edges = _clique(0, s) + _clique(s, s) + [(0, s), (1, s + 1), (2, s + 2)]
+    wts = [1] * (len(edges) - 3) + [1, 1, 5]
+    x, a, vw, aw = _unit(2 * s, edges, wts)
+    p, cut = orc.cpu_rb_partition(x, a, 2, vw, aw, seed=2)
+    assert cut == 7 and len(set(p[:s])) == 1 and len(set(p[s:])) == 1
