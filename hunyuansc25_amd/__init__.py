@@ -132,6 +132,7 @@ SIGNATURES = {
     "hgp_partition_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P, _P]),
     "hgp_check_graph_device": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I32]),
     "hgp_evaluate_device": (C.c_int, [_I32, _P, _P, _P, _P, _P, _I32, _I32, _P]),
+    "hgp_evaluate_device_noalloc": (C.c_int, [_I32, _P, _P, _P, _P, _P, _I32, _I32, _P]),
     "hgp_partition_cpu": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _U64, _P, _P]),
     "hgp_get_tuning": (C.c_int, [_P]),
     "hgp_set_tuning": (C.c_int, [_P]),
@@ -472,6 +473,14 @@ def evaluate_device(n, d_xadj, d_adjncy, d_adjwgt, d_vwgt, d_part, k, d_out, dev
     _check(lib().hgp_evaluate_device(int(n), C.c_void_p(d_xadj), C.c_void_p(d_adjncy),
                                      C.c_void_p(d_adjwgt or None), C.c_void_p(d_vwgt or None),
                                      C.c_void_p(d_part), int(k), int(device), C.c_void_p(d_out)))
+
+
+def evaluate_device_noalloc(n, d_xadj, d_adjncy, d_adjwgt, d_vwgt, d_part, k, d_out, device=0) -> None:
+    """evaluate_device with the library's fixed per-device scratch: allocates
+    no device memory (k <= 4096)."""
+    _check(lib().hgp_evaluate_device_noalloc(int(n), C.c_void_p(d_xadj), C.c_void_p(d_adjncy),
+                                             C.c_void_p(d_adjwgt or None), C.c_void_p(d_vwgt or None),
+                                             C.c_void_p(d_part), int(k), int(device), C.c_void_p(d_out)))
 
 
 # ------------------------------------------------------------------ Jet trace

@@ -610,6 +610,20 @@ int hgp_evaluate_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjnc
     });
 }
 
+int hgp_evaluate_device_noalloc(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy, const int32_t *d_adjwgt,
+                                const int32_t *d_vwgt, const int32_t *d_part, int32_t k, int32_t device,
+                                int64_t *d_out)
+{
+    return guarded([&] {
+        if (n < 0) throw Error{-1, "nvtxs < 0"};
+        if (k < 1) throw Error{-1, "nparts must be >= 1"};
+        if (k > kEvalMaxParts) throw Error{-4, "nparts > 4096 is not supported"};
+        if (!d_out) throw Error{-1, "out is NULL"};
+        if (n > 0 && (!d_xadj || !d_adjncy || !d_part)) throw Error{-1, "NULL device pointer"};
+        evaluate_fixed_scratch(device, n, d_xadj, d_adjncy, d_adjwgt, d_vwgt, d_part, k, d_out);
+    });
+}
+
 int hgp_coarsen(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt, const int32_t *adjwgt,
                 int32_t k, const hgp_options *opt, hgp_graph *coarsest, hgp_stats *stats)
 {

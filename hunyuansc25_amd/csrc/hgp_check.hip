@@ -244,6 +244,45 @@ void evaluate_launch(hipStream_t st, int32_t n, const eid_t *xadj, const vid_t *
     HGP_LAUNCH_CHECK();
 }
 
+// ------------------------------------------------------------------ fixed scratch
+
+// Evaluation scratch of the entry point that allocates nothing: kEvalHead
+// accumulators and up to kEvalMaxParts part weights (32.8 KB per device,
+// held until the process ends).  Host code never passes `scratch` itself to
+// a kernel: its name on the host is not a device address.  The address on
+// each device comes from hipGetSymbolAddress, taken once per device with
+// that device current.
+__device__ __align__(256) int64_t scratch[kEvalHead + kEvalMaxParts];
+
+namespace {
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu[kScratchDevices];
+int64_t *g_scratch_ptr[kScratchDevices] = {};
+}  // namespace
+
+void evaluate_fixed_scratch(int32_t device, int32_t n, const eid_t *xadj, const vid_t *adjncy,
+                            const wgt_t *adjwgt, const wgt_t *vwgt, const int32_t *part, int32_t k,
+                            int64_t *out)
+{
+    if (device < 0 || device >= kScratchDevices) throw Error{-1, "device ordinal out of range"};
+    int count = 0;
+    HGP_CHECK_HIP(hipGetDeviceCount(&count));
+    if (device >= count) throw Error{-1, "device ordinal out of range"};
+    // one call at a time per device: the scratch is shared by every call on it
+    std::lock_guard<std::mutex> lk(g_scratch_mu[device]);
+    HGP_CHECK_HIP(hipSetDevice(device));
+    if (!g_scratch_ptr[device]) {
+        int64_t *p = nullptr;
+        HGP_CHECK_HIP(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error{-2, "no device address for the evaluation scratch"};
+        g_scratch_ptr[device] = p;
+    }
+    // the null stream of `device`; the clear kernel zeroes the scratch, and
+    // the call waits, so the next call finds no work of this one in flight
+    evaluate_launch(nullptr, n, xadj, adjncy, adjwgt, vwgt, part, k, g_scratch_ptr[device], out);
+    HGP_CHECK_HIP(hipStreamSynchronize(nullptr));
+}
+
 namespace {
 const char *const kReasonText[kChkReasons] = {
     "xadj[0] != 0",

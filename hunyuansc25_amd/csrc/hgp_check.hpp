@@ -52,4 +52,12 @@ constexpr int32_t kEvalMaxParts = 4096;
 void evaluate_launch(hipStream_t st, int32_t n, const eid_t *xadj, const vid_t *adjncy, const wgt_t *adjwgt,
                      const wgt_t *vwgt, const int32_t *part, int32_t k, int64_t *scr, int64_t *out);
 
+// The same evaluation with the fixed per-device scratch instead of an arena
+// (k <= kEvalMaxParts, checked by the caller): no device memory is
+// allocated.  Runs on the null stream of `device` and waits for it; calls on
+// one device are serialised by a lock.
+void evaluate_fixed_scratch(int32_t device, int32_t n, const eid_t *xadj, const vid_t *adjncy,
+                            const wgt_t *adjwgt, const wgt_t *vwgt, const int32_t *part, int32_t k,
+                            int64_t *out);
+
 }  // namespace hgp

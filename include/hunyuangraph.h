@@ -139,6 +139,20 @@ int hgp_check_graph_device(int32_t n, const int64_t *d_xadj, const int32_t *d_ad
 int hgp_evaluate_device(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy, const int32_t *d_adjwgt,
                         const int32_t *d_vwgt, const int32_t *d_part, int32_t k, int32_t device, int64_t *d_out);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc): the same
+ * evaluation and arguments as hgp_evaluate_device, with its scratch in a
+ * fixed per-device array of the library (4 + 4096 int64 = 32.8 KB, held
+ * until the process ends) instead of the library's arena, so it calls no
+ * hipMalloc or hipFree and creates no stream.  k <= 4096 (larger k returns
+ * HGP_ERR_UNSUPPORTED before any launch).  Runs on the null stream of
+ * `device` and waits for it.  Every call on a device shares that device's
+ * array: calls are serialised by a per-device lock, and each call clears the
+ * array by a kernel before use. */
+int hgp_evaluate_device_noalloc(int32_t n, const int64_t *d_xadj, const int32_t *d_adjncy, const int32_t *d_adjwgt,
+                                const int32_t *d_vwgt, const int32_t *d_part, int32_t k, int32_t device,
+                                int64_t *d_out);
+
 
 /* ------------------------------------------------------------ CPU path
  * The reference's CPU partitioner, hunyuangraph_rbbisection
