@@ -315,7 +315,6 @@ struct SelArgs {
     uint64_t *ab;             // LP: the afterburner's (pregain, where, dest) records
     uint16_t *pd16;           // RW/RS: (where, dest) words of the move accounting
     uint32_t *pd32;
-    uint32_t *cbits;          // LP: one bit per LP candidate (may be null)
 };
 
 // The per-vertex inputs of a decision besides its connectivity: part, vertex
@@ -416,7 +415,6 @@ __device__ inline void sel_store(const SelArgs &A, int32_t mode, int32_t v, int3
     if (mode == kModeLP) {
         A.pregain[v] = o.val;
         A.ab[v] = ab_pack(o.val, p, o.d);
-        if (o.cand && A.cbits) atomicOr(&A.cbits[v >> 5], 1u << (v & 31));
     } else {
         A.key[v] = (uint32_t)o.val;
         store_pair(A.pd16, A.pd32, v, p, o.d);
@@ -1028,24 +1026,6 @@ __device__ inline int64_t ab_change(int32_t v, int32_t ig, int32_t p, int32_t be
     return ch;
 }
 
-// A neighbour that is no LP candidate reads as this record: pregain kNoGain,
-// which ab_change never counts (a candidate's own pregain is above it).
-constexpr uint64_t kAbNone = (uint64_t)(uint32_t)kNoGain;
-
-// Gathers the records of T neighbours (u < 0: none): the candidate bits
-// first, from a bitmap of n/8 bytes that stays in L2, then the 8-byte
-// records of the candidates alone.
-template <int T>
-__device__ inline void ab_gather(const uint32_t *__restrict__ cbits, const uint64_t *__restrict__ ab,
-                                 const int32_t (&u)[T], uint64_t (&x)[T])
-{
-    uint32_t bw[T];
-#pragma unroll
-    for (int t = 0; t < T; t++) bw[t] = u[t] >= 0 ? cbits[u[t] >> 5] : 0u;
-#pragma unroll
-    for (int t = 0; t < T; t++) x[t] = (u[t] >= 0 && ((bw[t] >> (u[t] & 31)) & 1u)) ? ab[u[t]] : kAbNone;
-}
-
 template <int G>
 __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restrict__ ids, int64_t cnt,
                                                         const eid_t *__restrict__ xadj,
@@ -1053,7 +1033,6 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
                                                         const wgt_t *__restrict__ adjwgt,
                                                         const int32_t *__restrict__ where,
                                                         const int32_t *__restrict__ dest, const uint64_t *__restrict__ ab,
-                                                        const uint32_t *__restrict__ cbits,
                                                         const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                         uint8_t *__restrict__ sel, unsigned long long *kb)
 {
@@ -1070,10 +1049,8 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner(const int32_t *__restric
             int32_t p = where[v], best = dest[v];
             int64_t change = 0;
             for (eid_t j = xadj[v] + lane; j < xadj[v + 1]; j += G) {
-                int32_t u[1] = {adjncy[j]}, w = adjwgt[j];
-                uint64_t x[1];
-                ab_gather<1>(cbits, ab, u, x);
-                change += ab_change(v, ig, p, best, u[0], w, x[0]);
+                int32_t u = adjncy[j], w = adjwgt[j];
+                change += ab_change(v, ig, p, best, u, w, ab[u]);
             }
             change = group_sum<G>(change);
             if (lane == 0) {
@@ -1095,7 +1072,6 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
                                                           const wgt_t *__restrict__ adjwgt,
                                                           const int32_t *__restrict__ where,
                                                           const int32_t *__restrict__ dest, const uint64_t *__restrict__ ab,
-                                                          const uint32_t *__restrict__ cbits,
                                                           const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                           uint8_t *__restrict__ sel, int32_t rpw,
                                                           unsigned long long *kb)
@@ -1140,7 +1116,8 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
             int32_t u[8], w[8];
             uint64_t x[8];
             load_entries<8, kQuarterG>(adjncy, adjwgt, b, e, lane % kQuarterG, u, w);
-            ab_gather<8>(cbits, ab, u, x);
+#pragma unroll
+            for (int t = 0; t < 8; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
             int64_t change = 0;
 #pragma unroll
             for (int t = 0; t < 8; t++)
@@ -1160,7 +1137,8 @@ __global__ void __launch_bounds__(kRowBS) k_afterburner_w(const int32_t *__restr
             int32_t u[S], w[S];
             uint64_t x[S];
             load_entries<S, kWave>(adjncy, adjwgt, b, e, lane, u, w);
-            ab_gather<S>(cbits, ab, u, x);
+#pragma unroll
+            for (int t = 0; t < S; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
             int64_t change = 0;
 #pragma unroll
             for (int t = 0; t < S; t++)
@@ -1184,7 +1162,6 @@ __global__ void __launch_bounds__(kSegBlk) k_afterburner_seg(RowSegs S, const ei
                                                             const int32_t *__restrict__ where,
                                                             const int32_t *__restrict__ dest,
                                                             const uint64_t *__restrict__ ab,
-                                                            const uint32_t *__restrict__ cbits,
                                                             const int32_t *__restrict__ pregain, const JetCtl *ctl,
                                                             uint8_t *__restrict__ sel,
                                                             unsigned long long *__restrict__ macc,
@@ -1208,7 +1185,8 @@ __global__ void __launch_bounds__(kSegBlk) k_afterburner_seg(RowSegs S, const ei
             int32_t u[T], w[T];
             uint64_t x[T];
             load_entries<T, kSegBlk>(adjncy, adjwgt, c0, j1, threadIdx.x, u, w);
-            ab_gather<T>(cbits, ab, u, x);
+#pragma unroll
+            for (int t = 0; t < T; t++) x[t] = u[t] >= 0 ? ab[u[t]] : 0ull;
 #pragma unroll
             for (int t = 0; t < T; t++)
                 if (u[t] >= 0) change += ab_change(v, ig, p, best, u[t], w[t], x[t]);
@@ -1239,13 +1217,11 @@ __global__ void k_afterburner_fin(RowSegs S, const int32_t *__restrict__ pregain
 // locks become exactly the selected set; without any, nothing changes.
 __global__ void k_lp_finish(int32_t n, const JetCtl *ctl, const uint8_t *__restrict__ sel,
                             const int32_t *__restrict__ where, int32_t *__restrict__ dest,
-                            uint8_t *__restrict__ lock, uint16_t *__restrict__ pd16, uint32_t *__restrict__ pd32,
-                            uint32_t *__restrict__ cbits)
+                            uint8_t *__restrict__ lock, uint16_t *__restrict__ pd16, uint32_t *__restrict__ pd32)
 {
     if (ctl->done || ctl->mode != kModeLP) return;
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (v >= n) return;
-    if ((v & 31) == 0) cbits[v >> 5] = 0u;   // the next LP selection starts from no candidates
     const int32_t w = where[v];
     int32_t d = w;
     if (ctl->ncand != 0) {
@@ -1814,11 +1790,11 @@ inline size_t sel_w_lds(int32_t k, bool q)
 void launch_select(Ctx &c, const DevGraph &g, int32_t k, const int32_t *where, const int64_t *pwgts,
                    const uint8_t *lock, const int32_t *under, JetCtl *ctl, int32_t *dest, int32_t *pregain,
                    uint32_t *key, int32_t *conn, NarrowParts wq, uint64_t *ab, uint16_t *pd16, uint32_t *pd32,
-                   uint32_t *cbits, const HubConn &hc)
+                   const HubConn &hc)
 {
     const RowBins &b = g.bins;
     SelArgs A{k,   g.xadj, g.adjncy, g.adjwgt, g.vwgt, where, pwgts, lock, under, ctl,
-              dest, pregain, key, nullptr, wq, ab, pd16, pd32, cbits};
+              dest, pregain, key, nullptr, wq, ab, pd16, pd32};
     const size_t tg_bytes = ((size_t)k + 3) & ~(size_t)3;   // part flags of the selection kernels
     // fixed bytes: ids, where, pwgts[p], vwgt, the decision (24 per row); eligible rows add xadj and
     // 12 B per entry (adjncy, adjwgt, where gather) in the kernel
@@ -2367,7 +2343,6 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     uint16_t *pd16 = byte_parts ? c.arena.ralloc<uint16_t>((size_t)n) : nullptr;
     uint32_t *pd32 = byte_parts ? nullptr : c.arena.ralloc<uint32_t>((size_t)n);
     uint64_t *ab = c.arena.ralloc<uint64_t>((size_t)n);
-    uint32_t *cbits = c.arena.ralloc<uint32_t>(((size_t)n + 31) / 32);
     const NarrowParts wq{w8, w16};
     const PairParts pq{pd16, pd32};
     int32_t *conn = c.arena.ralloc<int32_t>((size_t)g.segs.nmulti * (size_t)k);
@@ -2378,7 +2353,6 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     }
     int64_t *d_ne = c.arena.ralloc<int64_t>(1);
     fill<uint8_t>(c, lock, n, 0);
-    fill<uint32_t>(c, cbits, (n + 31) / 32, 0);
     HGP_CHECK_HIP(hipMemcpyAsync(best, where, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
     part_weights(c, n, g.vwgt, where, k, pwgts);
     k_jet_init<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, d_cut, maxwgt, opt, p.patience, finest ? 1 : 3);
@@ -2451,7 +2425,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         // the selection does not depend on the mode on the host: it runs while
         // the control block travels (and does nothing once the loop is done)
         read_scalars_start(c, ctl, sizeof(JetCtl));
-        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32, cbits, hc);
+        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32, hc);
         read_scalars_wait(c);
         JetCtl h = *(JetCtl *)c.pinned;
         if (jet_trace().on) {
@@ -2467,27 +2441,27 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                     KScope ks(c, "k_afterburner_few", 9.0 * (double)nb, true);
                     int32_t rpw = rows_per_wave(nb);
                     k_afterburner_w<<<wave_blocks(nb, rpw, 4096), kRowBS, 0, c.stream>>>(
-                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel, rpw,
+                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, rpw,
                         ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.few() ? 0 : g.bins.count(0)) {
                     KScope ks(c, "k_afterburner_s", 9.0 * (double)nb, true);
                     k_afterburner<8><<<grid_groups(nb, 8), kRowBS, 0, c.stream>>>(
-                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel, ks.dev());
+                        g.bins.list(0), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.few() ? 0 : g.bins.count(1)) {
                     KScope ks(c, "k_afterburner_m", 9.0 * (double)nb, true);
                     k_afterburner<32><<<grid_groups(nb, 32), kRowBS, 0, c.stream>>>(
-                        g.bins.list(1), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel, ks.dev());
+                        g.bins.list(1), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
                 if (int64_t nb = g.bins.few() ? 0 : g.bins.count(2)) {
                     KScope ks(c, "k_afterburner_w", 9.0 * (double)nb, true);
                     int32_t rpw = rows_per_wave(nb);
                     k_afterburner_w<<<wave_blocks(nb, rpw, 4096), kRowBS, 0, c.stream>>>(
-                        g.bins.list(2), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel, rpw,
+                        g.bins.list(2), nb, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel, rpw,
                         ks.dev());
                     HGP_LAUNCH_CHECK();
                 }
@@ -2495,7 +2469,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                     {
                         KScope ks(c, "k_afterburner_seg", 24.0 * (double)g.segs.nseg, true);
                         k_afterburner_seg<<<seg_grid(g.segs), kSegBlk, 0, c.stream>>>(
-                            g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, cbits, pregain, ctl, sel,
+                            g.segs, g.xadj, g.adjncy, g.adjwgt, where, dest, ab, pregain, ctl, sel,
                             (unsigned long long *)macc, ks.dev());
                         HGP_LAUNCH_CHECK();
                     }
@@ -2506,7 +2480,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                     }
                 }
             }
-            k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock, pd16, pd32, cbits);
+            k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock, pd16, pd32);
             HGP_LAUNCH_CHECK();
         } else {
             k_iota_u32<<<gv, kBS, 0, c.stream>>>(n, val);
