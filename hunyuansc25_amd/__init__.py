@@ -55,6 +55,7 @@ class HgpTuning(C.Structure):
         ("seg_grid", C.c_int32),
         ("batch_lanes", C.c_int32),
         ("hub_track", C.c_int32),
+        ("graph_max_vertices", C.c_int32),
     ]
 
 

@@ -5,7 +5,8 @@ The knobs choose launch shapes and which of two equivalent paths a level
 takes (one block for the whole refinement loop of a small level, one launch
 for all short rows of a level, how many waves the wave-per-row kernels fill,
 the grid of the long-row segment kernels, the batch lanes, whether rows of
-> 512 entries keep their part connectivity across iterations); none of them
+> 512 entries keep their part connectivity across iterations, up to which
+level size the refinement replays its launch sequences as HIP graphs); none of them
 changes a partition.  The tuner measures them by coordinate descent on two
 workloads and keeps a value only if it is faster:
 
@@ -40,6 +41,7 @@ CANDIDATES = {
     "seg_grid": [8192, 32768],
     "batch_lanes": [4, 6, 8, 12],
     "hub_track": [1],
+    "graph_max_vertices": [4096, 1 << 20],
 }
 
 

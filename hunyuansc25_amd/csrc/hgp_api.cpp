@@ -698,6 +698,7 @@ int hgp_get_tuning(hgp_tuning *t)
         t->seg_grid = x.seg_grid;
         t->batch_lanes = x.batch_lanes;
         t->hub_track = x.hub_track;
+        t->graph_max_vertices = x.graph_max_vertices;
     });
 }
 
@@ -706,7 +707,8 @@ int hgp_set_tuning(const hgp_tuning *t)
     return guarded([&] {
         if (!t) throw Error{-1, "tuning is NULL"};
         if (t->tiny_max_vertices < 0 || t->tiny_max_vertices > 4096 || t->few_rows < 0 || t->fill_waves < 1 ||
-            t->seg_grid < 1 || t->batch_lanes < 1 || t->batch_lanes > 64 || t->hub_track < 0 || t->hub_track > 1)
+            t->seg_grid < 1 || t->batch_lanes < 1 || t->batch_lanes > 64 || t->hub_track < 0 || t->hub_track > 1 ||
+            t->graph_max_vertices < 0)
             throw Error{-1, "tuning value out of range"};
         std::lock_guard<std::mutex> lk(g_api_mu);
         Tuning &x = tuning();
@@ -716,6 +718,7 @@ int hgp_set_tuning(const hgp_tuning *t)
         x.seg_grid = t->seg_grid;
         x.batch_lanes = t->batch_lanes;
         x.hub_track = t->hub_track;
+        x.graph_max_vertices = t->graph_max_vertices;
     });
 }
 

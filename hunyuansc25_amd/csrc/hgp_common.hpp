@@ -99,6 +99,8 @@ struct Tuning {
     int32_t batch_lanes = 8;     // lanes of hgp_partition_batch when the caller passes <= 0
     int32_t hub_track = 0;       // 1: long rows keep their connectivity rows current through a level's
                                  // iterations (k <= 512) instead of rebuilding them in each selection
+    int32_t graph_max_vertices = 0;   // levels of <= this many vertices replay their Jet launch
+                                      // sequences as HIP graphs (0: never)
 };
 Tuning &tuning();
 

@@ -2308,6 +2308,45 @@ __global__ void __launch_bounds__(kTinyBS) k_jet_tiny(TinyArgs T)
 }
 
 // hunyuangraph_malloc_krefine (hunyuangraph_GPU_uncoarsen.h:428-508) + k_refine.
+// Launch sequences of one level's Jet loop recorded as HIP graphs: slot 0
+// the selection, slots 1-3 the rest of an iteration in modes LP, RW and RS.
+// Each is recorded on its first use (stream capture, this thread only) and
+// replayed by every later iteration of the level; the kernels read the mode
+// and the stop flag from the control block themselves, and every temporary
+// they use comes from the same arena mark in each iteration.
+struct LevelGraphs {
+    bool on = false;
+    hipGraphExec_t ex[4] = {};
+    LevelGraphs() = default;
+    LevelGraphs(const LevelGraphs &) = delete;
+    LevelGraphs &operator=(const LevelGraphs &) = delete;
+    ~LevelGraphs()
+    {
+        for (hipGraphExec_t &e : ex)
+            if (e) (void)hipGraphExecDestroy(e);
+    }
+    template <class F>
+    void run(Ctx &c, int slot, F &&issue)
+    {
+        if (!ex[slot]) {
+            hipGraph_t gr = nullptr;
+            HGP_CHECK_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                issue();
+            } catch (...) {
+                (void)hipStreamEndCapture(c.stream, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            HGP_CHECK_HIP(hipStreamEndCapture(c.stream, &gr));
+            hipError_t e = hipGraphInstantiate(&ex[slot], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            HGP_CHECK_HIP(e);
+        }
+        HGP_CHECK_HIP(hipGraphLaunch(ex[slot], c.stream));
+    }
+};
+
 RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bool finest, const Params &p,
                          int32_t *where, int64_t *d_cut)
 {
@@ -2421,20 +2460,10 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 1);
         HGP_LAUNCH_CHECK();
     }
-    for (; !tiny;) {
-        // the selection does not depend on the mode on the host: it runs while
-        // the control block travels (and does nothing once the loop is done)
-        read_scalars_start(c, ctl, sizeof(JetCtl));
-        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32, hc);
-        read_scalars_wait(c);
-        JetCtl h = *(JetCtl *)c.pinned;
-        if (jet_trace().on) {
-            std::lock_guard<std::mutex> lk(jet_trace().mu);
-            jet_trace().recs.insert(jet_trace().recs.end(),
-                                    {(int64_t)n, (int64_t)h.iters, (int64_t)h.mode, h.cut, (int64_t)h.done});
-        }
-        if (h.done) break;
-        if (h.mode == kModeLP) {
+    // The rest of an iteration once its mode is known: the mode's moves, the
+    // move accounting, the commit and the bookkeeping.
+    auto issue_rest = [&](int32_t mode, int32_t iters) {
+        if (mode == kModeLP) {
             {
                 // fixed: ids, pregain, sel (9 B per row); candidates add xadj, where, dest and 16 B per entry
                 if (int64_t nb = g.bins.few() ? g.bins.short_rows() : 0) {
@@ -2492,7 +2521,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             k_rb_evict<<<gv, kBS, 0, c.stream>>>(n, k, key, val, wsc, g.vwgt, pstart, pwgts, ctl, where, dest, sel,
                                                  pd16, pd32);
             HGP_LAUNCH_CHECK();
-            if (h.mode == kModeRS) {
+            if (mode == kModeRS) {
                 compact_indices(c, sel, n, elist, d_ne);
                 k_gather_w<<<gv, kBS, 0, c.stream>>>(d_ne, elist, val, g.vwgt, bs);
                 HGP_LAUNCH_CHECK();
@@ -2544,7 +2573,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         {
             JetTrace &jt = jet_trace();
             std::lock_guard<std::mutex> lk(jt.mu);
-            if (jt.on && jt.dump_n == n && jt.dump_iter == h.iters) {
+            if (jt.on && jt.dump_n == n && jt.dump_iter == iters) {
                 jt.where.resize((size_t)n);
                 jt.dest.resize((size_t)n);
                 HGP_CHECK_HIP(hipMemcpyAsync(jt.where.data(), where, sizeof(int32_t) * (size_t)n,
@@ -2560,6 +2589,31 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         HGP_LAUNCH_CHECK();
         k_copy_best<<<gv, kBS, 0, c.stream>>>(n, ctl, where, best);
         HGP_LAUNCH_CHECK();
+    };
+    auto issue_select = [&] {
+        launch_select(c, g, k, where, pwgts, lock, under, ctl, dest, pregain, key, conn, wq, ab, pd16, pd32, hc);
+    };
+    // Levels of at most Tuning::graph_max_vertices vertices record each
+    // launch sequence once as a HIP graph and replay it (not while kernels are
+    // timed or iterations traced, which act between the launches).
+    LevelGraphs graphs;
+    graphs.on = !tiny && n <= tuning().graph_max_vertices && !ktimer().on && !jet_trace().on;
+    for (; !tiny;) {
+        // the selection does not depend on the mode on the host: it runs while
+        // the control block travels (and does nothing once the loop is done)
+        read_scalars_start(c, ctl, sizeof(JetCtl));
+        if (graphs.on) graphs.run(c, 0, issue_select);
+        else issue_select();
+        read_scalars_wait(c);
+        JetCtl h = *(JetCtl *)c.pinned;
+        if (jet_trace().on) {
+            std::lock_guard<std::mutex> lk(jet_trace().mu);
+            jet_trace().recs.insert(jet_trace().recs.end(),
+                                    {(int64_t)n, (int64_t)h.iters, (int64_t)h.mode, h.cut, (int64_t)h.done});
+        }
+        if (h.done) break;
+        if (graphs.on) graphs.run(c, 1 + h.mode, [&] { issue_rest(h.mode, h.iters); });
+        else issue_rest(h.mode, h.iters);
     }
     rs.iters = ((JetCtl *)c.pinned)->iters;
     HGP_CHECK_HIP(hipMemcpyAsync(where, best, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));

@@ -187,6 +187,9 @@ typedef struct hgp_tuning {
     int32_t hub_track;          /* 1: rows of > 512 entries keep their part-connectivity rows
                                    current through a level's iterations, updated by every move
                                    (k <= 512); 0: rebuilt by each selection (default 0) */
+    int32_t graph_max_vertices; /* levels of <= this many vertices record their refinement
+                                   launch sequences once as HIP graphs and replay them in
+                                   every iteration (0: never, the default) */
 } hgp_tuning;
 int hgp_get_tuning(hgp_tuning *t);
 int hgp_set_tuning(const hgp_tuning *t);

@@ -128,11 +128,12 @@ def test_struct_layouts_match_header(hgp, tmp_path):
 def test_tuning_round_trip_and_range(hgp):
     t0 = hgp.get_tuning()
     try:
-        hgp.set_tuning(tiny_max_vertices=1024, batch_lanes=3, hub_track=1)
+        hgp.set_tuning(tiny_max_vertices=1024, batch_lanes=3, hub_track=1, graph_max_vertices=77)
         t = hgp.get_tuning()
         assert t["tiny_max_vertices"] == 1024 and t["batch_lanes"] == 3 and t["hub_track"] == 1
+        assert t["graph_max_vertices"] == 77
         for bad in (dict(tiny_max_vertices=5000), dict(fill_waves=0), dict(batch_lanes=0), dict(seg_grid=0),
-                    dict(hub_track=2)):
+                    dict(hub_track=2), dict(graph_max_vertices=-1)):
             with pytest.raises(hgp.HgpError) as e:
                 hgp.set_tuning(**bad)
             assert e.value.code == -1
