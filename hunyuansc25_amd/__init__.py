@@ -56,6 +56,7 @@ class HgpTuning(C.Structure):
         ("batch_lanes", C.c_int32),
         ("hub_track", C.c_int32),
         ("graph_max_vertices", C.c_int32),
+        ("batch_fuse_max_vertices", C.c_int32),
     ]
 
 

@@ -9,12 +9,14 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <numeric>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace hgp {
@@ -388,6 +390,183 @@ std::vector<Ctx *> lane_contexts(int device, int count)
     return out;
 }
 
+// ---------------------------------------------------------------- batched mode, small levels fused
+
+// One graph of a fused batch group: its hierarchy, the partition buffers of
+// the levels the fused launch writes, and the one-block loop's scratch, all
+// in one device buffer (buf) that outlives the lane's arena between phases.
+struct FusedGraph {
+    int32_t i = -1;
+    bool fused = false;          // false: finished in phase A (or failed)
+    void *buf = nullptr;
+    std::vector<DevGraph> lv;
+    std::vector<int32_t *> cmaps;
+    std::vector<int32_t *> where;   // by level: lo - 1 .. hi
+    TinyScratch s;
+    int64_t tvwgt = 0, init_cut = 0;
+    int32_t lo = 0, hi = 0;
+    double ms_a = 0;
+};
+
+// Copies the arena's left region [m0, m1) into its own device buffer and
+// points every array of f into the copy.
+void detach_fused(Ctx &c, size_t m0, size_t m1, FusedGraph &f)
+{
+    const size_t bytes = m1 - m0;
+    HGP_CHECK_HIP(hipMalloc(&f.buf, bytes ? bytes : 1));
+    char *lo = c.arena.base() + m0, *hi = c.arena.base() + m1;
+    HGP_CHECK_HIP(hipMemcpyAsync(f.buf, lo, bytes, hipMemcpyDeviceToDevice, c.stream));
+    HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+    auto rb = [&](auto *&ptr) {
+        char *q = (char *)ptr;
+        if (q >= lo && q < hi) ptr = (std::remove_reference_t<decltype(ptr)>)((char *)f.buf + (q - lo));
+    };
+    for (DevGraph &g : f.lv) {
+        rb(g.xadj);
+        rb(g.adjncy);
+        rb(g.adjwgt);
+        rb(g.vwgt);
+        rb(g.bins.ids);
+        rb(g.segs.row);
+        rb(g.segs.j0);
+        rb(g.segs.mi);
+        rb(g.segs.mrow);
+        rb(g.segs.mseg);
+    }
+    for (int32_t *&p : f.cmaps) rb(p);
+    for (int32_t *&p : f.where) rb(p);
+    TinyScratch &t = f.s;
+    rb(t.dest), rb(t.pregain), rb(t.best), rb(t.elist), rb(t.under), rb(t.es), rb(t.ee), rb(t.pstart);
+    rb(t.iters), rb(t.lock), rb(t.sel), rb(t.w8), rb(t.w16), rb(t.pd16), rb(t.pd32), rb(t.key), rb(t.val);
+    rb(t.k2), rb(t.v2), rb(t.ab), rb(t.wsc), rb(t.bs), rb(t.pwgts), rb(t.cut), rb(t.ctl);
+}
+
+// Statistics of a fused batch graph from its level-0 partition part0.
+void finish_fused_stats(Ctx &c, const HostJob &j, const FusedGraph &f, const int32_t *part0, const int64_t *d_cut,
+                        const std::vector<int32_t> &iters)
+{
+    if (!j.stats) return;
+    hgp_stats *st = j.stats;
+    int64_t *pw = c.arena.lalloc<int64_t>((size_t)j.k);
+    part_weights(c, j.n, f.lv[0].vwgt, part0, j.k, pw);
+    std::vector<int64_t> hpw((size_t)j.k);
+    HGP_CHECK_HIP(hipMemcpyAsync(hpw.data(), pw, sizeof(int64_t) * (size_t)j.k, hipMemcpyDeviceToHost, c.stream));
+    read_scalars(c, d_cut, sizeof(int64_t));
+    st->cut = c.pinned[0];
+    int64_t mx = 0;
+    for (int64_t x : hpw) mx = x > mx ? x : mx;
+    st->max_pwgt = mx;
+    st->tvwgt = f.tvwgt;
+    const int32_t L = (int32_t)f.lv.size() - 1;
+    st->levels = L;
+    st->coarsest_n = f.lv[(size_t)L].n;
+    st->coarsest_m = f.lv[(size_t)L].m;
+    st->init_cut = f.init_cut;
+    std::vector<int64_t> ns, ms;
+    int64_t tot = 0;
+    for (size_t l = 0; l < f.lv.size(); l++) {
+        ns.push_back(f.lv[l].n);
+        ms.push_back(f.lv[l].m);
+        if (l < HGP_MAX_LEVELS) {
+            st->level_n[l] = f.lv[l].n;
+            st->level_m[l] = f.lv[l].m;
+            st->level_iters[l] = iters[l];
+        }
+        tot += iters[l];
+    }
+    st->refine_iters = tot;
+    st->alg_bytes = alg_bytes(ns, ms, iters);
+    st->arena_bytes = (int64_t)c.arena.capacity();
+    st->arena_peak = (int64_t)c.arena.high_water();
+}
+
+// Phase A of a fused batch graph: upload, coarsening, initial partition; the
+// levels of at most fuse_n vertices at the coarse end are left to the fused
+// launch.  A graph without such a level (or with k = 1, or k above the
+// one-block loop's range) is finished here.
+void fused_phase_a(Ctx &c, const hgp_options &o, const HostJob &j, int32_t fuse_n, FusedGraph &f)
+{
+    Params p = params_from(&o);
+    HGP_CHECK_HIP(hipSetDevice(c.device));
+    if (j.n == 0) return;
+    if (j.k == 1 || j.k > kTinyMaxK) {
+        exec_job(c, o, j, job_bytes(j));
+        return;
+    }
+    with_arena(c, job_bytes(j), [&] {
+        auto t0 = std::chrono::steady_clock::now();
+        const size_t m0 = c.arena.left_mark();
+        f.lv.assign(1, upload(c, j.n, j.xadj, j.adjncy, j.vwgt, j.adjwgt));
+        f.cmaps.clear();
+        int64_t *dsc = c.arena.lalloc<int64_t>(8);
+        f.tvwgt = total_vwgt(c, f.lv[0], dsc);
+        LevelProbe probe(false);
+        coarsen_levels(c, j.k, p, f.tvwgt, f.lv, f.cmaps, probe);
+        const int32_t L = (int32_t)f.lv.size() - 1;
+        int32_t lo = L + 1;
+        while (lo > 0 && f.lv[(size_t)lo - 1].n <= fuse_n) lo--;
+        f.where.assign((size_t)L + 1, nullptr);
+        f.where[(size_t)L] = c.arena.lalloc<int32_t>((size_t)f.lv[(size_t)L].n);
+        initial_partition(c, f.lv[(size_t)L], j.k, p, f.where[(size_t)L]);
+        if (lo > L) {
+            // no small level: the rest of the pipeline here
+            int64_t *d_cut = dsc + 1;
+            edge_cut(c, f.lv[(size_t)L], f.where[(size_t)L], d_cut);
+            read_scalars(c, d_cut, sizeof(int64_t));
+            f.init_cut = c.pinned[0];
+            int32_t *where = f.where[(size_t)L];
+            std::vector<int32_t> iters(f.lv.size(), 0);
+            for (int32_t l = L; l >= 0; l--) {
+                iters[(size_t)l] = refine_level(c, f.lv[(size_t)l], j.k, f.tvwgt, l == 0, p, where, d_cut).iters;
+                if (l == 0) break;
+                int32_t *wf = c.arena.lalloc<int32_t>((size_t)f.lv[(size_t)l - 1].n);
+                project(c, f.lv[(size_t)l - 1].n, f.cmaps[(size_t)l - 1], where, wf);
+                where = wf;
+            }
+            finish_fused_stats(c, j, f, where, d_cut, iters);
+            to_host(c, j.part, where, (size_t)j.n);
+            return;
+        }
+        f.s = alloc_tiny_scratch(c, f.lv[(size_t)lo].n, j.k);
+        edge_cut(c, f.lv[(size_t)L], f.where[(size_t)L], f.s.cut);
+        read_scalars(c, f.s.cut, sizeof(int64_t));
+        f.init_cut = c.pinned[0];
+        for (int32_t l = std::max(lo - 1, 0); l < L; l++)
+            f.where[(size_t)l] = c.arena.lalloc<int32_t>((size_t)f.lv[(size_t)l].n);
+        f.lo = lo;
+        f.hi = L;
+        detach_fused(c, m0, c.arena.left_mark(), f);
+        f.fused = true;
+        f.ms_a = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    });
+}
+
+// Phase C: the levels finer than the fused ones, final statistics, the
+// partition to the host; frees the graph's buffer.
+void fused_phase_c(Ctx &c, const hgp_options &o, const HostJob &j, FusedGraph &f)
+{
+    Params p = params_from(&o);
+    HGP_CHECK_HIP(hipSetDevice(c.device));
+    with_arena(c, resident_job_bytes(j.n, j.xadj[j.n], j.k), [&] {
+        int64_t *d_cut = c.arena.lalloc<int64_t>(1);
+        HGP_CHECK_HIP(hipMemcpyAsync(d_cut, f.s.cut, sizeof(int64_t), hipMemcpyDeviceToDevice, c.stream));
+        std::vector<int32_t> iters(f.lv.size(), 0);
+        HGP_CHECK_HIP(hipMemcpyAsync(iters.data(), f.s.iters, sizeof(int32_t) * iters.size(), hipMemcpyDeviceToHost,
+                                     c.stream));
+        HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+        int32_t *where = f.lo > 0 ? f.where[(size_t)f.lo - 1] : f.where[0];
+        for (int32_t l = f.lo - 1; l >= 0; l--) {
+            iters[(size_t)l] = refine_level(c, f.lv[(size_t)l], j.k, f.tvwgt, l == 0, p, where, d_cut).iters;
+            if (l == 0) break;
+            int32_t *wf = c.arena.lalloc<int32_t>((size_t)f.lv[(size_t)l - 1].n);
+            project(c, f.lv[(size_t)l - 1].n, f.cmaps[(size_t)l - 1], where, wf);
+            where = wf;
+        }
+        finish_fused_stats(c, j, f, where, d_cut, iters);
+        to_host(c, j.part, where, (size_t)j.n);
+    });
+}
+
 }  // namespace
 
 size_t job_bytes(const HostJob &j) { return estimate_arena(j.n, j.n > 0 ? j.xadj[j.n] : 0, j.k, true); }
@@ -512,9 +691,97 @@ int hgp_partition_batch(int32_t count, const int32_t *n, const int64_t *const *x
         std::iota(order.begin(), order.end(), 0);
         auto size_of = [&](int32_t i) { return xadj[i] && n[i] >= 0 ? xadj[i][n[i]] + n[i] : 0; };
         std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return size_of(a) > size_of(b); });
-        std::atomic<int32_t> next{0};
         std::vector<int32_t> st((size_t)count, 0);
         std::vector<std::string> msg((size_t)count);
+        auto job_of = [&](int32_t i) {
+            if (!xadj[i]) throw Error{-1, "xadj is NULL"};
+            return HostJob{n[i], xadj[i], adjncy[i], vwgt ? vwgt[i] : nullptr, adjwgt ? adjwgt[i] : nullptr, k[i],
+                           part[i], stats ? stats + i : nullptr};
+        };
+        // runs body(c, t) for t in [0, cnt) over the lanes
+        auto over_lanes = [&](int32_t cnt, auto body) {
+            std::atomic<int32_t> next{0};
+            auto work = [&](Ctx *c) {
+                for (int32_t t; (t = next.fetch_add(1)) < cnt;) body(c, t);
+            };
+            std::vector<std::thread> th;
+            for (int l = 1; l < std::min(nl, (int)cnt); l++) th.emplace_back(work, ctx[l]);
+            work(ctx[0]);
+            for (auto &t : th) t.join();
+        };
+        const int32_t fuse_n = std::min(tuning().batch_fuse_max, kTinyMaxN);
+        if (fuse_n > 0 && !o.level_stats) {
+            // Groups of graphs (largest first, up to ~8 GB of hierarchies):
+            // phase A coarsens and initially partitions each graph on the
+            // lanes and keeps its hierarchy; one launch refines the small
+            // levels of every graph of the group (a block per graph);
+            // phase C refines the rest on the lanes.
+            const size_t budget = (size_t)8 << 30;
+            for (int32_t g0 = 0; g0 < count;) {
+                int32_t g1 = g0;
+                size_t gb = 0;
+                while (g1 < count && (g1 == g0 || gb < budget)) {
+                    const int32_t i = order[(size_t)g1];
+                    gb += xadj[i] && n[i] >= 0 ? (size_t)(24 * (xadj[i][n[i]] + 2 * (int64_t)n[i])) + (1 << 20) : 0;
+                    g1++;
+                }
+                std::vector<FusedGraph> fg((size_t)(g1 - g0));
+                over_lanes(g1 - g0, [&](Ctx *c, int32_t t) {
+                    const int32_t i = order[(size_t)(g0 + t)];
+                    fg[(size_t)t].i = i;
+                    int rc = guarded([&] {
+                        HostJob j = job_of(i);
+                        check_job(o, j);
+                        fused_phase_a(*c, o, j, fuse_n, fg[(size_t)t]);
+                    });
+                    st[i] = rc;
+                    if (rc != HGP_OK) msg[i] = last_error_text();
+                });
+                std::vector<TinyRun> runs;
+                std::vector<size_t> idx;
+                for (size_t t = 0; t < fg.size(); t++) {
+                    FusedGraph &f = fg[t];
+                    if (!f.fused || st[f.i] != HGP_OK) continue;
+                    TinyRun r;
+                    r.k = k[f.i];
+                    r.hi = f.hi;
+                    r.lo = f.lo;
+                    r.tvwgt = f.tvwgt;
+                    r.p = params_from(&o);
+                    r.lv = f.lv;
+                    r.where = f.where;
+                    r.cmap.assign(f.cmaps.begin(), f.cmaps.end());
+                    r.s = f.s;
+                    runs.push_back(std::move(r));
+                    idx.push_back(t);
+                }
+                int brc = guarded([&] {
+                    HGP_CHECK_HIP(hipSetDevice(ctx[0]->device));
+                    with_arena(*ctx[0], (sizeof(TinyRun) + 2048) * (runs.size() + 1) * 4 + (4 << 20),
+                               [&] { tiny_batch(*ctx[0], runs); });
+                });
+                if (brc != HGP_OK)
+                    for (size_t t : idx) {
+                        st[fg[t].i] = brc;
+                        msg[fg[t].i] = last_error_text();
+                    }
+                over_lanes((int32_t)idx.size(), [&](Ctx *c, int32_t t) {
+                    FusedGraph &f = fg[idx[(size_t)t]];
+                    if (st[f.i] != HGP_OK) return;
+                    int rc = guarded([&] { fused_phase_c(*c, o, job_of(f.i), f); });
+                    st[f.i] = rc;
+                    if (rc != HGP_OK) msg[f.i] = last_error_text();
+                });
+                for (FusedGraph &f : fg)
+                    if (f.buf) (void)hipFree(f.buf);
+                g0 = g1;
+            }
+            if (status) std::copy(st.begin(), st.end(), status);
+            for (int32_t i = 0; i < count; i++)
+                if (st[i] != HGP_OK) throw Error{st[i], "graph " + std::to_string(i) + ": " + msg[i]};
+            return;
+        }
+        std::atomic<int32_t> next{0};
         auto work = [&](Ctx *c) {
             for (;;) {
                 int32_t t = next.fetch_add(1);
@@ -699,6 +966,7 @@ int hgp_get_tuning(hgp_tuning *t)
         t->batch_lanes = x.batch_lanes;
         t->hub_track = x.hub_track;
         t->graph_max_vertices = x.graph_max_vertices;
+        t->batch_fuse_max_vertices = x.batch_fuse_max;
     });
 }
 
@@ -708,7 +976,7 @@ int hgp_set_tuning(const hgp_tuning *t)
         if (!t) throw Error{-1, "tuning is NULL"};
         if (t->tiny_max_vertices < 0 || t->tiny_max_vertices > 4096 || t->few_rows < 0 || t->fill_waves < 1 ||
             t->seg_grid < 1 || t->batch_lanes < 1 || t->batch_lanes > 64 || t->hub_track < 0 || t->hub_track > 1 ||
-            t->graph_max_vertices < 0)
+            t->graph_max_vertices < 0 || t->batch_fuse_max_vertices < 0 || t->batch_fuse_max_vertices > 4096)
             throw Error{-1, "tuning value out of range"};
         std::lock_guard<std::mutex> lk(g_api_mu);
         Tuning &x = tuning();
@@ -719,6 +987,7 @@ int hgp_set_tuning(const hgp_tuning *t)
         x.batch_lanes = t->batch_lanes;
         x.hub_track = t->hub_track;
         x.graph_max_vertices = t->graph_max_vertices;
+        x.batch_fuse_max = t->batch_fuse_max_vertices;
     });
 }
 

@@ -5,6 +5,8 @@
 #include "hgp_common.hpp"
 #include "hgp_group.hpp"
 
+#include <vector>
+
 namespace hgp {
 
 // One level of the hierarchy in HBM: CSR with 64-bit offsets, int32 ids and
@@ -73,5 +75,46 @@ void project(Ctx &c, int32_t n, const int32_t *cmap, const int32_t *where_c, int
 // Exact cut into a device int64 (overwritten), part weights into pwgts[k].
 void edge_cut(Ctx &c, const DevGraph &g, const int32_t *where, int64_t *d_cut);
 void part_weights(Ctx &c, int32_t n, const wgt_t *vwgt, const int32_t *where, int32_t k, int64_t *pwgts);
+
+// ---------------------------------------------------------------- small levels of many graphs
+
+// Largest level (vertices) and k that the one-block Jet loop takes.
+constexpr int32_t kTinyMaxN = 4096;
+constexpr int32_t kTinyMaxK = 256;
+
+// Device scratch of the one-block Jet loop for levels of up to n vertices
+// and k parts (alloc_tiny_scratch: left end of the arena), plus the run's
+// cut (int64; in: the coarsest partition's cut, out: the cut after the
+// last level) and the Jet iterations of each level (HGP_MAX_LEVELS ints).
+struct TinyScratch {
+    int32_t *dest = nullptr, *pregain = nullptr, *best = nullptr, *elist = nullptr;
+    int32_t *under = nullptr, *es = nullptr, *ee = nullptr, *pstart = nullptr, *iters = nullptr;
+    uint8_t *lock = nullptr, *sel = nullptr, *w8 = nullptr;
+    uint16_t *w16 = nullptr, *pd16 = nullptr;
+    uint32_t *pd32 = nullptr, *key = nullptr, *val = nullptr, *k2 = nullptr, *v2 = nullptr;
+    uint64_t *ab = nullptr;
+    int64_t *wsc = nullptr, *bs = nullptr, *pwgts = nullptr, *cut = nullptr;
+    void *ctl = nullptr;   // the Jet control block
+};
+TinyScratch alloc_tiny_scratch(Ctx &c, int32_t n, int32_t k);
+
+// One graph's run of small levels: levels hi (the coarsest) down to lo, each
+// refined by the one-block Jet loop and projected to the next finer level;
+// the last projection fills where[lo - 1] when lo > 0.  lv, where and cmap
+// are indexed by level; cmap[l] maps level l to level l + 1.
+struct TinyRun {
+    int32_t k = 0, hi = 0, lo = 0;
+    int64_t tvwgt = 0;
+    Params p;
+    std::vector<DevGraph> lv;
+    std::vector<int32_t *> where;
+    std::vector<const int32_t *> cmap;
+    TinyScratch s;
+};
+
+// Every run in one launch, one block per run (k <= kTinyMaxK, levels of <=
+// kTinyMaxN vertices, checked by the caller); the same partitions as
+// refine_level and project level by level.  Waits for the device.
+void tiny_batch(Ctx &c, const std::vector<TinyRun> &runs);
 
 }  // namespace hgp

@@ -10,7 +10,10 @@
 #include "hgp_graph.hpp"
 #include "hgp_prims.hpp"
 
+#include "hunyuangraph.h"
+
 #include <algorithm>
+#include <cstring>
 #include <type_traits>
 
 namespace hgp {
@@ -1939,6 +1942,11 @@ struct TinyArgs {
     int64_t *pwgts, *wsc, *bs;
 };
 
+inline size_t tiny_lds(int32_t k)
+{
+    return sizeof(int32_t) * (size_t)kTinyWaves * k + (((size_t)k + 7) / 8) * 8 + sizeof(int64_t) * (size_t)k;
+}
+
 struct TinyShared {
     NextScratch nx;
     uint32_t cnt[256], base_d[256];
@@ -2031,10 +2039,11 @@ __device__ inline void tiny_sort(uint32_t *key, uint32_t *val, uint32_t *k2, uin
     __syncthreads();
 }
 
-__global__ void __launch_bounds__(kTinyBS) k_jet_tiny(TinyArgs T)
+// The whole Jet loop of one level in the calling block (kTinyBS threads).
+// dyn: per-wave k-int conn rows | k part flags (8-byte padded) | k int64
+// deltas (tiny_lds(k) bytes of dynamic LDS).
+__device__ inline void jet_tiny_loop(const TinyArgs &T, int32_t *dyn, TinyShared &S)
 {
-    extern __shared__ int32_t dyn[];   // per-wave k-int conn rows | k part flags (8-byte padded) | k int64 deltas
-    __shared__ TinyShared S;
     const SelArgs &A = T.A;
     JetCtl &C = *A.ctl;
     const int32_t n = T.n, k = A.k;
@@ -2307,6 +2316,106 @@ __global__ void __launch_bounds__(kTinyBS) k_jet_tiny(TinyArgs T)
     }
 }
 
+__global__ void __launch_bounds__(kTinyBS) k_jet_tiny(TinyArgs T)
+{
+    extern __shared__ int32_t dyn[];
+    __shared__ TinyShared S;
+    jet_tiny_loop(T, dyn, S);
+}
+
+// ---------------------------------------------------------------- small levels of many graphs
+
+// One level of a run for k_jet_tiny_batch: its loop arguments (where, n and
+// the graph set per level, scratch shared by the run's levels), the map of
+// the next finer level onto it and that level's partition and size (the
+// projection target; nfine = 0 after level 0).
+struct TinyStep {
+    TinyArgs T;
+    const int32_t *cmap_fine;
+    int32_t *where_fine;
+    int32_t nfine;
+    int32_t finest;
+};
+
+struct TinyRunDev {
+    int32_t first, count;    // steps [first, first + count), coarsest level first
+    int32_t patience, lo;
+    int64_t maxwgt, opt;
+    int64_t *cut;
+    int32_t *iters;          // indexed by level
+};
+
+// refine_level's small-level path and project, level after level, in one
+// block per run: lock cleared, best = where, part weights, the control block
+// (k_jet_init), the narrow part copy, the Jet loop, where = best, the cut
+// carried to the next level, the projection.
+__global__ void __launch_bounds__(kTinyBS) k_jet_tiny_batch(const TinyRunDev *__restrict__ runs,
+                                                           const TinyStep *__restrict__ steps)
+{
+    extern __shared__ int32_t dyn[];
+    __shared__ TinyShared S;
+    __shared__ int64_t cut;
+    const TinyRunDev &R = runs[blockIdx.x];
+    const int tid = threadIdx.x, lane = lane_id(), wv = tid / kWave;
+    if (tid == 0) cut = *R.cut;
+    __syncthreads();
+    for (int32_t i = 0; i < R.count; i++) {
+        const TinyStep &st = steps[R.first + i];
+        const TinyArgs &T = st.T;
+        const int32_t n = T.n, k = T.A.k;
+        JetCtl &C = *T.A.ctl;
+        for (int32_t v = tid; v < n; v += kTinyBS) {
+            const int32_t w = T.where[v];
+            T.lock[v] = 0;
+            T.best[v] = w;
+            if (T.w8) T.w8[v] = (uint8_t)w;
+            else T.w16[v] = (uint16_t)w;
+        }
+        for (int32_t q = tid; q < k; q += kTinyBS) T.pwgts[q] = 0;
+        __syncthreads();
+        for (int32_t v = tid; v < n; v += kTinyBS)
+            atomicAdd((unsigned long long *)&T.pwgts[T.where[v]], (unsigned long long)(int64_t)T.A.vwgt[v]);
+        __syncthreads();
+        int64_t mx = INT64_MIN;
+        for (int32_t q = tid; q < k; q += kTinyBS) mx = T.pwgts[q] > mx ? T.pwgts[q] : mx;
+        mx = wave_max(mx);
+        if (lane == 0) S.lsh[wv] = mx;
+        __syncthreads();
+        if (tid == 0) {
+            int64_t m = S.lsh[0];
+            for (int w = 1; w < kTinyWaves; w++) m = S.lsh[w] > m ? S.lsh[w] : m;
+            C.cut = C.best_cut = cut;
+            C.best_max = m;
+            C.maxwgt = R.maxwgt;
+            C.opt = R.opt;
+            const int64_t md = (99 * R.maxwgt) / 100;
+            C.maxdest = md < R.maxwgt - 100 ? R.maxwgt - 100 : md;
+            C.delta2 = 0;
+            C.count = 0;
+            C.balance_counter = 0;
+            C.mode = 0;
+            C.done = 0;
+            C.iters = 0;
+            C.patience = R.patience;
+            C.ncand = 0;
+            C.nunder = 0;
+            C.copy_best = 0;
+            C.filt_num = st.finest ? 1 : 3;
+        }
+        __syncthreads();
+        jet_tiny_loop(T, dyn, S);
+        for (int32_t v = tid; v < n; v += kTinyBS) T.where[v] = T.best[v];
+        if (tid == 0) {
+            cut = C.best_cut;
+            R.iters[R.lo + R.count - 1 - i] = C.iters;
+        }
+        __syncthreads();
+        for (int32_t v = tid; v < st.nfine; v += kTinyBS) st.where_fine[v] = T.where[st.cmap_fine[v]];
+        __syncthreads();
+    }
+    if (tid == 0) *R.cut = cut;
+}
+
 // hunyuangraph_malloc_krefine (hunyuangraph_GPU_uncoarsen.h:428-508) + k_refine.
 // Launch sequences of one level's Jet loop recorded as HIP graphs: slot 0
 // the selection, slots 1-3 the rest of an iteration in modes LP, RW and RS.
@@ -2448,7 +2557,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         T.pwgts = pwgts;
         T.wsc = wsc;
         T.bs = bs;
-        size_t lds = sizeof(int32_t) * (size_t)kTinyWaves * k + (((size_t)k + 7) / 8) * 8 + sizeof(int64_t) * (size_t)k;
+        const size_t lds = tiny_lds(k);
         {
             KScope ks(c, "k_jet_tiny", 0.0);
             k_jet_tiny<<<1, kTinyBS, lds, c.stream>>>(T);
@@ -2621,6 +2730,122 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     HGP_LAUNCH_CHECK();
     c.arena.right_release(mark);
     return rs;
+}
+
+
+TinyScratch alloc_tiny_scratch(Ctx &c, int32_t n, int32_t k)
+{
+    TinyScratch s;
+    const size_t N = (size_t)(n > 0 ? n : 1), K = (size_t)k;
+    s.dest = c.arena.lalloc<int32_t>(N);
+    s.pregain = c.arena.lalloc<int32_t>(N);
+    s.best = c.arena.lalloc<int32_t>(N);
+    s.elist = c.arena.lalloc<int32_t>(N);
+    s.under = c.arena.lalloc<int32_t>(K);
+    s.es = c.arena.lalloc<int32_t>(K);
+    s.ee = c.arena.lalloc<int32_t>(K);
+    s.pstart = c.arena.lalloc<int32_t>(K);
+    s.iters = c.arena.lalloc<int32_t>(HGP_MAX_LEVELS);
+    s.lock = c.arena.lalloc<uint8_t>(N);
+    s.sel = c.arena.lalloc<uint8_t>(N);
+    if (k <= 256) {
+        s.w8 = c.arena.lalloc<uint8_t>(N);
+        s.pd16 = c.arena.lalloc<uint16_t>(N);
+    } else {
+        s.w16 = c.arena.lalloc<uint16_t>(N);
+        s.pd32 = c.arena.lalloc<uint32_t>(N);
+    }
+    s.key = c.arena.lalloc<uint32_t>(N);
+    s.val = c.arena.lalloc<uint32_t>(N);
+    s.k2 = c.arena.lalloc<uint32_t>(N);
+    s.v2 = c.arena.lalloc<uint32_t>(N);
+    s.ab = c.arena.lalloc<uint64_t>(N);
+    s.wsc = c.arena.lalloc<int64_t>(N);
+    s.bs = c.arena.lalloc<int64_t>(N + 1);
+    s.pwgts = c.arena.lalloc<int64_t>(K);
+    s.cut = c.arena.lalloc<int64_t>(1);
+    s.ctl = c.arena.lalloc<JetCtl>(1);
+    fill<int32_t>(c, s.iters, HGP_MAX_LEVELS, 0);
+    return s;
+}
+
+void tiny_batch(Ctx &c, const std::vector<TinyRun> &runs)
+{
+    if (runs.empty()) return;
+    std::vector<TinyRunDev> rd;
+    std::vector<TinyStep> sd;
+    int32_t kmax = 1;
+    for (const TinyRun &r : runs) {
+        if (r.k > kTinyMaxK || r.k < 2) throw Error{-4, "tiny_batch: k out of range"};
+        kmax = std::max(kmax, r.k);
+        TinyRunDev d;
+        d.first = (int32_t)sd.size();
+        d.count = r.hi - r.lo + 1;
+        d.patience = r.p.patience;
+        d.lo = r.lo;
+        d.opt = r.tvwgt / r.k;
+        d.maxwgt = ((int64_t)r.p.ub_num * r.tvwgt) / ((int64_t)r.p.ub_den * r.k);
+        d.cut = r.s.cut;
+        d.iters = r.s.iters;
+        rd.push_back(d);
+        const TinyScratch &s = r.s;
+        for (int32_t l = r.hi; l >= r.lo; l--) {
+            const DevGraph &g = r.lv[(size_t)l];
+            if (g.n > kTinyMaxN) throw Error{-4, "tiny_batch: level too large"};
+            TinyStep st;
+            TinyArgs &T = st.T;
+            int32_t *where = r.where[(size_t)l];
+            T.A = SelArgs{r.k,      g.xadj,   g.adjncy, g.adjwgt, g.vwgt, where, s.pwgts, s.lock,
+                          s.under,  (JetCtl *)s.ctl, s.dest, s.pregain, s.key, nullptr, NarrowParts{s.w8, s.w16},
+                          s.ab, s.pd16, s.pd32};
+            T.n = g.n;
+            T.sort_bits = bits_for((uint64_t)r.k * kMaxBuckets);
+            T.where = where;
+            T.under = s.under;
+            T.best = s.best;
+            T.pstart = s.pstart;
+            T.elist = s.elist;
+            T.es = s.es;
+            T.ee = s.ee;
+            T.sel = s.sel;
+            T.lock = s.lock;
+            T.w8 = s.w8;
+            T.w16 = s.w16;
+            T.val = s.val;
+            T.k2 = s.k2;
+            T.v2 = s.v2;
+            T.pwgts = s.pwgts;
+            T.wsc = s.wsc;
+            T.bs = s.bs;
+            st.finest = l == 0;
+            if (l > 0 && (l > r.lo || r.lo > 0)) {
+                st.cmap_fine = r.cmap[(size_t)l - 1];
+                st.where_fine = r.where[(size_t)l - 1];
+                st.nfine = r.lv[(size_t)l - 1].n;
+            } else {
+                st.cmap_fine = nullptr;
+                st.where_fine = nullptr;
+                st.nfine = 0;
+            }
+            sd.push_back(st);
+        }
+    }
+    const size_t rb = sizeof(TinyRunDev) * rd.size(), sb = sizeof(TinyStep) * sd.size();
+    const size_t mark = c.arena.right_mark();
+    char *dev = c.arena.ralloc<char>(rb + sb + 256);
+    TinyRunDev *drun = (TinyRunDev *)dev;
+    TinyStep *dstep = (TinyStep *)(dev + ((rb + 255) & ~(size_t)255));
+    std::vector<char> host(((rb + 255) & ~(size_t)255) + sb);
+    std::memcpy(host.data(), rd.data(), rb);
+    std::memcpy(host.data() + ((rb + 255) & ~(size_t)255), sd.data(), sb);
+    HGP_CHECK_HIP(hipMemcpyAsync(dev, host.data(), host.size(), hipMemcpyHostToDevice, c.stream));
+    {
+        KScope ks(c, "k_jet_tiny_batch", 0.0);
+        k_jet_tiny_batch<<<(unsigned)rd.size(), kTinyBS, tiny_lds(kmax), c.stream>>>(drun, dstep);
+        HGP_LAUNCH_CHECK();
+    }
+    HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
+    c.arena.right_release(mark);
 }
 
 }  // namespace hgp

@@ -11,8 +11,10 @@ from conftest import random_graph
 pytestmark = pytest.mark.gpu
 
 SETTINGS = [
-    dict(tiny_max_vertices=0, few_rows=0, fill_waves=1, seg_grid=1, batch_lanes=1, graph_max_vertices=1 << 30),
-    dict(tiny_max_vertices=4096, few_rows=1 << 20, fill_waves=65536, seg_grid=4, batch_lanes=3, hub_track=1),
+    dict(tiny_max_vertices=0, few_rows=0, fill_waves=1, seg_grid=1, batch_lanes=1, graph_max_vertices=1 << 30,
+         batch_fuse_max_vertices=0),
+    dict(tiny_max_vertices=4096, few_rows=1 << 20, fill_waves=65536, seg_grid=4, batch_lanes=3, hub_track=1,
+         batch_fuse_max_vertices=300),
     dict(tiny_max_vertices=1024, few_rows=2048, fill_waves=4096, seg_grid=8192, batch_lanes=12, hub_track=1,
          graph_max_vertices=0),
 ]
