@@ -42,7 +42,7 @@ CANDIDATES = {
     "batch_lanes": [4, 6, 8, 12],
     "hub_track": [1],
     "graph_max_vertices": [4096, 1 << 20],
-    "batch_fuse_max_vertices": [0, 1024, 2048],
+    "batch_fuse_max_vertices": [0, 2048, 8192, 16384],
 }
 
 

@@ -976,7 +976,7 @@ int hgp_set_tuning(const hgp_tuning *t)
         if (!t) throw Error{-1, "tuning is NULL"};
         if (t->tiny_max_vertices < 0 || t->tiny_max_vertices > 4096 || t->few_rows < 0 || t->fill_waves < 1 ||
             t->seg_grid < 1 || t->batch_lanes < 1 || t->batch_lanes > 64 || t->hub_track < 0 || t->hub_track > 1 ||
-            t->graph_max_vertices < 0 || t->batch_fuse_max_vertices < 0 || t->batch_fuse_max_vertices > 4096)
+            t->graph_max_vertices < 0 || t->batch_fuse_max_vertices < 0 || t->batch_fuse_max_vertices > kTinyMaxN)
             throw Error{-1, "tuning value out of range"};
         std::lock_guard<std::mutex> lk(g_api_mu);
         Tuning &x = tuning();

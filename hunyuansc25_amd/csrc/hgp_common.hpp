@@ -101,8 +101,8 @@ struct Tuning {
                                  // iterations (k <= 512) instead of rebuilding them in each selection
     int32_t graph_max_vertices = 0;   // levels of <= this many vertices replay their Jet launch
                                       // sequences as HIP graphs (0: never)
-    int32_t batch_fuse_max = 4096;    // hgp_partition_batch refines the levels of <= this many vertices
-                                      // of all graphs in one launch (0: per graph; at most 4096)
+    int32_t batch_fuse_max = 8192;    // hgp_partition_batch refines the levels of <= this many vertices
+                                      // of all graphs in one launch (0: per graph; at most 16384)
 };
 Tuning &tuning();
 

@@ -79,7 +79,7 @@ void part_weights(Ctx &c, int32_t n, const wgt_t *vwgt, const int32_t *where, in
 // ---------------------------------------------------------------- small levels of many graphs
 
 // Largest level (vertices) and k that the one-block Jet loop takes.
-constexpr int32_t kTinyMaxN = 4096;
+constexpr int32_t kTinyMaxN = 16384;
 constexpr int32_t kTinyMaxK = 256;
 
 // Device scratch of the one-block Jet loop for levels of up to n vertices

@@ -1927,7 +1927,7 @@ void project(Ctx &c, int32_t n, const int32_t *cmap, const int32_t *where_c, int
 // and banded graphs, 512 vertices beat both 256 and 1024.
 constexpr int kTinyBS = 1024;
 constexpr int kTinyWaves = kTinyBS / kWave;
-constexpr int32_t kTinyN = 4096;   // capacity of the block-local scans
+constexpr int32_t kTinyN = 4096;   // chunk of the block-local scans and compactions
 constexpr int32_t kTinyK = 256;
 constexpr int kTinyChunk = 4;                         // entries per lane and chunk
 constexpr eid_t kTinyRow = (eid_t)kTinyChunk * kWave;  // row entries per chunk
@@ -1961,20 +1961,26 @@ struct TinyShared {
 __device__ inline void tiny_scan_i64(int64_t *a, int32_t n, TinyShared &S)
 {
     constexpr int E = kTinyN / kTinyBS;
-    int32_t b = threadIdx.x * E;
-    int64_t x[E], run = 0;
+    int64_t carry = 0;
+    for (int32_t base = 0; base < n; base += kTinyN) {   // chunks of kTinyN, the running sum carried
+        const int32_t m = min(n - base, kTinyN);
+        const int32_t b = threadIdx.x * E;
+        int64_t x[E], run = 0;
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        x[e] = b + e < n ? a[b + e] : 0;
-        run += x[e];
-    }
-    int64_t off = block_exclusive_scan<int64_t>(run, S.lsh, nullptr);
+        for (int e = 0; e < E; e++) {
+            x[e] = b + e < m ? a[base + b + e] : 0;
+            run += x[e];
+        }
+        int64_t tot;
+        int64_t off = carry + block_exclusive_scan<int64_t>(run, S.lsh, &tot);
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        off += x[e];
-        if (b + e < n) a[b + e] = off;
+        for (int e = 0; e < E; e++) {
+            off += x[e];
+            if (b + e < m) a[base + b + e] = off;
+        }
+        carry += tot;
+        __syncthreads();
     }
-    __syncthreads();
 }
 
 // Stable LSD radix sort of (key, val)[0..n) over bits [0, hi) in 8-bit
@@ -2206,19 +2212,22 @@ __device__ inline void jet_tiny_loop(const TinyArgs &T, int32_t *dyn, TinyShared
             if (mode == kModeRS) {
                 // the evicted list in sorted order, its weight prefix, the cookie cutter
                 constexpr int E = kTinyN / kTinyBS;
-                const int32_t b0 = tid * E;
-                int32_t f[E], cnt = 0;
+                int32_t ne = 0;
+                for (int32_t base = 0; base < n; base += kTinyN) {   // chunks of kTinyN, in order
+                    const int32_t b0 = base + tid * E;
+                    int32_t f[E], cnt = 0;
 #pragma unroll
-                for (int e = 0; e < E; e++) {
-                    f[e] = (b0 + e < n && T.sel[b0 + e]) ? 1 : 0;
-                    cnt += f[e];
+                    for (int e = 0; e < E; e++) {
+                        f[e] = (b0 + e < n && T.sel[b0 + e]) ? 1 : 0;
+                        cnt += f[e];
+                    }
+                    uint32_t tot;
+                    uint32_t off = (uint32_t)ne + block_exclusive_scan<uint32_t>((uint32_t)cnt, S.ssh, &tot);
+#pragma unroll
+                    for (int e = 0; e < E; e++)
+                        if (f[e]) T.elist[off++] = b0 + e;
+                    ne += (int32_t)tot;
                 }
-                uint32_t tot;
-                uint32_t off = block_exclusive_scan<uint32_t>((uint32_t)cnt, S.ssh, &tot);
-#pragma unroll
-                for (int e = 0; e < E; e++)
-                    if (f[e]) T.elist[off++] = b0 + e;
-                const int32_t ne = (int32_t)tot;
                 __syncthreads();
                 if (tid == 0) T.bs[0] = 0;
                 for (int32_t i = tid; i < ne; i += kTinyBS) T.bs[i + 1] = A.vwgt[T.val[T.elist[i]]];

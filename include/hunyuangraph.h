@@ -193,7 +193,7 @@ typedef struct hgp_tuning {
     int32_t batch_fuse_max_vertices; /* hgp_partition_batch refines the levels of <= this many
                                    vertices (the coarse end of each hierarchy, k <= 256) of
                                    every graph of a group in one launch, a block per graph
-                                   (0: each graph on its own; 0..4096, default 4096) */
+                                   (0: each graph on its own; 0..16384, default 8192) */
 } hgp_tuning;
 int hgp_get_tuning(hgp_tuning *t);
 int hgp_set_tuning(const hgp_tuning *t);

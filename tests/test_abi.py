@@ -133,7 +133,7 @@ def test_tuning_round_trip_and_range(hgp):
         assert t["tiny_max_vertices"] == 1024 and t["batch_lanes"] == 3 and t["hub_track"] == 1
         assert t["graph_max_vertices"] == 77
         for bad in (dict(tiny_max_vertices=5000), dict(fill_waves=0), dict(batch_lanes=0), dict(seg_grid=0),
-                    dict(hub_track=2), dict(graph_max_vertices=-1), dict(batch_fuse_max_vertices=4097)):
+                    dict(hub_track=2), dict(graph_max_vertices=-1), dict(batch_fuse_max_vertices=16385)):
             with pytest.raises(hgp.HgpError) as e:
                 hgp.set_tuning(**bad)
             assert e.value.code == -1

@@ -53,3 +53,25 @@ def test_batch_reports_bad_graph(hgp):
     with pytest.raises(hgp.HgpError) as e:
         hgp.partition_batch(gs, ks, lanes=4)
     assert e.value.code == -1 and "graph 2" in str(e.value)
+
+
+def test_batch_fused_small_levels_match_alone(hgp):
+    """The batch's fused launch over the small levels (one block per graph,
+    levels of up to 16384 vertices: the block's scans and compactions then
+    run in several chunks) gives every graph its partition alone."""
+    x, a, vw, aw = random_graph(3000, 0.004, seed=8, wmax=7, vmax=5)
+    gs = [hgp.gen_rgg3d(12000, hgp.rgg_radius(12000, 12), seed=2), hgp.gen_grid2d(110, 110),
+          hgp.gen_banded(9000, 32, 0.1, seed=3), (x, a, vw, aw), hgp.gen_rmat(13, 1 << 16, seed=4)]
+    ks = [8, 16, 32, 5, 64]
+    alone = [hgp.partition(g[0], g[1], k, g[2], g[3], seed=4)[0] for g, k in zip(gs, ks)]
+    t0 = hgp.get_tuning()
+    try:
+        for fuse in (16384, 4096, 0):
+            print(f"fuse {fuse}", flush=True)
+            hgp.set_tuning(batch_fuse_max_vertices=fuse)
+            parts, stats = hgp.partition_batch(gs, ks, lanes=3, seed=4)
+            for i, (p, q) in enumerate(zip(parts, alone)):
+                np.testing.assert_array_equal(p, q, err_msg=f"fuse {fuse} graph {i}")
+                assert stats[i]["cut"] > 0 and stats[i]["max_pwgt"] > 0
+    finally:
+        hgp.set_tuning(**t0)
