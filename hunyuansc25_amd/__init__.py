@@ -44,6 +44,7 @@ class HgpOptions(C.Structure):
         ("device", C.c_int32),
         ("level_stats", C.c_int32),
         ("fm_kind", C.c_int32),
+        ("refiner", C.c_int32),
     ]
 
 
@@ -162,6 +163,7 @@ SIGNATURES = {
     "hgp_contract": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P]),
     "hgp_initpart": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _P, _P]),
     "hgp_refine": (C.c_int, [_I32, _P, _P, _P, _P, _I32, _I64, _I32, _P, _P, _P, _P]),
+    "hgp_refine_2way_lp": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _P]),
     "hgp_graph_free": (None, [_P]),
     "hgp_gen_grid2d": (C.c_int, [_I32, _I32, _P]),
     "hgp_gen_rgg3d": (C.c_int, [_I32, _F64, _U64, _I32, _I32, _I32, _I32, _P]),
@@ -604,7 +606,8 @@ def initpart(xadj, adjncy, vwgt, adjwgt, k, **opts) -> np.ndarray:
 
 
 def refine(xadj, adjncy, vwgt, adjwgt, k, tvwgt, is_finest, where, cut, **opts):
-    """One level of Jet refinement; returns (where, cut, iters)."""
+    """One level of Jet refinement (refiner=1: the older label propagation);
+    returns (where, cut, iters)."""
     xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
     n = len(xadj) - 1
     o = options(**opts)
@@ -614,6 +617,19 @@ def refine(xadj, adjncy, vwgt, adjwgt, k, tvwgt, is_finest, where, cut, **opts):
     _check(lib().hgp_refine(n, _ptr(xadj), _ptr(adjncy), _ptr(vwgt), _ptr(adjwgt), int(k), int(tvwgt),
                             int(is_finest), C.byref(o), _ptr(w), _ptr(c), _ptr(it)))
     return w, int(c[0]), int(it[0])
+
+
+def refine_2way_lp(xadj, adjncy, vwgt, adjwgt, where, **opts):
+    """The 2-way label propagation pass (jetLP) on a 0/1 partition; returns
+    (where, transfer ran)."""
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    o = options(**opts)
+    w = _i32(where).copy()
+    ran = np.zeros(1, dtype=np.int32)
+    _check(lib().hgp_refine_2way_lp(n, _ptr(xadj), _ptr(adjncy), _ptr(vwgt), _ptr(adjwgt), C.byref(o), _ptr(w),
+                                    _ptr(ran)))
+    return w, bool(ran[0])
 
 
 # ------------------------------------------------------------------ generators

@@ -336,6 +336,7 @@ Params params_from(const hgp_options *o)
     p.patience = o->patience;
     p.level_stats = o->level_stats != 0;
     p.fm_kind = o->fm_kind;
+    p.refiner = o->refiner;
     return p;
 }
 
@@ -345,7 +346,7 @@ void check_params(const Params &p, int32_t k)
     if (k > 4096) throw Error{-4, "nparts > 4096 is not supported"};
     if (p.ub_num <= 0 || p.ub_den <= 0 || p.ub_num < p.ub_den) throw Error{-1, "balance bound must be >= 1"};
     if (p.match_rounds < 0 || p.trials < 1 || p.fm_passes < 0 || p.patience < 1 || p.fm_kind < -1 ||
-        p.fm_kind > 2)
+        p.fm_kind > 2 || p.refiner < 0 || p.refiner > 1)
         throw Error{-1, "invalid option values"};
 }
 
@@ -656,6 +657,7 @@ void hgp_default_options(hgp_options *o)
     o->validate = 0;
     o->device = 0;
     o->level_stats = 0;
+    o->refiner = 0;
 }
 
 int hgp_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
@@ -710,7 +712,7 @@ int hgp_partition_batch(int32_t count, const int32_t *n, const int64_t *const *x
             for (auto &t : th) t.join();
         };
         const int32_t fuse_n = std::min(tuning().batch_fuse_max, kTinyMaxN);
-        if (fuse_n > 0 && !o.level_stats) {
+        if (fuse_n > 0 && !o.level_stats && o.refiner == 0) {
             // Groups of graphs (largest first, up to ~8 GB of hierarchies):
             // phase A coarsens and initially partitions each graph on the
             // lanes and keeps its hierarchy; one launch refines the small
@@ -1256,6 +1258,29 @@ int hgp_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int3
             to_host(c, where, dw, (size_t)n);
             to_host(c, cut, dc, 1);
             if (iters) *iters = rs.iters;
+        });
+    });
+}
+
+int hgp_refine_2way_lp(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                       const int32_t *adjwgt, const hgp_options *opt, int32_t *where, int32_t *ran)
+{
+    return guarded([&] {
+        hgp_options o;
+        if (opt) o = *opt; else hgp_default_options(&o);
+        validate_host(n, xadj, adjncy, vwgt, adjwgt, false);
+        for (int32_t v = 0; v < n; v++)
+            if (where[v] < 0 || where[v] > 1) throw Error{-1, "side out of range (0 or 1)"};
+        if (ran) *ran = 0;
+        if (n == 0) return;
+        std::lock_guard<std::mutex> lk(g_api_mu);
+        Ctx &c = context(o.device);
+        with_arena(c, estimate_arena(n, xadj[n], 2, true), [&] {
+            DevGraph g = upload(c, n, xadj, adjncy, vwgt, adjwgt);
+            int32_t *dw = to_dev(c, where, (size_t)n);
+            const bool r = lp2_refine(c, g, dw);
+            to_host(c, where, dw, (size_t)n);
+            if (ran) *ran = r ? 1 : 0;
         });
     });
 }

@@ -33,6 +33,7 @@ struct Params {
     int32_t fm_passes = 10;
     int32_t patience = 6;
     int32_t fm_kind = -1;       // bisection-trial FM: 0 scan, 1 heap (same moves), 2 reference heap FM, -1 auto
+    int32_t refiner = 0;        // level refinement: 0 Jet, 1 the older label propagation
     bool level_stats = false;   // per-level edge-weight sums and build times
 };
 
@@ -68,6 +69,18 @@ struct RefineStats {
 // Jet refinement of one level in place.  *d_cut (device int64) in/out.
 RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bool finest,
                          const Params &p, int32_t *where, int64_t *d_cut);
+
+// The older k-way label propagation of one level in place
+// (hunyuangraph_k_refinement_me, hunyuangraph_GPU_LP.h:1245-1374); *d_cut
+// (device int64) receives the exact cut.  iters = rounds that moved a vertex
+// plus rebalancing rounds.
+RefineStats lp_refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, int32_t *where, int64_t *d_cut);
+int32_t lp_threshold(int32_t n, int32_t k);
+void lp_part_bounds(int32_t k, int64_t tvwgt, float *fmin, float *fmax);
+
+// The 2-way label propagation of jetLP (hunyuangraph_GPU_LP.h:673-781) on a
+// 0/1 partition in place; returns whether the transfer step ran.
+bool lp2_refine(Ctx &c, const DevGraph &g, int32_t *where);
 
 // where_f[v] = where_c[cmap[v]]
 void project(Ctx &c, int32_t n, const int32_t *cmap, const int32_t *where_c, int32_t *where_f);

@@ -2472,6 +2472,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     int32_t n = g.n;
     if (n <= 0 || k <= 1) return rs;
     if (k > 4096) throw Error{-4, "refine: k > 4096 is not supported"};
+    if (p.refiner == 1) return lp_refine_level(c, g, k, tvwgt, where, d_cut);
     size_t mark = c.arena.right_mark();
     int64_t opt = tvwgt / k;
     int64_t maxwgt = ((int64_t)p.ub_num * tvwgt) / ((int64_t)p.ub_den * k);

@@ -67,6 +67,10 @@ typedef struct hgp_options {
                             /* 2 the reference's priority-queue FM               */
                             /* (FM_2way_cut_refinement); -1 (default) picks 0 or */
                             /* 1 by the coarsest graph's size (same result)      */
+    int32_t refiner;        /* refinement of each level: 0 Jet (default), 1 the  */
+                            /* older label propagation the reference keeps beside*/
+                            /* it (hunyuangraph_k_refinement_me,                 */
+                            /* hunyuangraph_GPU_LP.h:1245)                       */
 } hgp_options;
 
 #define HGP_MAX_LEVELS 64
@@ -332,10 +336,23 @@ int hgp_initpart(int32_t n, const int64_t *xadj, const int32_t *adjncy, const in
                  const int32_t *adjwgt, int32_t k, const hgp_options *opt, int32_t *where);
 
 /* Jet refinement of one level (k_refine, hunyuangraph_GPU_krefine.h:4434);
- * where and *cut in/out, *iters out. */
+ * where and *cut in/out, *iters out.  With opt->refiner = 1 the level runs
+ * the older label propagation instead (hunyuangraph_k_refinement_me,
+ * hunyuangraph_GPU_LP.h:1245); *iters then counts its rounds that moved a
+ * vertex plus its rebalancing rounds. */
 int hgp_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
                const int32_t *adjwgt, int32_t k, int64_t tvwgt, int32_t is_finest,
                const hgp_options *opt, int32_t *where, int64_t *cut, int32_t *iters);
+
+/* The 2-way label propagation of jetLP (hunyuangraph_GPU_LP.h:673), which
+ * the reference keeps beside its 2-way FM (FM_2WayCutRefine_GPU,
+ * hunyuangraph_GPU_2wayrefine.h:653-674): one synchronous pass in which a
+ * vertex whose edges into its own side weigh no more than those to the other
+ * side changes side, then, when a side lies outside [tvwgt/2/1.03,
+ * tvwgt/2*1.03], a transfer of the best-gain prefix of the heavier side.
+ * where (0/1) in/out; *ran = 1 if the transfer ran.  Uses opt->device only. */
+int hgp_refine_2way_lp(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                       const int32_t *adjwgt, const hgp_options *opt, int32_t *where, int32_t *ran);
 
 /* ------------------------------------------------------------ graphs
  * No counterpart in the reference, which reads graphs from files

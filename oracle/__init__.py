@@ -28,6 +28,7 @@ class OracleOpts(C.Structure):
         ("fm_passes", C.c_int32),
         ("patience", C.c_int32),
         ("fm_kind", C.c_int32),
+        ("refiner", C.c_int32),
     ]
 
 
@@ -96,6 +97,10 @@ _SIG = {
     "oracle_jet_lp_step": (_I64, [_I32, _P, _P, _P, _I32, _P, _I32, _I32, _P, _P]),
     "oracle_gain_bucket": (_I32, [_I64, _I64]),
     "oracle_cpu_rb_partition": (_I64, [_I32, _P, _P, _P, _P, _I32, _U64, _P]),
+    "oracle_lp_threshold": (_I32, [_I32, _I32]),
+    "oracle_lp_part_bounds": (None, [_I32, _I64, _P, _P]),
+    "oracle_lp_refine": (None, [_I32, _P, _P, _P, _P, _I32, _I64, _P, _P, _P]),
+    "oracle_lp2_refine": (_I32, [_I32, _P, _P, _P, _P, _P]),
 }
 
 _lib = None
@@ -390,3 +395,35 @@ def fb_apply(kind, xadj, adjncy, adjwgt, vwgt, maxvwgt, match):
     lib().oracle_fb_apply(FALLBACKS[kind], len(xadj) - 1, _p(xadj), _p(adjncy), _p(adjwgt), _p(vwgt),
                           int(maxvwgt), _p(m))
     return m
+
+
+def lp_threshold(n, k):
+    return int(lib().oracle_lp_threshold(int(n), int(k)))
+
+
+def lp_part_bounds(k, tvwgt):
+    """(fmin, fmax) as the float32 values the older LP compares against."""
+    lo, hi = C.c_float(0), C.c_float(0)
+    lib().oracle_lp_part_bounds(int(k), int(tvwgt), C.byref(lo), C.byref(hi))
+    return np.float32(lo.value), np.float32(hi.value)
+
+
+def lp_refine(xadj, adjncy, vwgt, adjwgt, k, tvwgt, where):
+    """Older k-way label propagation of one level: (where, cut, (rounds with
+    moves, rebalancing rounds))."""
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    w = _i32(where).copy()
+    c = np.zeros(1, dtype=np.int64)
+    it = np.zeros(2, dtype=np.int32)
+    lib().oracle_lp_refine(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), int(k), int(tvwgt), _p(w), _p(c), _p(it))
+    return w, int(c[0]), (int(it[0]), int(it[1]))
+
+
+def lp2_refine(xadj, adjncy, vwgt, adjwgt, where):
+    """The 2-way label propagation: (where, transfer ran)."""
+    xadj, adjncy, vwgt, adjwgt = _i64(xadj), _i32(adjncy), _i32(vwgt), _i32(adjwgt)
+    n = len(xadj) - 1
+    w = _i32(where).copy()
+    ran = lib().oracle_lp2_refine(n, _p(xadj), _p(adjncy), _p(vwgt), _p(adjwgt), _p(w))
+    return w, bool(ran)

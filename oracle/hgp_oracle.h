@@ -91,6 +91,8 @@ typedef struct oracle_opts {
     int32_t patience;           /* Jet non-improving rounds, default 6           */
     int32_t fm_kind;            /* 0 wave arg-max FM, 1 heap FM with the same    */
                                 /* moves, 2 the reference's heap FM (default 0)  */
+    int32_t refiner;            /* 0 Jet (default), 1 the older label            */
+                                /* propagation (oracle_lp_refine)                */
 } oracle_opts;
 
 typedef struct oracle_stats {
@@ -210,6 +212,26 @@ void oracle_cookie_cutter(int32_t k, int32_t ne, const int64_t *bs, int64_t maxw
                           const int64_t *pwgts, int32_t *es, int32_t *ee);
 int oracle_jet_accept(int64_t cur_cut, int64_t curmax, int64_t maxwgt, int64_t *best_cut,
                       int64_t *best_max, int32_t *count);
+
+/* The older label propagation the reference keeps beside Jet
+ * (hunyuangraph_k_refinement_me, hunyuangraph_GPU_LP.h:1245-1374), one level
+ * in place: up to `threshold` rounds that move, into each part, the unmoved
+ * vertex of highest positive gain whose best part it is, then up to
+ * `threshold` rebalancing rounds between overweight and underweight parts.
+ * iters[0] = rounds that moved a vertex, iters[1] = rebalancing rounds;
+ * *cut receives the exact cut afterwards. */
+int32_t oracle_lp_threshold(int32_t n, int32_t k);
+void oracle_lp_part_bounds(int32_t k, int64_t tvwgt, float *fmin, float *fmax);
+void oracle_lp_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                      const int32_t *vwgt, const int32_t *adjwgt, int32_t k, int64_t tvwgt,
+                      int32_t *where, int64_t *cut, int32_t *iters);
+
+/* The reference's 2-way label propagation (jetLP, hunyuangraph_GPU_LP.h:
+ * 673-781): one synchronous flip pass, then, when a side is outside
+ * [tvwgt/2/1.03, tvwgt/2*1.03], one sort-based transfer from the heavier
+ * side.  where (0/1) in place; returns 1 if the transfer ran. */
+int32_t oracle_lp2_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
+                          const int32_t *vwgt, const int32_t *adjwgt, int32_t *where);
 
 /* ---------------------------------------------------------------- (3) */
 

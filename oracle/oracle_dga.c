@@ -51,6 +51,7 @@ void oracle_default_opts(oracle_opts *o)
     o->fm_passes = 10;
     o->fm_kind = 0;
     o->patience = 6;
+    o->refiner = 0;
 }
 
 /* hunyuangraph_partitiongraph.h:298-302 */
@@ -1300,6 +1301,246 @@ int32_t oracle_dga_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy,
     return iters;
 }
 
+/* ---------------------------------------- older label propagation (k-way, 2-way) */
+
+/* hunyuangraph_compute_log2 (hunyuangraph_common.h:26-31): floor(log2 a), 0 for a <= 1. */
+static int32_t ref_log2(int32_t a)
+{
+    int32_t i;
+    for (i = 1; a > 1; i++, a >>= 1) {}
+    return i - 1;
+}
+
+/* hunyuangraph_GPU_LP.h:1251: max(nvtxs / (6000 * log2(nparts)), 8 * nparts). */
+int32_t oracle_lp_threshold(int32_t n, int32_t k)
+{
+    int32_t a = n / (6000 * ref_log2(k)), b = 8 * k;
+    return a > b ? a : b;
+}
+
+/* calculate_overweight (hunyuangraph_GPU_LP.h:1008-1026) with tpwgts[0] =
+ * (float)(1.0 / nparts) (hunyuangraph_admin.h:21-23): each bound is a double
+ * expression stored in a float; a part weight is compared as a float. */
+void oracle_lp_part_bounds(int32_t k, int64_t tvwgt, float *fmin, float *fmax)
+{
+    float temp = (float)(1.0 / k);
+    int32_t tv = (int32_t)tvwgt;
+    *fmin = (float)(temp / 1.03 * tv);
+    *fmax = (float)(temp * 1.03 * tv);
+}
+
+static void lp_flags(const int64_t *pw, int32_t k, float fmin, float fmax, uint8_t *over, uint8_t *thin,
+                     int32_t *nover, int32_t *nthin)
+{
+    *nover = *nthin = 0;
+    for (int32_t p = 0; p < k; p++) {
+        float val = (float)(int32_t)pw[p];
+        over[p] = val > fmax;
+        thin[p] = val < fmin;
+        *nover += over[p];
+        *nthin += thin[p];
+    }
+}
+
+/* Arg-max of conn over all k parts, the lowest part id among equal values;
+ * only the parts in touched[0..nt) can be non-zero. */
+static int32_t lp_argmax(const int64_t *conn, const int32_t *touched, int32_t nt)
+{
+    int32_t best = -1;
+    int64_t bc = 0;
+    for (int32_t i = 0; i < nt; i++) {
+        int32_t q = touched[i];
+        if (conn[q] > bc || (conn[q] == bc && conn[q] > 0 && q < best)) {
+            bc = conn[q];
+            best = q;
+        }
+    }
+    return best < 0 ? 0 : best;   /* all zero: part 0 */
+}
+
+/* calculate_move (hunyuangraph_GPU_LP.h:990-1006): the chosen vertex of each
+ * destination part moves there; returns the number of moves. */
+static int32_t lp_apply(int32_t k, const int32_t *bv, const int32_t *vwgt, int64_t *pw, int32_t *where,
+                        uint8_t *moved)
+{
+    int32_t moves = 0;
+    for (int32_t p = 0; p < k; p++) {
+        int32_t v = bv[p];
+        if (v < 0) continue;
+        pw[where[v]] -= vwgt[v];
+        pw[p] += vwgt[v];
+        where[v] = p;
+        moved[v] = 1;
+        moves++;
+    }
+    return moves;
+}
+
+/* hunyuangraph_k_refinement_me (hunyuangraph_GPU_LP.h:1245-1374).
+ * Deterministic where the reference races:
+ *  - calculate_to / calculate_to_reba (:822-917, :1059-1162) reduce the
+ *    connectivity row in shared memory with unsynchronised steps (and, past
+ *    64 parts, pair a value with the index of another slot, :858-867); here
+ *    the destination is the part of largest connectivity, the lowest part id
+ *    among equal values, over all parts including the own one.
+ *  - calculate_Max (:930-988) and calculate_Max_reba (:1164-1179) pick, per
+ *    destination, whichever vertex of the largest gain wrote last; here the
+ *    lowest vertex id among equal gains.
+ *  - set_bnd_reba (:1028-1057) reads an uninitialised flag when no neighbour
+ *    qualifies; here such a vertex is not a boundary vertex.
+ *  - The first loop stops after a round without moves: nothing changed, so
+ *    every later round would repeat it. */
+void oracle_lp_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                      const int32_t *adjwgt, int32_t k, int64_t tvwgt, int32_t *where, int64_t *cut,
+                      int32_t *iters)
+{
+    iters[0] = iters[1] = 0;
+    if (n == 0 || k < 2) {
+        *cut = oracle_edge_cut(n, xadj, adjncy, adjwgt, where);
+        return;
+    }
+    int64_t *pw = (int64_t *)calloc((size_t)k, sizeof(int64_t));
+    int64_t *conn = (int64_t *)calloc((size_t)k, sizeof(int64_t));
+    int64_t *bg = (int64_t *)calloc((size_t)k, sizeof(int64_t));
+    int32_t *touched = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
+    int32_t *bv = (int32_t *)malloc(sizeof(int32_t) * (size_t)k);
+    uint8_t *seen = (uint8_t *)calloc((size_t)k, 1);
+    uint8_t *over = (uint8_t *)calloc((size_t)k, 1);
+    uint8_t *thin = (uint8_t *)calloc((size_t)k, 1);
+    uint8_t *moved = (uint8_t *)calloc((size_t)n, 1);
+    for (int32_t v = 0; v < n; v++) pw[where[v]] += vwgt[v];
+    const int32_t thr = oracle_lp_threshold(n, k);
+
+    /* first loop (:1283-1308): set_bnd, calculate_to, init_max, calculate_Max, calculate_move */
+    for (int32_t it = 0; it < thr; it++) {
+        for (int32_t p = 0; p < k; p++) { bv[p] = -1; bg[p] = 0; }
+        for (int32_t v = 0; v < n; v++) {
+            if (moved[v]) continue;                       /* set_bnd (:791-820) */
+            const int32_t me = where[v];
+            int bnd = 0;
+            for (int64_t e = xadj[v]; e < xadj[v + 1]; e++)
+                if (where[adjncy[e]] != me) { bnd = 1; break; }
+            if (!bnd) continue;
+            int32_t nt = 0;
+            for (int64_t e = xadj[v]; e < xadj[v + 1]; e++) {
+                int32_t q = where[adjncy[e]];
+                if (!seen[q]) { seen[q] = 1; touched[nt++] = q; }
+                conn[q] += adjwgt[e];
+            }
+            const int32_t best = lp_argmax(conn, touched, nt);
+            const int64_t gain = conn[best] - conn[me];
+            for (int32_t i = 0; i < nt; i++) { conn[touched[i]] = 0; seen[touched[i]] = 0; }
+            if (best == me || gain <= 0) continue;        /* moveto -1, or calculate_Max's val > 0 */
+            if (bv[best] < 0 || gain > bg[best]) { bv[best] = v; bg[best] = gain; }
+        }
+        if (!lp_apply(k, bv, vwgt, pw, where, moved)) break;
+        iters[0]++;
+    }
+
+    /* rebalancing loop (:1313-1364) */
+    float fmin, fmax;
+    oracle_lp_part_bounds(k, tvwgt, &fmin, &fmax);
+    int32_t nover, nthin;
+    lp_flags(pw, k, fmin, fmax, over, thin, &nover, &nthin);
+    int32_t it2 = 0;
+    while (nover > 0 && nthin > 0) {
+        if (it2 >= thr) break;
+        it2++;
+        for (int32_t p = 0; p < k; p++) { bv[p] = -1; bg[p] = 0; }
+        for (int32_t v = 0; v < n; v++) {
+            const int32_t me = where[v];
+            if (!over[me]) continue;                      /* set_bnd_reba (:1028-1057) */
+            int bnd = 0;
+            for (int64_t e = xadj[v]; e < xadj[v + 1]; e++) {
+                int32_t q = where[adjncy[e]];
+                if (q != me && thin[q]) { bnd = 1; break; }
+            }
+            if (!bnd) continue;
+            int32_t nt = 0;                                /* calculate_to_reba (:1059-1162) */
+            for (int64_t e = xadj[v]; e < xadj[v + 1]; e++) {
+                int32_t q = where[adjncy[e]];
+                if (q == me || !thin[q]) continue;
+                if (!seen[q]) { seen[q] = 1; touched[nt++] = q; }
+                conn[q] += adjwgt[e];
+            }
+            const int32_t best = lp_argmax(conn, touched, nt);
+            const int64_t gain = conn[best];
+            for (int32_t i = 0; i < nt; i++) { conn[touched[i]] = 0; seen[touched[i]] = 0; }
+            if (!thin[best]) continue;
+            if (bv[best] < 0 || gain > bg[best]) { bv[best] = v; bg[best] = gain; }   /* calculate_Max_reba */
+        }
+        lp_apply(k, bv, vwgt, pw, where, moved);
+        lp_flags(pw, k, fmin, fmax, over, thin, &nover, &nthin);
+    }
+    iters[1] = it2;
+    *cut = oracle_edge_cut(n, xadj, adjncy, adjwgt, where);
+    free(pw); free(conn); free(bg); free(touched); free(bv); free(seen); free(over); free(thin); free(moved);
+}
+
+typedef struct { int32_t key, ptr; } lpkp;
+
+/* compRule (hunyuangraph_GPU_LP.h:499-508): key descending, then id ascending. */
+static int cmp_lpkp(const void *a, const void *b)
+{
+    const lpkp *x = (const lpkp *)a, *y = (const lpkp *)b;
+    if (x->key != y->key) return x->key > y->key ? -1 : 1;
+    return x->ptr < y->ptr ? -1 : (x->ptr > y->ptr);
+}
+
+/* jetLP (hunyuangraph_GPU_LP.h:673-781).  filter_atomic (:253-296) lets a
+ * thread read sides its neighbours have already flipped; here every vertex
+ * decides on the sides before the pass (a synchronous pass). */
+int32_t oracle_lp2_refine(int32_t n, const int64_t *xadj, const int32_t *adjncy, const int32_t *vwgt,
+                          const int32_t *adjwgt, int32_t *where)
+{
+    if (n == 0) return 0;
+    int32_t *nw = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    for (int32_t v = 0; v < n; v++) {
+        int64_t same = 0, other = 0;
+        for (int64_t e = xadj[v]; e < xadj[v + 1]; e++) {
+            if (where[adjncy[e]] == where[v]) same += adjwgt[e];
+            else other += adjwgt[e];
+        }
+        nw[v] = same <= other ? 1 - where[v] : where[v];
+    }
+    memcpy(where, nw, sizeof(int32_t) * (size_t)n);
+    free(nw);
+    /* compute_pwgts + reduction6 (:410-466, :714-722) */
+    int64_t tv = 0, pw[2] = {0, 0};
+    for (int32_t v = 0; v < n; v++) {
+        tv += vwgt[v];
+        if (where[v] == 0) pw[0] += vwgt[v];
+    }
+    pw[1] = tv - pw[0];
+    const int32_t tvi = (int32_t)tv;
+    const double lo = tvi * 0.5 / 1.03, hi = tvi * 0.5 * 1.03;
+    if (pw[0] >= lo && pw[0] <= hi && pw[1] >= lo && pw[1] <= hi) return 0;
+    const int32_t from = pw[0] > pw[1] ? 0 : 1, to = 1 - from;
+    /* compute_gainkp (:468-497); nedges = the number of adjacency entries */
+    const int32_t nedges = (int32_t)xadj[n];
+    lpkp *kp = (lpkp *)malloc(sizeof(lpkp) * (size_t)n);
+    for (int32_t v = 0; v < n; v++) {
+        int32_t id = 0, ed = 0;
+        for (int64_t e = xadj[v]; e < xadj[v + 1]; e++) {
+            if (where[adjncy[e]] == where[v]) id += adjwgt[e];
+            else ed += adjwgt[e];
+        }
+        kp[v].key = where[v] == from ? ed - id : -nedges;
+        kp[v].ptr = v;
+    }
+    qsort(kp, (size_t)n, sizeof(lpkp), cmp_lpkp);
+    /* compute_gainvkp, inclusive_scan, rebalancekp (:510-532, :751-773) */
+    const int32_t num = (int32_t)((pw[from] - pw[to]) / 2);
+    int64_t prefix = 0;
+    for (int32_t i = 0; i < n; i++) {
+        prefix += vwgt[kp[i].ptr];
+        if (prefix > num) break;
+        where[kp[i].ptr] = to;
+    }
+    free(kp);
+    return 1;
+}
+
 /* ------------------------------------------------------ full pipeline */
 
 /* hunyuangraph_PartitionGraph (hunyuangraph_partitiongraph.h:284-347) ->
@@ -1428,8 +1669,15 @@ int oracle_dga_partition(int32_t n, const int64_t *xadj, const int32_t *adjncy,
     }
     for (int32_t l = L; l >= 0; l--) {
         olevel *g = &lv[l];
-        int32_t it = oracle_dga_refine(g->n, g->xadj, g->adjncy, g->vwgt, g->adjwgt, k, tvwgt,
-                                       l == 0, &o, g->where, &cut);
+        int32_t it;
+        if (o.refiner == 1) {
+            int32_t two[2];
+            oracle_lp_refine(g->n, g->xadj, g->adjncy, g->vwgt, g->adjwgt, k, tvwgt, g->where, &cut, two);
+            it = two[0] + two[1];
+        } else {
+            it = oracle_dga_refine(g->n, g->xadj, g->adjncy, g->vwgt, g->adjwgt, k, tvwgt, l == 0, &o, g->where,
+                                   &cut);
+        }
         if (st) { st->level_iters[l] = it; st->refine_iters += it; }
         if (l == 0) break;
         olevel *fg = &lv[l - 1];

@@ -102,6 +102,16 @@ def test_host_validation_rejects_bad_csr(hgp):
         hgp.partition(x, np.array([0, 0, 2, 1], np.int32), 2)        # self-loop at vertex 0
     with pytest.raises(hgp.HgpError):
         hgp.partition(x, a, 5000)                                  # k beyond the supported range
+    vw = np.ones(4, np.int32)
+    with pytest.raises(hgp.HgpError) as e:
+        hgp.partition(x, a, 2, refiner=2)                          # no such refiner
+    assert e.value.code == -1
+    with pytest.raises(hgp.HgpError) as e:
+        hgp.refine(x, a, vw, w, 2, 4, 1, np.zeros(4, np.int32), 0, refiner=-1)
+    assert e.value.code == -1
+    with pytest.raises(hgp.HgpError) as e:
+        hgp.refine_2way_lp(x, a, vw, w, np.array([0, 2, 1, 0], np.int32))   # side 2
+    assert e.value.code == -1
 
 
 def test_struct_layouts_match_header(hgp, tmp_path):
