@@ -322,6 +322,8 @@ def main():
     ap.add_argument("--no-others", action="store_true", help="skip the grid and RGG configurations")
     ap.add_argument("--no-fm-kinds", action="store_true",
                     help="skip the bisection-FM comparison (scan, queue, reference queue FM) on the headline graph")
+    ap.add_argument("--no-refiners", action="store_true",
+                    help="skip the older label-propagation refiners (k-way and 2-way) on the headline graph")
     ap.add_argument("--calib-copy", action="store_true",
                     help="one 1 GiB device-to-device copy before the steps (counter calibration)")
     ap.add_argument("--batch-graphs", type=int, default=4096, help="batched-mode graphs (0: skip)")
@@ -449,6 +451,33 @@ def main():
             del d_tmp
         log(f"bisection FM kinds: {fm_kinds}")
 
+    refiners = None
+    if not args.no_refiners:
+        # the older label propagation the reference keeps beside Jet, on the
+        # headline graph: the k-way rounds as the level refiner, and the 2-way
+        # pass on a random 30 % / 70 % split (host arrays in and out)
+        d_tmp = torch.zeros(n, dtype=torch.int32, device=dev)
+        s_l = hgp.partition_device(n, d_x.data_ptr(), d_a.data_ptr(), d_vw.data_ptr(), d_aw.data_ptr(), args.k,
+                                   d_tmp.data_ptr(), seed=args.seed, refiner=1)
+        lp_part = d_tmp.cpu().numpy()
+        del d_tmp
+        lcut, lmax = host_cut_and_maxpart(x, a, lp_part, args.k)
+        w0 = (np.random.default_rng(args.seed).random(n) < 0.3).astype(np.int32)
+        t2 = time.perf_counter()
+        w1, ran = hgp.refine_2way_lp(x, a, vw, aw, w0)
+        t2 = time.perf_counter() - t2
+        refiners = {
+            "jet": {"ms_total": round(st["ms_total"], 3), "ms_uncoarsen": round(st["ms_uncoarsen"], 3),
+                    "cut": cut, "imbalance": round(maxp * args.k / tvwgt, 5), "refine_iters": st["refine_iters"]},
+            "older_lp": {"ms_total": round(s_l["ms_total"], 3), "ms_uncoarsen": round(s_l["ms_uncoarsen"], 3),
+                         "cut": lcut, "imbalance": round(lmax * args.k / tvwgt, 5),
+                         "rounds": s_l["refine_iters"], "cut_check": s_l["cut"] == lcut},
+            "lp_2way_pass": {"wall_ms_with_copies": round(t2 * 1e3, 3), "transfer_ran": bool(ran),
+                             "cut_before": host_cut_and_maxpart(x, a, w0, 2)[0],
+                             "cut_after": host_cut_and_maxpart(x, a, w1, 2)[0]},
+        }
+        log(f"refiners: {refiners}")
+
     others = []
     gpu_rates = {}
     if not args.no_others:
@@ -514,6 +543,7 @@ def main():
         "roofline": roofline,
         "kernels": {k: [v[0], round(v[1], 3), v[2]] for k, v in kt.items()},
         "bisection_fm": fm_kinds,
+        "refiners": refiners,
         "cpu_baseline": cpu,
         "other_configs": others,
         "batched": batched,

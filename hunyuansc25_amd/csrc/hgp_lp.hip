@@ -8,10 +8,13 @@
 // overweight part with the most edge weight to it.  The reference recomputes
 // every vertex's best part in every round, O(m) work for at most k moves.
 // Here the candidates (best part, gain) of all vertices stay in HBM and a
-// round recomputes only the neighbours of the vertices it moved (listed,
-// without duplicates, by the move kernel); rows longer than a threshold keep
-// their edge weight per part in an HBM row that the moves update with
+// round recomputes only the vertices it moved and their neighbours (listed,
+// without duplicates, by the move kernel); a rebalancing round after which a
+// part crossed a bound recomputes every vertex.  Rows longer than a threshold
+// keep their edge weight per part in an HBM row that the moves update with
 // atomics, so a hub's candidate is an O(k) scan instead of a walk of its row.
+// A rebalancing round without a move leaves nothing changed, so every later
+// round would repeat it: the loop then ends and counts the rounds left.
 // Per destination part the winner is the largest (gain, -id) key of one
 // 64-bit atomicMax, so the result does not depend on scheduling.
 //
@@ -37,6 +40,8 @@ struct LpCtl {
     int32_t done, iters1, done2, iters2;
     int32_t moves[2];                     // moves of the rounds by parity
     int64_t list_n;                       // vertices listed for recomputation
+    int32_t full;                         // rebalancing: the bounds changed, recompute every vertex
+    int32_t pad;
 };
 
 struct LpArgs {
@@ -110,21 +115,23 @@ __global__ void __launch_bounds__(kLpBS) k_lp_tbuild(const int32_t *__restrict__
     }
 }
 
-// Candidates of the listed vertices (ids == nullptr: every vertex; with a
-// list the count is the control block's list_n).  PH 1: an unmoved vertex
+// Candidates of the listed vertices (LIST: the count is the control block's
+// list_n, and each vertex leaves the list; else every vertex, in PH 2 only
+// after a round that changed the bounds).  PH 1: an unmoved vertex
 // whose best other part (largest edge weight, lowest id among equal ones)
 // beats its own part by gain > 0.  PH 2: a vertex of an overweight part with
 // edge weight to an underweight part other than its own; the best such part,
 // gain = that weight.  A wave takes 64 vertices and works through the ones
 // with work together: a tracked row scans its k-int row, another row adds its
 // entries into the wave's k-int LDS row.
-template <int PH>
+template <int PH, bool LIST>
 __global__ void __launch_bounds__(kLpBS) k_lp_cand(const int32_t *__restrict__ ids, int64_t cnt, LpArgs A)
 {
     extern __shared__ int32_t acc[];
     const LpCtl &C = *A.ctl;
     if (PH == 1 ? C.done : C.done2) return;
-    if (ids) cnt = C.list_n;
+    if (PH == 2 && !LIST && !C.full) return;
+    if (LIST) cnt = C.list_n;
     const int k = A.k, lane = lane_id(), wv = threadIdx.x / kWave;
     int32_t *my = acc + (size_t)wv * k;
     for (int i = lane; i < k; i += kWave) my[i] = 0;
@@ -132,12 +139,13 @@ __global__ void __launch_bounds__(kLpBS) k_lp_cand(const int32_t *__restrict__ i
     for (int64_t base = ((int64_t)blockIdx.x * kLpWaves + wv) * kWave; base < cnt;
          base += (int64_t)gridDim.x * kLpWaves * kWave) {
         const int64_t i = base + lane;
-        const int32_t v = i < cnt ? (ids ? ids[i] : (int32_t)i) : -1;
+        const int32_t v = i < cnt ? (LIST ? ids[i] : (int32_t)i) : -1;
         bool work = false;
         if (v >= 0) {
+            uint32_t f = 0;
+            if (LIST || PH == 1) f = A.flag[v];
+            if (LIST) A.flag[v] = f & ~2u;
             if (PH == 1) {
-                const uint32_t f = A.flag[v];
-                if (ids) A.flag[v] = f & ~2u;
                 work = !(f & 1u);
             } else {
                 work = A.over[A.where[v]] != 0;
@@ -164,6 +172,25 @@ __global__ void __launch_bounds__(kLpBS) k_lp_cand(const int32_t *__restrict__ i
                     }
                 }
                 if (PH == 1) cme = row[me];
+            } else if (A.xadj[u + 1] - A.xadj[u] <= kWave) {
+                // one entry per lane, kept in registers over the three passes
+                const eid_t j = A.xadj[u] + lane;
+                const bool live = j < A.xadj[u + 1];
+                int32_t q = -1, w = 0;
+                if (live) {
+                    q = A.where[A.adjncy[j]];
+                    w = A.adjwgt[j];
+                }
+                if (live && (PH == 1 || (q != me && A.thin[q]))) atomicAdd(&my[q], w);
+                wave_lds_sync();
+                if (live && q != me && (PH == 1 || A.thin[q])) {
+                    const int32_t val = my[q];
+                    if (val > 0) bkey = lp_key(val, q);
+                }
+                if (PH == 1) cme = my[me];
+                wave_lds_sync();
+                if (live) my[q] = 0;
+                wave_lds_sync();
             } else {
                 const eid_t b = A.xadj[u], e = A.xadj[u + 1];
                 for (eid_t j = b + lane; j < e; j += kWave) {
@@ -217,8 +244,12 @@ __global__ void __launch_bounds__(kLpBS) k_lp_pick(int32_t it, LpArgs A)
             C.moves[it & 1] = 0;
             C.list_n = 0;
         }
-    } else if (C.done2) {
-        return;
+    } else {
+        if (C.done2) return;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            C.moves[it & 1] = 0;
+            C.list_n = 0;
+        }
     }
     const int k = A.k;
     for (int p = threadIdx.x; p < k; p += kLpBS) sb[p] = 0;
@@ -234,7 +265,8 @@ __global__ void __launch_bounds__(kLpBS) k_lp_pick(int32_t it, LpArgs A)
 
 // calculate_move (hunyuangraph_GPU_LP.h:990-1006): block p moves its winner
 // into part p, then walks the winner's row: tracked neighbours update their
-// part rows, and in PH 1 every neighbour not yet listed is listed once.
+// part rows, and every neighbour not yet listed is listed once (in PH 2 the
+// winner too: its candidate changes with its part).
 template <int PH>
 __global__ void __launch_bounds__(kLpBS) k_lp_apply(int32_t it, LpArgs A)
 {
@@ -256,6 +288,12 @@ __global__ void __launch_bounds__(kLpBS) k_lp_apply(int32_t it, LpArgs A)
         if (PH == 1) {
             atomicOr(&A.flag[v], 1u);
             if (atomicAdd(&C.moves[it & 1], 1) == 0) atomicAdd(&C.iters1, 1);
+        } else {
+            atomicAdd(&C.moves[it & 1], 1);
+            if (!(atomicOr(&A.flag[v], 2u) & 2u)) {
+                const unsigned long long pos = atomicAdd((unsigned long long *)&C.list_n, 1ull);
+                A.list[pos] = v;
+            }
         }
     }
     const int k = A.k, lane = lane_id();
@@ -267,7 +305,7 @@ __global__ void __launch_bounds__(kLpBS) k_lp_apply(int32_t it, LpArgs A)
             atomicAdd(&row[a], -w);
             atomicAdd(&row[p], w);
         }
-        if (PH == 1) {
+        {
             const bool add = !(atomicOr(&A.flag[u], 2u) & 2u);
             const uint64_t m = __ballot(add);
             if (m) {
@@ -283,17 +321,28 @@ __global__ void __launch_bounds__(kLpBS) k_lp_apply(int32_t it, LpArgs A)
 
 // calculate_overweight (hunyuangraph_GPU_LP.h:1008-1026) and the loop test
 // of the rebalancing rounds: part weights compared as floats with the float
-// bounds; the loop ends when no part is over or none is under.
-__global__ void __launch_bounds__(kLpBS) k_lp_flags(LpArgs A, float fmin, float fmax)
+// bounds; the loop ends when no part is over or none is under.  After a
+// round without moves nothing has changed, so the rounds up to the
+// threshold would all repeat it: they are counted and the loop ends.  full
+// is set when a part's flags changed (or in the first round).
+__global__ void __launch_bounds__(kLpBS) k_lp_flags(int32_t it, int32_t thr, LpArgs A, float fmin, float fmax)
 {
     LpCtl &C = *A.ctl;
     if (C.done2) return;
-    __shared__ int nover, nthin;
-    if (threadIdx.x == 0) nover = nthin = 0;
+    if (it > 0 && C.moves[(it + 1) & 1] == 0) {
+        if (threadIdx.x == 0) {
+            C.iters2 = thr;
+            C.done2 = 1;
+        }
+        return;
+    }
+    __shared__ int nover, nthin, changed;
+    if (threadIdx.x == 0) nover = nthin = changed = 0;
     __syncthreads();
     for (int p = threadIdx.x; p < A.k; p += kLpBS) {
         const float val = (float)(int32_t)A.pw[p];
-        const bool o = val > fmax, t = val < fmin;
+        const uint8_t o = val > fmax, t = val < fmin;
+        if (o != A.over[p] || t != A.thin[p]) changed = 1;
         A.over[p] = o;
         A.thin[p] = t;
         if (o) atomicAdd(&nover, 1);
@@ -301,6 +350,7 @@ __global__ void __launch_bounds__(kLpBS) k_lp_flags(LpArgs A, float fmin, float 
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        C.full = it == 0 || changed;
         if (nover > 0 && nthin > 0) C.iters2++;
         else C.done2 = 1;
     }
@@ -482,16 +532,16 @@ RefineStats lp_refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt,
     const size_t lds_cand = sizeof(int32_t) * (size_t)k * kLpWaves;
     const size_t lds_pick = sizeof(unsigned long long) * (size_t)k;
     const unsigned g_all = std::min<unsigned>(grid_for(n, kLpBS), 2048);
-    const unsigned g_list = std::min<unsigned>(grid_for(n, kLpBS), 1024);
+    const unsigned g_list = std::min<unsigned>(grid_for(n, kLpBS), 2048);
     constexpr int kChunk = 16;
 
     // first loop (hunyuangraph_GPU_LP.h:1283-1308)
-    k_lp_cand<1><<<g_all, kLpBS, lds_cand, c.stream>>>(nullptr, n, A);
+    k_lp_cand<1, false><<<g_all, kLpBS, lds_cand, c.stream>>>(nullptr, n, A);
     HGP_LAUNCH_CHECK();
     for (int32_t it = 0; it < thr; it++) {
         k_lp_pick<1><<<g_all, kLpBS, lds_pick, c.stream>>>(it, A);
         k_lp_apply<1><<<(unsigned)k, kLpBS, 0, c.stream>>>(it, A);
-        k_lp_cand<1><<<g_list, kLpBS, lds_cand, c.stream>>>(A.list, 0, A);
+        k_lp_cand<1, true><<<g_list, kLpBS, lds_cand, c.stream>>>(A.list, 0, A);
         HGP_LAUNCH_CHECK();
         if ((it + 1) % kChunk == 0 && it + 1 < thr) {
             read_scalars(c, A.ctl, sizeof(LpCtl));
@@ -503,8 +553,9 @@ RefineStats lp_refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt,
     float fmin, fmax;
     lp_part_bounds(k, tvwgt, &fmin, &fmax);
     for (int32_t it = 0; it < thr; it++) {
-        k_lp_flags<<<1, kLpBS, 0, c.stream>>>(A, fmin, fmax);
-        k_lp_cand<2><<<g_all, kLpBS, lds_cand, c.stream>>>(nullptr, n, A);
+        k_lp_flags<<<1, kLpBS, 0, c.stream>>>(it, thr, A, fmin, fmax);
+        k_lp_cand<2, true><<<g_list, kLpBS, lds_cand, c.stream>>>(A.list, 0, A);
+        k_lp_cand<2, false><<<g_all, kLpBS, lds_cand, c.stream>>>(nullptr, n, A);
         k_lp_pick<2><<<g_all, kLpBS, lds_pick, c.stream>>>(it, A);
         k_lp_apply<2><<<(unsigned)k, kLpBS, 0, c.stream>>>(it, A);
         HGP_LAUNCH_CHECK();
