@@ -9,9 +9,13 @@
 // pass proves the offsets monotone before the entry pass reads any row, so a
 // malformed xadj never sends a kernel outside adjncy.
 //
-// Entries are walked in tiles: a thread takes kEPT consecutive entries,
-// finds the row of the first by a binary search over xadj and then steps
-// through the rows, so hub rows cost no more than short ones.
+// Entries are walked in tiles of kEPT x 64 consecutive entries, one tile per
+// wave: lane l takes entries l, l + 64, ... (each load instruction reads 64
+// consecutive entries).  Two lanes find the rows of the tile's ends by a
+// binary search over xadj; every lane then finds the row of its first entry
+// by a search between those two rows (a few cached lines) and follows the
+// rows forward, so hub rows cost no more than short ones and xadj is read
+// about once.
 #include "hgp_check.hpp"
 #include "hgp_host.hpp"
 
@@ -24,7 +28,8 @@ namespace hgp {
 namespace {
 
 constexpr int kChkBS = 256;
-constexpr int kEPT = 16;                        // entries per thread
+constexpr int kEPT = 16;                        // entries per lane and tile
+constexpr int64_t kWaveTile = (int64_t)kWave * kEPT;
 constexpr int64_t kTile = (int64_t)kChkBS * kEPT;
 constexpr unsigned kEntryGrid = 2048;           // grid of the entry walkers (grid-stride)
 
@@ -39,14 +44,27 @@ __device__ inline int32_t row_of(const eid_t *__restrict__ xadj, int32_t n, int6
     return lo;
 }
 
-// Calls f(v, j) for the entries j in [j0, j1) of this thread (row v), in
-// order.  The row index never passes n - 1, whatever xadj holds.
+// Calls f(v, j) for this lane's entries j of the wave's tile [t0, t1)
+// (t0 + lane, t0 + lane + 64, ...; row v).  Every lane of the wave must call
+// it.  The row index stays in [0, n - 1], whatever xadj holds.
 template <class F>
-__device__ inline void walk_entries(const eid_t *__restrict__ xadj, int32_t n, int64_t j0, int64_t j1, F f)
+__device__ inline void walk_tile(const eid_t *__restrict__ xadj, int32_t n, int64_t t0, int64_t t1, F f)
 {
-    int32_t v = row_of(xadj, n, j0);
+    const int lane = lane_id();
+    int32_t r = 0;
+    if (lane == 0) r = row_of(xadj, n, t0);
+    if (lane == 1) r = row_of(xadj, n, t1 - 1);
+    const int32_t r0 = __shfl(r, 0, kWave), r1 = __shfl(r, 1, kWave);
+    int64_t j = t0 + lane;
+    if (j >= t1) return;
+    int32_t lo = r0, hi = r1 + 1;   // the row of j lies in [r0, r1] when xadj is monotone
+    while (hi - lo > 1) {
+        const int32_t mid = lo + (hi - lo) / 2;
+        if (xadj[mid] <= j) lo = mid; else hi = mid;
+    }
+    int32_t v = lo;
     eid_t nx = xadj[v + 1];
-    for (int64_t j = j0; j < j1; j++) {
+    for (; j < t1; j += kWave) {
         while (nx <= j && v < n - 1) nx = xadj[++v + 1];
         f(v, j);
     }
@@ -103,10 +121,10 @@ __global__ void __launch_bounds__(kChkBS) k_check_entries(int32_t n, int64_t m, 
                                                          CheckState *s)
 {
     uint64_t wsum = 0, h0 = 0, h1 = 0;
-    for (int64_t t0 = ((int64_t)blockIdx.x * kChkBS + threadIdx.x) * kEPT; t0 < m;
-         t0 += (int64_t)gridDim.x * kTile) {
-        int64_t t1 = min(t0 + (int64_t)kEPT, m);
-        walk_entries(xadj, n, t0, t1, [&](int32_t v, int64_t j) {
+    const int64_t wave = (int64_t)blockIdx.x * (kChkBS / kWave) + threadIdx.x / kWave;
+    for (int64_t t0 = wave * kWaveTile; t0 < m; t0 += (int64_t)gridDim.x * kTile) {
+        const int64_t t1 = min(t0 + kWaveTile, m);
+        walk_tile(xadj, n, t0, t1, [&](int32_t v, int64_t j) {
             int32_t u = adjncy[j];
             int32_t w = adjwgt ? adjwgt[j] : 1;
             if (u < 0 || u >= n) atomicMin(&s->first[kChkNeighbour], (unsigned long long)v);
@@ -170,11 +188,11 @@ __global__ void __launch_bounds__(kChkBS) k_eval_entries(int32_t n, const eid_t 
 {
     const int64_t m = xadj[n];
     uint64_t cut = 0, bad = 0;
-    for (int64_t t0 = ((int64_t)blockIdx.x * kChkBS + threadIdx.x) * kEPT; t0 < m;
-         t0 += (int64_t)gridDim.x * kTile) {
-        int64_t t1 = min(t0 + (int64_t)kEPT, m);
+    const int64_t wave = (int64_t)blockIdx.x * (kChkBS / kWave) + threadIdx.x / kWave;
+    for (int64_t t0 = wave * kWaveTile; t0 < m; t0 += (int64_t)gridDim.x * kTile) {
+        const int64_t t1 = min(t0 + kWaveTile, m);
         int32_t pv = -1, cv = -1;
-        walk_entries(xadj, n, t0, t1, [&](int32_t v, int64_t j) {
+        walk_tile(xadj, n, t0, t1, [&](int32_t v, int64_t j) {
             if (v != cv) { cv = v; pv = part[v]; }
             int32_t u = adjncy[j];
             if (u < 0 || u >= n) { bad++; return; }

@@ -406,3 +406,22 @@ def test_pipeline_level_paths(hgp, orc, n, k):
     np.testing.assert_array_equal(p_g, p_o, err_msg=f"n={n} k={k}")
     assert st_g["cut"] == st_o["cut"] == orc.edge_cut(x, a, aw, p_g)
     assert st_g["level_iters"] == st_o["level_iters"]
+
+
+def test_mostly_isolated_graph(hgp, orc):
+    """40,000 vertices, 97.5 % of them isolated, beside a 1000-vertex random
+    graph: after one level the coarse graph has fewer than n/2 entries, so
+    coarsening stops with ~20,000 vertices and the initial partition's trials
+    run on that graph from global scratch sized by it.  The result equals the
+    oracle (16 trials keep the oracle's run to seconds)."""
+    xs, as_, vs, ws = random_graph(1000, 0.008, seed=31, wmax=5, vmax=3)
+    n = 40000
+    x = np.concatenate([xs, np.full(n - 1000, xs[-1], dtype=np.int64)])
+    vw = np.concatenate([vs, np.ones(n - 1000, np.int32)])
+    for k in (4, 37):
+        p_g, st_g = hgp.partition(x, as_, k, vw, ws, seed=3, trials=16)
+        p_o, st_o = orc.dga_partition(x, as_, k, vw, ws, seed=3, trials=16)
+        np.testing.assert_array_equal(p_g, p_o, err_msg=f"k={k}")
+        assert st_g["cut"] == st_o["cut"] == orc.edge_cut(x, as_, ws, p_g)
+        assert orc.balanced(vw, p_g, k)
+        assert st_g["levels"] == st_o["levels"] == 1 and st_g["coarsest_n"] > 20000
