@@ -12,10 +12,10 @@
 // Entries are walked in tiles of kEPT x 64 consecutive entries, one tile per
 // wave: lane l takes entries l, l + 64, ... (each load instruction reads 64
 // consecutive entries).  Two lanes find the rows of the tile's ends by a
-// binary search over xadj; every lane then finds the rows of its kEPT
-// entries by kEPT interleaved searches between those two rows (the same
-// number of halvings for all, over a few cached lines of xadj), so hub rows
-// cost no more than short ones and xadj is read about once.
+// binary search over xadj; every lane then finds the row of its first entry
+// by a search between those two rows (a few cached lines) and follows the
+// rows forward, so hub rows cost no more than short ones and xadj is read
+// about once.
 #include "hgp_check.hpp"
 #include "hgp_host.hpp"
 
@@ -55,25 +55,18 @@ __device__ inline void walk_tile(const eid_t *__restrict__ xadj, int32_t n, int6
     if (lane == 0) r = row_of(xadj, n, t0);
     if (lane == 1) r = row_of(xadj, n, t1 - 1);
     const int32_t r0 = __shfl(r, 0, kWave), r1 = __shfl(r, 1, kWave);
-    // the row of each entry lies in [r0, r1] when xadj is monotone; the
-    // searches stay inside that range whatever xadj holds
-    int32_t span = r1 >= r0 ? r1 - r0 + 1 : 1, steps = 0;
-    while ((1 << steps) < span) steps++;
-    int32_t lo[kEPT];
-#pragma unroll
-    for (int e = 0; e < kEPT; e++) lo[e] = r0;
-    for (int s = steps - 1; s >= 0; s--) {
-#pragma unroll
-        for (int e = 0; e < kEPT; e++) {
-            const int32_t mid = lo[e] + (1 << s);
-            const int64_t j = t0 + lane + (int64_t)e * kWave;
-            if (mid <= r1 && xadj[mid] <= j) lo[e] = mid;
-        }
+    int64_t j = t0 + lane;
+    if (j >= t1) return;
+    int32_t lo = r0, hi = r1 + 1;   // the row of j lies in [r0, r1] when xadj is monotone
+    while (hi - lo > 1) {
+        const int32_t mid = lo + (hi - lo) / 2;
+        if (xadj[mid] <= j) lo = mid; else hi = mid;
     }
-#pragma unroll
-    for (int e = 0; e < kEPT; e++) {
-        const int64_t j = t0 + lane + (int64_t)e * kWave;
-        if (j < t1) f(lo[e], j);
+    int32_t v = lo;
+    eid_t nx = xadj[v + 1];
+    for (; j < t1; j += kWave) {
+        while (nx <= j && v < n - 1) nx = xadj[++v + 1];
+        f(v, j);
     }
 }
 
