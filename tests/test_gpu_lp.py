@@ -104,3 +104,22 @@ def test_lp2_large_rmat(hgp, orc):
     np.testing.assert_array_equal(gw, ow)
     assert gran == oran
 
+
+
+@pytest.mark.parametrize("k", [1024, 4096])
+def test_lp_level_many_parts(hgp, orc, k):
+    """k of 1024 and 4096 (LDS rows of k ints per wave, k 8-byte keys per
+    block), on a 6000-vertex weighted RGG and on graphs with fewer vertices
+    than parts."""
+    cases = [hgp.gen_rgg3d(6000, hgp.rgg_radius(6000, 12), seed=k, vw=(1, 3), ew=(1, 9))]
+    x, a, w = csr_from_edges(5, [(0, 1), (1, 2), (3, 4)])
+    cases.append((x, a, np.ones(5, np.int32), w))
+    for ci, (x, a, vw, aw) in enumerate(cases):
+        n = len(x) - 1
+        where = np.random.default_rng(k + ci).integers(0, k, size=n).astype(np.int32)
+        tv = int(vw.sum())
+        print(f"case {ci} n={n} k={k}", flush=True)
+        ow, ocut, (i1, i2) = orc.lp_refine(x, a, vw, aw, k, tv, where)
+        gw, gcut, git = hgp.refine(x, a, vw, aw, k, tv, 1, where, 0, refiner=1)
+        np.testing.assert_array_equal(gw, ow)
+        assert gcut == ocut and git == i1 + i2
