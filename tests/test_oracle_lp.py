@@ -65,9 +65,10 @@ def test_overweight_compared_as_float(p0, over):
     # does not start; 368,651,400 converts to 368,651,392 and does.  Part 1
     # (347,489,000) is under its bound either way.  Heavy edges inside each
     # part keep the first loop still; vertex 0 borders part 1 by one edge.
-    # When the loop starts, round 1 moves vertex 0 into part 1, which makes
-    # part 1 over and part 0 under; round 2 moves it back (the rebalancer does
-    # not check the destination's bound), and so on for all 8k = 24 rounds,
+    # When the loop starts, its first pass moves vertex 0 into part 1, which
+    # makes part 1 over and part 0 under; the next moves it back (the
+    # rebalancer does not check the destination's bound), and so on for all
+    # 8k = 24 rounds,
     # an even number: the partition ends where it began.
     tot = 2**30
     p1 = 347489000
@@ -97,19 +98,19 @@ def test_one_winner_per_part():
 
 def test_ties_lowest_id_and_moved_lock():
     # two separate edges 0-1 and 2-3, sides 1 0 1 0: every vertex has gain 1.
-    # Round 1: side 0 takes 0 (not 2), side 1 takes 1 (not 3) -> 0 1 1 0.
-    # Round 2: 0 and 1 are locked; 2 -> 0 and 3 -> 1 -> 0 1 0 1.
-    # Round 3: every vertex has moved; the loop ends with cut 2.
+    # First pass: side 0 takes 0 (not 2), side 1 takes 1 (not 3) -> 0 1 1 0.
+    # Second: 0 and 1 are locked; 2 -> 0 and 3 -> 1 -> 0 1 0 1.
+    # Third: every vertex has moved; the loop ends with cut 2.
     w, cut, it = lp(4, [(0, 1), (2, 3)], [1, 0, 1, 0], 2)
     assert w.tolist() == [0, 1, 0, 1] and cut == 2 and it == (2, 0)
 
 
 def test_destination_is_lowest_part_of_max():
     # star 0-1 (w 2), 0-2 (w 2), 0-3 (w 1), parts 2 1 0 2, k = 3.
-    # Round 1: vertex 0 sees 2 in parts 0 and 1, 1 in its own: destination
+    # First pass: vertex 0 sees 2 in parts 0 and 1, 1 in its own: destination
     # part 0 (the lower id), gain 1.  Vertices 1 and 2 both want part 2 with
-    # gain 2: vertex 1 wins.  -> 0 2 0 2.  Round 2: vertex 3 (own part 2,
-    # neighbour 0 now in part 0) moves to 0 -> 0 2 0 0; round 3 has no
+    # gain 2: vertex 1 wins.  -> 0 2 0 2.  Second: vertex 3 (own part 2,
+    # neighbour 0 now in part 0) moves to 0 -> 0 2 0 0; the third has no
     # candidate.  Rebalancing (bounds 1.29 / 1.37 of total 4): part 0 (3) is
     # over, parts 1 (0) and 2 (1) under; vertex 0 borders part 2 through
     # vertex 1 and moves there -> 2 2 0 0.  Then parts 0 and 2 are over and
@@ -121,7 +122,7 @@ def test_destination_is_lowest_part_of_max():
 
 
 def test_rebalance_rounds_without_moves_run_to_threshold():
-    # all of a path in part 0 but vertex 5: round 1 moves 5 into part 0
+    # all of a path in part 0 but vertex 5: the first pass moves 5 into part 0
     # (gain 1), after which part 1 is empty and nothing borders it.  The
     # rebalancing loop (part 0 over, part 1 under) finds no boundary vertex
     # and runs its full threshold of 16 rounds.
