@@ -145,11 +145,18 @@ def measured_traffic(workload, kernel):
     path = os.path.join(ROOT, "profiles", f"{workload}_pmc_traffic.md")
     if not os.path.exists(path):
         return None
+    col = None
     for line in open(path):
+        if not line.lstrip().startswith("|"):
+            continue
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) == 6 and cells[0] == kernel:
+        if col is None:
+            if "traffic bytes" in cells:
+                col = cells.index("traffic bytes")
+            continue
+        if len(cells) > col and cells[0] == kernel:
             try:
-                return int(float(cells[5]))
+                return int(float(cells[col]))
             except ValueError:
                 return None
     return None
