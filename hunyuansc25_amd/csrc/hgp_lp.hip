@@ -445,7 +445,8 @@ int32_t ref_log2(int32_t a)
 
 int32_t lp_threshold(int32_t n, int32_t k)
 {
-    // hunyuangraph_GPU_LP.h:1251
+    // hunyuangraph_GPU_LP.h:1251 (which divides by zero for one part: no rounds then)
+    if (k < 2) return 0;
     return std::max(n / (6000 * ref_log2(k)), 8 * k);
 }
 

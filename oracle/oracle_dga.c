@@ -1311,9 +1311,11 @@ static int32_t ref_log2(int32_t a)
     return i - 1;
 }
 
-/* hunyuangraph_GPU_LP.h:1251: max(nvtxs / (6000 * log2(nparts)), 8 * nparts). */
+/* hunyuangraph_GPU_LP.h:1251: max(nvtxs / (6000 * log2(nparts)), 8 * nparts).
+ * The reference divides by zero for one part; no rounds then. */
 int32_t oracle_lp_threshold(int32_t n, int32_t k)
 {
+    if (k < 2) return 0;
     int32_t a = n / (6000 * ref_log2(k)), b = 8 * k;
     return a > b ? a : b;
 }

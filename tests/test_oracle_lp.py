@@ -40,6 +40,7 @@ def test_threshold_closed_form():
     assert O.lp_threshold(10**7, 256) == 2048       # 10^7 / 48000 = 208
     assert O.lp_threshold(10**8, 8) == 5555         # 10^8 / 18000
     assert O.lp_threshold(10**8, 7) == 8333         # floor(log2 7) = 2
+    assert O.lp_threshold(1000, 1) == 0             # one part: no rounds
     for k in (2, 3, 5, 8, 9, 255, 256, 257, 4096):
         for n in (0, 1, 5999, 6000, 10**6, 2**31 - 1):
             lg = k.bit_length() - 1
