@@ -138,6 +138,7 @@ constexpr int kRsBS = 256;
 constexpr int kRsIPT = 16;                 // rounds per tile
 constexpr int kRsTile = kRsBS * kRsIPT;    // 4096 items
 constexpr int kRsWaves = kRsBS / kWave;
+constexpr int kRsFuseTiles = 64;           // up to this many tiles the scatter scans the histogram itself
 
 template <class K>
 __global__ void __launch_bounds__(kRsBS) k_rs_hist(const K *__restrict__ keys, int64_t n, int shift, int ntiles,
@@ -205,7 +206,9 @@ __device__ inline void scatter_round(const K *__restrict__ kin, const uint32_t *
 
 // Stable scatter of one tile.  incl = inclusive scan of hist (digit-major),
 // so the tile's first slot for digit d is incl[d*ntiles+tile] -
-// hist[d*ntiles+tile].
+// hist[d*ntiles+tile].  Without incl (at most kRsFuseTiles tiles) the block
+// computes the same from hist: thread d sums digit d over all tiles and over
+// the tiles before its own, and a block scan over the digits adds the rest.
 template <class K>
 __global__ void __launch_bounds__(kRsBS) k_rs_scatter(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
                                                       K *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
@@ -215,9 +218,19 @@ __global__ void __launch_bounds__(kRsBS) k_rs_scatter(const K *__restrict__ kin,
     __shared__ uint32_t base_d[256];
     __shared__ uint32_t wcnt[kRsWaves][256];
     int t = threadIdx.x;
-    {
+    if (incl) {
         int64_t h = (int64_t)t * ntiles + blockIdx.x;
         base_d[t] = incl[h] - hist[h];
+    } else {
+        __shared__ uint32_t wsh[kRsWaves + 1];
+        uint32_t tot = 0, before = 0;
+        const uint32_t *row = hist + (int64_t)t * ntiles;
+        for (int tt = 0; tt < ntiles; tt++) {
+            const uint32_t x = row[tt];
+            tot += x;
+            before += tt < (int)blockIdx.x ? x : 0u;
+        }
+        base_d[t] = block_exclusive_scan<uint32_t>(tot, wsh, nullptr) + before;
     }
     int64_t tbase = (int64_t)blockIdx.x * kRsTile;
     for (int r = 0; r < kRsIPT; r++) {
@@ -289,11 +302,13 @@ void radix_impl(Ctx &c, K *keys, uint32_t *vals, int64_t n, int lo, int hi)
             k_rs_hist<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, n, shift, ntiles, hist);
             HGP_LAUNCH_CHECK();
         }
-        scan_impl<int32_t>(c, (const int32_t *)hist, (int32_t *)incl, (int64_t)ntiles * 256);
+        const bool fuse = ntiles <= kRsFuseTiles;
+        if (!fuse) scan_impl<int32_t>(c, (const int32_t *)hist, (int32_t *)incl, (int64_t)ntiles * 256);
         {
             KScope ks(c, sizeof(K) == 8 ? "k_rs_scatter<u64>" : "k_rs_scatter<u32>",
                       2.0 * (double)(sizeof(K) + 4) * (double)n);
-            k_rs_scatter<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, src_v, dst_k, dst_v, n, shift, ntiles, hist, incl);
+            k_rs_scatter<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, src_v, dst_k, dst_v, n, shift, ntiles, hist,
+                                                            fuse ? nullptr : incl);
             HGP_LAUNCH_CHECK();
         }
         std::swap(src_k, dst_k);
