@@ -160,6 +160,8 @@ __global__ void __launch_bounds__(kRsBS) k_rs_hist(const K *__restrict__ keys, i
 // ranked in index order; ranks inside a wave come from an 8-ballot digit
 // match, ranks across waves from the LDS counters wcnt.  base_d[d] holds the
 // next output slot of digit d and is advanced past the round's items.
+// Without vin an item's value is its index (the first pass of a sort that
+// returns the permutation).
 template <class K>
 __device__ inline void scatter_round(const K *__restrict__ kin, const uint32_t *__restrict__ vin,
                                      K *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n, int shift,
@@ -169,7 +171,7 @@ __device__ inline void scatter_round(const K *__restrict__ kin, const uint32_t *
     int64_t i = first + t;
     bool valid = i < n;
     K key = valid ? kin[i] : (K)0;
-    uint32_t val = valid ? vin[i] : 0u;
+    uint32_t val = valid ? (vin ? vin[i] : (uint32_t)i) : 0u;
     uint32_t d = (uint32_t)(key >> shift) & 255u;
 #pragma unroll
     for (int ww = 0; ww < kRsWaves; ww++) wcnt[ww][t] = 0;
@@ -245,7 +247,7 @@ __global__ void __launch_bounds__(kRsBS) k_rs_scatter(const K *__restrict__ kin,
 // block wrote in the pass before (ordered by the barriers at block scope).
 template <class K>
 __global__ void __launch_bounds__(kRsBS) k_rs_small(K *keys, uint32_t *vals, K *k2, uint32_t *v2, int32_t n,
-                                                    int lo, int hi)
+                                                    int lo, int hi, int index_vals)
 {
     __shared__ uint32_t cnt[256];
     __shared__ uint32_t base_d[256];
@@ -261,8 +263,9 @@ __global__ void __launch_bounds__(kRsBS) k_rs_small(K *keys, uint32_t *vals, K *
         __syncthreads();
         base_d[t] = block_exclusive_scan<uint32_t>(cnt[t], wsh, nullptr);
         __syncthreads();
+        const uint32_t *vin = shift == lo && index_vals ? nullptr : src_v;
         for (int32_t first = 0; first < n; first += kRsBS)
-            scatter_round<K>(src_k, src_v, dst_k, dst_v, n, shift, first, base_d, wcnt);
+            scatter_round<K>(src_k, vin, dst_k, dst_v, n, shift, first, base_d, wcnt);
         K *tk = src_k; src_k = dst_k; dst_k = tk;
         uint32_t *tv = src_v; src_v = dst_v; dst_v = tv;
     }
@@ -275,8 +278,9 @@ __global__ void __launch_bounds__(kRsBS) k_rs_small(K *keys, uint32_t *vals, K *
 }
 
 template <class K>
-void radix_impl(Ctx &c, K *keys, uint32_t *vals, int64_t n, int lo, int hi)
+void radix_impl(Ctx &c, K *keys, uint32_t *vals, int64_t n, int lo, int hi, bool index_vals = false)
 {
+    if (index_vals && (n == 1 || (n > 1 && hi <= lo))) iota(c, (int32_t *)vals, n);
     if (n <= 1 || hi <= lo) return;
     if (n > 0x7fffffffll) throw Error{-4, "radix_sort: more than 2^31 items"};
     size_t mark = c.arena.right_mark();
@@ -286,7 +290,7 @@ void radix_impl(Ctx &c, K *keys, uint32_t *vals, int64_t n, int lo, int hi)
     if (ntiles == 1) {
         KScope ks(c, sizeof(K) == 8 ? "k_rs_small<u64>" : "k_rs_small<u32>",
                   (double)((hi - lo + 7) / 8) * 3.0 * (double)(sizeof(K) + 4) * (double)n);
-        k_rs_small<K><<<1, kRsBS, 0, c.stream>>>(keys, vals, k2, v2, (int32_t)n, lo, hi);
+        k_rs_small<K><<<1, kRsBS, 0, c.stream>>>(keys, vals, k2, v2, (int32_t)n, lo, hi, index_vals ? 1 : 0);
         HGP_LAUNCH_CHECK();
         c.arena.right_release(mark);
         return;
@@ -307,7 +311,8 @@ void radix_impl(Ctx &c, K *keys, uint32_t *vals, int64_t n, int lo, int hi)
         {
             KScope ks(c, sizeof(K) == 8 ? "k_rs_scatter<u64>" : "k_rs_scatter<u32>",
                       2.0 * (double)(sizeof(K) + 4) * (double)n);
-            k_rs_scatter<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, src_v, dst_k, dst_v, n, shift, ntiles, hist,
+            const uint32_t *vin = shift == lo && index_vals ? nullptr : src_v;
+            k_rs_scatter<K><<<ntiles, kRsBS, 0, c.stream>>>(src_k, vin, dst_k, dst_v, n, shift, ntiles, hist,
                                                             fuse ? nullptr : incl);
             HGP_LAUNCH_CHECK();
         }
@@ -601,6 +606,11 @@ void radix_sort(Ctx &c, uint32_t *keys, uint32_t *vals, int64_t n, int lo, int h
 void radix_sort(Ctx &c, uint64_t *keys, uint32_t *vals, int64_t n, int lo, int hi)
 {
     radix_impl<uint64_t>(c, keys, vals, n, lo, hi);
+}
+
+void radix_sort_index(Ctx &c, uint32_t *keys, uint32_t *idx, int64_t n, int lo, int hi)
+{
+    radix_impl<uint32_t>(c, keys, idx, n, lo, hi, true);
 }
 
 template <class T>

@@ -21,6 +21,9 @@ void offsets_from_counts(Ctx &c, const int32_t *cnt, int32_t *off, int64_t n);
 // listed range, in place.  Ranges are processed least significant first.
 void radix_sort(Ctx &c, uint32_t *keys, uint32_t *vals, int64_t n, int lo, int hi);
 void radix_sort(Ctx &c, uint64_t *keys, uint32_t *vals, int64_t n, int lo, int hi);
+// The same sort of keys alone; idx receives the permutation (the original
+// index of each sorted key), as a sort of (key, index) pairs would.
+void radix_sort_index(Ctx &c, uint32_t *keys, uint32_t *idx, int64_t n, int lo, int hi);
 
 template <class T> void fill(Ctx &c, T *p, int64_t n, T v);
 void iota(Ctx &c, int32_t *p, int64_t n);

@@ -1240,12 +1240,6 @@ __global__ void k_lp_finish(int32_t n, const JetCtl *ctl, const uint8_t *__restr
 
 // ---------------------------------------------------------------- rebalancers
 
-__global__ void k_iota_u32(int32_t n, uint32_t *__restrict__ val)
-{
-    int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
-    if (v < n) val[v] = (uint32_t)v;
-}
-
 // After the sort by (key, id): each candidate's weight (scanned next into
 // the prefix weights of its part's candidates) and the first position of
 // each part in the sorted list.
@@ -2631,9 +2625,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
             k_lp_finish<<<gv, kBS, 0, c.stream>>>(n, ctl, sel, where, dest, lock, pd16, pd32);
             HGP_LAUNCH_CHECK();
         } else {
-            k_iota_u32<<<gv, kBS, 0, c.stream>>>(n, val);
-            HGP_LAUNCH_CHECK();
-            radix_sort(c, key, val, n, 0, bits_for((uint64_t)k * kMaxBuckets));
+            radix_sort_index(c, key, val, n, 0, bits_for((uint64_t)k * kMaxBuckets));
             k_rb_weights<<<gv, kBS, 0, c.stream>>>(n, k, key, val, g.vwgt, wsc, pstart);
             HGP_LAUNCH_CHECK();
             scan_inclusive(c, wsc, wsc, n);
