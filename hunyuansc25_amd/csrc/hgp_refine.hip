@@ -1631,13 +1631,18 @@ __global__ void __launch_bounds__(kSegBlk) k_move_delta_seg(RowSegs S, int32_t k
     if (threadIdx.x == 0 && bdelta) atomicAdd(&ctl->delta2, bdelta);
 }
 
-// Applies dest and refreshes the narrow copy of where.
+// Applies dest and refreshes the narrow copy of where.  First, when the
+// previous iteration's k_jet_next kept its partition (copy_best, still set
+// until this iteration's k_jet_next), copies that partition to best: the
+// copy costs no launch of its own (after the last iteration k_copy_best
+// makes it).
 __global__ void k_move_commit(int32_t n, const JetCtl *ctl, const int32_t *__restrict__ dest, int32_t *__restrict__ where,
-                              uint8_t *__restrict__ w8, uint16_t *__restrict__ w16)
+                              uint8_t *__restrict__ w8, uint16_t *__restrict__ w16, int32_t *__restrict__ best)
 {
     if (ctl->done) return;
     int32_t v = (int32_t)((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (v >= n) return;
+    if (ctl->copy_best) best[v] = where[v];
     int32_t d = dest[v];
     where[v] = d;
     if (w8) w8[v] = (uint8_t)d;
@@ -2694,11 +2699,9 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
                 HGP_CHECK_HIP(hipStreamSynchronize(c.stream));
             }
         }
-        k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where, w8, w16);
+        k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where, w8, w16, best);
         HGP_LAUNCH_CHECK();
         k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 0);
-        HGP_LAUNCH_CHECK();
-        k_copy_best<<<gv, kBS, 0, c.stream>>>(n, ctl, where, best);
         HGP_LAUNCH_CHECK();
     };
     auto issue_select = [&] {
@@ -2727,6 +2730,11 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         else issue_rest(h.mode, h.iters);
     }
     rs.iters = ((JetCtl *)c.pinned)->iters;
+    if (!tiny) {
+        // the last iteration's keep decision (the next commit would have made it)
+        k_copy_best<<<gv, kBS, 0, c.stream>>>(n, ctl, where, best);
+        HGP_LAUNCH_CHECK();
+    }
     HGP_CHECK_HIP(hipMemcpyAsync(where, best, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, c.stream));
     k_jet_fini<<<1, 1, 0, c.stream>>>(ctl, d_cut);
     HGP_LAUNCH_CHECK();
