@@ -268,12 +268,19 @@ __device__ inline void jet_next(JetCtl &C, int32_t k, const int64_t *__restrict_
     __syncthreads();
 }
 
+// mirror (host memory mapped for the device, may be null) receives a copy of
+// the control block for the host to read once the kernel has completed, in
+// place of a copy launch.
 __global__ void __launch_bounds__(kBS) k_jet_next(JetCtl *ctl, int32_t k, const int64_t *__restrict__ pwgts,
-                                                  int32_t *__restrict__ under, int32_t first)
+                                                  int32_t *__restrict__ under, int32_t first, JetCtl *mirror)
 {
     __shared__ NextScratch S;
-    if (ctl->done) return;
-    jet_next(*ctl, k, pwgts, under, first, S);
+    __shared__ int32_t was_done;
+    if (threadIdx.x == 0) was_done = ctl->done;
+    __syncthreads();
+    if (!was_done) jet_next(*ctl, k, pwgts, under, first, S);
+    __syncthreads();
+    if (mirror && threadIdx.x == 0) *mirror = *ctl;
 }
 
 // ---------------------------------------------------------------- selection
@@ -2574,8 +2581,11 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         }
         read_scalars(c, ctl, sizeof(JetCtl));
     }
+    // k_jet_next leaves a copy of the control block in the pinned buffer
+    JetCtl *mirror = nullptr;
+    if (!tiny) HGP_CHECK_HIP(hipHostGetDevicePointer((void **)&mirror, c.pinned, 0));
     if (!tiny) {
-        k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 1);
+        k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 1, mirror);
         HGP_LAUNCH_CHECK();
     }
     // The rest of an iteration once its mode is known: the mode's moves, the
@@ -2701,7 +2711,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         }
         k_move_commit<<<gv, kBS, 0, c.stream>>>(n, ctl, dest, where, w8, w16, best);
         HGP_LAUNCH_CHECK();
-        k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 0);
+        k_jet_next<<<1, kBS, 0, c.stream>>>(ctl, k, pwgts, under, 0, mirror);
         HGP_LAUNCH_CHECK();
     };
     auto issue_select = [&] {
@@ -2712,14 +2722,22 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
     // timed or iterations traced, which act between the launches).
     LevelGraphs graphs;
     graphs.on = !tiny && n <= tuning().graph_max_vertices && !ktimer().on && !jet_trace().on;
+    int32_t rounds = 0;   // iterations issued: the control block's count once they are done
     for (; !tiny;) {
         // the selection does not depend on the mode on the host: it runs while
-        // the control block travels (and does nothing once the loop is done)
-        read_scalars_start(c, ctl, sizeof(JetCtl));
+        // the host waits for the last k_jet_next (and does nothing once the
+        // loop is done)
+        HGP_CHECK_HIP(hipEventRecord(c.ev[7], c.stream));
         if (graphs.on) graphs.run(c, 0, issue_select);
         else issue_select();
         read_scalars_wait(c);
-        JetCtl h = *(JetCtl *)c.pinned;
+        JetCtl h;
+        std::memcpy(&h, c.pinned, sizeof(JetCtl));
+        if (h.iters != rounds) {
+            // the copy did not show this iteration's count: read the block itself
+            read_scalars(c, ctl, sizeof(JetCtl));
+            h = *(JetCtl *)c.pinned;
+        }
         if (jet_trace().on) {
             std::lock_guard<std::mutex> lk(jet_trace().mu);
             jet_trace().recs.insert(jet_trace().recs.end(),
@@ -2728,6 +2746,7 @@ RefineStats refine_level(Ctx &c, const DevGraph &g, int32_t k, int64_t tvwgt, bo
         if (h.done) break;
         if (graphs.on) graphs.run(c, 1 + h.mode, [&] { issue_rest(h.mode, h.iters); });
         else issue_rest(h.mode, h.iters);
+        rounds++;
     }
     rs.iters = ((JetCtl *)c.pinned)->iters;
     if (!tiny) {
